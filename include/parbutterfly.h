@@ -1,0 +1,177 @@
+/*
+ * parbutterfly.h -- C ABI of the MI355X butterfly counting / peeling library
+ * (libparbutterfly_amd.so).  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Every entry point returns a pbf_status code (0 = success) and never exits or prints,
+ * replacing the reference's print-and-exit error convention (apps/hyper/butterfly_utils.h:
+ * 1013-1016, 1039-1041, 1049-1051).
+ *
+ * Graph convention (apps/hyper/butterfly_utils.h:184-197, bipartiteCSR): V has nv vertices,
+ * U has nu; offsetsV[nv+1] / edgesV[m] list the U-neighbours of each V vertex and
+ * offsetsU[nu+1] / edgesU[m] the V-neighbours of each U vertex.  Every list must be strictly
+ * ascending (sorted, duplicate free) -- the reference assumes this without checking; this
+ * library checks it (pbf_validate) and returns PBF_ERR_UNSORTED.  Ids and offsets are 32-bit
+ * as in the reference's default build (uintE / uintT, ligra/parallel.h:102-125); 2*m must be
+ * below 2^31 (the reference's intT offsets, Appendix-A limit) else PBF_ERR_CAPACITY.
+ *
+ * Count side: use_v != 0 means counts live on U (S = U, O = V), else on V (S = V, O = U),
+ * exactly the reference's cmpWedgeCounts flag (butterfly_utils.h:1080-1086).  Per-vertex
+ * outputs have n_S entries indexed by original id within S; per-edge outputs have m entries
+ * indexed by position in O's CSR (edgesV if use_v, else edgesU) -- the reference's output
+ * contract (butterfly_count.h:1600-1610, butterfly_ecount.h:1723-1741).  Counts are uint64.
+ *
+ * Memory: `mem` says where BOTH the graph arrays and the output / count arrays live:
+ * PBF_MEM_HOST (pageable host memory; the library copies) or PBF_MEM_DEVICE (HIP device
+ * pointers on the context's device; nothing crosses PCIe).
+ */
+#ifndef PARBUTTERFLY_H
+#define PARBUTTERFLY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBF_ABI_VERSION 1
+
+typedef enum {
+  PBF_OK = 0,
+  PBF_ERR_ARG = 1,         /* null pointer, bad enum value, bad size */
+  PBF_ERR_FORMAT = 2,      /* not an AdjacencyHypergraph file / length mismatch / mv != mu */
+  PBF_ERR_RANGE = 3,       /* neighbour id out of range (butterfly_utils.h:1039-1041, 1049-1051) */
+  PBF_ERR_UNSORTED = 4,    /* adjacency list not strictly ascending, or offsets decreasing */
+  PBF_ERR_OOM = 5,         /* host or device allocation failed */
+  PBF_ERR_CAPACITY = 6,    /* sizes beyond the 32-bit index domain, or a fixed-capacity limit */
+  PBF_ERR_DOMAIN = 7,      /* input outside the documented parity domain */
+  PBF_ERR_HIP = 8,         /* HIP runtime failure (no device, launch or copy error) */
+  PBF_ERR_UNSUPPORTED = 9, /* combination of options not implemented by this library */
+  PBF_ERR_IO = 10          /* file could not be opened / read / written */
+} pbf_status;
+
+/* Enums carry the reference's numeric values (apps/hyper/butterfly_utils.h:24-28); the GPU
+ * modes are appended after the reference's last value. */
+enum { PBF_COUNT_SORT = 0, PBF_COUNT_ASORT, PBF_COUNT_HASH, PBF_COUNT_AHASH, PBF_COUNT_HIST,
+       PBF_COUNT_AHIST, PBF_COUNT_BATCHS, PBF_COUNT_BATCHWA, PBF_COUNT_SERIAL, PBF_COUNT_GPU = 9 };
+enum { PBF_RANK_SIDE = 0, PBF_RANK_COCORE, PBF_RANK_ACOCORE, PBF_RANK_DEG, PBF_RANK_ADEG };
+enum { PBF_PEEL_SORT = 0, PBF_PEEL_HASH, PBF_PEEL_HIST, PBF_PEEL_BATCHS, PBF_PEEL_BATCHWA,
+       PBF_PEEL_GPU = 5 };
+enum { PBF_PER_VERT = 0, PBF_PER_EDGE, PBF_PER_TOTAL };
+enum { PBF_MEM_HOST = 0, PBF_MEM_DEVICE = 1 };
+
+typedef struct {
+  int64_t nv, nu, m;
+  const uint32_t* offsetsV; /* nv+1 */
+  const uint32_t* edgesV;   /* m    */
+  const uint32_t* offsetsU; /* nu+1 */
+  const uint32_t* edgesU;   /* m    */
+} pbf_bipartite_csr;
+
+/* Host graph owned by the library (reader / generators); release with pbf_graph_free. */
+typedef struct {
+  pbf_bipartite_csr csr;
+  void* storage;
+} pbf_graph;
+
+/* Phase timings of the last count / peel call on a context, from HIP events (ms). */
+typedef struct {
+  double rank_ms;     /* ranking (ADEG / ACOCORE keys, sort, rank maps) */
+  double relabel_ms;  /* ranked CSR build (rankGraph / rankGraphEdges) */
+  double count_ms;    /* wedge enumeration + aggregation + count scatter */
+  double convert_ms;  /* rank -> original id conversion */
+  double peel_ms;     /* peeling rounds */
+  double total_ms;    /* whole call, first kernel to last */
+  int64_t wedges;     /* W = rank-filtered wedges processed (rankWedgeF) */
+  int64_t rounds;     /* peeling rounds */
+  int64_t hub_vertices;   /* endpoints that took the HBM-histogram path */
+} pbf_timing;
+
+const char* pbf_status_str(int status);
+int pbf_abi_version(void);
+
+/* ---------------- host side: graphs, IO, generators (no GPU needed) ---------------- */
+
+/* readBipartite (apps/hyper/butterfly_utils.h:1008-1058): AdjacencyHypergraph text, or the
+ * library's binary "PBFCSR1" layout (detected by magic).  Validates like pbf_validate. */
+int pbf_read_bipartite(const char* path, pbf_graph* out);
+/* Writes AdjacencyHypergraph text (utils/graphIO.h:779-800 layout) or PBFCSR1 binary. */
+int pbf_write_bipartite(const char* path, const pbf_bipartite_csr* g);
+int pbf_write_binary(const char* path, const pbf_bipartite_csr* g);
+/* Range, order and size checks on a host graph. */
+int pbf_validate(const pbf_bipartite_csr* g);
+/* Allocate an empty host graph with the given sizes (offsets zeroed) / release any pbf_graph. */
+int pbf_graph_alloc(int64_t nv, int64_t nu, int64_t m, pbf_graph* out);
+void pbf_graph_free(pbf_graph* g);
+/* Build both CSRs from an edge list (v[i] in [0,nv), u[i] in [0,nu)); duplicates removed. */
+int pbf_graph_from_edges(int64_t nv, int64_t nu, int64_t ne, const uint32_t* v, const uint32_t* u,
+                         pbf_graph* out);
+
+/* Seeded generators.  Edges are drawn with a counter-based RNG; the graph is the first m
+ * DISTINCT pairs in draw order, so the same arguments give the same graph on any machine. */
+int pbf_gen_uniform(int64_t nv, int64_t nu, int64_t m, uint64_t seed, pbf_graph* out);
+/* Chung-Lu: endpoint weights w_i = (i+1)^(-1/(gamma-1)) on each side, endpoints drawn
+ * independently with probability proportional to weight. */
+int pbf_gen_chung_lu(int64_t nv, int64_t nu, int64_t m, double gamma_v, double gamma_u,
+                     uint64_t seed, pbf_graph* out);
+/* Rectangular R-MAT over 2^log_nv x 2^log_nu with quadrant probabilities a, b, c, d. */
+int pbf_gen_rmat(int log_nv, int log_nu, int64_t m, double a, double b, double c, double d,
+                 uint64_t seed, pbf_graph* out);
+/* Parameters of graph `index` of the mixed batch distribution: nv, nu ~ logU[64, 65536],
+ * average degree ~ logU[2, 32], m = round(avg*(nv+nu)/2) capped at nv*nu/2, gamma ~ U[2,3]. */
+int pbf_batch_graph_params(uint64_t seed, int64_t* nv, int64_t* nu, int64_t* m, double* gamma);
+int pbf_gen_batch_graph(uint64_t seed, pbf_graph* out);
+
+/* cmpWedgeCounts on a host graph (butterfly_utils.h:1080-1086). */
+int pbf_side_host(const pbf_bipartite_csr* g, int* use_v, int64_t* num_wedges);
+
+/* ---------------- GPU context ---------------- */
+
+typedef struct pbf_context pbf_context;
+
+/* Creates a context on HIP device `device` with its own default-priority stream.  Device
+ * scratch is cached in the context and grows on demand; one context per host thread. */
+int pbf_create(pbf_context** out, int device);
+int pbf_destroy(pbf_context* ctx);
+/* The context's HIP stream (hipStream_t) as an opaque pointer. */
+void* pbf_stream(pbf_context* ctx);
+int pbf_synchronize(pbf_context* ctx);
+int pbf_last_timing(pbf_context* ctx, pbf_timing* out);
+/* Releases cached device scratch. */
+int pbf_trim(pbf_context* ctx);
+
+/* cmpWedgeCounts (butterfly_utils.h:1061-1086) on device or host graphs. */
+int pbf_side(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int* use_v, int64_t* num_wedges);
+
+/* getRanks / getApproxDegRanks / getApproxCoCoreRanks (butterfly_utils.h:249-284, 448-503)
+ * plus rankWedgeF (:216-237).  order[r] = combined id of rank r (V ids 0..nv-1, U ids
+ * nv..nv+nu-1); rank_of[id] = r; *wedges = W under this order.  Ties inside a key are broken
+ * by ascending id (the reference's sampleSort leaves them unspecified).  Any of the output
+ * pointers may be NULL.  rank_type: PBF_RANK_ADEG or PBF_RANK_ACOCORE. */
+int pbf_rank(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int rank_type,
+             uint32_t* order, uint32_t* rank_of, int64_t* wedges);
+
+/* Count (butterfly_count.h:1634): per-vertex butterfly counts on S, n_S = use_v ? nu : nv. */
+int pbf_count_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
+                   int count_type, int rank_type, uint64_t* out);
+/* CountE (butterfly_ecount.h:1766): per-edge counts, m entries in O-CSR order. */
+int pbf_count_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
+                   int count_type, int rank_type, uint64_t* out);
+/* CountTotal (butterfly_count_total.h:1014): total count, written to host *total. */
+int pbf_count_total(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
+                    int count_type, int rank_type, uint64_t* total);
+/* edgeToIdxs (butterfly_utils.h:2034-2062): eti[p] = O-CSR position of S-CSR position p. */
+int pbf_edge_to_idxs(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
+                     uint32_t* eti);
+/* Peel (butterfly_peel.h:896): tip numbers on S from per-vertex counts (n_S entries). */
+int pbf_peel_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
+                  int peel_type, const uint64_t* counts, uint64_t* tip);
+/* PeelE (butterfly_epeel.h:519): wing numbers from per-edge counts (m entries, O-CSR order). */
+int pbf_peel_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
+                  int peel_type, const uint64_t* counts, uint64_t* wing);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARBUTTERFLY_H */

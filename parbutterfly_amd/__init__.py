@@ -1,0 +1,411 @@
+"""parbutterfly_amd -- butterfly counting and peeling on AMD MI355X (gfx950).
+
+Python view of the C ABI in include/parbutterfly.h (libparbutterfly_amd.so).  It mirrors the
+reference's interface for this path:
+
+  reference (apps/hyper)                       here
+  -----------------------------------------    -------------------------------------------
+  readBipartite (butterfly_utils.h:1008)       Graph.read(path)
+  cmpWedgeCounts (butterfly_utils.h:1080)      Context.side(g) -> (use_v, num_wedges)
+  getRanks / getApproxDegRanks / ...           Context.rank(g, rank_type)
+  Count (butterfly_count.h:1634)               Context.count(g, use_v, "VERT", ...)
+  CountE (butterfly_ecount.h:1766)             Context.count(g, use_v, "EDGE", ...)
+  CountTotal (butterfly_count_total.h:1014)    Context.count(g, use_v, "TOTAL", ...)
+  edgeToIdxs (butterfly_utils.h:2034)          Context.edge_to_idxs(g, use_v)
+  Peel (butterfly_peel.h:896)                  Context.peel(g, use_v, counts)
+  PeelE (butterfly_epeel.h:519)                Context.peel_edge(g, use_v, counts)
+
+Count / rank / peel types accept the reference's names ("BATCHS", "ADEG", ...) or its numeric
+enum values; every count type runs on the GPU ("GPU" names the mode explicitly).  Errors raise
+PbfError carrying the library's status code; nothing falls back to a CPU path: if the shared
+library is missing, importing this package raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libparbutterfly_amd.so")
+
+if not os.path.exists(_LIB_PATH):
+    raise ImportError(
+        f"{_LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+        "or `python parbutterfly_amd/build.py` first")
+_lib = ctypes.CDLL(_LIB_PATH)
+
+# enums (apps/hyper/butterfly_utils.h:24-28, values kept)
+COUNT_TYPES = {"SORT": 0, "ASORT": 1, "HASH": 2, "AHASH": 3, "HIST": 4, "AHIST": 5, "BATCHS": 6,
+               "BATCHWA": 7, "SERIAL": 8, "GPU": 9}
+RANK_TYPES = {"SIDE": 0, "COCORE": 1, "ACOCORE": 2, "DEG": 3, "ADEG": 4}
+PEEL_TYPES = {"SORT": 0, "HASH": 1, "HIST": 2, "BATCHS": 3, "BATCHWA": 4, "GPU": 5}
+PER_TYPES = {"VERT": 0, "EDGE": 1, "TOTAL": 2}
+MEM_HOST, MEM_DEVICE = 0, 1
+
+STATUS = {0: "PBF_OK", 1: "PBF_ERR_ARG", 2: "PBF_ERR_FORMAT", 3: "PBF_ERR_RANGE", 4: "PBF_ERR_UNSORTED",
+          5: "PBF_ERR_OOM", 6: "PBF_ERR_CAPACITY", 7: "PBF_ERR_DOMAIN", 8: "PBF_ERR_HIP",
+          9: "PBF_ERR_UNSUPPORTED", 10: "PBF_ERR_IO"}
+
+
+class PbfError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        self.name = STATUS.get(status, str(status))
+        msg = _lib.pbf_status_str(status).decode()
+        super().__init__(f"{where}: {self.name} ({msg})")
+
+
+class _CSR(ctypes.Structure):
+    _fields_ = [("nv", ctypes.c_int64), ("nu", ctypes.c_int64), ("m", ctypes.c_int64),
+                ("offsetsV", ctypes.c_void_p), ("edgesV", ctypes.c_void_p),
+                ("offsetsU", ctypes.c_void_p), ("edgesU", ctypes.c_void_p)]
+
+
+class _Graph(ctypes.Structure):
+    _fields_ = [("csr", _CSR), ("storage", ctypes.c_void_p)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("rank_ms", ctypes.c_double), ("relabel_ms", ctypes.c_double), ("count_ms", ctypes.c_double),
+                ("convert_ms", ctypes.c_double), ("peel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("wedges", ctypes.c_int64), ("rounds", ctypes.c_int64), ("hub_vertices", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+_INT = ctypes.c_int
+_PI = ctypes.POINTER(ctypes.c_int)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+_PCSR = ctypes.POINTER(_CSR)
+_PG = ctypes.POINTER(_Graph)
+
+_SIGS = {
+    "pbf_status_str": (ctypes.c_char_p, [_INT]),
+    "pbf_abi_version": (_INT, []),
+    "pbf_read_bipartite": (_INT, [ctypes.c_char_p, _PG]),
+    "pbf_write_bipartite": (_INT, [ctypes.c_char_p, _PCSR]),
+    "pbf_write_binary": (_INT, [ctypes.c_char_p, _PCSR]),
+    "pbf_validate": (_INT, [_PCSR]),
+    "pbf_graph_alloc": (_INT, [_I64, _I64, _I64, _PG]),
+    "pbf_graph_free": (None, [_PG]),
+    "pbf_graph_from_edges": (_INT, [_I64, _I64, _I64, _P, _P, _PG]),
+    "pbf_gen_uniform": (_INT, [_I64, _I64, _I64, _U64, _PG]),
+    "pbf_gen_chung_lu": (_INT, [_I64, _I64, _I64, ctypes.c_double, ctypes.c_double, _U64, _PG]),
+    "pbf_gen_rmat": (_INT, [_INT, _INT, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                            _U64, _PG]),
+    "pbf_batch_graph_params": (_INT, [_U64, _PI64, _PI64, _PI64, ctypes.POINTER(ctypes.c_double)]),
+    "pbf_gen_batch_graph": (_INT, [_U64, _PG]),
+    "pbf_side_host": (_INT, [_PCSR, _PI, _PI64]),
+    "pbf_create": (_INT, [ctypes.POINTER(_P), _INT]),
+    "pbf_destroy": (_INT, [_P]),
+    "pbf_stream": (_P, [_P]),
+    "pbf_synchronize": (_INT, [_P]),
+    "pbf_last_timing": (_INT, [_P, ctypes.POINTER(Timing)]),
+    "pbf_trim": (_INT, [_P]),
+    "pbf_side": (_INT, [_P, _PCSR, _INT, _PI, _PI64]),
+    "pbf_rank": (_INT, [_P, _PCSR, _INT, _INT, _P, _P, _PI64]),
+    "pbf_count_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
+    "pbf_count_edge": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
+    "pbf_count_total": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
+    "pbf_edge_to_idxs": (_INT, [_P, _PCSR, _INT, _INT, _P]),
+    "pbf_peel_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
+    "pbf_peel_edge": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+
+def _check(status, where):
+    if status != 0:
+        raise PbfError(status, where)
+
+
+def _enum(table, value, what):
+    if isinstance(value, str):
+        if value not in table:
+            raise ValueError(f"unknown {what} {value!r}")
+        return table[value]
+    return int(value)
+
+
+def library_path():
+    return _LIB_PATH
+
+
+class Graph:
+    """Host bipartite graph in the reference's bipartiteCSR layout (uint32 offsets / ids).
+
+    Arrays are numpy views.  Graphs made by the library (reader, generators) own C storage;
+    graphs made from numpy arrays keep references to them.
+    """
+
+    def __init__(self, nv, nu, offsetsV, edgesV, offsetsU, edgesU, _owner=None):
+        self.nv, self.nu = int(nv), int(nu)
+        self.offsetsV = np.ascontiguousarray(offsetsV, dtype=np.uint32)
+        self.edgesV = np.ascontiguousarray(edgesV, dtype=np.uint32)
+        self.offsetsU = np.ascontiguousarray(offsetsU, dtype=np.uint32)
+        self.edgesU = np.ascontiguousarray(edgesU, dtype=np.uint32)
+        self.m = int(self.edgesV.shape[0])
+        self._owner = _owner
+        self._csr = _CSR(self.nv, self.nu, self.m, self.offsetsV.ctypes.data, self.edgesV.ctypes.data,
+                         self.offsetsU.ctypes.data, self.edgesU.ctypes.data)
+
+    # -- construction --------------------------------------------------------------------
+    @staticmethod
+    def _adopt(g):
+        c = g.csr
+        nv, nu, m = c.nv, c.nu, c.m
+
+        def arr(ptr, n):
+            if n == 0:
+                return np.zeros(0, dtype=np.uint32)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint32)), shape=(n,))
+
+        owner = _Owner(g)
+        return Graph(nv, nu, arr(c.offsetsV, nv + 1), arr(c.edgesV, m), arr(c.offsetsU, nu + 1), arr(c.edgesU, m),
+                     _owner=owner)
+
+    @staticmethod
+    def read(path):
+        """readBipartite: AdjacencyHypergraph text or PBFCSR1 binary."""
+        g = _Graph()
+        _check(_lib.pbf_read_bipartite(os.fsencode(path), ctypes.byref(g)), "pbf_read_bipartite")
+        return Graph._adopt(g)
+
+    @staticmethod
+    def from_edges(nv, nu, v, u):
+        v = np.ascontiguousarray(v, dtype=np.uint32)
+        u = np.ascontiguousarray(u, dtype=np.uint32)
+        g = _Graph()
+        _check(_lib.pbf_graph_from_edges(nv, nu, v.shape[0], v.ctypes.data, u.ctypes.data, ctypes.byref(g)),
+               "pbf_graph_from_edges")
+        return Graph._adopt(g)
+
+    @staticmethod
+    def uniform(nv, nu, m, seed):
+        g = _Graph()
+        _check(_lib.pbf_gen_uniform(nv, nu, m, seed, ctypes.byref(g)), "pbf_gen_uniform")
+        return Graph._adopt(g)
+
+    @staticmethod
+    def chung_lu(nv, nu, m, gamma_v, gamma_u, seed):
+        g = _Graph()
+        _check(_lib.pbf_gen_chung_lu(nv, nu, m, gamma_v, gamma_u, seed, ctypes.byref(g)), "pbf_gen_chung_lu")
+        return Graph._adopt(g)
+
+    @staticmethod
+    def rmat(log_nv, log_nu, m, a, b, c, d, seed):
+        g = _Graph()
+        _check(_lib.pbf_gen_rmat(log_nv, log_nu, m, a, b, c, d, seed, ctypes.byref(g)), "pbf_gen_rmat")
+        return Graph._adopt(g)
+
+    @staticmethod
+    def batch_params(seed):
+        nv, nu, m, gamma = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        _check(_lib.pbf_batch_graph_params(seed, ctypes.byref(nv), ctypes.byref(nu), ctypes.byref(m),
+                                           ctypes.byref(gamma)), "pbf_batch_graph_params")
+        return nv.value, nu.value, m.value, gamma.value
+
+    @staticmethod
+    def batch_graph(seed):
+        g = _Graph()
+        _check(_lib.pbf_gen_batch_graph(seed, ctypes.byref(g)), "pbf_gen_batch_graph")
+        return Graph._adopt(g)
+
+    # -- IO / checks ---------------------------------------------------------------------
+    def write(self, path):
+        _check(_lib.pbf_write_bipartite(os.fsencode(path), ctypes.byref(self._csr)), "pbf_write_bipartite")
+
+    def write_binary(self, path):
+        _check(_lib.pbf_write_binary(os.fsencode(path), ctypes.byref(self._csr)), "pbf_write_binary")
+
+    def validate(self):
+        return _lib.pbf_validate(ctypes.byref(self._csr))
+
+    def side(self):
+        """cmpWedgeCounts on the host: (use_v, num_wedges)."""
+        uv, nw = ctypes.c_int(), ctypes.c_int64()
+        _check(_lib.pbf_side_host(ctypes.byref(self._csr), ctypes.byref(uv), ctypes.byref(nw)), "pbf_side_host")
+        return bool(uv.value), nw.value
+
+    @property
+    def n(self):
+        return self.nv + self.nu
+
+    def to_device(self, device="cuda"):
+        """Copies the four arrays to HIP device memory (torch tensors); returns a DeviceGraph."""
+        return DeviceGraph(self, device)
+
+
+class _Owner:
+    def __init__(self, g):
+        self.g = g
+
+    def __del__(self):
+        try:
+            _lib.pbf_graph_free(ctypes.byref(self.g))
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """A graph whose arrays live in HIP device memory (torch tensors on the GPU)."""
+
+    def __init__(self, g, device="cuda"):
+        import torch
+        self.nv, self.nu, self.m = g.nv, g.nu, g.m
+        t = lambda a: torch.from_numpy(a.view(np.int32)).to(device)
+        self.offsetsV, self.edgesV = t(g.offsetsV), t(g.edgesV)
+        self.offsetsU, self.edgesU = t(g.offsetsU), t(g.edgesU)
+        self._csr = _CSR(self.nv, self.nu, self.m, self.offsetsV.data_ptr(), self.edgesV.data_ptr(),
+                         self.offsetsU.data_ptr(), self.edgesU.data_ptr())
+
+    @property
+    def n(self):
+        return self.nv + self.nu
+
+
+class Context:
+    """A GPU context (device, stream, cached scratch).  One per host thread."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        _check(_lib.pbf_create(ctypes.byref(self._ctx), int(device)), "pbf_create")
+        self.device = int(device)
+
+    def close(self):
+        if self._ctx:
+            _lib.pbf_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return _lib.pbf_stream(self._ctx)
+
+    def synchronize(self):
+        _check(_lib.pbf_synchronize(self._ctx), "pbf_synchronize")
+
+    def trim(self):
+        _check(_lib.pbf_trim(self._ctx), "pbf_trim")
+
+    def timing(self):
+        t = Timing()
+        _check(_lib.pbf_last_timing(self._ctx, ctypes.byref(t)), "pbf_last_timing")
+        return t.as_dict()
+
+    @staticmethod
+    def _mem(g):
+        return MEM_DEVICE if isinstance(g, DeviceGraph) else MEM_HOST
+
+    def _alloc_out(self, g, n, dtype):
+        if isinstance(g, DeviceGraph):
+            import torch
+            td = {np.uint64: torch.int64, np.uint32: torch.int32}[dtype]
+            return torch.empty(max(n, 1), dtype=td, device=g.offsetsV.device)[:n] if n else \
+                torch.empty(0, dtype=td, device=g.offsetsV.device)
+        return np.zeros(n, dtype=dtype)
+
+    @staticmethod
+    def _ptr(a):
+        if isinstance(a, np.ndarray):
+            return a.ctypes.data
+        return a.data_ptr()
+
+    def side(self, g):
+        uv, nw = ctypes.c_int(), ctypes.c_int64()
+        _check(_lib.pbf_side(self._ctx, ctypes.byref(g._csr), self._mem(g), ctypes.byref(uv), ctypes.byref(nw)),
+               "pbf_side")
+        return bool(uv.value), nw.value
+
+    def rank(self, g, rank_type="ADEG"):
+        """Returns (order, rank_of, W) as uint32 arrays (torch tensors for device graphs)."""
+        order = self._alloc_out(g, g.n, np.uint32)
+        rank_of = self._alloc_out(g, g.n, np.uint32)
+        w = ctypes.c_int64()
+        _check(_lib.pbf_rank(self._ctx, ctypes.byref(g._csr), self._mem(g), _enum(RANK_TYPES, rank_type, "rank type"),
+                             self._ptr(order) if g.n else None, self._ptr(rank_of) if g.n else None, ctypes.byref(w)),
+               "pbf_rank")
+        return order, rank_of, w.value
+
+    def count(self, g, use_v, per="VERT", count_type="GPU", rank_type="ADEG", out=None):
+        """Count / CountE / CountTotal.  VERT -> uint64[n_S]; EDGE -> uint64[m]; TOTAL -> int."""
+        per = _enum(PER_TYPES, per, "per type")
+        ct = _enum(COUNT_TYPES, count_type, "count type")
+        rt = _enum(RANK_TYPES, rank_type, "rank type")
+        mem = self._mem(g)
+        if per == 2:
+            tot = np.zeros(1, dtype=np.uint64)
+            if mem == MEM_DEVICE:
+                import torch
+                dt = torch.zeros(1, dtype=torch.int64, device=g.offsetsV.device)
+                _check(_lib.pbf_count_total(self._ctx, ctypes.byref(g._csr), mem, int(bool(use_v)), ct, rt,
+                                            dt.data_ptr()), "pbf_count_total")
+                return int(dt.cpu().numpy().view(np.uint64)[0])
+            _check(_lib.pbf_count_total(self._ctx, ctypes.byref(g._csr), mem, int(bool(use_v)), ct, rt,
+                                        tot.ctypes.data), "pbf_count_total")
+            return int(tot[0])
+        n_out = (g.nu if use_v else g.nv) if per == 0 else g.m
+        if out is None:
+            out = self._alloc_out(g, n_out, np.uint64)
+        fn = _lib.pbf_count_vert if per == 0 else _lib.pbf_count_edge
+        _check(fn(self._ctx, ctypes.byref(g._csr), mem, int(bool(use_v)), ct, rt, self._ptr(out) if n_out else 1),
+               "pbf_count_vert" if per == 0 else "pbf_count_edge")
+        return out
+
+    def edge_to_idxs(self, g, use_v):
+        eti = self._alloc_out(g, g.m, np.uint32)
+        _check(_lib.pbf_edge_to_idxs(self._ctx, ctypes.byref(g._csr), self._mem(g), int(bool(use_v)),
+                                     self._ptr(eti) if g.m else 1), "pbf_edge_to_idxs")
+        return eti
+
+    def peel(self, g, use_v, counts, peel_type="GPU"):
+        """Peel: tip numbers on S from per-vertex counts."""
+        n_s = g.nu if use_v else g.nv
+        counts = self._as_u64(g, counts)
+        tip = self._alloc_out(g, n_s, np.uint64)
+        _check(_lib.pbf_peel_vert(self._ctx, ctypes.byref(g._csr), self._mem(g), int(bool(use_v)),
+                                  _enum(PEEL_TYPES, peel_type, "peel type"), self._ptr(counts) if n_s else 1,
+                                  self._ptr(tip) if n_s else 1), "pbf_peel_vert")
+        return tip
+
+    def peel_edge(self, g, use_v, counts, peel_type="GPU"):
+        """PeelE: wing numbers from per-edge counts (O-CSR order)."""
+        counts = self._as_u64(g, counts)
+        wing = self._alloc_out(g, g.m, np.uint64)
+        _check(_lib.pbf_peel_edge(self._ctx, ctypes.byref(g._csr), self._mem(g), int(bool(use_v)),
+                                  _enum(PEEL_TYPES, peel_type, "peel type"), self._ptr(counts) if g.m else 1,
+                                  self._ptr(wing) if g.m else 1), "pbf_peel_edge")
+        return wing
+
+    @staticmethod
+    def _as_u64(g, counts):
+        if isinstance(g, DeviceGraph):
+            return counts
+        return np.ascontiguousarray(counts, dtype=np.uint64)
+
+
+def as_u64(x):
+    """Device (torch int64) or host array -> numpy uint64."""
+    if isinstance(x, np.ndarray):
+        return x.astype(np.uint64, copy=False)
+    return x.detach().cpu().numpy().view(np.uint64)

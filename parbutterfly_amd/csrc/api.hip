@@ -1,0 +1,278 @@
+// C ABI of libparbutterfly_amd: context management and the count / rank / peel entry points.
+// Each entry point mirrors a reference function (cited in include/parbutterfly.h) and returns
+// a pbf_status code instead of printing and exiting.
+#include <cstring>
+
+#include "pbf_device.h"
+
+namespace {
+int alloc_dev(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return PBF_ERR_OOM;
+  }
+  return PBF_OK;
+}
+
+// Validates the descriptor sizes (the graph contents are validated by pbf_validate on host
+// graphs; device graphs are trusted to satisfy the same contract).
+int check_desc(const pbf_bipartite_csr* g) {
+  if (!g || g->nv < 0 || g->nu < 0 || g->m < 0) return PBF_ERR_ARG;
+  if (!g->offsetsV || !g->offsetsU || (g->m > 0 && (!g->edgesV || !g->edgesU))) return PBF_ERR_ARG;
+  if (g->nv + g->nu >= (int64_t(1) << 31) || 2 * g->m >= (int64_t(1) << 31)) return PBF_ERR_CAPACITY;
+  return PBF_OK;
+}
+
+// Resolves the graph to device pointers, uploading a host graph into the context.
+int resolve_graph(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, DevGraph* dg) {
+  PBF_CHECK(check_desc(g));
+  dg->nv = g->nv; dg->nu = g->nu; dg->m = g->m;
+  if (mem == PBF_MEM_DEVICE) {
+    dg->offV = g->offsetsV; dg->edgV = g->edgesV; dg->offU = g->offsetsU; dg->edgU = g->edgesU;
+    return PBF_OK;
+  }
+  if (mem != PBF_MEM_HOST) return PBF_ERR_ARG;
+  PBF_CHECK(pbf_validate(g));
+  size_t words = size_t(g->nv + 1) + size_t(g->m) + size_t(g->nu + 1) + size_t(g->m);
+  void* p;
+  PBF_CHECK(pbf_scratch(ctx, S_GRAPH, 4 * words, &p));
+  uint32_t* d = static_cast<uint32_t*>(p);
+  PBF_HIP(hipMemcpyAsync(d, g->offsetsV, 4 * size_t(g->nv + 1), hipMemcpyHostToDevice, ctx->stream));
+  PBF_HIP(hipMemcpyAsync(d + g->nv + 1, g->edgesV, 4 * size_t(g->m), hipMemcpyHostToDevice, ctx->stream));
+  PBF_HIP(hipMemcpyAsync(d + g->nv + 1 + g->m, g->offsetsU, 4 * size_t(g->nu + 1), hipMemcpyHostToDevice, ctx->stream));
+  PBF_HIP(hipMemcpyAsync(d + g->nv + 1 + g->m + g->nu + 1, g->edgesU, 4 * size_t(g->m), hipMemcpyHostToDevice, ctx->stream));
+  dg->offV = d;
+  dg->edgV = d + g->nv + 1;
+  dg->offU = d + g->nv + 1 + g->m;
+  dg->edgU = d + g->nv + 1 + g->m + g->nu + 1;
+  return PBF_OK;
+}
+
+// Output buffer: the caller's device pointer, or staging in the context for host outputs.
+int out_buffer(pbf_context* ctx, int mem, void* user, size_t bytes, void** dev) {
+  if (mem == PBF_MEM_DEVICE) { *dev = user; return PBF_OK; }
+  return pbf_scratch(ctx, S_OUT, bytes, dev);
+}
+
+int finish_output(pbf_context* ctx, int mem, void* user, const void* dev, size_t bytes) {
+  if (mem == PBF_MEM_HOST && bytes) PBF_HIP(hipMemcpyAsync(user, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  return PBF_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) { (void)hipGetLastError(); return 0; }
+  return ms;
+}
+
+bool rank_supported(int rank_type) { return rank_type == PBF_RANK_ADEG || rank_type == PBF_RANK_ACOCORE; }
+bool count_supported(int count_type) { return count_type >= PBF_COUNT_SORT && count_type <= PBF_COUNT_GPU; }
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+// Shared driver for the three count entry points.
+int count_common(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int count_type, int rank_type,
+                 int mode, void* out_user) {
+  if (!ctx || !out_user) return PBF_ERR_ARG;
+  if (!count_supported(count_type)) return PBF_ERR_ARG;
+  if (!rank_supported(rank_type)) return PBF_ERR_UNSUPPORTED;
+  DeviceGuard guard(ctx->device);
+  DevGraph dg;
+  PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  const int64_t ns = use_v ? g->nu : g->nv;
+  size_t out_bytes = mode == MODE_VERT ? 8 * (size_t)ns : mode == MODE_EDGE ? 8 * (size_t)g->m : 8;
+  void* out_dev;
+  if (mode == MODE_TOTAL) PBF_CHECK(pbf_scratch(ctx, S_OUT, 8, &out_dev));
+  else PBF_CHECK(out_buffer(ctx, mem, out_user, out_bytes, &out_dev));
+  memset(&ctx->timing, 0, sizeof(ctx->timing));
+  hipEvent_t* ev = ctx->ev;
+  PBF_HIP(hipEventRecord(ev[0], ctx->stream));
+  uint32_t* eti = nullptr;
+  if (mode == MODE_EDGE) {
+    void* ep;
+    PBF_CHECK(pbf_scratch(ctx, S_ETI, 4 * (size_t)(g->m ? g->m : 1), &ep));
+    eti = static_cast<uint32_t*>(ep);
+    PBF_CHECK(pbf_dev_edge_to_idxs(ctx, dg, use_v, eti));
+  }
+  PBF_HIP(hipEventRecord(ev[1], ctx->stream));
+  RankedGraph R{};
+  PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R));
+  PBF_HIP(hipEventRecord(ev[2], ctx->stream));
+  PBF_CHECK(pbf_dev_relabel(ctx, dg, use_v, mode == MODE_EDGE, &R));
+  PBF_HIP(hipEventRecord(ev[3], ctx->stream));
+  PBF_CHECK(pbf_dev_count(ctx, dg, use_v, R, mode, static_cast<uint64_t*>(out_dev), eti));
+  PBF_HIP(hipEventRecord(ev[4], ctx->stream));
+  if (mode == MODE_TOTAL) {
+    PBF_HIP(hipMemcpyAsync(out_user, out_dev, 8, mem == PBF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    PBF_CHECK(finish_output(ctx, mem, out_user, out_dev, out_bytes));
+  }
+  ctx->timing.convert_ms = elapsed(ev[0], ev[1]);
+  ctx->timing.rank_ms = elapsed(ev[1], ev[2]);
+  ctx->timing.relabel_ms = elapsed(ev[2], ev[3]);
+  ctx->timing.count_ms = elapsed(ev[3], ev[4]);
+  ctx->timing.total_ms = elapsed(ev[0], ev[4]);
+  ctx->timing.wedges = R.wedges;
+  return PBF_OK;
+}
+}  // namespace
+
+int pbf_scratch(pbf_context* ctx, int slot, size_t bytes, void** out) {
+  if (slot < 0 || slot >= S_NUM_SLOTS) return PBF_ERR_ARG;
+  if (ctx->cap[slot] < bytes || !ctx->buf[slot]) {
+    if (ctx->buf[slot]) {
+      // the stream may still use the old buffer
+      PBF_HIP(hipStreamSynchronize(ctx->stream));
+      (void)hipFree(ctx->buf[slot]);
+      ctx->buf[slot] = nullptr;
+      ctx->cap[slot] = 0;
+    }
+    size_t want = bytes + bytes / 8 + 256;
+    PBF_CHECK(alloc_dev(&ctx->buf[slot], want));
+    ctx->cap[slot] = want;
+  }
+  *out = ctx->buf[slot];
+  return PBF_OK;
+}
+
+extern "C" int pbf_create(pbf_context** out, int device) {
+  if (!out) return PBF_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { (void)hipGetLastError(); return PBF_ERR_HIP; }
+  if (device < 0 || device >= ndev) return PBF_ERR_ARG;
+  DeviceGuard guard(device);
+  pbf_context* ctx = new (std::nothrow) pbf_context();
+  if (!ctx) return PBF_ERR_OOM;
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return PBF_ERR_HIP; }
+  for (auto& e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) { pbf_destroy(ctx); return PBF_ERR_HIP; }
+  if (hipHostMalloc(reinterpret_cast<void**>(&ctx->pinned), 4096, hipHostMallocDefault) != hipSuccess) {
+    pbf_destroy(ctx);
+    return PBF_ERR_OOM;
+  }
+  *out = ctx;
+  return PBF_OK;
+}
+
+extern "C" int pbf_trim(pbf_context* ctx) {
+  if (!ctx) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int s = 0; s < S_NUM_SLOTS; s++) {
+    if (ctx->buf[s]) (void)hipFree(ctx->buf[s]);
+    ctx->buf[s] = nullptr;
+    ctx->cap[s] = 0;
+  }
+  return PBF_OK;
+}
+
+extern "C" int pbf_destroy(pbf_context* ctx) {
+  if (!ctx) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  pbf_trim(ctx);
+  for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PBF_OK;
+}
+
+extern "C" void* pbf_stream(pbf_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+extern "C" int pbf_synchronize(pbf_context* ctx) {
+  if (!ctx) return PBF_ERR_ARG;
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  return PBF_OK;
+}
+
+extern "C" int pbf_last_timing(pbf_context* ctx, pbf_timing* out) {
+  if (!ctx || !out) return PBF_ERR_ARG;
+  *out = ctx->timing;
+  return PBF_OK;
+}
+
+extern "C" int pbf_side(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int* use_v, int64_t* num_wedges) {
+  if (!ctx || !use_v || !num_wedges) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  DevGraph dg;
+  PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  return pbf_dev_side(ctx, dg, use_v, num_wedges);
+}
+
+extern "C" int pbf_rank(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int rank_type, uint32_t* order,
+                        uint32_t* rank_of, int64_t* wedges) {
+  if (!ctx) return PBF_ERR_ARG;
+  if (!rank_supported(rank_type)) return PBF_ERR_UNSUPPORTED;
+  DeviceGuard guard(ctx->device);
+  DevGraph dg;
+  PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  RankedGraph R{};
+  PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R));
+  if (wedges) {
+    PBF_CHECK(pbf_dev_relabel(ctx, dg, 1, false, &R));
+    *wedges = R.wedges;
+  }
+  const size_t nb = 4 * (size_t)R.n;
+  hipMemcpyKind kind = mem == PBF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (order && nb) PBF_HIP(hipMemcpyAsync(order, R.order, nb, kind, ctx->stream));
+  if (rank_of && nb) PBF_HIP(hipMemcpyAsync(rank_of, R.rank_of, nb, kind, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  return PBF_OK;
+}
+
+extern "C" int pbf_count_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int count_type,
+                              int rank_type, uint64_t* out) {
+  return count_common(ctx, g, mem, use_v, count_type, rank_type, MODE_VERT, out);
+}
+
+extern "C" int pbf_count_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int count_type,
+                              int rank_type, uint64_t* out) {
+  return count_common(ctx, g, mem, use_v, count_type, rank_type, MODE_EDGE, out);
+}
+
+extern "C" int pbf_count_total(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int count_type,
+                               int rank_type, uint64_t* total) {
+  return count_common(ctx, g, mem, use_v, count_type, rank_type, MODE_TOTAL, total);
+}
+
+extern "C" int pbf_edge_to_idxs(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, uint32_t* eti) {
+  if (!ctx || !eti) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  DevGraph dg;
+  PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  void* dev;
+  PBF_CHECK(out_buffer(ctx, mem, eti, 4 * (size_t)g->m, &dev));
+  PBF_CHECK(pbf_dev_edge_to_idxs(ctx, dg, use_v, static_cast<uint32_t*>(dev)));
+  return finish_output(ctx, mem, eti, dev, 4 * (size_t)g->m);
+}
+
+extern "C" int pbf_peel_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int peel_type,
+                             const uint64_t* counts, uint64_t* tip) {
+  (void)ctx; (void)g; (void)mem; (void)use_v; (void)peel_type; (void)counts; (void)tip;
+  return PBF_ERR_UNSUPPORTED;
+}
+
+extern "C" int pbf_peel_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int peel_type,
+                             const uint64_t* counts, uint64_t* wing) {
+  (void)ctx; (void)g; (void)mem; (void)use_v; (void)peel_type; (void)counts; (void)wing;
+  return PBF_ERR_UNSUPPORTED;
+}

@@ -1,0 +1,132 @@
+// Device-side shared declarations of libparbutterfly_amd (HIP, gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "parbutterfly.h"
+
+#define PBF_WAVE 64
+
+// ------------------------------------------------------------------------------------------
+// Context: one HIP stream, a grow-only scratch arena (named slots), timing events.
+// ------------------------------------------------------------------------------------------
+enum ScratchSlot {
+  S_GRAPH = 0,   // uploaded host graph (offV, edgV, offU, edgU packed)
+  S_KEY,         // u8 key per vertex (ADEG / ACOCORE level)
+  S_ORDER,       // u32 order[n]
+  S_RANKOF,      // u32 rank_of[n]
+  S_HIST,        // u32 key histogram per block
+  S_SCAN,        // scan block sums
+  S_ROFF,        // u32 ranked offsets [n+1]
+  S_REDG,        // u32 ranked adjacency [2m]
+  S_RPOS,        // u32 original CSR position of each ranked slot [2m]
+  S_OIDX,        // u32 O-CSR index of each ranked slot [2m]
+  S_INPREF,      // u32 in(r) [n]
+  S_BOUND,       // u32 endpoint bound [n]
+  S_BINS,        // u32 vertex lists per kernel class
+  S_COUNTS,      // u64 counts in rank space [n]
+  S_MISC,        // small counters / totals (u64)
+  S_ETI,         // u32 eti [m]
+  S_ITE,         // u32 ite [m]
+  S_POOL,        // HBM fallback dense histograms
+  S_OUT,         // staging for host outputs
+  S_PEEL0, S_PEEL1, S_PEEL2, S_PEEL3, S_PEEL4, S_PEEL5, S_PEEL6, S_PEEL7,
+  S_ACORE0, S_ACORE1, S_ACORE2,
+  S_NUM_SLOTS
+};
+
+struct pbf_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* buf[S_NUM_SLOTS] = {};
+  size_t cap[S_NUM_SLOTS] = {};
+  hipEvent_t ev[8] = {};
+  pbf_timing timing = {};
+  uint64_t* pinned = nullptr;  // small pinned host buffer (readbacks)
+  int num_cus = 256;
+};
+
+// Returns a device buffer of at least `bytes` for the slot (grow-only; contents undefined).
+int pbf_scratch(pbf_context* ctx, int slot, size_t bytes, void** out);
+
+#define PBF_HIP(call)                                  \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return PBF_ERR_HIP;          \
+  } while (0)
+
+#define PBF_CHECK(call)                                \
+  do {                                                 \
+    int _s = (call);                                   \
+    if (_s != PBF_OK) return _s;                       \
+  } while (0)
+
+// Device view of the input graph.
+struct DevGraph {
+  int64_t nv, nu, m;
+  const uint32_t* offV;
+  const uint32_t* edgV;
+  const uint32_t* offU;
+  const uint32_t* edgU;
+};
+
+// Ranked graph (rankGraph / rankGraphEdges, butterfly_utils.h:579-700) on device.
+struct RankedGraph {
+  int64_t n, m2;
+  uint32_t* order;    // order[r] = combined id
+  uint32_t* rank_of;  // rank_of[id] = r
+  uint32_t* roff;     // [n+1]
+  uint32_t* redg;     // [2m] (rank << 1) | flag(neighbour in S)
+  uint32_t* rpos;     // [2m] position in owner's original CSR (EDGE only, else null)
+  uint32_t* inpref;   // [n]  neighbours of lower rank
+  int64_t wedges;     // W under this ranking
+};
+
+// scan.hip
+int pbf_scan_excl_u32(pbf_context* ctx, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev);
+int pbf_scan_excl_u64(pbf_context* ctx, const uint64_t* in, uint64_t* out, int64_t n, uint64_t* total_dev);
+
+// rank.hip
+int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedges);
+int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R);
+int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_pos, RankedGraph* R);
+int pbf_dev_edge_to_idxs(pbf_context* ctx, const DevGraph& g, int use_v, uint32_t* eti);
+
+// count.hip
+enum CountMode { MODE_VERT = 0, MODE_EDGE = 1, MODE_TOTAL = 2 };
+int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGraph& R, int mode,
+                  uint64_t* out_dev /* VERT: n_S, EDGE: m, TOTAL: 1 */, const uint32_t* eti);
+
+// peel.hip
+int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint64_t* counts, uint64_t* tip);
+int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint64_t* counts, uint64_t* wing);
+
+// ---------------- device helpers ----------------
+__device__ __forceinline__ uint32_t pbf_bitlen(uint32_t d) { return d ? 32u - __clz(d) : 0u; }
+
+__device__ __forceinline__ uint64_t pbf_lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Inclusive wave scan (64 lanes).
+template <typename T>
+__device__ __forceinline__ T pbf_wave_incl_scan(T x) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T pbf_wave_sum(T x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
