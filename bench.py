@@ -1,0 +1,153 @@
+"""Benchmark: per-vertex butterfly counting throughput (Gwedges/s) on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): Chung-Lu power-law bipartite
+graph, nv = 2^20, nu = 2^18, m = 2^24 distinct edges, degree exponent 2.1 on both sides, seed 2,
+uint32 ids, ADEG ranking, per-vertex uint64 counts.  One step = the reference's timed region
+for Count (apps/hyper/BipartiteButterflies.C:219-222): ranking + relabel + wedge counting +
+rank-to-id conversion, from the CSR resident in HBM to the counts resident in HBM.
+
+value = W / (seconds per step) / 1e9, W = rank-filtered wedges (rankWedgeF) under the ranking.
+roofline: the counting phase, algorithmic bytes B_alg(VERT) = 2 * (4 W + 32 m) + 4 (n + 1) + 8 n_S
+over its duration measured with HIP events on the library's stream, against 8 TB/s.
+cpu_baseline: the reference's own BATCHS path (oracle/_ref, compiled from the reference's
+headers) on the same graph, all host cores, median of its timed rounds.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--traffic JSON]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+CFG = dict(nv=1 << 20, nu=1 << 18, m=1 << 24, gamma=2.1, seed=2)
+
+
+def b_alg_vert(W, m, n, ns):
+    return 2 * (4 * W + 32 * m) + 4 * (n + 1) + 8 * ns
+
+
+def cpu_baseline(graph, use_v, rounds=2):
+    """Times the compiled reference (BATCHS, ADEG, per vertex) on the same graph."""
+    from oracle import oracle as O
+    if not O.ref_available():
+        return None
+    cores = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+        cores = min(cores, aff)
+    except AttributeError:
+        pass
+    cores = min(cores, 16)
+    with tempfile.TemporaryDirectory() as td:
+        gp = os.path.join(td, "g.bcsr")
+        graph.write_binary(gp)
+        t0 = time.time()
+        uv, _, counts, _, info = O.run_ref("vert", gp, os.path.join(td, "o.bin"), "BATCHS", "ADEG",
+                                           rounds=rounds, threads=cores, timeout=900)
+        wall = time.time() - t0
+    secs = sorted(info["seconds"])
+    return dict(seconds=secs[len(secs) // 2], all_seconds=info["seconds"], cores=cores, wall=wall,
+                use_v=uv, counts=counts)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import parbutterfly_amd as pb
+
+    t0 = time.time()
+    g = pb.Graph.chung_lu(CFG["nv"], CFG["nu"], CFG["m"], CFG["gamma"], CFG["gamma"], CFG["seed"])
+    gen_s = time.time() - t0
+    print(f"# generated graph nv={g.nv} nu={g.nu} m={g.m} in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    ctx = pb.Context(0)
+    dg = g.to_device(ctx)
+    use_v, side_w = ctx.side(dg)
+    ns = g.nu if use_v else g.nv
+    n = g.nv + g.nu
+    out = pb.DeviceArray(ctx, ns, np.uint64)
+
+    # H2D-inclusive rate (host graph in, host counts out), reported separately
+    t_h = time.time()
+    host_counts = ctx.count(g, use_v, "VERT", "GPU", "ADEG")
+    h2d_s = time.time() - t_h
+
+    for _ in range(args.warmup):
+        ctx.count(dg, use_v, "VERT", "GPU", "ADEG", out=out)
+    ctx.synchronize()
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.count(dg, use_v, "VERT", "GPU", "ADEG", out=out)
+        phases.append(ctx.timing())
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    W = phases[-1]["wedges"]
+    assert all(p["wedges"] == W for p in phases)
+    gwedges = W / (ms_per_step * 1e-3) / 1e9
+    count_ms = float(np.mean([p["count_ms"] for p in phases]))
+    total_ms = float(np.mean([p["total_ms"] for p in phases]))
+    balg = b_alg_vert(W, g.m, n, ns)
+    achieved = balg / (count_ms * 1e-3) / 1e9
+    gpu_counts = pb.as_u64(out)
+    assert np.array_equal(gpu_counts, pb.as_u64(host_counts)), "device and host paths disagree"
+
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    rec = {
+        "metric": "Gwedges/s per-vertex count",
+        "value": round(gwedges, 3),
+        "unit": "Gwedges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 ids / u64 counts",
+        "data": "synthetic (seeded Chung-Lu generator)",
+        "config": {"workload": "chung-lu bipartite nv=2^20 nu=2^18 m=2^24 gamma=2.1 seed=2, ADEG, per-vertex counts",
+                   "wedges": W, "use_v": use_v, "n_S": ns},
+        "phases_ms": {k: round(float(np.mean([p[k] for p in phases])), 4)
+                      for k in ("rank_ms", "relabel_ms", "count_ms", "total_ms")},
+        "roofline": {"bound": "hbm", "kernel": "count phase (k_bin + k_count)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "b_alg_bytes": balg, "count_ms": round(count_ms, 4)},
+        "host_inclusive_gwedges_s": round(W / h2d_s / 1e9, 3),
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(g, use_v)
+        if cb is not None:
+            ok = cb["use_v"] == use_v and np.array_equal(cb["counts"].astype(np.uint64), gpu_counts)
+            rec["cpu_baseline"] = {"value": round(W / cb["seconds"] / 1e9, 4), "unit": "Gwedges/s",
+                                   "cores": cb["cores"], "kind": "reference",
+                                   "sample": "full workload (same graph), reference BATCHS/ADEG Count(), "
+                                             f"median of {len(cb['all_seconds'])} rounds",
+                                   "seconds": round(cb["seconds"], 3), "bit_exact_vs_gpu": bool(ok)}
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

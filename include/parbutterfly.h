@@ -140,6 +140,14 @@ int pbf_last_timing(pbf_context* ctx, pbf_timing* out);
 /* Releases cached device scratch. */
 int pbf_trim(pbf_context* ctx);
 
+/* Device buffers on the context's device (for PBF_MEM_DEVICE callers without their own HIP
+ * allocator).  Copies are synchronous with respect to the host and ordered on the context's
+ * stream.  No counterpart in the reference (it has no device memory). */
+int pbf_device_alloc(pbf_context* ctx, size_t bytes, void** ptr);
+int pbf_device_free(pbf_context* ctx, void* ptr);
+int pbf_copy_to_device(pbf_context* ctx, void* dst, const void* src, size_t bytes);
+int pbf_copy_to_host(pbf_context* ctx, void* dst, const void* src, size_t bytes);
+
 /* cmpWedgeCounts (butterfly_utils.h:1061-1086) on device or host graphs. */
 int pbf_side(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int* use_v, int64_t* num_wedges);
 

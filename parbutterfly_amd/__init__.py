@@ -1,6 +1,7 @@
 """parbutterfly_amd -- butterfly counting and peeling on AMD MI355X (gfx950).
 
-Python view of the C ABI in include/parbutterfly.h (libparbutterfly_amd.so).  It mirrors the
+Python view of the C ABI in include/parbutterfly.h (libparbutterfly_amd.so).  Device memory is
+the library's own (DeviceArray / DeviceGraph); torch is not needed.  It mirrors the
 reference's interface for this path:
 
   reference (apps/hyper)                       here
@@ -106,6 +107,10 @@ _SIGS = {
     "pbf_synchronize": (_INT, [_P]),
     "pbf_last_timing": (_INT, [_P, ctypes.POINTER(Timing)]),
     "pbf_trim": (_INT, [_P]),
+    "pbf_device_alloc": (_INT, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "pbf_device_free": (_INT, [_P, _P]),
+    "pbf_copy_to_device": (_INT, [_P, _P, _P, ctypes.c_size_t]),
+    "pbf_copy_to_host": (_INT, [_P, _P, _P, ctypes.c_size_t]),
     "pbf_side": (_INT, [_P, _PCSR, _INT, _PI, _PI64]),
     "pbf_rank": (_INT, [_P, _PCSR, _INT, _INT, _P, _P, _PI64]),
     "pbf_count_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
@@ -240,9 +245,9 @@ class Graph:
     def n(self):
         return self.nv + self.nu
 
-    def to_device(self, device="cuda"):
-        """Copies the four arrays to HIP device memory (torch tensors); returns a DeviceGraph."""
-        return DeviceGraph(self, device)
+    def to_device(self, ctx):
+        """Copies the four arrays to HBM on ctx's device; returns a DeviceGraph."""
+        return DeviceGraph(self, ctx)
 
 
 class _Owner:
@@ -256,17 +261,56 @@ class _Owner:
             pass
 
 
-class DeviceGraph:
-    """A graph whose arrays live in HIP device memory (torch tensors on the GPU)."""
+class DeviceArray:
+    """A 1-D array in device memory, allocated with the library's own allocator."""
 
-    def __init__(self, g, device="cuda"):
-        import torch
+    def __init__(self, ctx, n, dtype, host=None):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(_lib.pbf_device_alloc(ctx._ctx, max(self.nbytes, 16), ctypes.byref(p)), "pbf_device_alloc")
+        self.ptr = p.value
+        if host is not None:
+            h = np.ascontiguousarray(host, dtype=self.dtype)
+            assert h.size == self.n
+            if self.nbytes:
+                _check(_lib.pbf_copy_to_device(ctx._ctx, self.ptr, h.ctypes.data, self.nbytes), "pbf_copy_to_device")
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        if self.nbytes:
+            _check(_lib.pbf_copy_to_host(self.ctx._ctx, out.ctypes.data, self.ptr, self.nbytes), "pbf_copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx._ctx:
+            _lib.pbf_device_free(self.ctx._ctx, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return self.n
+
+
+class DeviceGraph:
+    """A graph whose four arrays live in HBM (PBF_MEM_DEVICE)."""
+
+    def __init__(self, g, ctx):
         self.nv, self.nu, self.m = g.nv, g.nu, g.m
-        t = lambda a: torch.from_numpy(a.view(np.int32)).to(device)
-        self.offsetsV, self.edgesV = t(g.offsetsV), t(g.edgesV)
-        self.offsetsU, self.edgesU = t(g.offsetsU), t(g.edgesU)
-        self._csr = _CSR(self.nv, self.nu, self.m, self.offsetsV.data_ptr(), self.edgesV.data_ptr(),
-                         self.offsetsU.data_ptr(), self.edgesU.data_ptr())
+        self.ctx = ctx
+        self.offsetsV = DeviceArray(ctx, g.nv + 1, np.uint32, g.offsetsV)
+        self.edgesV = DeviceArray(ctx, g.m, np.uint32, g.edgesV)
+        self.offsetsU = DeviceArray(ctx, g.nu + 1, np.uint32, g.offsetsU)
+        self.edgesU = DeviceArray(ctx, g.m, np.uint32, g.edgesU)
+        self._csr = _CSR(self.nv, self.nu, self.m, self.offsetsV.ptr, self.edgesV.ptr, self.offsetsU.ptr,
+                         self.edgesU.ptr)
 
     @property
     def n(self):
@@ -319,17 +363,14 @@ class Context:
 
     def _alloc_out(self, g, n, dtype):
         if isinstance(g, DeviceGraph):
-            import torch
-            td = {np.uint64: torch.int64, np.uint32: torch.int32}[dtype]
-            return torch.empty(max(n, 1), dtype=td, device=g.offsetsV.device)[:n] if n else \
-                torch.empty(0, dtype=td, device=g.offsetsV.device)
+            return DeviceArray(self, n, dtype)
         return np.zeros(n, dtype=dtype)
 
     @staticmethod
     def _ptr(a):
         if isinstance(a, np.ndarray):
             return a.ctypes.data
-        return a.data_ptr()
+        return a.ptr
 
     def side(self, g):
         uv, nw = ctypes.c_int(), ctypes.c_int64()
@@ -356,11 +397,10 @@ class Context:
         if per == 2:
             tot = np.zeros(1, dtype=np.uint64)
             if mem == MEM_DEVICE:
-                import torch
-                dt = torch.zeros(1, dtype=torch.int64, device=g.offsetsV.device)
+                dt = DeviceArray(self, 1, np.uint64)
                 _check(_lib.pbf_count_total(self._ctx, ctypes.byref(g._csr), mem, int(bool(use_v)), ct, rt,
-                                            dt.data_ptr()), "pbf_count_total")
-                return int(dt.cpu().numpy().view(np.uint64)[0])
+                                            dt.ptr), "pbf_count_total")
+                return int(dt.numpy()[0])
             _check(_lib.pbf_count_total(self._ctx, ctypes.byref(g._csr), mem, int(bool(use_v)), ct, rt,
                                         tot.ctypes.data), "pbf_count_total")
             return int(tot[0])
@@ -397,15 +437,16 @@ class Context:
                                   self._ptr(wing) if g.m else 1), "pbf_peel_edge")
         return wing
 
-    @staticmethod
-    def _as_u64(g, counts):
+    def _as_u64(self, g, counts):
         if isinstance(g, DeviceGraph):
-            return counts
-        return np.ascontiguousarray(counts, dtype=np.uint64)
+            if isinstance(counts, DeviceArray):
+                return counts
+            return DeviceArray(self, len(counts), np.uint64, counts)
+        return np.ascontiguousarray(as_u64(counts), dtype=np.uint64)
 
 
 def as_u64(x):
-    """Device (torch int64) or host array -> numpy uint64."""
-    if isinstance(x, np.ndarray):
-        return x.astype(np.uint64, copy=False)
-    return x.detach().cpu().numpy().view(np.uint64)
+    """DeviceArray or host array -> numpy uint64."""
+    if isinstance(x, DeviceArray):
+        return x.numpy().view(np.uint64)
+    return np.asarray(x).astype(np.uint64, copy=False)

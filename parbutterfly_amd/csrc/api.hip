@@ -210,6 +210,37 @@ extern "C" int pbf_last_timing(pbf_context* ctx, pbf_timing* out) {
   return PBF_OK;
 }
 
+extern "C" int pbf_device_alloc(pbf_context* ctx, size_t bytes, void** ptr) {
+  if (!ctx || !ptr) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  return alloc_dev(ptr, bytes);
+}
+
+extern "C" int pbf_device_free(pbf_context* ctx, void* ptr) {
+  if (!ctx) return PBF_ERR_ARG;
+  if (!ptr) return PBF_OK;
+  DeviceGuard guard(ctx->device);
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  PBF_HIP(hipFree(ptr));
+  return PBF_OK;
+}
+
+extern "C" int pbf_copy_to_device(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  return PBF_OK;
+}
+
+extern "C" int pbf_copy_to_host(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  return PBF_OK;
+}
+
 extern "C" int pbf_side(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int* use_v, int64_t* num_wedges) {
   if (!ctx || !use_v || !num_wedges) return PBF_ERR_ARG;
   DeviceGuard guard(ctx->device);
@@ -265,14 +296,44 @@ extern "C" int pbf_edge_to_idxs(pbf_context* ctx, const pbf_bipartite_csr* g, in
   return finish_output(ctx, mem, eti, dev, 4 * (size_t)g->m);
 }
 
+namespace {
+// Shared driver of the two peel entry points: counts in, peel numbers out (n items).
+int peel_common(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int peel_type,
+                const uint64_t* counts, uint64_t* out, bool edges) {
+  if (!ctx || !counts || !out) return PBF_ERR_ARG;
+  if (peel_type < PBF_PEEL_SORT || peel_type > PBF_PEEL_GPU) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  DevGraph dg;
+  PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  const int64_t n = edges ? g->m : (use_v ? g->nu : g->nv);
+  const size_t bytes = 8 * (size_t)n;
+  const uint64_t* cdev = counts;
+  void* odev;
+  if (mem == PBF_MEM_HOST) {
+    void* cp;
+    PBF_CHECK(pbf_scratch(ctx, S_COUNTS, bytes ? bytes : 8, &cp));
+    if (bytes) PBF_HIP(hipMemcpyAsync(cp, counts, bytes, hipMemcpyHostToDevice, ctx->stream));
+    cdev = static_cast<const uint64_t*>(cp);
+  }
+  PBF_CHECK(out_buffer(ctx, mem, out, bytes ? bytes : 8, &odev));
+  memset(&ctx->timing, 0, sizeof(ctx->timing));
+  PBF_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  if (edges) PBF_CHECK(pbf_dev_peel_edge(ctx, dg, use_v, cdev, static_cast<uint64_t*>(odev)));
+  else PBF_CHECK(pbf_dev_peel_vert(ctx, dg, use_v, cdev, static_cast<uint64_t*>(odev)));
+  PBF_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  PBF_CHECK(finish_output(ctx, mem, out, odev, bytes));
+  ctx->timing.peel_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+  ctx->timing.total_ms = ctx->timing.peel_ms;
+  return PBF_OK;
+}
+}  // namespace
+
 extern "C" int pbf_peel_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int peel_type,
                              const uint64_t* counts, uint64_t* tip) {
-  (void)ctx; (void)g; (void)mem; (void)use_v; (void)peel_type; (void)counts; (void)tip;
-  return PBF_ERR_UNSUPPORTED;
+  return peel_common(ctx, g, mem, use_v, peel_type, counts, tip, false);
 }
 
 extern "C" int pbf_peel_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int peel_type,
                              const uint64_t* counts, uint64_t* wing) {
-  (void)ctx; (void)g; (void)mem; (void)use_v; (void)peel_type; (void)counts; (void)wing;
-  return PBF_ERR_UNSUPPORTED;
+  return peel_common(ctx, g, mem, use_v, peel_type, counts, wing, true);
 }

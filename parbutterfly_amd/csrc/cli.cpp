@@ -1,0 +1,221 @@
+// bipartite_butterflies: the command-line driver, with the flags and output lines of the
+// reference's apps/hyper/BipartiteButterflies.C (Compute :75-277, parallel_main :283-355),
+// running every count / peel on the GPU through libparbutterfly_amd's C ABI.
+//
+//   bipartite_butterflies [-countType T | -t N] [-rankType R | -w N] [-peelType P | -tp N]
+//                         [-per VERT|EDGE|TOTAL | -e N | -total] [-r rounds] [-m max_wedges]
+//                         [-a max_array_size] [-nopeel] [-device D] <inFile>
+//
+// Count types are the reference's (ASORT ... SERIAL, numeric aliases of :96-107) plus GPU;
+// they are aggregation strategies with identical results, and all of them run the GPU kernels
+// here.  Rank types ADEG and ACOCORE are supported on the GPU (the configurations this build
+// targets); SIDE / DEG / COCORE return PBF_ERR_UNSUPPORTED.  Sparsification (-sparseType) is not
+// part of this build and is rejected.  The graph is re-read every round, as the reference does
+// (:331-341).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "parbutterfly.h"
+
+namespace {
+struct Args {
+  int argc;
+  char** argv;
+  bool has(const char* opt) const {
+    for (int i = 1; i < argc - 1; i++) if (!strcmp(argv[i], opt)) return true;
+    return false;
+  }
+  std::string str(const char* opt, const std::string& def) const {
+    for (int i = 1; i < argc - 1; i++) if (!strcmp(argv[i], opt)) return argv[i + 1];
+    return def;
+  }
+  long num(const char* opt, long def) const {
+    for (int i = 1; i < argc - 1; i++) if (!strcmp(argv[i], opt)) return atol(argv[i + 1]);
+    return def;
+  }
+};
+
+const long NONE = -0x7fffffffffffffffL;
+
+// CountTypeToStr (BipartiteButterflies.C:41-55), plus the GPU mode.
+const char* count_label(int t) {
+  switch (t) {
+    case PBF_COUNT_ASORT: return "Sort";
+    case PBF_COUNT_SORT: return "SortCE";
+    case PBF_COUNT_AHASH: return "Hash";
+    case PBF_COUNT_HASH: return "HashCE";
+    case PBF_COUNT_AHIST: return "Hist";
+    case PBF_COUNT_HIST: return "HistCE";
+    case PBF_COUNT_BATCHS: return "Par";
+    case PBF_COUNT_SERIAL: return "Serial";
+    case PBF_COUNT_BATCHWA: return "WedgePar";
+    case PBF_COUNT_GPU: return "GPU";
+  }
+  return "";
+}
+const char* peel_label(int t) {
+  switch (t) {
+    case PBF_PEEL_SORT: return "SortCE Peel";
+    case PBF_PEEL_HASH: return "HashCE Peel";
+    case PBF_PEEL_HIST: return "HistCE Peel";
+    case PBF_PEEL_BATCHS: return "Par Peel";
+    case PBF_PEEL_BATCHWA: return "WedgePar Peel";
+    case PBF_PEEL_GPU: return "GPU Peel";
+  }
+  return "";
+}
+
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+int fail(const char* what, int st) {
+  fprintf(stderr, "%s: %s (%d)\n", what, pbf_status_str(st), st);
+  return st ? st : 1;
+}
+
+int compute(pbf_context* ctx, const pbf_bipartite_csr& g, const Args& P) {
+  // count type (:79-109)
+  long tl = P.num("-t", NONE);
+  std::string ts = P.str("-countType", "");
+  int ty = PBF_COUNT_GPU;
+  if (tl == NONE) {
+    if (ts == "ASORT") ty = PBF_COUNT_ASORT; else if (ts == "SORT") ty = PBF_COUNT_SORT;
+    else if (ts == "AHASH") ty = PBF_COUNT_AHASH; else if (ts == "HASH") ty = PBF_COUNT_HASH;
+    else if (ts == "AHIST") ty = PBF_COUNT_AHIST; else if (ts == "HIST") ty = PBF_COUNT_HIST;
+    else if (ts == "BATCHS") ty = PBF_COUNT_BATCHS; else if (ts == "BATCHWA") ty = PBF_COUNT_BATCHWA;
+    else if (ts == "SERIAL") ty = PBF_COUNT_SERIAL; else ty = PBF_COUNT_GPU;
+  } else {
+    switch (tl) {
+      case 0: ty = PBF_COUNT_ASORT; break; case 1: ty = PBF_COUNT_SORT; break;
+      case 2: ty = PBF_COUNT_AHASH; break; case 3: ty = PBF_COUNT_HASH; break;
+      case 4: ty = PBF_COUNT_AHIST; break; case 6: ty = PBF_COUNT_HIST; break;
+      case 5: case 7: case 11: ty = PBF_COUNT_BATCHS; break;
+      case 9: case 12: ty = PBF_COUNT_SERIAL; break;
+      case 8: ty = PBF_COUNT_BATCHWA; break;
+      default: ty = PBF_COUNT_GPU; break;
+    }
+  }
+  // rank type (:111-129)
+  long wl = P.num("-w", NONE);
+  std::string ws = P.str("-rankType", "");
+  int tw = PBF_RANK_ADEG;
+  if (wl == NONE) {
+    if (ws == "SIDE") tw = PBF_RANK_SIDE; else if (ws == "COCORE") tw = PBF_RANK_COCORE;
+    else if (ws == "ACOCORE") tw = PBF_RANK_ACOCORE; else if (ws == "DEG") tw = PBF_RANK_DEG;
+    else tw = PBF_RANK_ADEG;
+  } else if (wl >= 0 && wl <= 4) {
+    tw = (int)wl;
+  }
+  // peel type (:131-153)
+  long pl = P.num("-tp", NONE);
+  std::string ps = P.str("-peelType", "");
+  int tp = PBF_PEEL_GPU;
+  bool nopeel = false;
+  if (pl == NONE) {
+    if (ps == "SORT") tp = PBF_PEEL_SORT; else if (ps == "HASH") tp = PBF_PEEL_HASH;
+    else if (ps == "HIST") tp = PBF_PEEL_HIST; else if (ps == "BATCHS") tp = PBF_PEEL_BATCHS;
+    else if (ps == "BATCHWA") tp = PBF_PEEL_BATCHWA; else if (ps == "GPU") tp = PBF_PEEL_GPU;
+    else nopeel = true;
+  } else {
+    nopeel = P.has("-nopeel");
+    switch (pl) {
+      case 0: tp = PBF_PEEL_HASH; break; case 1: tp = PBF_PEEL_SORT; break; case 2: tp = PBF_PEEL_HIST; break;
+      case 3: case 4: tp = PBF_PEEL_BATCHS; break; case 5: tp = PBF_PEEL_BATCHWA; break;
+      case 6: tp = PBF_PEEL_GPU; break;
+      default: nopeel = true; break;
+    }
+  }
+  // per type (:155-168)
+  std::string pers = P.str("-per", "");
+  long te = P.num("-e", NONE);
+  int per = PBF_PER_VERT;
+  if (te == NONE) {
+    if (pers == "EDGE") per = PBF_PER_EDGE; else if (pers == "TOTAL") per = PBF_PER_TOTAL;
+  } else {
+    per = P.has("-total") ? PBF_PER_TOTAL : (te == 0 ? PBF_PER_VERT : PBF_PER_EDGE);
+  }
+  if (P.num("-s", 0) != 0 || !P.str("-sparseType", "").empty()) {
+    fprintf(stderr, "sparsification (-sparseType) is not supported by this build\n");
+    return PBF_ERR_UNSUPPORTED;
+  }
+  if (per == PBF_PER_TOTAL) nopeel = true;
+
+  double t0 = now();
+  int use_v = 0;
+  int64_t nw = 0;
+  int st = pbf_side(ctx, &g, PBF_MEM_HOST, &use_v, &nw);
+  if (st) return fail("pbf_side", st);
+  printf("preprocess (wedge counts) : %.3g\n", now() - t0);
+
+  if (per == PBF_PER_TOTAL) {
+    uint64_t total = 0;
+    t0 = now();
+    st = pbf_count_total(ctx, &g, PBF_MEM_HOST, use_v, ty, tw, &total);
+    if (st) return fail("pbf_count_total", st);
+    printf("%s : %.3g\n", count_label(ty), now() - t0);
+    printf("number of butterflies: %llu\n", (unsigned long long)total);
+    return 0;
+  }
+  if (per == PBF_PER_VERT) {
+    const int64_t ns = use_v ? g.nu : g.nv;
+    std::vector<uint64_t> b((size_t)ns);
+    t0 = now();
+    st = pbf_count_vert(ctx, &g, PBF_MEM_HOST, use_v, ty, tw, b.data());
+    if (st) return fail("pbf_count_vert", st);
+    printf("%s : %.3g\n", count_label(ty), now() - t0);
+    unsigned long long sum = 0;
+    for (auto x : b) sum += x;
+    printf("number of butterflies: %llu\n", sum / 2);
+    if (!nopeel) {
+      std::vector<uint64_t> tip((size_t)ns);
+      t0 = now();
+      st = pbf_peel_vert(ctx, &g, PBF_MEM_HOST, use_v, tp, b.data(), tip.data());
+      if (st) return fail("pbf_peel_vert", st);
+      printf("%s : %.3g\n", peel_label(tp), now() - t0);
+    }
+    return 0;
+  }
+  std::vector<uint64_t> e((size_t)g.m);
+  t0 = now();
+  st = pbf_count_edge(ctx, &g, PBF_MEM_HOST, use_v, ty, tw, e.data());
+  if (st) return fail("pbf_count_edge", st);
+  printf("%s : %.3g\n", count_label(ty), now() - t0);
+  unsigned long long sum = 0;
+  for (auto x : e) sum += x;
+  printf("number of edge butterflies: %llu\n", sum / 4);
+  if (!nopeel) {
+    std::vector<uint64_t> wing((size_t)g.m);
+    t0 = now();
+    st = pbf_peel_edge(ctx, &g, PBF_MEM_HOST, use_v, tp, e.data(), wing.data());
+    if (st) return fail("pbf_peel_edge", st);
+    printf("%s : %.3g\n", peel_label(tp), now() - t0);
+  }
+  return 0;
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: %s [options] <inFile>\n", argv[0]);
+    return 2;
+  }
+  Args P{argc, argv};
+  const char* file = argv[argc - 1];
+  long rounds = P.num("-r", 3);
+  pbf_context* ctx = nullptr;
+  int st = pbf_create(&ctx, (int)P.num("-device", 0));
+  if (st) return fail("pbf_create", st);
+  int rc = 0;
+  for (long r = 0; r < rounds + 1 && rc == 0; r++) {
+    pbf_graph g;
+    st = pbf_read_bipartite(file, &g);
+    if (st) { rc = fail("readBipartite", st); break; }
+    rc = compute(ctx, g.csr, P);
+    pbf_graph_free(&g);
+  }
+  pbf_destroy(ctx);
+  return rc;
+}

@@ -22,10 +22,11 @@
 // second kernel that uses a dense per-workgroup histogram in HBM.  Counts are scattered with
 // 64-bit atomics; integer atomic adds commute, so results are exact and deterministic.
 #include "pbf_device.h"
+#include "team.h"
+
+using namespace pbf;
 
 namespace {
-constexpr uint32_t EMPTY = 0xFFFFFFFFu;
-
 enum TableKind { T_LDS = 0, T_HBM = 1 };
 
 struct CountArgs {
@@ -44,123 +45,6 @@ struct CountArgs {
   uint32_t* pool;          // T_HBM: per-block dense counters [n] followed by used list [n]
   int64_t n;
 };
-
-__device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t shift) {
-  return (key * 0x9E3779B1u) >> shift;
-}
-
-template <int TEAM>
-struct TeamShared {
-  uint32_t P[TEAM];    // inclusive scan of segment lengths
-  uint32_t vs[TEAM];   // first redg index of segment
-  uint32_t vv[TEAM];   // centre vertex v of segment
-  unsigned long long acc[TEAM];
-  uint32_t scan[TEAM / 64 + 2];
-  unsigned long long red[TEAM / 64 + 1];
-  uint32_t item, nused, ovf, pad;
-};
-
-template <int TEAM>
-__device__ __forceinline__ void team_sync() {
-  __syncthreads();  // a team is a whole workgroup (TEAM == blockDim.x)
-}
-
-// Inclusive team scan; *total receives the team total.
-template <int TEAM>
-__device__ __forceinline__ uint32_t team_incl_scan(uint32_t x, TeamShared<TEAM>& sh, uint32_t* total) {
-  uint32_t incl = pbf_wave_incl_scan(x);
-  if constexpr (TEAM == 64) {
-    *total = __shfl(incl, 63, 64);
-    return incl;
-  } else {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 63) sh.scan[wid] = incl;
-    __syncthreads();
-    uint32_t add = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < TEAM / 64; w++) {
-      uint32_t s = sh.scan[w];
-      if (w < wid) add += s;
-      tot += s;
-    }
-    __syncthreads();
-    *total = tot;
-    return incl + add;
-  }
-}
-
-template <int TEAM>
-__device__ __forceinline__ unsigned long long team_sum(unsigned long long x, TeamShared<TEAM>& sh) {
-  x = pbf_wave_sum(x);
-  if constexpr (TEAM == 64) {
-    return x;
-  } else {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh.red[wid] = x;
-    __syncthreads();
-    unsigned long long t = 0;
-#pragma unroll
-    for (int w = 0; w < TEAM / 64; w++) t += sh.red[w];
-    __syncthreads();
-    return t;
-  }
-}
-
-// First index s in [0, TEAM) with P[s] > w (P non-decreasing, P[TEAM-1] > w).
-template <int TEAM>
-__device__ __forceinline__ uint32_t seg_of(const uint32_t* P, uint32_t w) {
-  uint32_t lo = 0, hi = TEAM - 1;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (P[mid] > w) hi = mid; else lo = mid + 1;
-  }
-  return lo;
-}
-
-// Number of entries of N(v)[0, len) whose rank is below i (entries are (rank << 1) | flag).
-__device__ __forceinline__ uint32_t rank_lower_bound(const uint32_t* __restrict__ a, uint32_t len, uint32_t i) {
-  const uint32_t key = i << 1;
-  uint32_t lo = 0, hi = len;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] < key) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-// Table operations -------------------------------------------------------------------------
-struct LdsTable {
-  uint32_t* keys;
-  uint32_t* cnts;
-  uint32_t* used;   // may be null
-  uint32_t shift, mask, maxfill;
-};
-
-template <int TEAM>
-__device__ __forceinline__ void lds_insert(const LdsTable& t, TeamShared<TEAM>& sh, uint32_t key) {
-  uint32_t h = hash_slot(key, t.shift);
-  while (true) {
-    uint32_t old = atomicCAS(&t.keys[h], EMPTY, key);
-    if (old == EMPTY) {
-      atomicAdd(&t.cnts[h], 1u);
-      uint32_t pos = atomicAdd(&sh.nused, 1u);
-      if (t.used) t.used[pos] = h;
-      if (pos >= t.maxfill) sh.ovf = 1;
-      return;
-    }
-    if (old == key) {
-      atomicAdd(&t.cnts[h], 1u);
-      return;
-    }
-    h = (h + 1) & t.mask;
-  }
-}
-
-__device__ __forceinline__ uint32_t lds_lookup(const LdsTable& t, uint32_t key) {
-  uint32_t h = hash_slot(key, t.shift);
-  while (t.keys[h] != key) h = (h + 1) & t.mask;
-  return t.cnts[h];
-}
 
 // Core: process the endpoints of one list with one team per endpoint.
 //   KIND == T_LDS: LDS hash table of capacity CAP (USED: keep a used-slot list so that the

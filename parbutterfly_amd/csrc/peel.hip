@@ -1,0 +1,560 @@
+// Butterfly peeling on the GPU: tip numbers (vertices of S) and wing numbers (edges).
+//
+// Semantics follow the reference's bucketed peeling (Peel / PeelOrigParallel, apps/hyper/
+// butterfly_peel.h:647-711, 896-934; PeelE / PeelEOrigParallel, butterfly_epeel.h:339-443,
+// 519-569; updateBuckets, butterfly_putils.h:1027-1115; increasing buckets of ligra/bucket.h):
+//   repeat: k = min D over live items; every live item with D == k is peeled in this round;
+//   the peeled items' butterflies are subtracted from the live items they share them with;
+//   every updated live item gets D = max(B, k).  The tip / wing number is D at removal.
+// The reference's bucket structure is replaced by min-count frontiers: per round a min
+// reduction, a selection (stream compaction of D == k), a two-hop walk of the frontier with
+// 64-bit atomic decrements, and an update of the touched items.  Rounds run in batches on the
+// device; the host reads the live count once per batch.
+//
+// Vertex peel walk (PeelOrigParallel :670-702): for a peeled u, every u2 != u reached through a
+// centre v loses C(c(u,u2), 2), c counted in an LDS hash (same team enumeration as counting).
+// Edge peel walk (PeelEOrigParallel :364-429): for a peeled edge e = (v,u) and every live
+// u2 in N(v), the butterflies {u,u2} x {v,v2} with v2 in N(u2) & N(u) and all four edges live
+// are charged once, by their minimum-index peeled edge; each of the other three edges loses one.
+#include "pbf_device.h"
+#include "team.h"
+
+using namespace pbf;
+
+namespace {
+constexpr unsigned long long KNONE = ~0ull;
+
+inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (unsigned)b;
+}
+
+// ---------------- shared round kernels ----------------
+// ctl layout (u64): [0] k, [1] live count; u32 view from word 4: list lengths / queues.
+struct Ctl {
+  unsigned long long* k;
+  unsigned long long* live;
+  uint32_t* len;    // [0] small, [1] large, [2] ovf, [3] ntouched
+  uint32_t* queue;  // [0] small, [1] large, [2] ovf
+};
+
+__global__ void k_round_begin(Ctl c) {
+  if (threadIdx.x == 0) *c.k = KNONE;
+  if (threadIdx.x < 4) c.len[threadIdx.x] = 0;
+  if (threadIdx.x < 3) c.queue[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    unsigned long long y = __shfl_xor(x, d, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+// k = min D over live items (state != 0).
+__global__ void k_min_live(const unsigned long long* __restrict__ D, const uint8_t* __restrict__ state, int64_t n, Ctl c) {
+  unsigned long long mn = KNONE;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+    if (state[x]) { unsigned long long d = D[x]; mn = d < mn ? d : mn; }
+  mn = wave_min_u64(mn);
+  if ((threadIdx.x & 63) == 0 && mn != KNONE) atomicMin(c.k, mn);
+}
+
+// Frontier selection: live items with D == k become current (state 2) and are listed by work
+// class (small / large by `work`); `live` is decremented by the number selected.
+__global__ void k_select(const unsigned long long* __restrict__ D, uint8_t* __restrict__ state, int64_t n,
+                         const uint32_t* __restrict__ work, uint32_t small_max, uint32_t* __restrict__ small_list,
+                         uint32_t* __restrict__ large_list, Ctl c) {
+  const unsigned long long k = *c.k;
+  if (k == KNONE) return;
+  uint32_t nsel = 0;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    if (state[x] == 1 && D[x] == k) {
+      state[x] = 2;
+      nsel++;
+      const uint32_t w = work[x];
+      if (w == 0) continue;
+      if (w <= small_max) small_list[atomicAdd(&c.len[0], 1u)] = (uint32_t)x;
+      else large_list[atomicAdd(&c.len[1], 1u)] = (uint32_t)x;
+    }
+  }
+  nsel = pbf_wave_sum(nsel);
+  if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
+}
+
+// Touched live items: D = max(B, k) (updateBuckets, butterfly_putils.h:1036-1043).
+__global__ void k_update(const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
+                         const unsigned long long* __restrict__ B, unsigned long long* __restrict__ D, Ctl c) {
+  const unsigned long long k = *c.k;
+  const uint32_t nt = c.len[3];
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    const uint32_t x = touched[t];
+    tflag[x] = 0;
+    const unsigned long long b = B[x];
+    D[x] = b > k ? b : k;
+  }
+}
+
+// Current items die at the end of the round.
+__global__ void k_retire(uint8_t* __restrict__ state, int64_t n, Ctl c) {
+  if (*c.k == KNONE) return;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+    if (state[x] == 2) state[x] = 0;
+}
+
+__device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* touched, uint32_t* ntouched) {
+  if (atomicExch(&tflag[x], 1u) == 0u) touched[atomicAdd(ntouched, 1u)] = x;
+}
+
+// ---------------- vertex peel walk ----------------
+enum TableKind { T_LDS = 0, T_HBM = 1 };
+
+struct PeelVArgs {
+  const uint32_t* offS;
+  const uint32_t* edgS;
+  const uint32_t* offO;
+  const uint32_t* edgO;
+  int64_t ns;
+  unsigned long long* B;
+  const unsigned long long* D;
+  const uint32_t* bound;
+  const uint32_t* list;
+  const uint32_t* list_len;
+  uint32_t* queue;
+  uint32_t* ovf_list;
+  uint32_t* ovf_len;
+  uint32_t* tflag;
+  uint32_t* touched;
+  uint32_t* ntouched;
+  uint32_t* pool;
+  const unsigned long long* k;
+};
+
+template <int TEAM, int CAP, bool USED, int KIND>
+__global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
+  __shared__ TeamShared<TEAM> sh;
+  __shared__ uint32_t lkeys[KIND == T_LDS ? CAP : 1];
+  __shared__ uint32_t lcnts[KIND == T_LDS ? CAP : 1];
+  __shared__ uint32_t lused[(KIND == T_LDS && USED) ? CAP : 1];
+  const uint32_t tid = threadIdx.x;
+  const unsigned long long k = *a.k;
+  if (k == KNONE) return;
+  const uint32_t nlist = *a.list_len;
+  uint32_t* dense = nullptr;
+  uint32_t* dused = nullptr;
+  if constexpr (KIND == T_HBM) {
+    dense = a.pool + (size_t)blockIdx.x * (size_t)a.ns * 2;
+    dused = dense + a.ns;
+  }
+  if constexpr (KIND == T_LDS && USED) {
+    for (uint32_t s = tid; s < CAP; s += TEAM) { lkeys[s] = EMPTY; lcnts[s] = 0; }
+  }
+  while (true) {
+    if (tid == 0) sh.item = atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const uint32_t idx = sh.item;
+    __syncthreads();
+    if (idx >= nlist) break;
+    const uint32_t u = a.list[idx];
+    const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
+    LdsTable t;
+    if constexpr (KIND == T_LDS) {
+      const uint32_t bnd = a.bound[u];
+      uint32_t cap = CAP;
+      if constexpr (!USED) {
+        uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
+        cap = 256;
+        while (cap < want && cap < CAP) cap <<= 1;
+      }
+      const uint32_t lg = 31 - __clz(cap);
+      t.keys = lkeys; t.cnts = lcnts; t.used = USED ? lused : nullptr;
+      t.shift = 32 - lg; t.mask = cap - 1;
+      t.maxfill = (cap == CAP) ? (CAP / 4) * 3 - TEAM : 0xFFFFFFFFu;
+      if constexpr (!USED) {
+        for (uint32_t s = tid; s < cap; s += TEAM) { lkeys[s] = EMPTY; lcnts[s] = 0; }
+      }
+    }
+    if (tid == 0) { sh.nused = 0; sh.ovf = 0; }
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < deg; c0 += TEAM) {
+      const uint32_t j = c0 + tid;
+      uint32_t L = 0, vo = 0;
+      if (j < deg) {
+        const uint32_t v = a.edgS[uo + j];
+        vo = a.offO[v];
+        L = a.offO[v + 1] - vo;
+      }
+      uint32_t T;
+      const uint32_t incl = team_incl_scan<TEAM>(L, sh, &T);
+      sh.P[tid] = incl;
+      sh.vs[tid] = vo;
+      __syncthreads();
+      for (uint32_t w = tid; w < T; w += TEAM) {
+        const uint32_t s = seg_of<TEAM>(sh.P, w);
+        const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
+        const uint32_t u2 = a.edgO[sh.vs[s] + kk];
+        if (u2 == u) continue;
+        if constexpr (KIND == T_LDS) {
+          if (!sh.ovf) lds_insert<TEAM>(t, sh, u2);
+        } else {
+          if (atomicAdd(&dense[u2], 1u) == 0u) dused[atomicAdd(&sh.nused, 1u)] = u2;
+        }
+      }
+      __syncthreads();
+    }
+    const bool overflowed = sh.ovf != 0;
+    if (!overflowed) {
+      const uint32_t extent = (KIND == T_HBM || USED) ? sh.nused : (t.mask + 1);
+      for (uint32_t q = tid; q < extent; q += TEAM) {
+        uint32_t key, c;
+        if constexpr (KIND == T_LDS) {
+          const uint32_t slot = USED ? lused[q] : q;
+          key = lkeys[slot];
+          c = (key == EMPTY) ? 0u : lcnts[slot];
+        } else {
+          key = dused[q];
+          c = dense[key];
+        }
+        if (c > 1u && a.D[key] > k) {
+          atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
+          touch(key, a.tflag, a.touched, a.ntouched);
+        }
+      }
+    } else if (tid == 0) {
+      a.ovf_list[atomicAdd(a.ovf_len, 1u)] = u;
+    }
+    __syncthreads();
+    if constexpr (KIND == T_LDS && USED) {
+      const uint32_t nu_ = sh.nused < CAP ? sh.nused : CAP;
+      for (uint32_t q = tid; q < nu_; q += TEAM) { const uint32_t slot = lused[q]; lkeys[slot] = EMPTY; lcnts[slot] = 0; }
+    }
+    if constexpr (KIND == T_HBM) {
+      for (uint32_t q = tid; q < sh.nused; q += TEAM) dense[dused[q]] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// bound[u] = min(sum_{v in N(u)} deg(v), ns): distinct two-hop neighbours of u.
+__global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ edgS,
+                                                     const uint32_t* __restrict__ offO, int64_t ns, uint32_t* __restrict__ bound) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int64_t u = (int64_t)blockIdx.x * 4 + wid; u < ns; u += (int64_t)gridDim.x * 4) {
+    uint64_t s = 0;
+    for (uint32_t p = offS[u] + lane; p < offS[u + 1]; p += 64) {
+      const uint32_t v = edgS[p];
+      s += offO[v + 1] - offO[v];
+    }
+    s = pbf_wave_sum(s);
+    if (lane == 0) bound[u] = (uint32_t)(s < (uint64_t)ns ? s : (uint64_t)ns);
+  }
+}
+
+// ---------------- edge peel walk ----------------
+struct PeelEArgs {
+  const uint32_t* offS;
+  const uint32_t* edgS;
+  const uint32_t* offO;
+  const uint32_t* edgO;
+  const uint32_t* eti;      // S-CSR position -> O-CSR index
+  const uint32_t* owner_o;  // O vertex of each O-CSR index
+  const uint8_t* state;     // 0 dead, 1 live, 2 peeled this round
+  unsigned long long* B;
+  const uint32_t* list;
+  const uint32_t* list_len;
+  uint32_t* queue;
+  uint32_t* tflag;
+  uint32_t* touched;
+  uint32_t* ntouched;
+  const unsigned long long* k;
+};
+
+// Edge f is charged by peeled edge e when it is live and, if peeled too, has a larger index.
+__device__ __forceinline__ bool usable(const uint8_t* state, uint32_t f, uint32_t e) {
+  const uint8_t s = state[f];
+  return s == 1 || (s == 2 && f > e);
+}
+
+template <int TEAM, int UCAP>
+__global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
+  __shared__ TeamShared<TEAM> sh;
+  __shared__ uint32_t nu_v[UCAP];    // N(u) (O vertices, ascending)
+  __shared__ uint32_t nu_f[UCAP];    // O-CSR index of (v2, u)
+  const uint32_t tid = threadIdx.x;
+  if (*a.k == KNONE) return;
+  const uint32_t nlist = *a.list_len;
+  while (true) {
+    if (tid == 0) sh.item = atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const uint32_t idx = sh.item;
+    __syncthreads();
+    if (idx >= nlist) break;
+    const uint32_t e = a.list[idx];
+    const uint32_t v = a.owner_o[e], u = a.edgO[e];
+    const uint32_t vo = a.offO[v], vdeg = a.offO[v + 1] - vo;
+    const uint32_t uo = a.offS[u], udeg = a.offS[u + 1] - uo;
+    const bool cached = udeg <= (uint32_t)UCAP;
+    if (cached)
+      for (uint32_t q = tid; q < udeg; q += TEAM) { nu_v[q] = a.edgS[uo + q]; nu_f[q] = a.eti[uo + q]; }
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < vdeg; c0 += TEAM) {
+      const uint32_t j = c0 + tid;
+      uint32_t L = 0, so = 0, u2 = 0;
+      if (j < vdeg) {
+        const uint32_t q = vo + j;  // f1 = (v, u2)
+        u2 = a.edgO[q];
+        if (u2 != u && usable(a.state, q, e)) {
+          so = a.offS[u2];
+          L = a.offS[u2 + 1] - so;
+        }
+      }
+      uint32_t T;
+      const uint32_t incl = team_incl_scan<TEAM>(L, sh, &T);
+      sh.P[tid] = incl;
+      sh.vs[tid] = so;
+      sh.acc[tid] = 0;
+      __syncthreads();
+      for (uint32_t w = tid; w < T; w += TEAM) {
+        const uint32_t s = seg_of<TEAM>(sh.P, w);
+        const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
+        const uint32_t p2 = sh.vs[s] + kk;
+        const uint32_t v2 = a.edgS[p2];
+        if (v2 == v) continue;
+        const uint32_t f2 = a.eti[p2];
+        if (!usable(a.state, f2, e)) continue;
+        // f3 = (v2, u): binary search v2 in N(u)
+        uint32_t lo = 0, hi = udeg;
+        if (cached) {
+          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nu_v[mid] < v2) lo = mid + 1; else hi = mid; }
+          if (lo == udeg || nu_v[lo] != v2) continue;
+        } else {
+          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (a.edgS[uo + mid] < v2) lo = mid + 1; else hi = mid; }
+          if (lo == udeg || a.edgS[uo + lo] != v2) continue;
+        }
+        const uint32_t f3 = cached ? nu_f[lo] : a.eti[uo + lo];
+        if (!usable(a.state, f3, e)) continue;
+        atomicAdd(&sh.acc[s], 1ull);
+        if (a.state[f2] == 1) { atomicAdd(&a.B[f2], ~0ull); touch(f2, a.tflag, a.touched, a.ntouched); }
+        if (a.state[f3] == 1) { atomicAdd(&a.B[f3], ~0ull); touch(f3, a.tflag, a.touched, a.ntouched); }
+      }
+      __syncthreads();
+      if (j < vdeg) {
+        const unsigned long long cnt = sh.acc[tid];
+        const uint32_t q = vo + j;
+        if (cnt && a.state[q] == 1) {
+          atomicAdd(&a.B[q], (unsigned long long)(-(long long)cnt));
+          touch(q, a.tflag, a.touched, a.ntouched);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// work[e] = sum_{u2 in N(v)} deg(u2) for e = (v, u): merge work of the edge's walk (saturating).
+__global__ __launch_bounds__(256) void k_peele_work(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ offO,
+                                                    const uint32_t* __restrict__ edgO, int64_t no, uint32_t* __restrict__ work,
+                                                    uint32_t* __restrict__ owner_o) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int64_t v = (int64_t)blockIdx.x * 4 + wid; v < no; v += (int64_t)gridDim.x * 4) {
+    uint64_t s = 0;
+    const uint32_t b = offO[v], e = offO[v + 1];
+    for (uint32_t q = b + lane; q < e; q += 64) {
+      const uint32_t u2 = edgO[q];
+      s += offS[u2 + 1] - offS[u2];
+      owner_o[q] = (uint32_t)v;
+    }
+    s = pbf_wave_sum(s);
+    const uint32_t w = (uint32_t)(s < 0xFFFFFFFFull ? s : 0xFFFFFFFFull);
+    for (uint32_t q = b + lane; q < e; q += 64) work[q] = w;
+  }
+}
+
+__global__ void k_init_peel(const unsigned long long* __restrict__ counts, unsigned long long* __restrict__ B,
+                            unsigned long long* __restrict__ D, uint8_t* __restrict__ state, uint32_t* __restrict__ tflag,
+                            int64_t n) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long c = counts[x];
+    B[x] = c;
+    D[x] = c;
+    state[x] = 1;
+    tflag[x] = 0;
+  }
+}
+
+constexpr int V_SMALL_TEAM = 64, V_SMALL_CAP = 1024, V_LARGE_TEAM = 256, V_LARGE_CAP = 8192, V_HBM_BLOCKS = 32;
+constexpr int E_SMALL_TEAM = 64, E_LARGE_TEAM = 256, E_UCAP = 2048;
+constexpr int ROUND_BATCH = 16;
+
+// Peel-state buffers common to both peels.
+struct PeelBufs {
+  unsigned long long *B, *D;
+  uint8_t* state;
+  uint32_t *tflag, *touched, *work, *lists, *owner_o;
+  Ctl c;
+};
+
+int peel_bufs(pbf_context* ctx, int64_t n, PeelBufs* pb) {
+  void *p0, *p1, *p2, *p3, *p4, *p5, *mp;
+  const size_t nn = (size_t)(n ? n : 1);
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL0, 8 * nn, &p0));
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL1, 8 * nn, &p1));
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL2, nn, &p2));
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL3, 4 * nn * 2, &p3));  // tflag | touched
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL4, 4 * nn * 2, &p4));  // work | owner_o
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL5, 4 * nn * 3, &p5));  // lists: small | large | ovf
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL6, 256, &mp));
+  pb->B = static_cast<unsigned long long*>(p0);
+  pb->D = static_cast<unsigned long long*>(p1);
+  pb->state = static_cast<uint8_t*>(p2);
+  pb->tflag = static_cast<uint32_t*>(p3);
+  pb->touched = pb->tflag + nn;
+  pb->work = static_cast<uint32_t*>(p4);
+  pb->owner_o = pb->work + nn;
+  pb->lists = static_cast<uint32_t*>(p5);
+  unsigned long long* m64 = static_cast<unsigned long long*>(mp);
+  pb->c.k = m64;
+  pb->c.live = m64 + 1;
+  pb->c.len = reinterpret_cast<uint32_t*>(m64 + 4);
+  pb->c.queue = reinterpret_cast<uint32_t*>(m64 + 8);
+  return PBF_OK;
+}
+
+// Runs rounds in batches until no live item remains; returns the number of rounds.
+template <class RoundF>
+int run_rounds(pbf_context* ctx, PeelBufs& pb, int64_t n, RoundF round, int64_t* rounds_out) {
+  ctx->pinned[33] = (uint64_t)n;
+  PBF_HIP(hipMemcpyAsync(pb.c.live, ctx->pinned + 33, 8, hipMemcpyHostToDevice, ctx->stream));
+  int64_t rounds = 0;
+  // every round peels at least one live item, so n rounds always suffice
+  while (rounds <= n) {
+    for (int r = 0; r < ROUND_BATCH; r++) PBF_CHECK(round());
+    rounds += ROUND_BATCH;
+    PBF_HIP(hipMemcpyAsync(ctx->pinned + 32, pb.c.live, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->pinned[32] == 0) break;
+  }
+  *rounds_out = rounds;
+  return ctx->pinned[32] == 0 ? PBF_OK : PBF_ERR_DOMAIN;
+}
+}  // namespace
+
+int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint64_t* counts, uint64_t* tip) {
+  const int64_t ns = use_v ? g.nu : g.nv;
+  if (ns == 0) return PBF_OK;
+  PeelBufs pb;
+  PBF_CHECK(peel_bufs(ctx, ns, &pb));
+  PeelVArgs a{};
+  a.offS = use_v ? g.offU : g.offV;
+  a.edgS = use_v ? g.edgU : g.edgV;
+  a.offO = use_v ? g.offV : g.offU;
+  a.edgO = use_v ? g.edgV : g.edgU;
+  a.ns = ns;
+  a.B = pb.B;
+  a.D = pb.D;
+  a.bound = pb.work;
+  a.tflag = pb.tflag;
+  a.touched = pb.touched;
+  a.ntouched = pb.c.len + 3;
+  a.k = pb.c.k;
+  hipLaunchKernelGGL(k_init_peel, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<const unsigned long long*>(counts), pb.B, pb.D, pb.state, pb.tflag, ns);
+  hipLaunchKernelGGL(k_peelv_bound, dim3(grid_for(ns, 4, 16384)), dim3(256), 0, ctx->stream, a.offS, a.edgS, a.offO, ns,
+                     pb.work);
+  PBF_HIP(hipGetLastError());
+  // HBM fallback pool, only when some vertex could overflow the largest LDS table
+  const uint32_t small_max = (V_SMALL_CAP / 4) * 3 - V_SMALL_TEAM;
+  void* pp;
+  PBF_CHECK(pbf_scratch(ctx, S_POOL, sizeof(uint32_t) * 2 * (size_t)ns * V_HBM_BLOCKS, &pp));
+  PBF_HIP(hipMemsetAsync(pp, 0, sizeof(uint32_t) * 2 * (size_t)ns * V_HBM_BLOCKS, ctx->stream));
+  a.pool = static_cast<uint32_t*>(pp);
+  uint32_t* small_list = pb.lists;
+  uint32_t* large_list = pb.lists + ns;
+  uint32_t* ovf_list = pb.lists + 2 * ns;
+  const unsigned sel_grid = grid_for(ns, 256, 4096);
+  auto round = [&]() -> int {
+    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
+    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, ns, pb.c);
+    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, ns, pb.work, small_max,
+                       small_list, large_list, pb.c);
+    PeelVArgs s = a;
+    s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
+    s.ovf_list = ovf_list; s.ovf_len = pb.c.len + 2;
+    hipLaunchKernelGGL((k_peelv_walk<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS>), dim3(ctx->num_cus * 12), dim3(V_SMALL_TEAM), 0,
+                       ctx->stream, s);
+    s.list = large_list; s.list_len = pb.c.len + 1; s.queue = pb.c.queue + 1;
+    hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS>), dim3(ctx->num_cus * 2), dim3(V_LARGE_TEAM), 0,
+                       ctx->stream, s);
+    s.list = ovf_list; s.list_len = pb.c.len + 2; s.queue = pb.c.queue + 2;
+    hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>), dim3(V_HBM_BLOCKS), dim3(V_LARGE_TEAM), 0, ctx->stream, s);
+    hipLaunchKernelGGL(k_update, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
+                       pb.c);
+    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, ns, pb.c);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+  };
+  int64_t rounds = 0;
+  PBF_CHECK(run_rounds(ctx, pb, ns, round, &rounds));
+  ctx->timing.rounds = rounds;
+  PBF_HIP(hipMemcpyAsync(tip, pb.D, 8 * (size_t)ns, hipMemcpyDeviceToDevice, ctx->stream));
+  return PBF_OK;
+}
+
+int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint64_t* counts, uint64_t* wing) {
+  const int64_t m = g.m;
+  if (m == 0) return PBF_OK;
+  PeelBufs pb;
+  PBF_CHECK(peel_bufs(ctx, m, &pb));
+  void* ep;
+  PBF_CHECK(pbf_scratch(ctx, S_ETI, 4 * (size_t)m, &ep));
+  uint32_t* eti = static_cast<uint32_t*>(ep);
+  PBF_CHECK(pbf_dev_edge_to_idxs(ctx, g, use_v, eti));
+  PeelEArgs a{};
+  a.offS = use_v ? g.offU : g.offV;
+  a.edgS = use_v ? g.edgU : g.edgV;
+  a.offO = use_v ? g.offV : g.offU;
+  a.edgO = use_v ? g.edgV : g.edgU;
+  a.eti = eti;
+  a.owner_o = pb.owner_o;
+  a.state = pb.state;
+  a.B = pb.B;
+  a.tflag = pb.tflag;
+  a.touched = pb.touched;
+  a.ntouched = pb.c.len + 3;
+  a.k = pb.c.k;
+  const int64_t no = use_v ? g.nv : g.nu;
+  hipLaunchKernelGGL(k_init_peel, dim3(grid_for(m, 256, 16384)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<const unsigned long long*>(counts), pb.B, pb.D, pb.state, pb.tflag, m);
+  hipLaunchKernelGGL(k_peele_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, a.offS, a.offO, a.edgO, no, pb.work,
+                     pb.owner_o);
+  PBF_HIP(hipGetLastError());
+  const uint32_t small_max = 4096;  // merge work below which one wavefront handles the edge
+  uint32_t* small_list = pb.lists;
+  uint32_t* large_list = pb.lists + m;
+  const unsigned sel_grid = grid_for(m, 256, 8192);
+  auto round = [&]() -> int {
+    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
+    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, m, pb.c);
+    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, m, pb.work, small_max, small_list,
+                       large_list, pb.c);
+    PeelEArgs s = a;
+    s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
+    hipLaunchKernelGGL((k_peele_walk<E_SMALL_TEAM, E_UCAP>), dim3(ctx->num_cus * 8), dim3(E_SMALL_TEAM), 0, ctx->stream, s);
+    s.list = large_list; s.list_len = pb.c.len + 1; s.queue = pb.c.queue + 1;
+    hipLaunchKernelGGL((k_peele_walk<E_LARGE_TEAM, E_UCAP>), dim3(ctx->num_cus * 4), dim3(E_LARGE_TEAM), 0, ctx->stream, s);
+    hipLaunchKernelGGL(k_update, dim3(grid_for(m, 256, 4096)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
+                       pb.c);
+    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, m, pb.c);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+  };
+  int64_t rounds = 0;
+  PBF_CHECK(run_rounds(ctx, pb, m, round, &rounds));
+  ctx->timing.rounds = rounds;
+  PBF_HIP(hipMemcpyAsync(wing, pb.D, 8 * (size_t)m, hipMemcpyDeviceToDevice, ctx->stream));
+  return PBF_OK;
+}

@@ -1,0 +1,71 @@
+"""Writes the golden vectors in tests/golden/ from the compiled reference (oracle/_ref).
+
+Run in a container that has the reference tree (oracle/Makefile builds oracle/_ref/ref_harness
+from the reference's own headers):
+    python tests/golden/make_golden.py
+For every graph below it stores the AdjacencyHypergraph input (<name>.adj) and the reference's
+outputs (<name>.npz): cmpWedgeCounts, per-vertex counts (SERIAL and BATCHS), per-edge counts,
+totals, tip numbers (PBATCHS), wing numbers (PBATCHS), and the reference's ADEG / ACOCORE orders
+with the wedge count W each gives.  The reference prints only totals (BipartiteButterflies.C:
+215, 234, 262); the harness dumps the arrays that Compute discards.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import parbutterfly_amd as pb  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def graphs():
+    kab = np.repeat(np.arange(4, dtype=np.uint32), 5), np.tile(np.arange(5, dtype=np.uint32), 4)
+    return [
+        ("uniform_1000x1000_m10000_s1", pb.Graph.uniform(1000, 1000, 10000, 1)),
+        ("chunglu_300x200_m2000_g21_s7", pb.Graph.chung_lu(300, 200, 2000, 2.1, 2.1, 7)),
+        ("chunglu_2000x500_m12000_g21_s2", pb.Graph.chung_lu(2000, 500, 12000, 2.1, 2.1, 2)),
+        ("rmat_8x8_m3000_s3", pb.Graph.rmat(8, 8, 3000, 0.5, 0.1, 0.1, 0.3, 3)),
+        ("k_4_5", pb.Graph.from_edges(4, 5, *kab)),
+    ]
+
+
+def main():
+    O.build()
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/ref_harness is not built (needs the reference tree)")
+    tmp = tempfile.mkdtemp()
+    out_bin = os.path.join(tmp, "o.bin")
+    for name, g in graphs():
+        adj = os.path.join(HERE, name + ".adj")
+        g.write(adj)
+        rec = {}
+        use_v, sw, c, _, _ = O.run_ref("side", adj, out_bin)
+        rec["use_v"] = np.array([use_v])
+        rec["side_wedges"] = np.array([sw], np.int64)
+        for ct in ("SERIAL", "BATCHS"):
+            for rt in ("ADEG", "ACOCORE"):
+                _, _, c, _, _ = O.run_ref("vert", adj, out_bin, ct, rt)
+                rec[f"vert_{ct}_{rt}"] = c
+                _, _, c, _, _ = O.run_ref("edge", adj, out_bin, ct, rt)
+                rec[f"edge_{ct}_{rt}"] = c
+                _, _, c, _, _ = O.run_ref("total", adj, out_bin, ct, rt)
+                rec[f"total_{ct}_{rt}"] = c
+        _, _, _, tip, _ = O.run_ref("peelv", adj, out_bin, "BATCHS", "ADEG", "BATCHS")
+        rec["tip_PBATCHS"] = tip
+        _, _, _, wing, _ = O.run_ref("peele", adj, out_bin, "BATCHS", "ADEG", "BATCHS")
+        rec["wing_PBATCHS"] = wing
+        for rt in ("ADEG", "ACOCORE"):
+            _, _, order, w, _ = O.run_ref("rank", adj, out_bin, "BATCHS", rt)
+            rec[f"order_{rt}"] = order.astype(np.uint32)
+            rec[f"W_{rt}"] = w
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print(name, "m", g.m, "total", int(rec["total_SERIAL_ADEG"][0]))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,26 @@
+"""Graph fixtures shared by the tests (generated with the library's seeded generators)."""
+import numpy as np
+
+import parbutterfly_amd as pb
+
+
+def complete_bipartite(a, b):
+    v = np.repeat(np.arange(a, dtype=np.uint32), b)
+    u = np.tile(np.arange(b, dtype=np.uint32), a)
+    return pb.Graph.from_edges(a, b, v, u)
+
+
+def empty_graph(nv, nu):
+    return pb.Graph.from_edges(nv, nu, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+
+
+def small_graphs():
+    """(tag, graph) pairs small enough for every oracle function, brute force included."""
+    out = []
+    for s in range(4):
+        out.append((f"uniform-30x25-s{s}", pb.Graph.uniform(30, 25, 150, s)))
+        out.append((f"chunglu-60x40-s{s}", pb.Graph.chung_lu(60, 40, 200, 2.1, 2.1, s)))
+    out.append(("K_4_5", complete_bipartite(4, 5)))
+    out.append(("K_1_7", complete_bipartite(1, 7)))
+    out.append(("rmat-6x6", pb.Graph.rmat(6, 6, 600, 0.5, 0.1, 0.1, 0.3, 3)))
+    return out

@@ -1,0 +1,63 @@
+"""GPU parity of tip (vertex) and wing (edge) peeling against the CPU oracle, bit-exact."""
+import numpy as np
+import pytest
+
+import parbutterfly_amd as pb
+from oracle import oracle as O
+from tests.graphs import complete_bipartite, small_graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_peel_small(ctx):
+    for tag, g in small_graphs():
+        print("case", tag, flush=True)
+        use_v, _ = ctx.side(g)
+        v = O.count_vert(g, use_v)
+        tip = pb.as_u64(ctx.peel(g, use_v, v))
+        assert np.array_equal(tip, O.peel_vert(g, use_v, v)), tag
+        e = O.count_edge(g, use_v)
+        wing = pb.as_u64(ctx.peel_edge(g, use_v, e))
+        assert np.array_equal(wing, O.peel_edge(g, use_v, e)), tag
+
+
+def test_peel_closed_form(ctx):
+    for a, b in [(3, 4), (6, 5)]:
+        g = complete_bipartite(a, b)
+        use_v, _ = ctx.side(g)
+        ns, no = (b, a) if use_v else (a, b)
+        v = pb.as_u64(ctx.count(g, use_v, "VERT"))
+        tip = pb.as_u64(ctx.peel(g, use_v, v))
+        assert (tip == (ns - 1) * (no * (no - 1) // 2)).all()
+        e = pb.as_u64(ctx.count(g, use_v, "EDGE"))
+        wing = pb.as_u64(ctx.peel_edge(g, use_v, e))
+        assert (wing == (a - 1) * (b - 1)).all()
+
+
+def test_peel_medium(ctx):
+    g = pb.Graph.rmat(11, 11, 30000, 0.5, 0.1, 0.1, 0.3, 3)
+    use_v, _ = ctx.side(g)
+    v = pb.as_u64(ctx.count(g, use_v, "VERT", "GPU", "ACOCORE"))
+    tip = pb.as_u64(ctx.peel(g, use_v, v))
+    assert np.array_equal(tip, O.peel_vert(g, use_v, v))
+    e = pb.as_u64(ctx.count(g, use_v, "EDGE", "GPU", "ACOCORE"))
+    wing = pb.as_u64(ctx.peel_edge(g, use_v, e))
+    assert np.array_equal(wing, O.peel_edge(g, use_v, e))
+
+
+def test_peel_chung_lu_hubs(ctx):
+    # hubs with large two-hop neighbourhoods exercise the workgroup and HBM-histogram walks
+    g = pb.Graph.chung_lu(4000, 3000, 60000, 2.1, 2.1, 8)
+    use_v, _ = ctx.side(g)
+    v = pb.as_u64(ctx.count(g, use_v, "VERT"))
+    tip = pb.as_u64(ctx.peel(g, use_v, v))
+    assert np.array_equal(tip, O.peel_vert(g, use_v, v))
+
+
+def test_peel_device_memory(ctx):
+    g = pb.Graph.rmat(9, 9, 5000, 0.5, 0.1, 0.1, 0.3, 5)
+    dg = g.to_device(ctx)
+    use_v, _ = ctx.side(dg)
+    e = ctx.count(dg, use_v, "EDGE", "GPU", "ACOCORE")
+    wing = ctx.peel_edge(dg, use_v, e)
+    assert np.array_equal(pb.as_u64(wing), O.peel_edge(g, use_v, pb.as_u64(e)))
