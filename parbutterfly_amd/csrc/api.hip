@@ -182,6 +182,7 @@ extern "C" int pbf_trim(pbf_context* ctx) {
     ctx->buf[s] = nullptr;
     ctx->cap[s] = 0;
   }
+  ctx->hot_clear = false;
   return PBF_OK;
 }
 

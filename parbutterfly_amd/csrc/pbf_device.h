@@ -35,6 +35,15 @@ enum ScratchSlot {
   S_OUT,         // staging for host outputs
   S_PEEL0, S_PEEL1, S_PEEL2, S_PEEL3, S_PEEL4, S_PEEL5, S_PEEL6, S_PEEL7,
   S_ACORE0, S_ACORE1, S_ACORE2,
+  S_HOT,         // replicated hub accumulators (kept zeroed between calls)
+  S_BIG,         // long-list classification: list, chunk counts, chunk bases
+  S_UB,          // u64 work bound of long lists
+  S_RLONG,       // relabel: long-list ids, chunk counts, chunk bases
+  S_RHIST,       // relabel: per-chunk key histograms / offsets
+  S_SPLIT,       // split endpoints: list, chunk counts, chunk bases
+  S_GROWS,       // split endpoints: dense count rows in HBM
+  S_LPOS,        // per-slot position of the owner inside the neighbour's list
+  S_ROWN,        // per-slot owner rank
   S_NUM_SLOTS
 };
 
@@ -47,6 +56,7 @@ struct pbf_context {
   pbf_timing timing = {};
   uint64_t* pinned = nullptr;  // small pinned host buffer (readbacks)
   int num_cus = 256;
+  bool hot_clear = false;  // S_HOT has been zeroed
 };
 
 // Returns a device buffer of at least `bytes` for the slot (grow-only; contents undefined).
@@ -82,6 +92,7 @@ struct RankedGraph {
   uint32_t* redg;     // [2m] (rank << 1) | flag(neighbour in S)
   uint32_t* rpos;     // [2m] position in owner's original CSR (EDGE only, else null)
   uint32_t* inpref;   // [n]  neighbours of lower rank
+  uint32_t* rown;     // [2m] owner rank of each slot
   int64_t wedges;     // W under this ranking
 };
 

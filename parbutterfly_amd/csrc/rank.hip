@@ -17,6 +17,7 @@
 
 namespace {
 constexpr int KEYS = 33;  // key values 0..32
+constexpr uint32_t RL_CHUNK = 1024;  // lists longer than this are relabelled in chunks
 
 __global__ void k_side_partial(const uint32_t* __restrict__ off, int64_t n, unsigned long long* acc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,7 +99,7 @@ __global__ void k_ranked_degree(DevGraph g, const uint32_t* __restrict__ order, 
 __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const uint8_t* __restrict__ key,
                                                  const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
                                                  uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
-                                                 uint32_t* __restrict__ inpref) {
+                                                 uint32_t* __restrict__ inpref, uint32_t* __restrict__ rown) {
   __shared__ uint32_t cnt[4][KEYS + 1];
   __shared__ uint32_t run[4][KEYS + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -108,6 +109,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     const bool xv = x < g.nv;
     const uint32_t obase = xv ? g.offV[x] : g.offU[x - g.nv];
     const uint32_t deg = xv ? g.offV[x + 1] - obase : g.offU[x - g.nv + 1] - obase;
+    if (deg > RL_CHUNK) continue;  // long lists: k_rl_* below
     const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
     const int64_t nbr_shift = xv ? g.nv : 0;  // combined id of neighbour = adj[j] + shift
     const uint32_t flag = use_v ? (xv ? 1u : 0u) : (xv ? 0u : 1u);
@@ -144,6 +146,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
       if (act) {
         slot = rbase + run[wid][ky] + rk;
         redg[slot] = (rank_of[y] << 1) | flag;
+        rown[slot] = r;
         if (rpos) rpos[slot] = obase + j;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -151,6 +154,159 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
       if (act && rk == 0) run[wid][ky] += __popcll(mm);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+// Long lists (more than RL_CHUNK entries) are relabelled in chunks so that a hub's list is
+// spread over many wavefronts: (1) each chunk's key histogram, (2) per list, each chunk's
+// starting offset inside every key class plus the class starts, (3) a stable scatter of each
+// chunk.  Chunk c of long vertex b is item chunk_base[b] + c.
+
+__device__ __forceinline__ uint32_t find_item_owner(const uint32_t* __restrict__ chunk_base, uint32_t nlong, uint32_t item) {
+  uint32_t lo = 0, hi = nlong;  // last b with chunk_base[b] <= item
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (chunk_base[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+struct LongLists {
+  const uint32_t* list;        // original ids of long vertices
+  const uint32_t* chunk_base;  // exclusive scan of chunk counts
+  const uint32_t* nlong;       // device count of long vertices
+  const uint32_t* nitems;      // device total of chunks
+  uint32_t* hist;              // [item][KEYS] counts, then offsets
+};
+
+__device__ __forceinline__ void vertex_list(const DevGraph& g, uint32_t x, uint32_t* obase, uint32_t* deg,
+                                            const uint32_t** adj, int64_t* shift) {
+  const bool xv = x < g.nv;
+  *obase = xv ? g.offV[x] : g.offU[x - g.nv];
+  *deg = xv ? g.offV[x + 1] - *obase : g.offU[x - g.nv + 1] - *obase;
+  *adj = xv ? g.edgV + *obase : g.edgU + *obase;
+  *shift = xv ? g.nv : 0;
+}
+
+__global__ __launch_bounds__(256) void k_rl_hist(DevGraph g, const uint8_t* __restrict__ key, LongLists L) {
+  __shared__ uint32_t h[4][KEYS + 1];
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t nl = *L.nlong, tot = *L.nitems;
+  for (uint32_t item = blockIdx.x * 4 + wid; item < tot; item += gridDim.x * 4) {
+    const uint32_t b = find_item_owner(L.chunk_base, nl, item);
+    const uint32_t x = L.list[b], c = item - L.chunk_base[b];
+    uint32_t obase, deg;
+    const uint32_t* adj;
+    int64_t sh;
+    vertex_list(g, x, &obase, &deg, &adj, &sh);
+    if (lane < KEYS + 1) h[wid][lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t j1 = min(deg, (c + 1) * RL_CHUNK);
+    for (uint32_t j = c * RL_CHUNK + lane; j < j1; j += 64) atomicAdd(&h[wid][key[adj[j] + sh]], 1u);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < KEYS) L.hist[(size_t)item * KEYS + lane] = h[wid][lane];
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// One wavefront per long vertex; lane k owns key k: running offsets over the chunks, then the
+// class starts (descending key order) are added, and in(r) is written.
+__global__ __launch_bounds__(256) void k_rl_offsets(DevGraph g, const uint8_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ rank_of, LongLists L,
+                                                    uint32_t* __restrict__ inpref) {
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t nl = *L.nlong;
+  for (uint32_t b = blockIdx.x * 4 + wid; b < nl; b += gridDim.x * 4) {
+    const uint32_t x = L.list[b];
+    const uint32_t i0 = L.chunk_base[b];
+    const uint32_t i1 = (b + 1 < nl) ? L.chunk_base[b + 1] : *L.nitems;
+    uint32_t run = 0;
+    if (lane < KEYS)
+      for (uint32_t it = i0; it < i1; it++) {
+        const uint32_t c = L.hist[(size_t)it * KEYS + lane];
+        L.hist[(size_t)it * KEYS + lane] = run;
+        run += c;
+      }
+    // class start of key k = sum of totals of keys > k
+    const uint32_t rv = lane < KEYS ? run : 0u;
+    const uint32_t srcl = lane < KEYS ? (KEYS - 1 - lane) : lane;  // reversed order for the scan
+    const uint32_t rrev = __shfl(rv, (int)srcl, 64);
+    const uint32_t inc = pbf_wave_incl_scan(lane < KEYS ? rrev : 0u);
+    const uint32_t start_rev = inc - (lane < KEYS ? rrev : 0u);  // start of key KEYS-1-lane
+    const uint32_t start = __shfl(start_rev, (int)srcl, 64);    // start of key lane
+    if (lane < KEYS)
+      for (uint32_t it = i0; it < i1; it++) L.hist[(size_t)it * KEYS + lane] += start;
+    const uint32_t kx = key[x];
+    const uint32_t sx = __shfl(start, (int)kx, 64), cx = __shfl(rv, (int)kx, 64);
+    if (lane == 0) inpref[rank_of[x]] = sx + (x < g.nv ? 0u : cx);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const uint8_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
+                                                    LongLists L, uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
+                                                    uint32_t* __restrict__ rown) {
+  __shared__ uint32_t run[4][KEYS + 1];
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t nl = *L.nlong, tot = *L.nitems;
+  for (uint32_t item = blockIdx.x * 4 + wid; item < tot; item += gridDim.x * 4) {
+    const uint32_t b = find_item_owner(L.chunk_base, nl, item);
+    const uint32_t x = L.list[b], c = item - L.chunk_base[b];
+    uint32_t obase, deg;
+    const uint32_t* adj;
+    int64_t sh;
+    vertex_list(g, x, &obase, &deg, &adj, &sh);
+    const bool xv = x < g.nv;
+    const uint32_t flag = use_v ? (xv ? 1u : 0u) : (xv ? 0u : 1u);
+    const uint32_t rx = rank_of[x];
+    const uint32_t rbase = roff[rx];
+    if (lane < KEYS) run[wid][lane] = L.hist[(size_t)item * KEYS + lane];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t j1 = min(deg, (c + 1) * RL_CHUNK);
+    for (uint32_t j0 = c * RL_CHUNK; j0 < j1; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      const bool act = j < j1;
+      uint32_t y = 0, ky = 0;
+      if (act) { y = (uint32_t)(adj[j] + sh); ky = key[y]; }
+      const uint64_t mm = match_key6(ky, __ballot(act));
+      const uint32_t rk = __popcll(mm & pbf_lanemask_lt());
+      if (act) {
+        const uint32_t slot = rbase + run[wid][ky] + rk;
+        redg[slot] = (rank_of[y] << 1) | flag;
+        rown[slot] = rx;
+        if (rpos) rpos[slot] = obase + j;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (act && rk == 0) run[wid][ky] += __popcll(mm);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+// Lists vertices whose lists exceed RL_CHUNK entries, with their chunk counts.
+__global__ void k_rl_find_long(DevGraph g, uint32_t* __restrict__ list, uint32_t* __restrict__ chunks,
+                               uint32_t* __restrict__ nlong) {
+  const int64_t n = g.nv + g.nu;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+    const int64_t x = base + (threadIdx.x & 63);
+    uint32_t d = 0;
+    if (x < n) d = vertex_degree(g, x);
+    const bool lg = d > RL_CHUNK;
+    const uint64_t m = __ballot(lg);
+    uint32_t b = 0;
+    if ((threadIdx.x & 63) == 0 && m) b = atomicAdd(nlong, (uint32_t)__popcll(m));
+    b = __shfl(b, 0, 64);
+    if (lg) {
+      const uint32_t slot = b + __popcll(m & pbf_lanemask_lt());
+      list[slot] = (uint32_t)x;
+      chunks[slot] = (d + RL_CHUNK - 1) / RL_CHUNK;
     }
   }
 }
@@ -271,6 +427,10 @@ __global__ void k_acore_update(const uint32_t* __restrict__ touched, const uint3
 
 __global__ void k_reset_u32(uint32_t* p) { *p = 0; }
 
+__global__ void k_fill_u32b(uint32_t* p, uint32_t v, int64_t n) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) p[x] = v;
+}
+
 inline unsigned grid_for(int64_t n, int threads, int max_blocks = 65535 * 4) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -355,6 +515,9 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
   PBF_CHECK(pbf_scratch(ctx, S_REDG, 4 * (size_t)(m2 ? m2 : 1), &re));
   PBF_CHECK(pbf_scratch(ctx, S_INPREF, 4 * (size_t)(n ? n : 1), &ip));
   PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
+  void* rwp;
+  PBF_CHECK(pbf_scratch(ctx, S_ROWN, 4 * (size_t)(m2 ? m2 : 1), &rwp));
+  R->rown = static_cast<uint32_t*>(rwp);
   R->roff = static_cast<uint32_t*>(ro);
   R->redg = static_cast<uint32_t*>(re);
   R->inpref = static_cast<uint32_t*>(ip);
@@ -368,7 +531,27 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
   hipLaunchKernelGGL(k_ranked_degree, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, R->order, R->roff);
   PBF_CHECK(pbf_scan_excl_u32(ctx, R->roff, R->roff, n + 1, nullptr));
   hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 4, 16384)), dim3(256), 0, ctx->stream, g, use_v, key, R->rank_of, R->roff,
-                     R->redg, R->rpos, R->inpref);
+                     R->redg, R->rpos, R->inpref, R->rown);
+  // long lists, chunk-parallel
+  {
+    const int64_t max_items = 2 * (2 * g.m / RL_CHUNK) + 64;
+    void *lp, *hp2;
+    PBF_CHECK(pbf_scratch(ctx, S_RLONG, 4 * (size_t)(3 * n + 64), &lp));
+    PBF_CHECK(pbf_scratch(ctx, S_RHIST, 4 * (size_t)max_items * KEYS, &hp2));
+    uint32_t* llist = static_cast<uint32_t*>(lp);
+    uint32_t* lchunks = llist + n;
+    uint32_t* lbase = llist + 2 * n;
+    uint32_t* lctl = llist + 3 * n;  // [0] nlong, [1] nitems
+    hipLaunchKernelGGL(k_fill_u32b, dim3(grid_for(n + 64, 256, 4096)), dim3(256), 0, ctx->stream, lchunks, 0u, n);
+    hipLaunchKernelGGL(k_fill_u32b, dim3(1), dim3(64), 0, ctx->stream, lctl, 0u, (int64_t)2);
+    hipLaunchKernelGGL(k_rl_find_long, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, g, llist, lchunks, lctl);
+    PBF_CHECK(pbf_scan_excl_u32(ctx, lchunks, lbase, n, lctl + 1));
+    LongLists L{llist, lbase, lctl, lctl + 1, static_cast<uint32_t*>(hp2)};
+    hipLaunchKernelGGL(k_rl_hist, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, key, L);
+    hipLaunchKernelGGL(k_rl_offsets, dim3(grid_for(n, 4, 2048)), dim3(256), 0, ctx->stream, g, key, R->rank_of, L, R->inpref);
+    hipLaunchKernelGGL(k_rl_scatter, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, use_v, key, R->rank_of, R->roff, L,
+                       R->redg, R->rpos, R->rown);
+  }
   unsigned long long* acc = static_cast<unsigned long long*>(mp) + 8;
   PBF_HIP(hipMemsetAsync(acc, 0, sizeof(uint64_t), ctx->stream));
   hipLaunchKernelGGL(k_rank_wedges, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, R->roff, R->inpref, n, acc);

@@ -53,6 +53,18 @@ def test_chung_lu_medium(ctx, rank_type):
     assert 2 * t == int(v.sum())
 
 
+def test_chung_lu_split_hubs(ctx):
+    # hubs above 32768 neighbours take the multi-workgroup (split) path; checked on every mode
+    g = pb.Graph.chung_lu(1 << 18, 1 << 16, 1 << 22, 2.1, 2.1, 5)
+    assert max(np.diff(g.offsetsV).max(), np.diff(g.offsetsU).max()) > 32768
+    use_v, _ = ctx.side(g)
+    v = pb.as_u64(ctx.count(g, use_v, "VERT"))
+    assert np.array_equal(v, O.count_vert(g, use_v))
+    e = pb.as_u64(ctx.count(g, use_v, "EDGE"))
+    assert np.array_equal(e, O.count_edge(g, use_v))
+    assert ctx.count(g, use_v, "TOTAL") == int(v.sum()) // 2
+
+
 def test_rmat_medium_edge(ctx):
     g = pb.Graph.rmat(14, 14, 200000, 0.5, 0.1, 0.1, 0.3, 3)
     use_v, _ = ctx.side(g)
