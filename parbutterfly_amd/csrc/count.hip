@@ -10,17 +10,15 @@
 //   * edge credit:     E[i->v] += c - 1 and E[v->u2] += c - 1 for every wedge with c > 1;
 //   * total:           sum of C(c,2) over all groups.
 //
-// GPU mapping.  A "team" (one wavefront, or one 256-thread workgroup for heavy endpoints)
-// owns one endpoint i at a time, taken from a device work queue.  The team enumerates i's
-// wedges flattened: for a chunk of TEAM neighbours v, each thread computes its segment length
-// L_v (in(v) when v < i, else the position of i inside N(v) by binary search), a team scan of
-// L gives a dense wedge index space, and each thread takes wedges w, w+TEAM, ... locating its
-// segment by binary search over the scanned lengths (so consecutive threads read consecutive
-// entries of the same N(v): coalesced loads, full lanes whatever the degrees).  Wedges are
-// aggregated by u2 in an open-addressing hash table in LDS (atomic CAS on the key, atomic add
-// on the count).  Heavy endpoints whose distinct-u2 bound exceeds the LDS table are moved to a
-// second kernel that uses a dense per-workgroup histogram in HBM.  Counts are scattered with
-// 64-bit atomics; integer atomic adds commute, so results are exact and deterministic.
+// GPU mapping.  One wavefront (light endpoints) or one 256-thread workgroup (heavy ones) owns
+// an endpoint i at a time, taken from a device work queue.  The wedges of a chunk of 64
+// neighbours are flattened: lane j holds L_j, the length of N(v_j)'s prefix below min(i, v_j),
+// a wave scan of L numbers the wedges densely, and windows of 256 consecutive wedge indices
+// are handed out, 4 consecutive indices per lane (consecutive lanes read consecutive entries
+// of the same N(v): coalesced loads, full lanes whatever the degrees).  The wedge key u2 is
+// aggregated in LDS: dense counters indexed by u2 when u2 < i is small, otherwise an
+// open-addressing hash.  Counts are scattered with 64-bit atomics; integer adds commute, so
+// the results are exact and deterministic.
 #include "pbf_device.h"
 #include "team.h"
 
@@ -34,7 +32,7 @@ struct CountArgs {
   const uint32_t* redg;
   const uint32_t* inpref;
   const uint32_t* lpos;    // per slot (i -> v) with v > i: position of i inside N(v)
-  const uint32_t* bound;   // distinct-u2 bound per endpoint
+  const uint32_t* bound;   // distinct-u2 bound per endpoint: min(wedges, i)
   const uint32_t* oidx;    // EDGE: O-CSR index of each ranked slot
   unsigned long long* B;   // VERT: counts in rank space; EDGE: counts in O-CSR order
   unsigned long long* total;
@@ -54,28 +52,16 @@ struct CountArgs {
   uint32_t cb;             // count bits of a packed 32-bit hash slot (32 - key bits)
 };
 
-// Core: process the endpoints of one list with one team per endpoint.
-//   KIND == T_LDS: LDS hash table of capacity CAP (USED: keep a used-slot list so that the
-//                  table is cleared lazily; otherwise clear the first `cap` slots per vertex).
-//   KIND == T_HBM: dense counters in HBM (pool slice per block), used list in HBM.
-template <int MODE, int TEAM, int CAP, bool USED, int KIND>
-__global__ __launch_bounds__(TEAM) void k_count(CountArgs a) {
+// HBM fallback: dense counters in HBM (pool slice per block), used list in HBM, one workgroup
+// per endpoint.  Only endpoints that no LDS table can hold come here (a workgroup hash that
+// overflowed, or an endpoint of degree >= 2^24).
+template <int MODE, int TEAM>
+__global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
   __shared__ TeamShared<TEAM> sh;
-  __shared__ uint32_t lkeys[KIND == T_LDS ? CAP : 1];
-  __shared__ uint32_t lcnts[KIND == T_LDS ? CAP : 1];
-  __shared__ uint32_t lused[(KIND == T_LDS && USED) ? CAP : 1];
   const uint32_t tid = threadIdx.x;
   const uint32_t nlist = *a.list_len;
-
-  uint32_t* dense = nullptr;
-  uint32_t* dused = nullptr;
-  if constexpr (KIND == T_HBM) {
-    dense = a.pool + (size_t)blockIdx.x * (size_t)a.n * 2;
-    dused = dense + a.n;
-  }
-  if constexpr (KIND == T_LDS && USED) {
-    for (uint32_t s = tid; s < CAP; s += TEAM) { lkeys[s] = EMPTY; lcnts[s] = 0; }
-  }
+  uint32_t* dense = a.pool + (size_t)blockIdx.x * (size_t)a.n * 2;
+  uint32_t* dused = dense + a.n;
   unsigned long long block_total = 0;
 
   while (true) {
@@ -87,36 +73,10 @@ __global__ __launch_bounds__(TEAM) void k_count(CountArgs a) {
     const uint32_t i = a.list[idx];
     const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
     const bool i_in_S = (a.redg[io] & 1u) == 0u;  // neighbours of i carry flag 1 iff i is on O
-
-    LdsTable t;
-    if constexpr (KIND == T_LDS) {
-      uint32_t bnd = a.bound[i];
-      uint32_t cap = CAP;
-      if constexpr (!USED) {
-        uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
-        cap = 256;
-        while (cap < want && cap < CAP) cap <<= 1;
-      }
-      uint32_t lg = 31 - __clz(cap);
-      t.keys = lkeys; t.cnts = lcnts; t.used = USED ? lused : nullptr;
-      t.shift = 32 - lg; t.mask = cap - 1;
-      // Only a full-size table can overflow: a smaller one was sized from a bound on the
-      // number of distinct keys.  After the flag is raised every thread completes at most its
-      // current insert, so occupancy stays below maxfill + TEAM < CAP and probing ends.
-      t.maxfill = (cap == CAP) ? (CAP / 4) * 3 - TEAM : 0xFFFFFFFFu;
-      if constexpr (!USED) {
-        for (uint32_t s = tid; s < cap; s += TEAM) { lkeys[s] = EMPTY; lcnts[s] = 0; }
-      }
-    }
-    if (tid == 0) { sh.nused = 0; sh.ovf = 0; }
+    if (tid == 0) sh.nused = 0;
     team_sync<TEAM>();
-
     const bool need_pass2 = (MODE == MODE_EDGE) || (MODE == MODE_VERT && !i_in_S);
     for (int pass = 1; pass <= (need_pass2 ? 2 : 1); pass++) {
-      if (pass == 2) {
-        // endpoint groups are complete: abandon on overflow (never reached when bound fits)
-        if (sh.ovf) break;
-      }
       for (uint32_t c0 = 0; c0 < deg; c0 += TEAM) {
         const uint32_t j = c0 + tid;
         uint32_t L = 0, vo = 0, v = 0;
@@ -130,7 +90,6 @@ __global__ __launch_bounds__(TEAM) void k_count(CountArgs a) {
         const uint32_t incl = team_incl_scan<TEAM>(L, sh, &T);
         sh.P[tid] = incl;
         sh.vs[tid] = vo;
-        sh.vv[tid] = v;
         if (pass == 2) sh.acc[tid] = 0;
         team_sync<TEAM>();
         for (uint32_t w = tid; w < T; w += TEAM) {
@@ -139,15 +98,9 @@ __global__ __launch_bounds__(TEAM) void k_count(CountArgs a) {
           const uint32_t p = sh.vs[s] + k;
           const uint32_t u2 = a.redg[p] >> 1;
           if (pass == 1) {
-            if constexpr (KIND == T_LDS) {
-              if (!sh.ovf) lds_insert<TEAM>(t, sh, u2);
-            } else {
-              if (atomicAdd(&dense[u2], 1u) == 0u) dused[atomicAdd(&sh.nused, 1u)] = u2;
-            }
+            if (atomicAdd(&dense[u2], 1u) == 0u) dused[atomicAdd(&sh.nused, 1u)] = u2;
           } else {
-            uint32_t c;
-            if constexpr (KIND == T_LDS) c = lds_lookup(t, u2);
-            else c = dense[u2];
+            const uint32_t c = dense[u2];
             if (c > 1u) {
               atomicAdd(&sh.acc[s], (unsigned long long)(c - 1u));
               if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[p]], (unsigned long long)(c - 1u));
@@ -164,48 +117,26 @@ __global__ __launch_bounds__(TEAM) void k_count(CountArgs a) {
         }
         team_sync<TEAM>();
       }
-      if (pass == 1) {
-        team_sync<TEAM>();
-        if (sh.ovf) break;
-        if (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S)) {
-          // endpoint credit: C(c,2) per group
-          unsigned long long sum = 0;
-          const uint32_t extent = (KIND == T_HBM || USED) ? sh.nused : (t.mask + 1);
-          for (uint32_t q = tid; q < extent; q += TEAM) {
-            uint32_t key, c;
-            if constexpr (KIND == T_LDS) {
-              const uint32_t slot = USED ? lused[q] : q;
-              key = lkeys[slot];
-              c = (key == EMPTY) ? 0u : lcnts[slot];
-            } else {
-              key = dused[q];
-              c = dense[key];
-            }
-            if (c > 1u) {
-              const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
-              sum += b;
-              if constexpr (MODE == MODE_VERT) atomicAdd(&a.B[key], b);
-            }
+      if (pass == 1 && (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S))) {
+        unsigned long long sum = 0;
+        for (uint32_t q = tid; q < sh.nused; q += TEAM) {
+          const uint32_t key = dused[q];
+          const uint32_t c = dense[key];
+          if (c > 1u) {
+            const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
+            sum += b;
+            if constexpr (MODE == MODE_VERT) atomicAdd(&a.B[key], b);
           }
-          sum = team_sum<TEAM>(sum, sh);
-          if (tid == 0 && sum) {
-            if constexpr (MODE == MODE_VERT) atomicAdd(&a.B[i], sum);
-            else block_total += sum;
-          }
+        }
+        sum = team_sum<TEAM>(sum, sh);
+        if (tid == 0 && sum) {
+          if constexpr (MODE == MODE_VERT) atomicAdd(&a.B[i], sum);
+          else block_total += sum;
         }
       }
     }
     team_sync<TEAM>();
-    const bool overflowed = sh.ovf != 0;
-    if (overflowed && tid == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = i;
-    // restore the table for the next endpoint
-    if constexpr (KIND == T_LDS && USED) {
-      const uint32_t nu_ = sh.nused < CAP ? sh.nused : CAP;
-      for (uint32_t q = tid; q < nu_; q += TEAM) { const uint32_t slot = lused[q]; lkeys[slot] = EMPTY; lcnts[slot] = 0; }
-    }
-    if constexpr (KIND == T_HBM) {
-      for (uint32_t q = tid; q < sh.nused; q += TEAM) dense[dused[q]] = 0;
-    }
+    for (uint32_t q = tid; q < sh.nused; q += TEAM) dense[dused[q]] = 0;
     team_sync<TEAM>();
   }
   if constexpr (MODE == MODE_TOTAL) {
@@ -215,49 +146,43 @@ __global__ __launch_bounds__(TEAM) void k_count(CountArgs a) {
 
 // ------------------------------------------------------------------------------------------
 // Fused counting kernel (one launch): a workgroup queue of heavy endpoints, then a wavefront
-// queue of light ones.
-//
-// Per endpoint i the wedges are enumerated segment by segment (segment = N(v)[0, L_v) for a
-// neighbour v).  Segments of at least LONG_SEG entries are streamed by a whole wavefront
-// (64 consecutive entries per step: coalesced, no per-wedge search); shorter segments are
-// walked by the lane that owns them.  The wedge key u2 is aggregated in LDS:
+// queue of light ones.  Tables:
 //   * dense u32 counters indexed by u2 when i < D32_MAX (u2 < i),
 //   * dense u16 counters (two per word) when i < D16_MAX and deg(i) < 65536 (so c < 65536),
-//   * otherwise an open-addressing hash of 64-bit slots (u2 << 32 | count).
-// Light endpoints (bound <= WAVE_BOUND distinct keys, bounded work) get a private 1024-slot
-// hash per wavefront; heavy ones share a workgroup table.  A workgroup hash that fills up
-// sends its endpoint to the HBM dense-histogram kernel.  Credits to ranks < HOT_N (the hubs,
-// which receive most credits) go to HOT_R replicated accumulators to spread atomic
-// contention; k_hot_reduce folds them into the counts.
+//   * otherwise open addressing: packed 32-bit slots (key << cb | count) when the counts fit
+//     in cb bits, else 64-bit slots (key << 32 | count).
+// An endpoint is light when its distinct-key bound min(wedges, i) fits a wavefront's private
+// 2048-slot table at load <= 3/4 and its wedges are few; such a table cannot overflow.  A
+// workgroup hash that fills up sends its endpoint to the HBM kernel.  Credits to ranks <
+// HOT_N (the hubs, which receive most credits) go to HOT_R replicated accumulators to spread
+// atomic contention; k_hot_reduce folds them into the counts.
 // ------------------------------------------------------------------------------------------
 constexpr int NWAVE = 4;
 constexpr int TPB = NWAVE * 64;
-constexpr uint32_t POOL_WORDS = 8192;          // 32 KB of LDS per workgroup
-constexpr uint32_t D32_MAX = POOL_WORDS;       // dense u32: i < 8192
-constexpr uint32_t D16_MAX = 2 * POOL_WORDS;   // dense u16: i < 16384
-constexpr uint32_t HB_CAP = POOL_WORDS / 2;    // workgroup hash: 4096 64-bit slots
-constexpr uint32_t HW_CAP = POOL_WORDS / 2 / NWAVE;  // wavefront hash: 1024 64-bit slots
-constexpr uint32_t WAVE_BOUND = (HW_CAP / 4) * 3 - 64;  // light endpoint: <= 704 distinct keys
-constexpr uint32_t WAVE_WORK = 1u << 14;             // ... and at most 16384 wedges (bound)
-constexpr uint32_t LONG_SEG = 32;
+constexpr uint32_t POOL_WORDS = 8192;             // 32 KB of LDS per workgroup
+constexpr uint32_t D32_MAX = POOL_WORDS;          // dense u32: i < 8192
+constexpr uint32_t D16_MAX = 2 * POOL_WORDS;      // dense u16: i < 16384
+constexpr uint32_t LT_WORDS = POOL_WORDS / NWAVE; // wavefront table: 2048 packed slots
+constexpr uint32_t LIGHT_MAX = LT_WORDS / 4 * 3 - 1;  // light endpoint: <= 1535 distinct keys ...
+constexpr uint32_t LIGHT_WORK = 1u << 14;         // ... and at most 16384 wedges
+static_assert(LIGHT_MAX + LIGHT_MAX / 3 + 1 <= LT_WORDS, "a light table must fit its wavefront's slots");
+constexpr uint32_t WIDE_DEG = 1u << 24;           // per-window u32 sums need deg(i) < 2^24
+constexpr uint32_t WIN = 4;                       // wedge indices per lane per window
+constexpr uint32_t WIN_SPAN = 64 * WIN;           // window of 256 consecutive wedge indices
 constexpr uint32_t HOT_N = 4096, HOT_R = 16;
-constexpr unsigned long long EMPTY64 = 0xFFFFFFFF00000000ull;
+constexpr uint32_t EPOCH_MAX = 1u << 26;          // window tags are epoch << 6 | segment
 
 enum TabMode { TM_D32 = 0, TM_D16 = 1, TM_HASH = 2, TM_HASH32 = 3, TM_G32 = 4 };
 
-// TM_HASH32 packs a slot into 32 bits, key << cb | count, usable when every count of the
-// endpoint fits in cb bits (c <= deg(i) < 2^cb) and every key fits in 32 - cb bits; the all-ones
-// word is the empty slot.  TM_HASH uses 64-bit slots, key << 32 | count.
 // LDS pointers carry address space 3 so that every access compiles to ds_* instructions
 // (through a generic pointer the compiler emits flat accesses, which also wait for all
 // outstanding global loads).
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 
 __device__ __forceinline__ uint32_t lds_load(lds_u32* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ unsigned long long lds_load(lds_u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void lds_store(lds_u32* p, uint32_t v) {
@@ -280,95 +205,138 @@ __device__ __forceinline__ unsigned long long lds_cas(lds_u64* p, unsigned long 
   return expect;
 }
 
+// Orders this wavefront's LDS accesses around cross-lane exchanges through LDS.  LDS
+// operations of one wavefront are performed in issue order, so only the compiler needs
+// fencing here.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------- wavefront scans on DPP (no LDS traffic) ----------------
+// row_shr:d (0x110 + d) within rows of 16 lanes, row_bcast:15 (0x142) into rows 1 and 3,
+// row_bcast:31 (0x143) into rows 2 and 3, wave_shr:1 (0x138), wave_shl:1 (0x130).  Lanes
+// without a source read 0 (bound_ctrl), rows outside row_mask keep 0.
+template <int CTRL, int ROWM = 0xf>
+__device__ __forceinline__ uint32_t dpp0(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWM, 0xf, true);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
+  x += dpp0<0x111>(x);
+  x += dpp0<0x112>(x);
+  x += dpp0<0x114>(x);
+  x += dpp0<0x118>(x);
+  x += dpp0<0x142, 0xa>(x);
+  x += dpp0<0x143, 0xc>(x);
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t x) {
+  x = max(x, dpp0<0x111>(x));
+  x = max(x, dpp0<0x112>(x));
+  x = max(x, dpp0<0x114>(x));
+  x = max(x, dpp0<0x118>(x));
+  x = max(x, dpp0<0x142, 0xa>(x));
+  x = max(x, dpp0<0x143, 0xc>(x));
+  return x;
+}
+
+// Segmented inclusive sum: lanes with equal keys s form runs (s non-decreasing over lanes).
+__device__ __forceinline__ uint32_t wave_seg_incl_add(uint32_t x, uint32_t s) {
+#define PBF_SEG_STEP(CTRL, ROWM)                  \
+  {                                               \
+    const uint32_t y = dpp0<CTRL, ROWM>(x);       \
+    const uint32_t t = dpp0<CTRL, ROWM>(s);       \
+    x += (t == s) ? y : 0u;                       \
+  }
+  PBF_SEG_STEP(0x111, 0xf)
+  PBF_SEG_STEP(0x112, 0xf)
+  PBF_SEG_STEP(0x114, 0xf)
+  PBF_SEG_STEP(0x118, 0xf)
+  PBF_SEG_STEP(0x142, 0xa)
+  PBF_SEG_STEP(0x143, 0xc)
+#undef PBF_SEG_STEP
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t x) { return dpp0<0x138>(x); }
+__device__ __forceinline__ uint32_t wave_shl1_or(uint32_t x, uint32_t fill) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t lane_read(uint32_t x, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)lane);
+}
+
+// ---------------- tables ----------------
 struct Tab {
-  lds_u32* w;                    // pool words
-  lds_u64* s;                    // same memory as 64-bit slots
-  uint32_t mode, shift, mask, maxfill, cb;
-  lds_u32* nused;
-  lds_u32* ovf;
-  const uint32_t* g;             // TM_G32: dense counts in HBM (lookups only)
+  lds_u32* w;        // LDS words
+  uint32_t mask, shift, cb;
+  const uint32_t* g; // TM_G32: dense counts in HBM (lookups only)
 };
 
-__device__ __forceinline__ void tab_insert(const Tab& t, uint32_t u2) {
-  if (t.mode == TM_D32) {
+// Inserts key u2: returns 1 when the key is new, 0 when it was present, 2 when the probe
+// sequence found no room (the table is full).
+template <int TM>
+__device__ __forceinline__ uint32_t tab_ins(const Tab& t, uint32_t u2) {
+  if constexpr (TM == TM_D32) {
     lds_add(t.w + u2, 1u);
-  } else if (t.mode == TM_D16) {
+    return 0;
+  } else if constexpr (TM == TM_D16) {
     lds_add(t.w + (u2 >> 1), 1u << ((u2 & 1u) << 4));
-  } else if (t.mode == TM_HASH32) {
+    return 0;
+  } else if constexpr (TM == TM_HASH32) {
     uint32_t h = hash_slot(u2, t.shift);
     const uint32_t mine = (u2 << t.cb) | 1u;
     for (uint32_t probe = 0; probe <= t.mask; probe++) {
-      const uint32_t cur = lds_load(t.w + h);
-      if (cur != EMPTY && (cur >> t.cb) == u2) { lds_add(t.w + h, 1u); return; }
-      if (cur == EMPTY) {
-        const uint32_t old = lds_cas(t.w + h, EMPTY, mine);
-        if (old == EMPTY) {
-          if (lds_add(t.nused, 1u) >= t.maxfill) lds_store(t.ovf, 1u);
-          return;
-        }
-        if ((old >> t.cb) == u2) { lds_add(t.w + h, 1u); return; }
-      }
+      const uint32_t old = lds_cas(t.w + h, EMPTY, mine);
+      if (old == EMPTY) return 1;
+      if ((old >> t.cb) == u2) { lds_add(t.w + h, 1u); return 0; }
       h = (h + 1) & t.mask;
     }
-    lds_store(t.ovf, 1u);
+    return 2;
   } else {
+    lds_u64* s = (lds_u64*)t.w;
     uint32_t h = hash_slot(u2, t.shift);
     const unsigned long long mine = ((unsigned long long)u2 << 32) | 1ull;
     for (uint32_t probe = 0; probe <= t.mask; probe++) {
-      const unsigned long long cur = lds_load(t.s + h);
-      if ((uint32_t)(cur >> 32) == u2) { lds_add(t.s + h, 1ull); return; }
-      if (cur == EMPTY64) {
-        const unsigned long long old = lds_cas(t.s + h, EMPTY64, mine);
-        if (old == EMPTY64) {
-          if (lds_add(t.nused, 1u) >= t.maxfill) lds_store(t.ovf, 1u);
-          return;
-        }
-        if ((uint32_t)(old >> 32) == u2) { lds_add(t.s + h, 1ull); return; }
-      }
+      const unsigned long long old = lds_cas(s + h, ~0ull, mine);
+      if (old == ~0ull) return 1;
+      if ((uint32_t)(old >> 32) == u2) { lds_add(s + h, 1ull); return 0; }
       h = (h + 1) & t.mask;
     }
-    lds_store(t.ovf, 1u);
+    return 2;
   }
 }
 
-__device__ __forceinline__ uint32_t tab_lookup(const Tab& t, uint32_t u2) {
-  if (t.mode == TM_D32) return t.w[u2];
-  if (t.mode == TM_G32) return t.g[u2];
-  if (t.mode == TM_D16) return (t.w[u2 >> 1] >> ((u2 & 1u) << 4)) & 0xFFFFu;
-  uint32_t h = hash_slot(u2, t.shift);
-  if (t.mode == TM_HASH32) {
+template <int TM>
+__device__ __forceinline__ uint32_t tab_get(const Tab& t, uint32_t u2) {
+  if constexpr (TM == TM_D32) {
+    return t.w[u2];
+  } else if constexpr (TM == TM_G32) {
+    return t.g[u2];
+  } else if constexpr (TM == TM_D16) {
+    return (t.w[u2 >> 1] >> ((u2 & 1u) << 4)) & 0xFFFFu;
+  } else if constexpr (TM == TM_HASH32) {
+    uint32_t h = hash_slot(u2, t.shift);
     for (uint32_t probe = 0; probe <= t.mask; probe++) {
       const uint32_t cur = t.w[h];
+      if ((cur >> t.cb) == u2) return cur & ((1u << t.cb) - 1u);  // EMPTY never matches a key
       if (cur == EMPTY) return 0u;
-      if ((cur >> t.cb) == u2) return cur & ((1u << t.cb) - 1u);
+      h = (h + 1) & t.mask;
+    }
+    return 0u;
+  } else {
+    const lds_u64* s = (const lds_u64*)t.w;
+    uint32_t h = hash_slot(u2, t.shift);
+    for (uint32_t probe = 0; probe <= t.mask; probe++) {
+      const unsigned long long cur = s[h];
+      if ((uint32_t)(cur >> 32) == u2 && cur != ~0ull) return (uint32_t)cur;
+      if (cur == ~0ull) return 0u;
       h = (h + 1) & t.mask;
     }
     return 0u;
   }
-  for (uint32_t probe = 0; probe <= t.mask; probe++) {
-    const unsigned long long cur = t.s[h];
-    if ((uint32_t)(cur >> 32) == u2) return (uint32_t)cur;
-    if (cur == EMPTY64) return 0u;
-    h = (h + 1) & t.mask;
-  }
-  return 0u;
-}
-
-// Number of table cells and the (key, count) of cell q.
-__device__ __forceinline__ uint32_t tab_cells(const Tab& t, uint32_t i) {
-  return (t.mode == TM_HASH || t.mode == TM_HASH32) ? t.mask + 1 : i;
-}
-__device__ __forceinline__ uint32_t tab_cell(const Tab& t, uint32_t q, uint32_t* key) {
-  if (t.mode == TM_D32) { *key = q; return t.w[q]; }
-  if (t.mode == TM_D16) { *key = q; return (t.w[q >> 1] >> ((q & 1u) << 4)) & 0xFFFFu; }
-  if (t.mode == TM_HASH32) {
-    const uint32_t cur = t.w[q];
-    *key = cur >> t.cb;
-    return cur == EMPTY ? 0u : (cur & ((1u << t.cb) - 1u));
-  }
-  const unsigned long long cur = t.s[q];
-  *key = (uint32_t)(cur >> 32);
-  return cur == EMPTY64 ? 0u : (uint32_t)cur;
 }
 
 // Credit to rank x: hub ranks go to replicated accumulators.
@@ -377,207 +345,348 @@ __device__ __forceinline__ void credit(const CountArgs& a, uint32_t x, unsigned 
   else atomicAdd(&a.B[x], val);
 }
 
-// Enumerates the wedges of endpoint i: the neighbours are taken 64 at a time; their segment
-// lengths L_v are scanned into a flattened wedge range [0, T), and the wavefronts of the group
-// share that range (wave w of nw takes indices w*64 + lane, then every 64*nw).  Each index is
-// mapped to its segment by a binary search over the scanned lengths held in lanes (six
-// shuffles), so any mix of short and long segments keeps every lane busy, and U indices are
-// resolved before their loads are issued so that U loads per lane are in flight.  PASS2 looks
-// the counts up and credits centres (VERT) or edges (EDGE): per-segment sums are reduced over
-// the contiguous lanes of a segment and accumulated in the wave's LDS slots `acc`.
-constexpr int UNROLL = 4;
+// ---------------- windowed wedge enumeration ----------------
+// Per wavefront LDS: the window's segment-start flags and the per-segment partial sums.
+struct WaveCtx {
+  lds_u32* flags;   // WIN_SPAN words, 16-byte aligned
+  lds_u64* acc;     // 64 partial sums (pass 2), kept zero between chunks
+  uint32_t epoch;   // tag generation of the flags
+};
 
-template <int MODE, bool PASS2>
-__device__ __forceinline__ void enum_wedges(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t jbeg,
-                                            uint32_t deg, uint32_t w, uint32_t nw, lds_u64* acc) {
-  const uint32_t lane = threadIdx.x & 63;
-  // high-degree endpoints: the waves take different neighbour chunks; otherwise they share the
-  // wedge range of each chunk
-  const bool split_chunks = nw > 1 && deg - jbeg >= 8 * 64;
-  const uint32_t cstart = split_chunks ? w * 64 : 0u, cstep = split_chunks ? nw * 64 : 64u;
-  const uint32_t wstart = split_chunks ? 0u : w * 64, wstep = split_chunks ? 64u : nw * 64;
-  for (uint32_t c0 = jbeg + cstart; c0 < deg; c0 += cstep) {
+// Overflow tracking of a shared hash table (one workgroup).
+struct Track {
+  lds_u32* nused;
+  lds_u32* ovf;
+  uint32_t maxfill;
+  bool on;
+};
+
+// One window: lane l holds wedge indices b0 + 4l + k, k < nval.
+struct Win {
+  uint32_t ent[WIN];  // ranked entry (rank << 1 | flag) of u2
+  uint32_t pos[WIN];  // its slot in the ranked adjacency
+  uint32_t seg[WIN];  // neighbour (segment) index within the chunk
+  uint32_t nval;
+};
+
+__device__ __forceinline__ uint32_t next_tag(WaveCtx& wc) {
+  if (++wc.epoch == EPOCH_MAX) {
+    // tags would wrap: forget every stale start
+    wave_lds_fence();
+    *(lds_u32x4*)(wc.flags + 4 * __lane_id()) = u32x4{0u, 0u, 0u, 0u};
+    wave_lds_fence();
+    wc.epoch = 1;
+  }
+  return wc.epoch << 6;
+}
+
+// Maps the window at b0 to (segment, slot) per wedge index and issues its loads.  A segment
+// s with L_s > 0 starting inside the window writes its tag (epoch << 6 | s) at its start; an
+// inclusive max-scan over the window then yields, per index, the last segment started at or
+// before it.  The segment covering b0 itself (it may have started in an earlier window) is
+// the highest lane with L > 0 and P <= b0.  Stale flags from earlier windows of the chunk
+// hold smaller segments, those of earlier chunks smaller epochs: the max ignores both.
+__device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint32_t b0, uint32_t T, uint32_t P, uint32_t L,
+                                        uint32_t base, uint32_t tag, Win& wv) {
+  const uint32_t lane = __lane_id();
+  if (L != 0u && P >= b0 && P - b0 < WIN_SPAN) flags[P - b0] = tag | lane;
+  const uint64_t cm = __ballot(L != 0u && P <= b0);
+  const uint32_t carry = tag | (63u - (uint32_t)__clzll(cm));
+  wave_lds_fence();
+  const u32x4 f = *(lds_u32x4*)(flags + 4 * lane);
+  wave_lds_fence();
+  const uint32_t m0 = f.x, m1 = max(m0, f.y), m2 = max(m1, f.z), m3 = max(m2, f.w);
+  const uint32_t cin = max(wave_shr1(wave_incl_max(m3)), carry);
+  const uint32_t q0 = b0 + WIN * lane;
+  wv.nval = q0 < T ? min(T - q0, WIN) : 0u;
+  const uint32_t mm[WIN] = {m0, m1, m2, m3};
+#pragma unroll
+  for (uint32_t k = 0; k < WIN; k++) {
+    const uint32_t s = max(cin, mm[k]) & 63u;
+    wv.seg[k] = s;
+    wv.pos[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(s << 2), (int)base) + q0 + k;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < WIN; k++) wv.ent[k] = k < wv.nval ? a.redg[wv.pos[k]] : 0u;
+}
+
+// Pass 1 on one window: returns the lane's number of new keys, plus 256 on a failed insert.
+template <int TM>
+__device__ __forceinline__ uint32_t win_insert(const Tab& t, const Win& wv) {
+  uint32_t r = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < WIN; k++) {
+    if (k < wv.nval) {
+      const uint32_t s = tab_ins<TM>(t, wv.ent[k] >> 1);
+      r += (s == 2u) ? 256u : s;
+    }
+  }
+  return r;
+}
+
+// Pass 2 on one window: c - 1 per wedge with c > 1, credited to the edge (v -> u2) (EDGE) and
+// summed per segment into acc.  A lane folds its own wedges of its last segment; the others
+// (a segment boundary inside the lane) go straight to acc.  Lane sums are then reduced per
+// segment with a segmented scan; the last lane of each run adds the run's sum.  The u32 sums
+// hold at most 256 values below deg(i) < 2^24.
+template <int MODE, int TM>
+__device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, const Win& wv, lds_u64* acc) {
+  uint32_t val[WIN];
+  uint32_t last = wv.seg[0];
+#pragma unroll
+  for (uint32_t k = 0; k < WIN; k++) {
+    val[k] = 0;
+    if (k < wv.nval) {
+      last = wv.seg[k];
+      const uint32_t c = tab_get<TM>(t, wv.ent[k] >> 1);
+      if (c > 1u) {
+        val[k] = c - 1u;
+        if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[wv.pos[k]]], (unsigned long long)(c - 1u));
+      }
+    }
+  }
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < WIN; k++) {
+    if (wv.seg[k] == last) sum += val[k];
+    else if (val[k]) lds_add(acc + wv.seg[k], (unsigned long long)val[k]);
+  }
+  if (__ballot(sum != 0u) == 0ull) return;
+  const uint32_t key = wv.nval ? last : 64u;  // lanes past the end form a trailing run
+  const uint32_t x = wave_seg_incl_add(sum, key);
+  if (key < 64u && x && wave_shl1_or(key, ~0u) != key) lds_add(acc + key, (unsigned long long)x);
+}
+
+// Enumerates the wedges of endpoint i over neighbours [jbeg, jend) with wave w of nw.  For
+// long lists (several waves) the waves take different 64-neighbour chunks; otherwise they
+// share the windows of each chunk.  The next window is mapped and its loads issued before the
+// current one is aggregated.  Returns false when a tracked table overflowed.
+template <int MODE, bool PASS2, int TM>
+__device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t jbeg,
+                                          uint32_t jend, uint32_t w, uint32_t nw, WaveCtx& wc, const Track& tr) {
+  const uint32_t lane = __lane_id();
+  const bool split = nw > 1 && jend - jbeg >= 8 * 64;
+  const uint32_t cstart = split ? w * 64 : 0u, cstep = split ? nw * 64 : 64u;
+  const uint32_t wstart = split ? 0u : w * WIN_SPAN, wstep = split ? WIN_SPAN : nw * WIN_SPAN;
+  for (uint32_t c0 = jbeg + cstart; c0 < jend; c0 += cstep) {
     const uint32_t j = c0 + lane;
     uint32_t v = 0, vo = 0, L = 0;
-    if (j < deg) {
+    if (j < jend) {
       v = a.redg[io + j] >> 1;
       vo = a.roff[v];
-      const uint32_t ip = a.inpref[v];
-      // at most i entries of N(v) rank below i
       // the prefix of N(v) below min(i, v): all of in(v) when v < i, else up to i's own slot
-      L = (v < i) ? ip : a.lpos[io + j];
+      L = (v < i) ? a.inpref[v] : a.lpos[io + j];
     }
-    const uint32_t incl = pbf_wave_incl_scan(L);
-    const uint32_t T = __shfl(incl, 63, 64);
-    if (T == 0) continue;
-    const uint32_t P = incl - L;  // first flattened index of this lane's segment
-    if (PASS2) {
-      acc[lane] = 0;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-    }
-    for (uint32_t b0 = wstart; b0 < T; b0 += wstep * UNROLL) {
-      uint32_t ent[UNROLL], seg[UNROLL], pos[UNROLL];
-#pragma unroll
-      for (int u = 0; u < UNROLL; u++) {
-        const uint32_t widx = b0 + (uint32_t)u * wstep + lane;
-        // largest s with P_s <= widx (zero-length segments share a start with the next one,
-        // and the larger index wins)
-        uint32_t s = 0;
-#pragma unroll
-        for (uint32_t bit = 32; bit; bit >>= 1) {
-          const uint32_t cand = s + bit;
-          const uint32_t pc = __shfl(P, (int)(cand & 63u), 64);
-          if (cand < 64u && pc <= widx) s = cand;
+    const uint32_t incl = wave_incl_add(L);
+    const uint32_t T = lane_read(incl, 63);
+    if (T <= wstart) continue;
+    const uint32_t P = incl - L;
+    const uint32_t base = vo - P;  // slot of wedge index x in segment s: base_s + x
+    const uint32_t tag = next_tag(wc);
+    Win cur, nxt;
+    uint32_t b0 = wstart;
+    win_map(a, wc.flags, b0, T, P, L, base, tag, cur);
+    while (true) {
+      const uint32_t nb = b0 + wstep;
+      const bool more = nb < T;
+      if (more) win_map(a, wc.flags, nb, T, P, L, base, tag, nxt);
+      if constexpr (!PASS2) {
+        const uint32_t r = win_insert<TM>(t, cur);
+        const bool failed = __ballot(r >= 256u) != 0ull;
+        if (tr.on) {
+          const uint32_t nnew = lane_read(wave_incl_add(r & 255u), 63);
+          if (lane == 0 && (failed || lds_add(tr.nused, nnew) + nnew >= tr.maxfill)) lds_store(tr.ovf, 1u);
+          wave_lds_fence();
+          if (__builtin_amdgcn_readfirstlane(lds_load(tr.ovf))) return false;
+        } else if (failed) {
+          if (lane == 0) lds_store(tr.ovf, 1u);
+          return false;
         }
-        const uint32_t ps = __shfl(P, (int)s, 64), vs = __shfl(vo, (int)s, 64);
-        pos[u] = vs + (widx - ps);
-        seg[u] = widx < T ? s : 64u;
-        ent[u] = widx < T ? a.redg[pos[u]] : 0u;
+      } else {
+        win_credit<MODE, TM>(a, t, cur, wc.acc);
       }
-#pragma unroll
-      for (int u = 0; u < UNROLL; u++) {
-        if (!PASS2) {
-          if (seg[u] < 64u && !lds_load(t.ovf)) tab_insert(t, ent[u] >> 1);
-        } else {
-          unsigned long long val = 0;
-          if (seg[u] < 64u) {
-            const uint32_t c = tab_lookup(t, ent[u] >> 1);
-            if (c > 1u) {
-              val = c - 1u;
-              if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[pos[u]]], val);
-            }
-          }
-          // segmented inclusive scan over the contiguous lanes of each segment
-          const uint32_t sg = seg[u];
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) {
-            const unsigned long long y = __shfl_up(val, d, 64);
-            const uint32_t sy = __shfl_up(sg, d, 64);
-            if ((int)lane >= d && sy == sg) val += y;
-          }
-          const uint32_t snext = __shfl_down(sg, 1, 64);
-          const bool tail = lane == 63 || snext != sg;
-          if (tail && sg < 64u && val) lds_add(acc + sg, val);
-        }
-      }
+      if (!more) break;
+      cur = nxt;
+      b0 = nb;
     }
-    if (PASS2) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      const unsigned long long mine = acc[lane];
-      if (j < deg && mine) {
+    if constexpr (PASS2) {
+      wave_lds_fence();
+      const unsigned long long mine = wc.acc[lane];
+      if (mine) {
+        wc.acc[lane] = 0;
         if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[io + j]], mine);
         else credit(a, v, mine);
       }
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_fence();
     }
   }
+  return true;
 }
 
-template <bool BLOCK>
-__device__ __forceinline__ void group_sync() {
-  if constexpr (BLOCK) {
-    __syncthreads();
-  } else {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+// Endpoint credits from a finished table: C(c,2) to every key (VERT) and the sum (returned).
+// Cells q = gt, gt + gn, ... of `cells`.
+template <int MODE, int TM>
+__device__ __forceinline__ unsigned long long sweep(const CountArgs& a, const Tab& t, uint32_t cells, uint32_t gt,
+                                                    uint32_t gn) {
+  unsigned long long sum = 0;
+  for (uint32_t q = gt; q < cells; q += gn) {
+    uint32_t key, c;
+    if constexpr (TM == TM_D32) {
+      key = q; c = t.w[q];
+    } else if constexpr (TM == TM_D16) {
+      key = q; c = (t.w[q >> 1] >> ((q & 1u) << 4)) & 0xFFFFu;
+    } else if constexpr (TM == TM_HASH32) {
+      const uint32_t cur = t.w[q];
+      key = cur >> t.cb; c = cur == EMPTY ? 0u : (cur & ((1u << t.cb) - 1u));
+    } else {
+      const unsigned long long cur = ((const lds_u64*)t.w)[q];
+      key = (uint32_t)(cur >> 32); c = cur == ~0ull ? 0u : (uint32_t)cur;
+    }
+    if (c > 1u) {
+      const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
+      sum += b;
+      if constexpr (MODE == MODE_VERT) credit(a, key, b);
+    }
   }
+  return sum;
 }
 
-// Processes endpoint i with a group of waves (the workgroup, or one wavefront) on table t.
-// Returns false when the table overflowed (nothing was credited).
-template <int MODE, bool BLOCK>
-__device__ bool process_endpoint(const CountArgs& a, Tab& t, uint32_t i, uint32_t bound, unsigned long long* red,
-                                 unsigned long long* total_acc, lds_u64* acc) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t w = BLOCK ? (threadIdx.x >> 6) : 0u, nw = BLOCK ? NWAVE : 1u;
-  const uint32_t gt = BLOCK ? threadIdx.x : lane, gn = BLOCK ? TPB : 64u;
+// Heavy endpoint, whole workgroup.  Returns false when its hash table overflowed (nothing was
+// credited).
+template <int MODE, int TM>
+__device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t deg, bool i_in_S,
+                              uint32_t cells, WaveCtx& wc, const Track& tr, unsigned long long* red,
+                              unsigned long long* total_acc) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  enum_pass<MODE, false, TM>(a, t, i, io, 0, deg, w, NWAVE, wc, tr);
+  __syncthreads();
+  if (lds_load(tr.ovf)) return false;
+  if (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S)) {
+    unsigned long long sum = pbf_wave_sum(sweep<MODE, TM>(a, t, cells, tid, TPB));
+    if (lane == 0) red[w] = sum;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long tot = 0;
+      for (int q = 0; q < NWAVE; q++) tot += red[q];
+      if (tot) {
+        if constexpr (MODE == MODE_VERT) credit(a, i, tot);
+        else *total_acc += tot;
+      }
+    }
+  }
+  if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S)) enum_pass<MODE, true, TM>(a, t, i, io, 0, deg, w, NWAVE, wc, tr);
+  __syncthreads();
+  return true;
+}
+
+template <int MODE>
+__device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCtx& wc, lds_u32* nused, lds_u32* ovf,
+                           unsigned long long* red, unsigned long long* total_acc) {
+  const uint32_t tid = threadIdx.x;
   const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
   const bool i_in_S = (a.redg[io] & 1u) == 0u;
-  // choose and clear the table
-  uint32_t words;
-  if (BLOCK && i < D32_MAX) { t.mode = TM_D32; words = i; }
-  else if (BLOCK && i < D16_MAX && deg < 65536u) { t.mode = TM_D16; words = (i + 1) >> 1; }
-  else {
-    // packed 32-bit slots when the counts (<= deg) and the keys (< n) both fit
-    const bool packed = t.cb < 32u && deg < (1u << t.cb);
-    const uint32_t capmax = (BLOCK ? POOL_WORDS : POOL_WORDS / NWAVE) / (packed ? 1u : 2u);
-    const uint64_t want = (uint64_t)bound + bound / 3 + 1;
+  Tab t{pool, 0, 0, a.cb, nullptr};
+  Track tr{nused, ovf, 0xFFFFFFFFu, false};
+  int tm;
+  uint32_t words, fill = 0, cells = i;
+  if (i < D32_MAX) {
+    tm = TM_D32; words = i;
+  } else if (i < D16_MAX && deg < 65536u) {
+    tm = TM_D16; words = (i + 1) >> 1;
+  } else {
+    const uint32_t bnd = a.bound[i];
+    const bool packed = a.cb < 32u && deg < (1u << a.cb);
+    const uint32_t capmax = packed ? POOL_WORDS : POOL_WORDS / 2;
+    const uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
     uint32_t cap = 64;
     while (cap < want && cap < capmax) cap <<= 1;
-    t.mode = packed ? TM_HASH32 : TM_HASH;
+    tm = packed ? TM_HASH32 : TM_HASH;
     t.mask = cap - 1;
     t.shift = 32 - (31 - __clz(cap));
-    t.maxfill = (cap == capmax) ? (capmax / 4) * 3 - gn : 0xFFFFFFFFu;
+    if (cap == capmax) {
+      // the bound does not fit: count keys; every wave may insert one more window after
+      // the flag is raised, so occupancy stays below 3/4
+      tr.on = true;
+      tr.maxfill = (capmax / 4) * 3 - NWAVE * WIN_SPAN;
+    }
     words = packed ? cap : 2 * cap;
+    fill = EMPTY;
+    cells = cap;
   }
-  if (t.mode == TM_HASH) {
-    for (uint32_t q = gt; q <= t.mask; q += gn) t.s[q] = EMPTY64;
-  } else if (t.mode == TM_HASH32) {
-    for (uint32_t q = gt; q <= t.mask; q += gn) t.w[q] = EMPTY;
-  } else {
-    for (uint32_t q = gt; q < words; q += gn) t.w[q] = 0u;
+  for (uint32_t q = tid; q < words; q += TPB) pool[q] = fill;
+  if (tid == 0) { *nused = 0; *ovf = 0; }
+  __syncthreads();
+  switch (tm) {
+    case TM_D32: return block_item_tm<MODE, TM_D32>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
+    case TM_D16: return block_item_tm<MODE, TM_D16>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
+    case TM_HASH32: return block_item_tm<MODE, TM_HASH32>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
+    default: return block_item_tm<MODE, TM_HASH>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
   }
-  if (gt == 0) { t.nused[0] = 0; t.ovf[0] = 0; }
-  group_sync<BLOCK>();
-  enum_wedges<MODE, false>(a, t, i, io, 0, deg, w, nw, acc);
-  group_sync<BLOCK>();
-  if (lds_load(t.ovf)) return false;
+}
+
+// Light endpoint, one wavefront, private packed table tw of up to LT_WORDS slots.  wm is the
+// extent of tw that may hold stale slots (cleared lazily; a sweep leaves the table empty).
+template <int MODE>
+__device__ bool light_item(const CountArgs& a, uint32_t i, lds_u32* tw, uint32_t& wm, WaveCtx& wc, lds_u32* ovf,
+                           unsigned long long* total_acc) {
+  const uint32_t lane = __lane_id();
+  const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
+  const uint32_t bnd = a.bound[i];
+  const bool i_in_S = (a.redg[io] & 1u) == 0u;
+  uint32_t cap = 64;
+  while (cap < bnd + bnd / 3 + 1) cap <<= 1;  // <= LT_WORDS for a light endpoint
+  const uint32_t dirty = min(wm, cap);
+  for (uint32_t q = WIN * lane; q < dirty; q += WIN_SPAN) *(lds_u32x4*)(tw + q) = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
+  if (lane == 0) lds_store(ovf, 0u);
+  wm = max(wm, cap);
+  wave_lds_fence();
+  const Tab t{tw, cap - 1, 32u - (31u - (uint32_t)__clz(cap)), a.cb, nullptr};
+  const Track tr{nullptr, ovf, 0xFFFFFFFFu, false};
+  if (!enum_pass<MODE, false, TM_HASH32>(a, t, i, io, 0, deg, 0, 1, wc, tr)) return false;
+  wave_lds_fence();
   if (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S)) {
     unsigned long long sum = 0;
-    const uint32_t cells = tab_cells(t, i);
-    for (uint32_t q = gt; q < cells; q += gn) {
-      uint32_t key;
-      const uint32_t c = tab_cell(t, q, &key);
-      if (c > 1u) {
-        const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
-        sum += b;
-        if constexpr (MODE == MODE_VERT) credit(a, key, b);
-      }
-    }
-    sum = pbf_wave_sum(sum);
-    if constexpr (BLOCK) {
-      if (lane == 0) red[w] = sum;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        unsigned long long tot = 0;
-        for (int q = 0; q < NWAVE; q++) tot += red[q];
-        if (tot) {
-          if constexpr (MODE == MODE_VERT) credit(a, i, tot);
-          else *total_acc += tot;
+    const uint32_t cmask = (1u << a.cb) - 1u;
+    for (uint32_t q = WIN * lane; q < cap; q += WIN_SPAN) {
+      const u32x4 f = *(lds_u32x4*)(tw + q);
+      *(lds_u32x4*)(tw + q) = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
+      const uint32_t cur[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t c = cur[k] == EMPTY ? 0u : (cur[k] & cmask);
+        if (c > 1u) {
+          const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
+          sum += b;
+          if constexpr (MODE == MODE_VERT) credit(a, cur[k] >> a.cb, b);
         }
       }
-    } else {
-      if (lane == 0 && sum) {
-        if constexpr (MODE == MODE_VERT) credit(a, i, sum);
-        else *total_acc += sum;
-      }
+    }
+    if (cap >= wm) wm = 0;  // swept clean
+    sum = pbf_wave_sum(sum);
+    if (lane == 0 && sum) {
+      if constexpr (MODE == MODE_VERT) credit(a, i, sum);
+      else *total_acc += sum;
     }
   }
-  if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S)) enum_wedges<MODE, true>(a, t, i, io, 0, deg, w, nw, acc);
-  group_sync<BLOCK>();
+  if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S)) enum_pass<MODE, true, TM_HASH32>(a, t, i, io, 0, deg, 0, 1, wc, tr);
   return true;
 }
 
 template <int MODE>
 __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   __shared__ __align__(16) uint32_t pool[POOL_WORDS];
-  __shared__ uint32_t s_item, s_nused[NWAVE + 1], s_ovf[NWAVE + 1];
-  __shared__ unsigned long long s_red[NWAVE];
+  __shared__ __align__(16) uint32_t s_flags[NWAVE][WIN_SPAN];
   __shared__ unsigned long long s_acc[NWAVE][64];
+  __shared__ unsigned long long s_red[NWAVE];
+  __shared__ uint32_t s_item, s_nused, s_ovf, s_wovf[NWAVE];
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned long long total_acc = 0;
-  Tab t;
-  t.cb = a.cb;
+  *(lds_u32x4*)((lds_u32*)s_flags[wid] + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
+  s_acc[wid][lane] = 0;
+  __syncthreads();
+  WaveCtx wc{(lds_u32*)s_flags[wid], (lds_u64*)s_acc[wid], 0};
   // phase 1: heavy endpoints, whole workgroup
   const uint32_t nb = a.lens[1];
-  t.w = (lds_u32*)pool;
-  t.s = (lds_u64*)pool;
-  t.nused = (lds_u32*)&s_nused[NWAVE];
-  t.ovf = (lds_u32*)&s_ovf[NWAVE];
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(&a.queues[0], 1u);
     __syncthreads();
@@ -585,16 +694,15 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
     __syncthreads();
     if (idx >= nb) break;
     const uint32_t i = a.blist[idx];
-    if (!process_endpoint<MODE, true>(a, t, i, a.bound[i], s_red, &total_acc, (lds_u64*)s_acc[wid])) {
+    if (!block_item<MODE>(a, i, (lds_u32*)pool, wc, (lds_u32*)&s_nused, (lds_u32*)&s_ovf, s_red, &total_acc)) {
       if (threadIdx.x == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = i;
+      __syncthreads();
     }
   }
   // phase 2: light endpoints, one per wavefront, private quarter of the pool
   const uint32_t nwl = a.lens[0];
-  t.w = (lds_u32*)(pool + wid * (POOL_WORDS / NWAVE));
-  t.s = (lds_u64*)(pool + wid * (POOL_WORDS / NWAVE));
-  t.nused = (lds_u32*)&s_nused[wid];
-  t.ovf = (lds_u32*)&s_ovf[wid];
+  lds_u32* tw = (lds_u32*)(pool + wid * LT_WORDS);
+  uint32_t wm = LT_WORDS;  // phase 1 left arbitrary words
   // the next item is claimed before the current one is processed, hiding the queue latency
   uint32_t next = 0;
   if (lane == 0) next = atomicAdd(&a.queues[1], 1u);
@@ -603,7 +711,7 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
     const uint32_t idx = next;
     if (lane == 0) next = atomicAdd(&a.queues[1], 1u);
     const uint32_t i = a.wlist[idx];
-    if (!process_endpoint<MODE, false>(a, t, i, a.bound[i], nullptr, &total_acc, (lds_u64*)s_acc[wid])) {
+    if (!light_item<MODE>(a, i, tw, wm, wc, (lds_u32*)&s_wovf[wid], &total_acc)) {
       if (lane == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = i;
     }
     next = __shfl(next, 0, 64);
@@ -624,75 +732,96 @@ __global__ void k_hot_reduce(unsigned long long* __restrict__ hot, unsigned long
   }
 }
 
-// Work classification: UB_i = sum_{v in N(i)} in(v) bounds both the wedges and (with i) the
-// distinct keys of endpoint i.  Lists of at most 64 entries are summed by their own lane;
-// longer lists are cut into chunks of CL_CHUNK entries summed by separate wavefronts
-// (k_ub_chunks), so a hub's list does not serialise one wavefront.  Appends to the work lists
-// are aggregated per wavefront (one atomic per class per wave).
+// Work classification by the exact wedge count of each endpoint, T_i = sum over v in N(i) of
+// L_v (in(v) when v < i, else the position of i inside N(v)); min(T_i, i) bounds its
+// distinct keys.  Lists of at most 64 entries are summed by their own lane; longer lists are
+// cut into chunks of CL_CHUNK entries summed by separate wavefronts (k_wedge_chunks), so a
+// hub's list does not serialise one wavefront.  Appends to the work lists are aggregated per
+// wavefront (one atomic per class per wave).
 constexpr uint32_t CL_CHUNK = 2048;
 
-__device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t ub, uint32_t* __restrict__ bound,
-                                                uint32_t* __restrict__ wlist, uint32_t* __restrict__ blist,
+__device__ __forceinline__ uint32_t seg_len(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ inpref,
+                                            const uint32_t* __restrict__ lpos, uint32_t i, uint32_t slot) {
+  const uint32_t v = redg[slot] >> 1;
+  return v < i ? inpref[v] : lpos[slot];
+}
+
+// lens: [0] light, [1] heavy, [6] HBM-fallback (the overflow list)
+__device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, uint32_t cb,
+                                                uint32_t* __restrict__ bound, uint32_t* __restrict__ wlist,
+                                                uint32_t* __restrict__ blist, uint32_t* __restrict__ olist,
                                                 uint32_t* __restrict__ lens) {
   const uint32_t lane = threadIdx.x & 63;
-  uint32_t cls = 0;  // 0 none, 1 wavefront, 2 workgroup
-  if (valid && ub) {
-    const uint32_t b = (uint32_t)(ub < (uint64_t)i ? ub : (uint64_t)i);
-    // a wavefront takes every endpoint of bounded work (optimistically: its table may
-    // overflow, and the endpoint is then redone by a workgroup); the rest go to workgroups
-    if (b) cls = ub <= WAVE_WORK ? 1u : 2u;
+  uint32_t cls = 0;  // 0 none, 1 wavefront, 2 workgroup, 3 HBM
+  if (valid && T) {
+    const uint32_t b = (uint32_t)(T < (uint64_t)i ? T : (uint64_t)i);
+    if (b) {
+      const bool light = b <= LIGHT_MAX && T <= LIGHT_WORK && cb < 32u && deg < (1u << cb);
+      cls = deg >= WIDE_DEG ? 3u : light ? 1u : 2u;
+    }
     bound[i] = b;
   }
-  const uint64_t m1 = __ballot(cls == 1), m2 = __ballot(cls == 2);
-  uint32_t base1 = 0, base2 = 0;
+  const uint64_t m1 = __ballot(cls == 1), m2 = __ballot(cls == 2), m3 = __ballot(cls == 3);
+  uint32_t base1 = 0, base2 = 0, base3 = 0;
   if (lane == 0) {
     if (m1) base1 = atomicAdd(&lens[0], (uint32_t)__popcll(m1));
     if (m2) base2 = atomicAdd(&lens[1], (uint32_t)__popcll(m2));
+    if (m3) base3 = atomicAdd(&lens[6], (uint32_t)__popcll(m3));
   }
   base1 = __shfl(base1, 0, 64);
   base2 = __shfl(base2, 0, 64);
+  base3 = __shfl(base3, 0, 64);
   const uint64_t lt = pbf_lanemask_lt();
   if (cls == 1) wlist[base1 + __popcll(m1 & lt)] = i;
   if (cls == 2) blist[base2 + __popcll(m2 & lt)] = i;
+  if (cls == 3) olist[base3 + __popcll(m3 & lt)] = i;
 }
+
+struct ClassOut {
+  uint32_t* bound;
+  uint32_t* wlist;
+  uint32_t* blist;
+  uint32_t* olist;
+  uint32_t* lens;
+  uint32_t cb;
+};
 
 // Short lists: classified here.  Long lists: listed in `big` with their chunk counts.
 __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
-                                                  const uint32_t* __restrict__ inpref, int64_t n, uint32_t* __restrict__ bound,
-                                                  uint32_t* __restrict__ wlist, uint32_t* __restrict__ blist,
-                                                  uint32_t* __restrict__ lens, uint32_t* __restrict__ big,
-                                                  uint32_t* __restrict__ big_chunks, unsigned long long* __restrict__ ub64) {
+                                                  const uint32_t* __restrict__ inpref, const uint32_t* __restrict__ lpos,
+                                                  int64_t n, ClassOut co, uint32_t* __restrict__ big,
+                                                  uint32_t* __restrict__ big_chunks, unsigned long long* __restrict__ w64) {
   const uint32_t lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; base < n; base += nwaves * 64) {
     const int64_t i = base + lane;
     uint32_t io = 0, deg = 0;
     if (i < n) { io = roff[i]; deg = roff[i + 1] - io; }
-    uint64_t ub = 0;
+    uint64_t T = 0;
     const bool small = deg <= 64;
     if (small)
-      for (uint32_t j = 0; j < deg; j++) ub += inpref[redg[io + j] >> 1];
-    classify_append((uint32_t)i, i < n && small, ub, bound, wlist, blist, lens);
+      for (uint32_t j = 0; j < deg; j++) T += seg_len(redg, inpref, lpos, (uint32_t)i, io + j);
+    classify_append((uint32_t)i, i < n && small, T, deg, co.cb, co.bound, co.wlist, co.blist, co.olist, co.lens);
     const uint64_t mb = __ballot(!small);
     uint32_t bb = 0;
-    if (lane == 0 && mb) bb = atomicAdd(&lens[2], (uint32_t)__popcll(mb));
+    if (lane == 0 && mb) bb = atomicAdd(&co.lens[2], (uint32_t)__popcll(mb));
     bb = __shfl(bb, 0, 64);
     if (!small) {
       const uint32_t slot = bb + __popcll(mb & pbf_lanemask_lt());
       big[slot] = (uint32_t)i;
       big_chunks[slot] = (deg + CL_CHUNK - 1) / CL_CHUNK;
-      ub64[i] = 0;
+      w64[i] = 0;
     }
   }
 }
 
 // One wavefront per chunk of a long list (chunk c of big vertex b, located by a binary search
-// over the scanned chunk counts); partial sums are added into ub64.
-__global__ __launch_bounds__(256) void k_ub_chunks(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
-                                                   const uint32_t* __restrict__ inpref, const uint32_t* __restrict__ big,
-                                                   const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ lens,
-                                                   const uint32_t* __restrict__ total_chunks,
-                                                   unsigned long long* __restrict__ ub64) {
+// over the scanned chunk counts); partial wedge counts are added into w64.
+__global__ __launch_bounds__(256) void k_wedge_chunks(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
+                                                      const uint32_t* __restrict__ inpref, const uint32_t* __restrict__ lpos,
+                                                      const uint32_t* __restrict__ big, const uint32_t* __restrict__ chunk_base,
+                                                      const uint32_t* __restrict__ lens, const uint32_t* __restrict__ total_chunks,
+                                                      unsigned long long* __restrict__ w64) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nbig = lens[2], tot = *total_chunks;
   const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
@@ -708,11 +837,11 @@ __global__ __launch_bounds__(256) void k_ub_chunks(const uint32_t* __restrict__ 
     uint64_t part = 0;
     uint32_t j = b0 + lane;
     for (; j + 192 < b1; j += 256)
-      part += (uint64_t)inpref[redg[io + j] >> 1] + inpref[redg[io + j + 64] >> 1] + inpref[redg[io + j + 128] >> 1] +
-              inpref[redg[io + j + 192] >> 1];
-    for (; j < b1; j += 64) part += inpref[redg[io + j] >> 1];
+      part += (uint64_t)seg_len(redg, inpref, lpos, i, io + j) + seg_len(redg, inpref, lpos, i, io + j + 64) +
+              seg_len(redg, inpref, lpos, i, io + j + 128) + seg_len(redg, inpref, lpos, i, io + j + 192);
+    for (; j < b1; j += 64) part += seg_len(redg, inpref, lpos, i, io + j);
     part = pbf_wave_sum(part);
-    if (lane == 0 && part) atomicAdd(&ub64[i], part);
+    if (lane == 0 && part) atomicAdd(&w64[i], part);
   }
 }
 
@@ -722,10 +851,8 @@ constexpr uint32_t SPLIT_DEG = 1u << 15, SPLIT_CHUNK = 1u << 13;
 
 __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ big,
                                                       const uint32_t* __restrict__ lens_in,
-                                                      const unsigned long long* __restrict__ ub64, uint32_t* __restrict__ bound,
-                                                      uint32_t* __restrict__ wlist, uint32_t* __restrict__ blist,
-                                                      uint32_t* __restrict__ lens, uint32_t* __restrict__ split,
-                                                      uint32_t* __restrict__ split_chunks) {
+                                                      const unsigned long long* __restrict__ w64, ClassOut co,
+                                                      uint32_t* __restrict__ split, uint32_t* __restrict__ split_chunks) {
   const uint32_t nbig = lens_in[2];
   const uint32_t lane = threadIdx.x & 63;
   // every wavefront runs the same number of iterations (the ballots inside need the whole wave)
@@ -735,12 +862,12 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
     const bool valid = t < nbig;
     const uint32_t i = valid ? big[t] : 0u;
     const uint32_t deg = valid ? roff[i + 1] - roff[i] : 0u;
-    const unsigned long long ub = valid ? ub64[i] : 0ull;
-    const bool sp = valid && ub && i > 0 && deg > SPLIT_DEG && i < D32_MAX;
-    classify_append(i, valid && !sp, ub, bound, wlist, blist, lens);
+    const unsigned long long T = valid ? w64[i] : 0ull;
+    const bool sp = valid && T && i > 0 && deg > SPLIT_DEG && deg < WIDE_DEG && i < D32_MAX;
+    classify_append(i, valid && !sp, T, deg, co.cb, co.bound, co.wlist, co.blist, co.olist, co.lens);
     const uint64_t ms = __ballot(sp);
     uint32_t b = 0;
-    if (lane == 0 && ms) b = atomicAdd(&lens[8], (uint32_t)__popcll(ms));
+    if (lane == 0 && ms) b = atomicAdd(&co.lens[8], (uint32_t)__popcll(ms));
     b = __shfl(b, 0, 64);
     if (sp) {
       const uint32_t slot = b + __popcll(ms & pbf_lanemask_lt());
@@ -758,23 +885,26 @@ __global__ __launch_bounds__(TPB) void k_split_pass1(CountArgs a, const uint32_t
                                                      const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ ctl,
                                                      uint32_t* __restrict__ G) {
   __shared__ __align__(16) uint32_t pool[POOL_WORDS];
-  __shared__ uint32_t s_nused, s_ovf;
+  __shared__ __align__(16) uint32_t s_flags[NWAVE][WIN_SPAN];
   __shared__ unsigned long long s_acc[NWAVE][64];
+  __shared__ uint32_t s_ovf;
   const uint32_t nsplit = ctl[8], tot = ctl[9];
-  const uint32_t wid = threadIdx.x >> 6;
-  Tab t;
-  t.w = (lds_u32*)pool; t.s = nullptr; t.mode = TM_D32; t.nused = (lds_u32*)&s_nused; t.ovf = (lds_u32*)&s_ovf;
-  t.g = nullptr; t.cb = a.cb;
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  *(lds_u32x4*)((lds_u32*)s_flags[wid] + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
+  s_acc[wid][lane] = 0;
+  if (threadIdx.x == 0) s_ovf = 0;
+  WaveCtx wc{(lds_u32*)s_flags[wid], (lds_u64*)s_acc[wid], 0};
+  const Tab t{(lds_u32*)pool, 0, 0, a.cb, nullptr};
+  const Track tr{nullptr, (lds_u32*)&s_ovf, 0xFFFFFFFFu, false};
   for (uint32_t item = blockIdx.x; item < tot; item += gridDim.x) {
     uint32_t lo = 0, hi = nsplit;
     while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (chunk_base[mid] <= item) lo = mid; else hi = mid; }
     const uint32_t i = split[lo], c = item - chunk_base[lo];
     const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
     for (uint32_t q = threadIdx.x; q < i; q += TPB) pool[q] = 0u;
-    if (threadIdx.x == 0) { s_nused = 0; s_ovf = 0; }
     __syncthreads();
     const uint32_t j1 = min(deg, (c + 1) * SPLIT_CHUNK);
-    enum_wedges<MODE, false>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, (lds_u64*)s_acc[wid]);
+    enum_pass<MODE, false, TM_D32>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
     __syncthreads();
     uint32_t* row = G + (size_t)lo * D32_MAX;
     for (uint32_t q = threadIdx.x; q < i; q += TPB)
@@ -823,12 +953,17 @@ template <int MODE>
 __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t* __restrict__ split,
                                                      const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ ctl,
                                                      const uint32_t* __restrict__ G) {
+  __shared__ __align__(16) uint32_t s_flags[NWAVE][WIN_SPAN];
   __shared__ unsigned long long s_acc[NWAVE][64];
   __shared__ uint32_t s_ovf;
   const uint32_t nsplit = ctl[8], tot = ctl[9];
-  const uint32_t wid = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  *(lds_u32x4*)((lds_u32*)s_flags[wid] + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
+  s_acc[wid][lane] = 0;
   if (threadIdx.x == 0) s_ovf = 0;
   __syncthreads();
+  WaveCtx wc{(lds_u32*)s_flags[wid], (lds_u64*)s_acc[wid], 0};
+  const Track tr{nullptr, (lds_u32*)&s_ovf, 0xFFFFFFFFu, false};
   for (uint32_t item = blockIdx.x; item < tot; item += gridDim.x) {
     uint32_t lo = 0, hi = nsplit;
     while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (chunk_base[mid] <= item) lo = mid; else hi = mid; }
@@ -836,12 +971,9 @@ __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t
     const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
     const bool i_in_S = (a.redg[io] & 1u) == 0u;
     if (!(MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S))) continue;
-    Tab t;
-    t.w = nullptr; t.s = nullptr; t.mode = TM_G32; t.g = G + (size_t)lo * D32_MAX; t.ovf = (lds_u32*)&s_ovf;
-    t.nused = nullptr;
-    t.cb = a.cb;
+    const Tab t{nullptr, 0, 0, a.cb, G + (size_t)lo * D32_MAX};
     const uint32_t j1 = min(deg, (c + 1) * SPLIT_CHUNK);
-    enum_wedges<MODE, true>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, (lds_u64*)s_acc[wid]);
+    enum_pass<MODE, true, TM_G32>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
   }
 }
 
@@ -896,29 +1028,31 @@ inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
 constexpr int HBM_TEAM = 256, HBM_BLOCKS = 32;
 
 // Runs the counting phase: classification, the fused kernel, the hub fold and (only if some
-// workgroup table overflowed) the HBM fallback.  lens/queues are cleared with a fill kernel.
+// endpoint needs it) the HBM fallback.  lens/queues are cleared with a fill kernel.
 template <int MODE>
 int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lists, uint32_t* ctl) {
   const int64_t n = R.n;
-  // ctl: [0] wlist len, [1] blist len, [2] big len, [3] q block, [4] q wave, [5] q ovf,
-  //      [6] ovf len, [7] total big chunks, [8] split len, [9] total split items,
-  //      [10], [11] queues of the overflow pass, [12] its overflow len, [13] HBM queue
+  // ctl: [0] light len, [1] heavy len, [2] big len, [3] heavy queue, [4] light queue,
+  //      [6] HBM-fallback len, [7] total big chunks, [8] split len, [9] total split items,
+  //      [13] HBM queue
+  // lists: [0, n) light, [n, 2n) heavy, [2n, 3n) HBM fallback
   void *bp2, *up;
   PBF_CHECK(pbf_scratch(ctx, S_BIG, 4 * (size_t)(3 * n + 64), &bp2));
   PBF_CHECK(pbf_scratch(ctx, S_UB, 8 * (size_t)(n ? n : 1), &up));
   uint32_t* big = static_cast<uint32_t*>(bp2);
   uint32_t* big_chunks = big + n;
   uint32_t* chunk_base = big + 2 * n;
-  unsigned long long* ub64 = static_cast<unsigned long long*>(up);
+  unsigned long long* w64 = static_cast<unsigned long long*>(up);
   hipLaunchKernelGGL(k_lpos, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.rown,
                      R.inpref, R.m2, const_cast<uint32_t*>(a.lpos));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctl, 0u, (int64_t)16);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
-  hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, n,
-                     const_cast<uint32_t*>(a.bound), lists, lists + n, ctl, big, big_chunks, ub64);
+  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, lists + 2 * n, ctl, a.cb};
+  hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.lpos,
+                     n, co, big, big_chunks, w64);
   PBF_CHECK(pbf_scan_excl_u32(ctx, big_chunks, chunk_base, n, ctl + 7));
-  hipLaunchKernelGGL(k_ub_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, big,
-                     chunk_base, ctl, ctl + 7, ub64);
+  hipLaunchKernelGGL(k_wedge_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.lpos,
+                     big, chunk_base, ctl, ctl + 7, w64);
   // split endpoints: list, chunk counts, chunk bases, dense rows
   void *sp, *gp;
   const int64_t max_split = 2 * g_m2_over(R.m2, SPLIT_DEG) + 8;
@@ -929,8 +1063,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   uint32_t* split_base = split + 2 * max_split;
   uint32_t* G = static_cast<uint32_t*>(gp);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split, 256, 64)), dim3(256), 0, ctx->stream, split_chunks, 0u, max_split);
-  hipLaunchKernelGGL(k_classify_big, dim3(64), dim3(256), 0, ctx->stream, R.roff, big, ctl, ub64,
-                     const_cast<uint32_t*>(a.bound), lists, lists + n, ctl, split, split_chunks);
+  hipLaunchKernelGGL(k_classify_big, dim3(64), dim3(256), 0, ctx->stream, R.roff, big, ctl, w64, co, split, split_chunks);
   PBF_CHECK(pbf_scan_excl_u32(ctx, split_chunks, split_base, max_split, ctl + 9));
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split * D32_MAX, 256, 4096)), dim3(256), 0, ctx->stream, G, 0u,
                      max_split * (int64_t)D32_MAX);
@@ -945,31 +1078,22 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   a.ovf_list = lists + 2 * n;
   a.ovf_len = ctl + 6;
   hipLaunchKernelGGL(k_count2<MODE>, dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
-  // endpoints whose wavefront table overflowed: again, one workgroup each (lens = {ctl[5] = 0,
-  // ctl[6] = overflow count})
-  a.blist = lists + 2 * n;
-  a.lens = ctl + 5;
-  a.queues = ctl + 10;
-  a.ovf_list = lists + 3 * n;
-  a.ovf_len = ctl + 12;
-  hipLaunchKernelGGL(k_count2<MODE>, dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
   PBF_HIP(hipGetLastError());
   if (MODE != MODE_EDGE && MODE != MODE_TOTAL)
     hipLaunchKernelGGL(k_hot_reduce, dim3(grid_for(HOT_N, 256, 64)), dim3(256), 0, ctx->stream, a.hot, a.B, n);
-  PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 12, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 6, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   const uint32_t novf = (uint32_t)(ctx->pinned[16] & 0xFFFFFFFFu);
   ctx->timing.hub_vertices = novf;
   if (novf) {
-    // workgroup tables overflowed too: dense HBM histograms
+    // endpoints no LDS table could hold: dense HBM histograms
     void* pp;
     const int blocks = (int)(novf < (uint32_t)HBM_BLOCKS ? novf : (uint32_t)HBM_BLOCKS);
     PBF_CHECK(pbf_scratch(ctx, S_POOL, sizeof(uint32_t) * 2 * (size_t)n * blocks, &pp));
     PBF_HIP(hipMemsetAsync(pp, 0, sizeof(uint32_t) * 2 * (size_t)n * blocks, ctx->stream));
     a.pool = static_cast<uint32_t*>(pp);
-    a.list = lists + 3 * n; a.list_len = ctl + 12; a.queue = ctl + 13;
-    a.ovf_list = lists; a.ovf_len = ctl + 14;  // never written by the HBM kernel
-    hipLaunchKernelGGL((k_count<MODE, HBM_TEAM, 1, true, T_HBM>), dim3(blocks), dim3(HBM_TEAM), 0, ctx->stream, a);
+    a.list = lists + 2 * n; a.list_len = ctl + 6; a.queue = ctl + 13;
+    hipLaunchKernelGGL((k_count_hbm<MODE, HBM_TEAM>), dim3(blocks), dim3(HBM_TEAM), 0, ctx->stream, a);
     PBF_HIP(hipGetLastError());
   }
   return PBF_OK;
@@ -981,13 +1105,13 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
   const int64_t n = R.n, m2 = R.m2;
   void *bp, *lp, *cp, *mp, *hp, *op = nullptr;
   PBF_CHECK(pbf_scratch(ctx, S_BOUND, 4 * (size_t)(n ? n : 1), &bp));
-  PBF_CHECK(pbf_scratch(ctx, S_BINS, 4 * (size_t)(4 * n + 64), &lp));
+  PBF_CHECK(pbf_scratch(ctx, S_BINS, 4 * (size_t)(3 * n + 64), &lp));
   PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
   PBF_CHECK(pbf_scratch(ctx, S_HOT, 8 * (size_t)HOT_N * HOT_R, &hp));
   void* lpp;
   PBF_CHECK(pbf_scratch(ctx, S_LPOS, 4 * (size_t)(m2 ? m2 : 1), &lpp));
   uint32_t* lists = static_cast<uint32_t*>(lp);
-  uint32_t* ctl = lists + 4 * n;
+  uint32_t* ctl = lists + 3 * n;
   CountArgs a{};
   a.roff = R.roff; a.redg = R.redg; a.inpref = R.inpref; a.bound = static_cast<uint32_t*>(bp);
   a.lpos = static_cast<uint32_t*>(lpp);
