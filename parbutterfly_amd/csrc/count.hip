@@ -45,9 +45,10 @@ struct CountArgs {
   int64_t n;
   // fused kernel (k_count2): two work lists and their queues
   const uint32_t* blist;   // endpoints processed by a whole workgroup
-  const uint32_t* wlist;   // endpoints processed by one wavefront
-  const uint32_t* lens;    // [0] wlist length, [1] blist length
-  uint32_t* queues;        // [0] block queue, [1] wave queue
+  const uint32_t* wlist;   // light endpoints, grouped (see light_group)
+  const uint32_t* gstart;  // first wlist index of each light group, and the end
+  const uint32_t* lens;    // [1] blist length, [5] light groups, [10] groups on side S
+  uint32_t* queues;        // [0] block queue, [1] light-group queue
   unsigned long long* hot; // replicated accumulators for ranks < HOT_N (HOT_R replicas each)
   uint32_t cb;             // count bits of a packed 32-bit hash slot (32 - key bits)
 };
@@ -361,11 +362,40 @@ struct Track {
   bool on;
 };
 
+// A chunk: up to 64 segments (endpoint i, centre v), one per lane.  Segment s covers the
+// wedge indices [P, P + L) of the chunk, i.e. the entries N(v)[0, L) below min(i, v).
+struct Chunk {
+  uint32_t v;      // centre rank
+  uint32_t slot;   // ranked slot of (i -> v): the edge credited in EDGE mode
+  uint32_t L, P;   // segment length and first wedge index
+  uint32_t T;      // wedges of the chunk (wave-uniform)
+  uint32_t base;   // slot of wedge index x of this segment: base + x
+  uint32_t td;     // grouped tables: the endpoint's sub-table (word offset | log2 size << 16)
+};
+
+__device__ __forceinline__ Chunk load_chunk(const CountArgs& a, uint32_t i, uint32_t slot, bool valid, uint32_t td) {
+  Chunk ch;
+  ch.v = 0; ch.L = 0; ch.slot = slot; ch.td = td;
+  uint32_t vo = 0;
+  if (valid) {
+    ch.v = a.redg[slot] >> 1;
+    vo = a.roff[ch.v];
+    // the prefix of N(v) below min(i, v): all of in(v) when v < i, else up to i's own slot
+    ch.L = (ch.v < i) ? a.inpref[ch.v] : a.lpos[slot];
+  }
+  const uint32_t incl = wave_incl_add(ch.L);
+  ch.T = lane_read(incl, 63);
+  ch.P = incl - ch.L;
+  ch.base = vo - ch.P;
+  return ch;
+}
+
 // One window: lane l holds wedge indices b0 + 4l + k, k < nval.
 struct Win {
   uint32_t ent[WIN];  // ranked entry (rank << 1 | flag) of u2
   uint32_t pos[WIN];  // its slot in the ranked adjacency
-  uint32_t seg[WIN];  // neighbour (segment) index within the chunk
+  uint32_t seg[WIN];  // segment (lane) within the chunk
+  uint32_t td[WIN];   // grouped tables: sub-table of the segment
   uint32_t nval;
 };
 
@@ -386,11 +416,12 @@ __device__ __forceinline__ uint32_t next_tag(WaveCtx& wc) {
 // before it.  The segment covering b0 itself (it may have started in an earlier window) is
 // the highest lane with L > 0 and P <= b0.  Stale flags from earlier windows of the chunk
 // hold smaller segments, those of earlier chunks smaller epochs: the max ignores both.
-__device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint32_t b0, uint32_t T, uint32_t P, uint32_t L,
-                                        uint32_t base, uint32_t tag, Win& wv) {
+template <bool MULTI>
+__device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint32_t b0, const Chunk& ch, uint32_t tag,
+                                        Win& wv) {
   const uint32_t lane = __lane_id();
-  if (L != 0u && P >= b0 && P - b0 < WIN_SPAN) flags[P - b0] = tag | lane;
-  const uint64_t cm = __ballot(L != 0u && P <= b0);
+  if (ch.L != 0u && ch.P >= b0 && ch.P - b0 < WIN_SPAN) flags[ch.P - b0] = tag | lane;
+  const uint64_t cm = __ballot(ch.L != 0u && ch.P <= b0);
   const uint32_t carry = tag | (63u - (uint32_t)__clzll(cm));
   wave_lds_fence();
   const u32x4 f = *(lds_u32x4*)(flags + 4 * lane);
@@ -398,26 +429,33 @@ __device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint
   const uint32_t m0 = f.x, m1 = max(m0, f.y), m2 = max(m1, f.z), m3 = max(m2, f.w);
   const uint32_t cin = max(wave_shr1(wave_incl_max(m3)), carry);
   const uint32_t q0 = b0 + WIN * lane;
-  wv.nval = q0 < T ? min(T - q0, WIN) : 0u;
+  wv.nval = q0 < ch.T ? min(ch.T - q0, WIN) : 0u;
   const uint32_t mm[WIN] = {m0, m1, m2, m3};
 #pragma unroll
   for (uint32_t k = 0; k < WIN; k++) {
     const uint32_t s = max(cin, mm[k]) & 63u;
     wv.seg[k] = s;
-    wv.pos[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(s << 2), (int)base) + q0 + k;
+    wv.pos[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(s << 2), (int)ch.base) + q0 + k;
+    if constexpr (MULTI) wv.td[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(s << 2), (int)ch.td);
   }
 #pragma unroll
   for (uint32_t k = 0; k < WIN; k++) wv.ent[k] = k < wv.nval ? a.redg[wv.pos[k]] : 0u;
 }
 
+// The sub-table of a grouped endpoint.
+__device__ __forceinline__ Tab sub_tab(const Tab& t, uint32_t td) {
+  const uint32_t lg = td >> 16;
+  return Tab{t.w + (td & 0xFFFFu), (1u << lg) - 1u, 32u - lg, t.cb, nullptr};
+}
+
 // Pass 1 on one window: returns the lane's number of new keys, plus 256 on a failed insert.
-template <int TM>
+template <int TM, bool MULTI>
 __device__ __forceinline__ uint32_t win_insert(const Tab& t, const Win& wv) {
   uint32_t r = 0;
 #pragma unroll
   for (uint32_t k = 0; k < WIN; k++) {
     if (k < wv.nval) {
-      const uint32_t s = tab_ins<TM>(t, wv.ent[k] >> 1);
+      const uint32_t s = tab_ins<TM>(MULTI ? sub_tab(t, wv.td[k]) : t, wv.ent[k] >> 1);
       r += (s == 2u) ? 256u : s;
     }
   }
@@ -429,7 +467,7 @@ __device__ __forceinline__ uint32_t win_insert(const Tab& t, const Win& wv) {
 // (a segment boundary inside the lane) go straight to acc.  Lane sums are then reduced per
 // segment with a segmented scan; the last lane of each run adds the run's sum.  The u32 sums
 // hold at most 256 values below deg(i) < 2^24.
-template <int MODE, int TM>
+template <int MODE, int TM, bool MULTI>
 __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, const Win& wv, lds_u64* acc) {
   uint32_t val[WIN];
   uint32_t last = wv.seg[0];
@@ -438,7 +476,7 @@ __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, con
     val[k] = 0;
     if (k < wv.nval) {
       last = wv.seg[k];
-      const uint32_t c = tab_get<TM>(t, wv.ent[k] >> 1);
+      const uint32_t c = tab_get<TM>(MULTI ? sub_tab(t, wv.td[k]) : t, wv.ent[k] >> 1);
       if (c > 1u) {
         val[k] = c - 1u;
         if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[wv.pos[k]]], (unsigned long long)(c - 1u));
@@ -457,68 +495,67 @@ __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, con
   if (key < 64u && x && wave_shl1_or(key, ~0u) != key) lds_add(acc + key, (unsigned long long)x);
 }
 
+// Runs windows wstart, wstart + wstep, ... of a chunk.  The next window is mapped and its
+// loads issued before the current one is aggregated.  Pass 2 ends with the per-segment
+// credits (centre v, or edge i -> v).  Returns false when a table overflowed in pass 1.
+template <int MODE, bool PASS2, int TM, bool MULTI>
+__device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, const Chunk& ch, uint32_t wstart,
+                                              uint32_t wstep, WaveCtx& wc, const Track& tr) {
+  const uint32_t lane = __lane_id();
+  if (ch.T <= wstart) return true;
+  const uint32_t tag = next_tag(wc);
+  Win cur, nxt;
+  uint32_t b0 = wstart;
+  win_map<MULTI>(a, wc.flags, b0, ch, tag, cur);
+  while (true) {
+    const uint32_t nb = b0 + wstep;
+    const bool more = nb < ch.T;
+    if (more) win_map<MULTI>(a, wc.flags, nb, ch, tag, nxt);
+    if constexpr (!PASS2) {
+      const uint32_t r = win_insert<TM, MULTI>(t, cur);
+      const bool failed = __ballot(r >= 256u) != 0ull;
+      if (tr.on) {
+        const uint32_t nnew = lane_read(wave_incl_add(r & 255u), 63);
+        if (lane == 0 && (failed || lds_add(tr.nused, nnew) + nnew >= tr.maxfill)) lds_store(tr.ovf, 1u);
+        wave_lds_fence();
+        if (__builtin_amdgcn_readfirstlane(lds_load(tr.ovf))) return false;
+      } else if (failed) {
+        if (lane == 0) lds_store(tr.ovf, 1u);
+        return false;
+      }
+    } else {
+      win_credit<MODE, TM, MULTI>(a, t, cur, wc.acc);
+    }
+    if (!more) break;
+    cur = nxt;
+    b0 = nb;
+  }
+  if constexpr (PASS2) {
+    wave_lds_fence();
+    const unsigned long long mine = wc.acc[lane];
+    if (mine) {
+      wc.acc[lane] = 0;
+      if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[ch.slot]], mine);
+      else credit(a, ch.v, mine);
+    }
+    wave_lds_fence();
+  }
+  return true;
+}
+
 // Enumerates the wedges of endpoint i over neighbours [jbeg, jend) with wave w of nw.  For
 // long lists (several waves) the waves take different 64-neighbour chunks; otherwise they
-// share the windows of each chunk.  The next window is mapped and its loads issued before the
-// current one is aggregated.  Returns false when a tracked table overflowed.
+// share the windows of each chunk.  Returns false when a tracked table overflowed.
 template <int MODE, bool PASS2, int TM>
 __device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t jbeg,
                                           uint32_t jend, uint32_t w, uint32_t nw, WaveCtx& wc, const Track& tr) {
-  const uint32_t lane = __lane_id();
   const bool split = nw > 1 && jend - jbeg >= 8 * 64;
   const uint32_t cstart = split ? w * 64 : 0u, cstep = split ? nw * 64 : 64u;
   const uint32_t wstart = split ? 0u : w * WIN_SPAN, wstep = split ? WIN_SPAN : nw * WIN_SPAN;
   for (uint32_t c0 = jbeg + cstart; c0 < jend; c0 += cstep) {
-    const uint32_t j = c0 + lane;
-    uint32_t v = 0, vo = 0, L = 0;
-    if (j < jend) {
-      v = a.redg[io + j] >> 1;
-      vo = a.roff[v];
-      // the prefix of N(v) below min(i, v): all of in(v) when v < i, else up to i's own slot
-      L = (v < i) ? a.inpref[v] : a.lpos[io + j];
-    }
-    const uint32_t incl = wave_incl_add(L);
-    const uint32_t T = lane_read(incl, 63);
-    if (T <= wstart) continue;
-    const uint32_t P = incl - L;
-    const uint32_t base = vo - P;  // slot of wedge index x in segment s: base_s + x
-    const uint32_t tag = next_tag(wc);
-    Win cur, nxt;
-    uint32_t b0 = wstart;
-    win_map(a, wc.flags, b0, T, P, L, base, tag, cur);
-    while (true) {
-      const uint32_t nb = b0 + wstep;
-      const bool more = nb < T;
-      if (more) win_map(a, wc.flags, nb, T, P, L, base, tag, nxt);
-      if constexpr (!PASS2) {
-        const uint32_t r = win_insert<TM>(t, cur);
-        const bool failed = __ballot(r >= 256u) != 0ull;
-        if (tr.on) {
-          const uint32_t nnew = lane_read(wave_incl_add(r & 255u), 63);
-          if (lane == 0 && (failed || lds_add(tr.nused, nnew) + nnew >= tr.maxfill)) lds_store(tr.ovf, 1u);
-          wave_lds_fence();
-          if (__builtin_amdgcn_readfirstlane(lds_load(tr.ovf))) return false;
-        } else if (failed) {
-          if (lane == 0) lds_store(tr.ovf, 1u);
-          return false;
-        }
-      } else {
-        win_credit<MODE, TM>(a, t, cur, wc.acc);
-      }
-      if (!more) break;
-      cur = nxt;
-      b0 = nb;
-    }
-    if constexpr (PASS2) {
-      wave_lds_fence();
-      const unsigned long long mine = wc.acc[lane];
-      if (mine) {
-        wc.acc[lane] = 0;
-        if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[io + j]], mine);
-        else credit(a, v, mine);
-      }
-      wave_lds_fence();
-    }
+    const uint32_t j = c0 + __lane_id();
+    const Chunk ch = load_chunk(a, i, io + j, j < jend, 0u);
+    if (!chunk_windows<MODE, PASS2, TM, false>(a, t, ch, wstart, wstep, wc, tr)) return false;
   }
   return true;
 }
@@ -624,51 +661,125 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
   }
 }
 
-// Light endpoint, one wavefront, private packed table tw of up to LT_WORDS slots.  wm is the
-// extent of tw that may hold stale slots (cleared lazily; a sweep leaves the table empty).
-template <int MODE>
-__device__ bool light_item(const CountArgs& a, uint32_t i, lds_u32* tw, uint32_t& wm, WaveCtx& wc, lds_u32* ovf,
-                           unsigned long long* total_acc) {
-  const uint32_t lane = __lane_id();
-  const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
-  const uint32_t bnd = a.bound[i];
-  const bool i_in_S = (a.redg[io] & 1u) == 0u;
+// Table size of a light endpoint with distinct-key bound b (load <= 3/4).
+__device__ __forceinline__ uint32_t light_cap(uint32_t b) {
   uint32_t cap = 64;
-  while (cap < bnd + bnd / 3 + 1) cap <<= 1;  // <= LT_WORDS for a light endpoint
-  const uint32_t dirty = min(wm, cap);
+  while (cap < b + b / 3 + 1) cap <<= 1;
+  return cap;
+}
+
+// Largest k < nk with key_k <= x, keys held by lanes 0..nk-1 in increasing order (nk <= 32).
+__device__ __forceinline__ uint32_t lane_upper(uint32_t key, uint32_t nk, uint32_t x) {
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t bit = 16; bit; bit >>= 1) {
+    const uint32_t cand = s + bit;
+    const uint32_t kc = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((cand & 63u) << 2), (int)key);
+    if (cand < nk && kc <= x) s = cand;
+  }
+  return s;
+}
+
+// A group of light endpoints (one wavefront): up to 32 endpoints whose tables, sorted by
+// decreasing power-of-two size, tile the wavefront's LDS slots exactly (buddy-aligned).  Their
+// adjacency lists are concatenated into one flattened slot range, taken 64 slots per chunk;
+// every wedge is aggregated in its endpoint's sub-table.  One enumeration thus serves several
+// endpoints, and the dependent loads of an endpoint's setup are paid once per group.  All
+// endpoints of a group lie on the same side.  Returns false if a table overflowed (then
+// nothing was credited).
+template <int MODE>
+__device__ bool light_group(const CountArgs& a, uint32_t gs, uint32_t ge, bool o_side, lds_u32* tw, uint32_t& wm,
+                            WaveCtx& wc, lds_u64* iacc, lds_u32* ovf, unsigned long long* total_acc) {
+  const uint32_t lane = __lane_id();
+  const uint32_t nk = ge - gs;
+  uint32_t i_k = 0, io_k = 0, deg_k = 0, cap_k = 0, lg_k = 0;
+  if (lane < nk) {
+    i_k = a.wlist[gs + lane];
+    io_k = a.roff[i_k];
+    deg_k = a.roff[i_k + 1] - io_k;
+    cap_k = light_cap(a.bound[i_k]);
+    lg_k = 31u - (uint32_t)__clz(cap_k);
+  }
+  const uint32_t cinc = wave_incl_add(cap_k);
+  const uint32_t words = lane_read(cinc, 63);  // <= LT_WORDS by construction of the groups
+  const uint32_t off_k = cinc - cap_k;
+  const uint32_t dinc = wave_incl_add(deg_k);
+  const uint32_t DS = lane_read(dinc, 63);
+  const uint32_t dex_k = dinc - deg_k;
+  const uint32_t td_k = off_k | (lg_k << 16);
+  const uint32_t dirty = min(wm, words);
   for (uint32_t q = WIN * lane; q < dirty; q += WIN_SPAN) *(lds_u32x4*)(tw + q) = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
   if (lane == 0) lds_store(ovf, 0u);
-  wm = max(wm, cap);
+  wm = max(wm, words);
   wave_lds_fence();
-  const Tab t{tw, cap - 1, 32u - (31u - (uint32_t)__clz(cap)), a.cb, nullptr};
+  const Tab t{tw, 0, 0, a.cb, nullptr};
   const Track tr{nullptr, ovf, 0xFFFFFFFFu, false};
-  if (!enum_pass<MODE, false, TM_HASH32>(a, t, i, io, 0, deg, 0, 1, wc, tr)) return false;
-  wave_lds_fence();
-  if (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S)) {
-    unsigned long long sum = 0;
+  const bool need_pass2 = MODE == MODE_EDGE || (MODE == MODE_VERT && o_side);
+  // chunk c0 of the concatenated lists: lane -> endpoint (largest k with dex_k <= f; every
+  // light endpoint has a neighbour, so dex_k is strictly increasing)
+  auto chunk_at = [&](uint32_t c0) {
+    const uint32_t f = c0 + lane;
+    const uint32_t k = lane_upper(dex_k, nk, f);
+    const int kb = (int)(k << 2);
+    const uint32_t i = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)i_k);
+    const uint32_t slot = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)io_k) + f -
+                          (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)dex_k);
+    const uint32_t td = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)td_k);
+    return load_chunk(a, i, slot, f < DS, td);
+  };
+  if (DS <= 64) {
+    // one chunk: its setup serves both passes
+    const Chunk ch = chunk_at(0);
+    if (!chunk_windows<MODE, false, TM_HASH32, true>(a, t, ch, 0, WIN_SPAN, wc, tr)) return false;
+    wave_lds_fence();
+    if (need_pass2) chunk_windows<MODE, true, TM_HASH32, true>(a, t, ch, 0, WIN_SPAN, wc, tr);
+  } else {
+    for (uint32_t c0 = 0; c0 < DS; c0 += 64)
+      if (!chunk_windows<MODE, false, TM_HASH32, true>(a, t, chunk_at(c0), 0, WIN_SPAN, wc, tr)) return false;
+    wave_lds_fence();
+    if (need_pass2)
+      for (uint32_t c0 = 0; c0 < DS; c0 += 64) chunk_windows<MODE, true, TM_HASH32, true>(a, t, chunk_at(c0), 0, WIN_SPAN, wc, tr);
+  }
+  if (MODE == MODE_TOTAL || (MODE == MODE_VERT && !o_side)) {
+    // endpoint credits: each lane sweeps 4 words of one sub-table per step (sub-tables are
+    // multiples of 64 words) and clears them; per-endpoint sums are reduced with a segmented
+    // scan (an endpoint's sum of C(c,2) is at most T^2 / 2 < 2^32) into iacc
     const uint32_t cmask = (1u << a.cb) - 1u;
-    for (uint32_t q = WIN * lane; q < cap; q += WIN_SPAN) {
-      const u32x4 f = *(lds_u32x4*)(tw + q);
-      *(lds_u32x4*)(tw + q) = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
+    for (uint32_t q0 = 0; q0 < words; q0 += WIN_SPAN) {
+      const uint32_t q = q0 + WIN * lane;
+      const bool in = q < words;
+      u32x4 f = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
+      if (in) {
+        f = *(lds_u32x4*)(tw + q);
+        *(lds_u32x4*)(tw + q) = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
+      }
       const uint32_t cur[4] = {f.x, f.y, f.z, f.w};
+      uint32_t sum = 0;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const uint32_t c = cur[k] == EMPTY ? 0u : (cur[k] & cmask);
         if (c > 1u) {
-          const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
+          const uint32_t b = c * (c - 1u) / 2u;
           sum += b;
           if constexpr (MODE == MODE_VERT) credit(a, cur[k] >> a.cb, b);
         }
       }
+      const uint32_t k = in ? lane_upper(off_k, nk, q) : 64u;
+      const uint32_t x = wave_seg_incl_add(sum, k);
+      if (k < 64u && x && wave_shl1_or(k, ~0u) != k) lds_add(iacc + k, (unsigned long long)x);
     }
-    if (cap >= wm) wm = 0;  // swept clean
-    sum = pbf_wave_sum(sum);
-    if (lane == 0 && sum) {
-      if constexpr (MODE == MODE_VERT) credit(a, i, sum);
-      else *total_acc += sum;
+    if (words >= wm) wm = 0;  // swept clean
+    wave_lds_fence();
+    if (lane < nk) {
+      const unsigned long long s = iacc[lane];
+      if (s) {
+        iacc[lane] = 0;
+        if constexpr (MODE == MODE_VERT) credit(a, i_k, s);
+        else *total_acc += s;
+      }
     }
+    wave_lds_fence();
   }
-  if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S)) enum_pass<MODE, true, TM_HASH32>(a, t, i, io, 0, deg, 0, 1, wc, tr);
   return true;
 }
 
@@ -677,12 +788,14 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   __shared__ __align__(16) uint32_t pool[POOL_WORDS];
   __shared__ __align__(16) uint32_t s_flags[NWAVE][WIN_SPAN];
   __shared__ unsigned long long s_acc[NWAVE][64];
+  __shared__ unsigned long long s_iacc[NWAVE][32];
   __shared__ unsigned long long s_red[NWAVE];
   __shared__ uint32_t s_item, s_nused, s_ovf, s_wovf[NWAVE];
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned long long total_acc = 0;
   *(lds_u32x4*)((lds_u32*)s_flags[wid] + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
   s_acc[wid][lane] = 0;
+  if (lane < 32) s_iacc[wid][lane] = 0;
   __syncthreads();
   WaveCtx wc{(lds_u32*)s_flags[wid], (lds_u64*)s_acc[wid], 0};
   // phase 1: heavy endpoints, whole workgroup
@@ -696,27 +809,30 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
     const uint32_t i = a.blist[idx];
     if (!block_item<MODE>(a, i, (lds_u32*)pool, wc, (lds_u32*)&s_nused, (lds_u32*)&s_ovf, s_red, &total_acc)) {
       if (threadIdx.x == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = i;
-      __syncthreads();
     }
   }
-  // phase 2: light endpoints, one per wavefront, private quarter of the pool
-  const uint32_t nwl = a.lens[0];
+  // phase 2: groups of light endpoints, one per wavefront, private quarter of the pool
+  const uint32_t ng = a.lens[5], ng0 = a.lens[10];
   lds_u32* tw = (lds_u32*)(pool + wid * LT_WORDS);
   uint32_t wm = LT_WORDS;  // phase 1 left arbitrary words
-  // the next item is claimed before the current one is processed, hiding the queue latency
+  // the next group is claimed before the current one is processed, hiding the queue latency
   uint32_t next = 0;
   if (lane == 0) next = atomicAdd(&a.queues[1], 1u);
   next = __shfl(next, 0, 64);
-  while (next < nwl) {
-    const uint32_t idx = next;
+  while (next < ng) {
+    const uint32_t g = next;
     if (lane == 0) next = atomicAdd(&a.queues[1], 1u);
-    const uint32_t i = a.wlist[idx];
-    if (!light_item<MODE>(a, i, tw, wm, wc, (lds_u32*)&s_wovf[wid], &total_acc)) {
-      if (lane == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = i;
+    const uint32_t gs = a.gstart[g], ge = a.gstart[g + 1];
+    if (!light_group<MODE>(a, gs, ge, g >= ng0, tw, wm, wc, (lds_u64*)s_iacc[wid], (lds_u32*)&s_wovf[wid], &total_acc)) {
+      if (lane == 0) {
+        const uint32_t o = atomicAdd(a.ovf_len, ge - gs);
+        for (uint32_t k = gs; k < ge; k++) a.ovf_list[o + k - gs] = a.wlist[k];
+      }
     }
     next = __shfl(next, 0, 64);
   }
   if constexpr (MODE == MODE_TOTAL) {
+    total_acc = pbf_wave_sum(total_acc);
     if (lane == 0 && total_acc) atomicAdd(a.total, total_acc);
   }
 }
@@ -877,6 +993,98 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
   }
 }
 
+// ---------------- grouping of the light endpoints ----------------
+// Light endpoints are sorted by (side, table size descending) with a counting sort over 12
+// buckets: bucket = side * 6 + (11 - log2 cap), side 1 when a VERT endpoint lies on O (its
+// groups run pass 2 instead of the sweep).  Within a side the tables then tile consecutive
+// LT_WORDS-word groups exactly: the word prefix of every endpoint is a multiple of its own
+// power-of-two size, so a group starts wherever the prefix is a multiple of LT_WORDS.
+// plan (u32): [0, 12) bucket counts, [12, 24) bucket starts, [24, 36) scatter counters,
+// [36, 38) first group of each side; wbase (u64): word prefix of each bucket within its side.
+constexpr uint32_t LBUCKETS = 12;
+
+template <int MODE>
+__device__ __forceinline__ uint32_t light_bucket(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
+                                                 const uint32_t* __restrict__ bound, uint32_t i) {
+  const uint32_t side = (MODE == MODE_VERT && (redg[roff[i]] & 1u)) ? 1u : 0u;
+  return side * 6u + (11u - (31u - (uint32_t)__clz(light_cap(bound[i]))));
+}
+
+// Adds 1 per active lane to cnt[b] with one atomic per distinct bucket in the wavefront;
+// returns the lane's position (the old value plus its rank among the lanes of its bucket).
+__device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* cnt, uint32_t b, bool act) {
+  uint64_t todo = __ballot(act);
+  uint32_t pos = 0;
+  while (todo) {
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
+    const uint32_t bl = lane_read(b, leader);
+    const uint64_t m = __ballot(act && b == bl);
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(&cnt[bl], (uint32_t)__popcll(m));
+    base = lane_read(base, leader);
+    if (act && b == bl) pos = base + (uint32_t)__popcll(m & pbf_lanemask_lt());
+    todo &= ~m;
+  }
+  return pos;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_light_hist(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
+                                                    const uint32_t* __restrict__ bound, const uint32_t* __restrict__ wl,
+                                                    const uint32_t* __restrict__ lens, uint32_t* __restrict__ plan) {
+  const uint32_t nl = lens[0];
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nl; base += gridDim.x * blockDim.x) {
+    const uint32_t t = base + (threadIdx.x & 63);
+    const bool act = t < nl;
+    const uint32_t b = act ? light_bucket<MODE>(roff, redg, bound, wl[t]) : 0u;
+    wave_bucket_add(plan, b, act);
+  }
+}
+
+__global__ void k_light_plan(uint32_t* __restrict__ plan, unsigned long long* __restrict__ wbase, uint32_t* __restrict__ ctl,
+                             uint32_t* __restrict__ gstart) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t start = 0, groups = 0;
+  for (uint32_t side = 0; side < 2; side++) {
+    unsigned long long words = 0;
+    for (uint32_t r = 0; r < 6; r++) {
+      const uint32_t b = side * 6 + r;
+      plan[12 + b] = start;
+      start += plan[b];
+      wbase[b] = words;
+      words += (unsigned long long)plan[b] * (LT_WORDS >> r);
+      plan[24 + b] = 0;
+    }
+    plan[36 + side] = groups;
+    groups += (uint32_t)((words + LT_WORDS - 1) / LT_WORDS);
+    if (side == 0) ctl[10] = groups;
+  }
+  ctl[5] = groups;
+  gstart[groups] = start;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_light_scatter(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
+                                                       const uint32_t* __restrict__ bound, const uint32_t* __restrict__ wl,
+                                                       const uint32_t* __restrict__ lens, uint32_t* __restrict__ plan,
+                                                       const unsigned long long* __restrict__ wbase, uint32_t* __restrict__ out,
+                                                       uint32_t* __restrict__ gstart) {
+  const uint32_t nl = lens[0];
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nl; base += gridDim.x * blockDim.x) {
+    const uint32_t t = base + (threadIdx.x & 63);
+    const bool act = t < nl;
+    const uint32_t i = act ? wl[t] : 0u;
+    const uint32_t b = act ? light_bucket<MODE>(roff, redg, bound, i) : 0u;
+    const uint32_t p = wave_bucket_add(plan + 24, b, act);
+    if (act) {
+      const uint32_t pos = plan[12 + b] + p;
+      out[pos] = i;
+      const unsigned long long prefix = wbase[b] + (unsigned long long)p * (LT_WORDS >> (b % 6u));
+      if ((prefix & (LT_WORDS - 1)) == 0) gstart[plan[36 + b / 6u] + (uint32_t)(prefix / LT_WORDS)] = pos;
+    }
+  }
+}
+
 // Split endpoints, pass 1: one workgroup per (endpoint, slice of SPLIT_CHUNK neighbours) counts
 // its wedges in a dense LDS table and adds the non-zero counts into the endpoint's dense HBM
 // array G (row of D32_MAX words).
@@ -1032,11 +1240,12 @@ constexpr int HBM_TEAM = 256, HBM_BLOCKS = 32;
 template <int MODE>
 int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lists, uint32_t* ctl) {
   const int64_t n = R.n;
-  // ctl: [0] light len, [1] heavy len, [2] big len, [3] heavy queue, [4] light queue,
-  //      [6] HBM-fallback len, [7] total big chunks, [8] split len, [9] total split items,
-  //      [13] HBM queue
-  // lists: [0, n) light, [n, 2n) heavy, [2n, 3n) HBM fallback
-  void *bp2, *up;
+  // ctl: [0] light len, [1] heavy len, [2] big len, [3] heavy queue, [4] light-group queue,
+  //      [5] light groups, [6] HBM-fallback len, [7] total big chunks, [8] split len,
+  //      [9] total split items, [10] light groups on side S, [13] HBM queue
+  // lists: [0, n) light (classification order), [n, 2n) heavy, [2n, 3n) HBM fallback,
+  //        [3n, 4n) light grouped, [4n, 5n + 1) group starts
+  void *bp2, *up, *pl;
   PBF_CHECK(pbf_scratch(ctx, S_BIG, 4 * (size_t)(3 * n + 64), &bp2));
   PBF_CHECK(pbf_scratch(ctx, S_UB, 8 * (size_t)(n ? n : 1), &up));
   uint32_t* big = static_cast<uint32_t*>(bp2);
@@ -1045,7 +1254,11 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   unsigned long long* w64 = static_cast<unsigned long long*>(up);
   hipLaunchKernelGGL(k_lpos, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.rown,
                      R.inpref, R.m2, const_cast<uint32_t*>(a.lpos));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctl, 0u, (int64_t)16);
+  PBF_CHECK(pbf_scratch(ctx, S_LPLAN, 64 * 4 + LBUCKETS * 8, &pl));
+  uint32_t* plan = static_cast<uint32_t*>(pl);
+  unsigned long long* wbase = reinterpret_cast<unsigned long long*>(plan + 64);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctl, 0u, (int64_t)64);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, plan, 0u, (int64_t)64);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
   const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, lists + 2 * n, ctl, a.cb};
   hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.lpos,
@@ -1071,7 +1284,14 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   hipLaunchKernelGGL(k_split_credit<MODE>, dim3(256), dim3(TPB), 0, ctx->stream, a, split, ctl, G);
   if (MODE != MODE_TOTAL)
     hipLaunchKernelGGL(k_split_pass2<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G);
-  a.wlist = lists;
+  // light endpoints: sort into groups
+  hipLaunchKernelGGL(k_light_hist<MODE>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound,
+                     lists, ctl, plan);
+  hipLaunchKernelGGL(k_light_plan, dim3(1), dim3(64), 0, ctx->stream, plan, wbase, ctl, lists + 4 * n);
+  hipLaunchKernelGGL(k_light_scatter<MODE>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, ctx->stream, R.roff, R.redg,
+                     a.bound, lists, ctl, plan, wbase, lists + 3 * n, lists + 4 * n);
+  a.wlist = lists + 3 * n;
+  a.gstart = lists + 4 * n;
   a.blist = lists + n;
   a.lens = ctl;
   a.queues = ctl + 3;
@@ -1105,13 +1325,13 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
   const int64_t n = R.n, m2 = R.m2;
   void *bp, *lp, *cp, *mp, *hp, *op = nullptr;
   PBF_CHECK(pbf_scratch(ctx, S_BOUND, 4 * (size_t)(n ? n : 1), &bp));
-  PBF_CHECK(pbf_scratch(ctx, S_BINS, 4 * (size_t)(3 * n + 64), &lp));
+  PBF_CHECK(pbf_scratch(ctx, S_BINS, 4 * (size_t)(5 * n + 128), &lp));
   PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
   PBF_CHECK(pbf_scratch(ctx, S_HOT, 8 * (size_t)HOT_N * HOT_R, &hp));
   void* lpp;
   PBF_CHECK(pbf_scratch(ctx, S_LPOS, 4 * (size_t)(m2 ? m2 : 1), &lpp));
   uint32_t* lists = static_cast<uint32_t*>(lp);
-  uint32_t* ctl = lists + 3 * n;
+  uint32_t* ctl = lists + 5 * n + 64;
   CountArgs a{};
   a.roff = R.roff; a.redg = R.redg; a.inpref = R.inpref; a.bound = static_cast<uint32_t*>(bp);
   a.lpos = static_cast<uint32_t*>(lpp);

@@ -44,6 +44,7 @@ enum ScratchSlot {
   S_GROWS,       // split endpoints: dense count rows in HBM
   S_LPOS,        // per-slot position of the owner inside the neighbour's list
   S_ROWN,        // per-slot owner rank
+  S_LPLAN,       // grouping plan of the light endpoints (bucket counts, starts, word bases)
   S_NUM_SLOTS
 };
 
