@@ -31,7 +31,7 @@ struct CountArgs {
   const uint32_t* roff;
   const uint32_t* redg;
   const uint32_t* inpref;
-  const uint32_t* lpos;    // per slot (i -> v) with v > i: position of i inside N(v)
+  const uint2* sinfo;      // per slot (i -> v): roff[v] and L, the length of N(v)'s prefix below min(i, v)
   const uint32_t* bound;   // distinct-u2 bound per endpoint: min(wedges, i)
   const uint32_t* oidx;    // EDGE: O-CSR index of each ranked slot
   unsigned long long* B;   // VERT: counts in rank space; EDGE: counts in O-CSR order
@@ -47,8 +47,14 @@ struct CountArgs {
   const uint32_t* blist;   // endpoints processed by a whole workgroup
   const uint32_t* wlist;   // light endpoints, grouped (see light_group)
   const uint32_t* gstart;  // first wlist index of each light group, and the end
-  const uint32_t* lens;    // [1] blist length, [5] light groups, [10] groups on side S
+  const uint4* lrec;       // per grouped light endpoint: i, roff[i], deg(i), log2 table size
+  const uint32_t* nblock;  // blist length
+  const uint32_t* ngroups; // light groups, and those of them on side S
+  const uint32_t* ngroups0;
   uint32_t* queues;        // [0] block queue, [1] light-group queue
+  uint32_t* rerun_list;    // light endpoints whose group table overflowed (optimistic size)
+  uint32_t* rerun_len;
+  uint32_t rerun;          // this launch re-runs such endpoints with full-size tables
   unsigned long long* hot; // replicated accumulators for ranks < HOT_N (HOT_R replicas each)
   uint32_t cb;             // count bits of a packed 32-bit hash slot (32 - key bits)
 };
@@ -168,6 +174,7 @@ constexpr uint32_t LIGHT_MAX = LT_WORDS / 4 * 3 - 1;  // light endpoint: <= 1535
 constexpr uint32_t LIGHT_WORK = 1u << 14;         // ... and at most 16384 wedges
 static_assert(LIGHT_MAX + LIGHT_MAX / 3 + 1 <= LT_WORDS, "a light table must fit its wavefront's slots");
 constexpr uint32_t WIDE_DEG = 1u << 24;           // per-window u32 sums need deg(i) < 2^24
+constexpr uint32_t PROBE_MAX = 64;                // probes per insert into a light sub-table
 constexpr uint32_t WIN = 4;                       // wedge indices per lane per window
 constexpr uint32_t WIN_SPAN = 64 * WIN;           // window of 256 consecutive wedge indices
 constexpr uint32_t HOT_N = 4096, HOT_R = 16;
@@ -274,6 +281,7 @@ struct Tab {
   lds_u32* w;        // LDS words
   uint32_t mask, shift, cb;
   const uint32_t* g; // TM_G32: dense counts in HBM (lookups only)
+  uint32_t plim;     // inserts: probes beyond the first before giving up
 };
 
 // Inserts key u2: returns 1 when the key is new, 0 when it was present, 2 when the probe
@@ -289,7 +297,7 @@ __device__ __forceinline__ uint32_t tab_ins(const Tab& t, uint32_t u2) {
   } else if constexpr (TM == TM_HASH32) {
     uint32_t h = hash_slot(u2, t.shift);
     const uint32_t mine = (u2 << t.cb) | 1u;
-    for (uint32_t probe = 0; probe <= t.mask; probe++) {
+    for (uint32_t probe = 0; probe <= t.plim; probe++) {
       const uint32_t old = lds_cas(t.w + h, EMPTY, mine);
       if (old == EMPTY) return 1;
       if ((old >> t.cb) == u2) { lds_add(t.w + h, 1u); return 0; }
@@ -300,7 +308,7 @@ __device__ __forceinline__ uint32_t tab_ins(const Tab& t, uint32_t u2) {
     lds_u64* s = (lds_u64*)t.w;
     uint32_t h = hash_slot(u2, t.shift);
     const unsigned long long mine = ((unsigned long long)u2 << 32) | 1ull;
-    for (uint32_t probe = 0; probe <= t.mask; probe++) {
+    for (uint32_t probe = 0; probe <= t.plim; probe++) {
       const unsigned long long old = lds_cas(s + h, ~0ull, mine);
       if (old == ~0ull) return 1;
       if ((uint32_t)(old >> 32) == u2) { lds_add(s + h, 1ull); return 0; }
@@ -340,6 +348,27 @@ __device__ __forceinline__ uint32_t tab_get(const Tab& t, uint32_t u2) {
   }
 }
 
+// Continuations of a packed-hash insert / lookup whose first probe (slot h - 1) collided.
+__device__ __forceinline__ uint32_t tab_ins_from(const Tab& t, uint32_t u2, uint32_t h) {
+  const uint32_t mine = (u2 << t.cb) | 1u;
+  for (uint32_t probe = 1; probe <= t.plim; probe++) {
+    const uint32_t old = lds_cas(t.w + h, EMPTY, mine);
+    if (old == EMPTY) return 1;
+    if ((old >> t.cb) == u2) { lds_add(t.w + h, 1u); return 0; }
+    h = (h + 1) & t.mask;
+  }
+  return 2;
+}
+__device__ __forceinline__ uint32_t tab_get_from(const Tab& t, uint32_t u2, uint32_t h) {
+  for (uint32_t probe = 1; probe <= t.mask; probe++) {
+    const uint32_t cur = t.w[h];
+    if ((cur >> t.cb) == u2) return cur & ((1u << t.cb) - 1u);
+    if (cur == EMPTY) return 0u;
+    h = (h + 1) & t.mask;
+  }
+  return 0u;
+}
+
 // Credit to rank x: hub ranks go to replicated accumulators.
 __device__ __forceinline__ void credit(const CountArgs& a, uint32_t x, unsigned long long val) {
   if (x < HOT_N) atomicAdd(&a.hot[x * HOT_R + (blockIdx.x & (HOT_R - 1))], val);
@@ -373,15 +402,15 @@ struct Chunk {
   uint32_t td;     // grouped tables: the endpoint's sub-table (word offset | log2 size << 16)
 };
 
-__device__ __forceinline__ Chunk load_chunk(const CountArgs& a, uint32_t i, uint32_t slot, bool valid, uint32_t td) {
+__device__ __forceinline__ Chunk load_chunk(const CountArgs& a, uint32_t slot, bool valid, uint32_t td) {
   Chunk ch;
   ch.v = 0; ch.L = 0; ch.slot = slot; ch.td = td;
   uint32_t vo = 0;
   if (valid) {
+    const uint2 si = a.sinfo[slot];
     ch.v = a.redg[slot] >> 1;
-    vo = a.roff[ch.v];
-    // the prefix of N(v) below min(i, v): all of in(v) when v < i, else up to i's own slot
-    ch.L = (ch.v < i) ? a.inpref[ch.v] : a.lpos[slot];
+    vo = si.x;
+    ch.L = si.y;
   }
   const uint32_t incl = wave_incl_add(ch.L);
   ch.T = lane_read(incl, 63);
@@ -445,21 +474,91 @@ __device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint
 // The sub-table of a grouped endpoint.
 __device__ __forceinline__ Tab sub_tab(const Tab& t, uint32_t td) {
   const uint32_t lg = td >> 16;
-  return Tab{t.w + (td & 0xFFFFu), (1u << lg) - 1u, 32u - lg, t.cb, nullptr};
+  const uint32_t mask = (1u << lg) - 1u;
+  return Tab{t.w + (td & 0xFFFFu), mask, 32u - lg, t.cb, nullptr, mask < PROBE_MAX ? mask : PROBE_MAX - 1u};
 }
 
 // Pass 1 on one window: returns the lane's number of new keys, plus 256 on a failed insert.
+// Packed hash tables take the first probe of all WIN wedges at once (their CAS round trips
+// overlap); only collisions continue probing one wedge at a time.
 template <int TM, bool MULTI>
 __device__ __forceinline__ uint32_t win_insert(const Tab& t, const Win& wv) {
   uint32_t r = 0;
+  if constexpr (TM == TM_HASH32) {
+    uint32_t h[WIN], old[WIN];
 #pragma unroll
-  for (uint32_t k = 0; k < WIN; k++) {
-    if (k < wv.nval) {
-      const uint32_t s = tab_ins<TM>(MULTI ? sub_tab(t, wv.td[k]) : t, wv.ent[k] >> 1);
-      r += (s == 2u) ? 256u : s;
+    for (uint32_t k = 0; k < WIN; k++) {
+      const Tab tk = MULTI ? sub_tab(t, wv.td[k]) : t;
+      const uint32_t u2 = wv.ent[k] >> 1;
+      h[k] = hash_slot(u2, tk.shift);
+      old[k] = k < wv.nval ? lds_cas(tk.w + h[k], EMPTY, (u2 << t.cb) | 1u) : EMPTY;
+    }
+    uint32_t slow = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) {
+      if (k < wv.nval) {
+        const uint32_t u2 = wv.ent[k] >> 1;
+        if (old[k] == EMPTY) r += 1u;
+        else if ((old[k] >> t.cb) == u2) lds_add((MULTI ? sub_tab(t, wv.td[k]) : t).w + h[k], 1u);
+        else slow |= 1u << k;
+      }
+    }
+    if (__ballot(slow != 0u)) {
+#pragma unroll
+      for (uint32_t k = 0; k < WIN; k++) {
+        if (slow & (1u << k)) {
+          const Tab tk = MULTI ? sub_tab(t, wv.td[k]) : t;
+          const uint32_t s = tab_ins_from(tk, wv.ent[k] >> 1, (h[k] + 1u) & tk.mask);
+          r += (s == 2u) ? 256u : s;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) {
+      if (k < wv.nval) {
+        const uint32_t s = tab_ins<TM>(MULTI ? sub_tab(t, wv.td[k]) : t, wv.ent[k] >> 1);
+        r += (s == 2u) ? 256u : s;
+      }
     }
   }
   return r;
+}
+
+// Counts of the WIN wedges of a lane; packed hash tables read all first probes at once.
+template <int TM, bool MULTI>
+__device__ __forceinline__ void win_get(const Tab& t, const Win& wv, uint32_t* c) {
+  if constexpr (TM == TM_HASH32) {
+    uint32_t h[WIN], cur[WIN];
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) {
+      const Tab tk = MULTI ? sub_tab(t, wv.td[k]) : t;
+      h[k] = hash_slot(wv.ent[k] >> 1, tk.shift);
+      cur[k] = k < wv.nval ? tk.w[h[k]] : EMPTY;
+    }
+    const uint32_t cmask = (1u << t.cb) - 1u;
+    uint32_t slow = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) {
+      c[k] = 0;
+      if (k < wv.nval) {
+        if ((cur[k] >> t.cb) == (wv.ent[k] >> 1)) c[k] = cur[k] & cmask;
+        else if (cur[k] != EMPTY) slow |= 1u << k;
+      }
+    }
+    if (__ballot(slow != 0u)) {
+#pragma unroll
+      for (uint32_t k = 0; k < WIN; k++) {
+        if (slow & (1u << k)) {
+          const Tab tk = MULTI ? sub_tab(t, wv.td[k]) : t;
+          c[k] = tab_get_from(tk, wv.ent[k] >> 1, (h[k] + 1u) & tk.mask);
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) c[k] = k < wv.nval ? tab_get<TM>(MULTI ? sub_tab(t, wv.td[k]) : t, wv.ent[k] >> 1) : 0u;
+  }
 }
 
 // Pass 2 on one window: c - 1 per wedge with c > 1, credited to the edge (v -> u2) (EDGE) and
@@ -469,14 +568,15 @@ __device__ __forceinline__ uint32_t win_insert(const Tab& t, const Win& wv) {
 // hold at most 256 values below deg(i) < 2^24.
 template <int MODE, int TM, bool MULTI>
 __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, const Win& wv, lds_u64* acc) {
-  uint32_t val[WIN];
+  uint32_t val[WIN], cnt[WIN];
   uint32_t last = wv.seg[0];
+  win_get<TM, MULTI>(t, wv, cnt);
 #pragma unroll
   for (uint32_t k = 0; k < WIN; k++) {
     val[k] = 0;
     if (k < wv.nval) {
       last = wv.seg[k];
-      const uint32_t c = tab_get<TM>(MULTI ? sub_tab(t, wv.td[k]) : t, wv.ent[k] >> 1);
+      const uint32_t c = cnt[k];
       if (c > 1u) {
         val[k] = c - 1u;
         if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[wv.pos[k]]], (unsigned long long)(c - 1u));
@@ -554,7 +654,7 @@ __device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint
   const uint32_t wstart = split ? 0u : w * WIN_SPAN, wstep = split ? WIN_SPAN : nw * WIN_SPAN;
   for (uint32_t c0 = jbeg + cstart; c0 < jend; c0 += cstep) {
     const uint32_t j = c0 + __lane_id();
-    const Chunk ch = load_chunk(a, i, io + j, j < jend, 0u);
+    const Chunk ch = load_chunk(a, io + j, j < jend, 0u);
     if (!chunk_windows<MODE, PASS2, TM, false>(a, t, ch, wstart, wstep, wc, tr)) return false;
   }
   return true;
@@ -622,7 +722,7 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
   const uint32_t tid = threadIdx.x;
   const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
   const bool i_in_S = (a.redg[io] & 1u) == 0u;
-  Tab t{pool, 0, 0, a.cb, nullptr};
+  Tab t{pool, 0, 0, a.cb, nullptr, 0};
   Track tr{nused, ovf, 0xFFFFFFFFu, false};
   int tm;
   uint32_t words, fill = 0, cells = i;
@@ -635,10 +735,11 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
     const bool packed = a.cb < 32u && deg < (1u << a.cb);
     const uint32_t capmax = packed ? POOL_WORDS : POOL_WORDS / 2;
     const uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
-    uint32_t cap = 64;
+    uint32_t cap = a.rerun ? capmax : 64u;  // a re-run endpoint's bound was optimistic
     while (cap < want && cap < capmax) cap <<= 1;
     tm = packed ? TM_HASH32 : TM_HASH;
     t.mask = cap - 1;
+    t.plim = cap - 1;
     t.shift = 32 - (31 - __clz(cap));
     if (cap == capmax) {
       // the bound does not fit: count keys; every wave may insert one more window after
@@ -685,21 +786,17 @@ __device__ __forceinline__ uint32_t lane_upper(uint32_t key, uint32_t nk, uint32
 // adjacency lists are concatenated into one flattened slot range, taken 64 slots per chunk;
 // every wedge is aggregated in its endpoint's sub-table.  One enumeration thus serves several
 // endpoints, and the dependent loads of an endpoint's setup are paid once per group.  All
-// endpoints of a group lie on the same side.  Returns false if a table overflowed (then
-// nothing was credited).
-template <int MODE>
-__device__ bool light_group(const CountArgs& a, uint32_t gs, uint32_t ge, bool o_side, lds_u32* tw, uint32_t& wm,
-                            WaveCtx& wc, lds_u64* iacc, lds_u32* ovf, unsigned long long* total_acc) {
+// endpoints of a group lie on the same side.  Lane k < nk holds endpoint k's record rc =
+// (i, roff[i], deg(i), log2 table size).  at_b runs after pass 1 and at_c at the end (the
+// caller's prefetches of the next groups).  Returns false if a table overflowed (then nothing
+// was credited).
+template <int MODE, typename FB, typename FC>
+__device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bool o_side, lds_u32* tw, uint32_t& wm,
+                            WaveCtx& wc, lds_u64* iacc, lds_u32* ovf, unsigned long long* total_acc, FB at_b, FC at_c) {
   const uint32_t lane = __lane_id();
-  const uint32_t nk = ge - gs;
-  uint32_t i_k = 0, io_k = 0, deg_k = 0, cap_k = 0, lg_k = 0;
-  if (lane < nk) {
-    i_k = a.wlist[gs + lane];
-    io_k = a.roff[i_k];
-    deg_k = a.roff[i_k + 1] - io_k;
-    cap_k = light_cap(a.bound[i_k]);
-    lg_k = 31u - (uint32_t)__clz(cap_k);
-  }
+  const bool own = lane < nk;
+  const uint32_t i_k = own ? rc.x : 0u, io_k = own ? rc.y : 0u, deg_k = own ? rc.z : 0u;
+  const uint32_t lg_k = own ? rc.w : 0u, cap_k = own ? 1u << rc.w : 0u;
   const uint32_t cinc = wave_incl_add(cap_k);
   const uint32_t words = lane_read(cinc, 63);  // <= LT_WORDS by construction of the groups
   const uint32_t off_k = cinc - cap_k;
@@ -712,7 +809,7 @@ __device__ bool light_group(const CountArgs& a, uint32_t gs, uint32_t ge, bool o
   if (lane == 0) lds_store(ovf, 0u);
   wm = max(wm, words);
   wave_lds_fence();
-  const Tab t{tw, 0, 0, a.cb, nullptr};
+  const Tab t{tw, 0, 0, a.cb, nullptr, 0};
   const Track tr{nullptr, ovf, 0xFFFFFFFFu, false};
   const bool need_pass2 = MODE == MODE_EDGE || (MODE == MODE_VERT && o_side);
   // chunk c0 of the concatenated lists: lane -> endpoint (largest k with dex_k <= f; every
@@ -721,26 +818,28 @@ __device__ bool light_group(const CountArgs& a, uint32_t gs, uint32_t ge, bool o
     const uint32_t f = c0 + lane;
     const uint32_t k = lane_upper(dex_k, nk, f);
     const int kb = (int)(k << 2);
-    const uint32_t i = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)i_k);
     const uint32_t slot = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)io_k) + f -
                           (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)dex_k);
     const uint32_t td = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)td_k);
-    return load_chunk(a, i, slot, f < DS, td);
+    return load_chunk(a, slot, f < DS, td);
   };
+  bool ok = true;
   if (DS <= 64) {
     // one chunk: its setup serves both passes
     const Chunk ch = chunk_at(0);
-    if (!chunk_windows<MODE, false, TM_HASH32, true>(a, t, ch, 0, WIN_SPAN, wc, tr)) return false;
+    ok = chunk_windows<MODE, false, TM_HASH32, true>(a, t, ch, 0, WIN_SPAN, wc, tr);
     wave_lds_fence();
-    if (need_pass2) chunk_windows<MODE, true, TM_HASH32, true>(a, t, ch, 0, WIN_SPAN, wc, tr);
+    at_b();
+    if (ok && need_pass2) chunk_windows<MODE, true, TM_HASH32, true>(a, t, ch, 0, WIN_SPAN, wc, tr);
   } else {
-    for (uint32_t c0 = 0; c0 < DS; c0 += 64)
-      if (!chunk_windows<MODE, false, TM_HASH32, true>(a, t, chunk_at(c0), 0, WIN_SPAN, wc, tr)) return false;
+    for (uint32_t c0 = 0; c0 < DS && ok; c0 += 64)
+      ok = chunk_windows<MODE, false, TM_HASH32, true>(a, t, chunk_at(c0), 0, WIN_SPAN, wc, tr);
     wave_lds_fence();
-    if (need_pass2)
+    at_b();
+    if (ok && need_pass2)
       for (uint32_t c0 = 0; c0 < DS; c0 += 64) chunk_windows<MODE, true, TM_HASH32, true>(a, t, chunk_at(c0), 0, WIN_SPAN, wc, tr);
   }
-  if (MODE == MODE_TOTAL || (MODE == MODE_VERT && !o_side)) {
+  if (ok && (MODE == MODE_TOTAL || (MODE == MODE_VERT && !o_side))) {
     // endpoint credits: each lane sweeps 4 words of one sub-table per step (sub-tables are
     // multiples of 64 words) and clears them; per-endpoint sums are reduced with a segmented
     // scan (an endpoint's sum of C(c,2) is at most T^2 / 2 < 2^32) into iacc
@@ -780,10 +879,12 @@ __device__ bool light_group(const CountArgs& a, uint32_t gs, uint32_t ge, bool o
     }
     wave_lds_fence();
   }
-  return true;
+  at_c();
+  return ok;
 }
 
-template <int MODE>
+// PH: 1 = heavy endpoints (workgroup queue), 2 = light groups (wavefront queue), 3 = both.
+template <int MODE, int PH>
 __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   __shared__ __align__(16) uint32_t pool[POOL_WORDS];
   __shared__ __align__(16) uint32_t s_flags[NWAVE][WIN_SPAN];
@@ -798,38 +899,68 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   if (lane < 32) s_iacc[wid][lane] = 0;
   __syncthreads();
   WaveCtx wc{(lds_u32*)s_flags[wid], (lds_u64*)s_acc[wid], 0};
-  // phase 1: heavy endpoints, whole workgroup
-  const uint32_t nb = a.lens[1];
-  while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&a.queues[0], 1u);
-    __syncthreads();
-    const uint32_t idx = s_item;
-    __syncthreads();
-    if (idx >= nb) break;
+  // phase 1: heavy endpoints, whole workgroup; the next one is claimed while this one runs
+  if constexpr ((PH & 1) != 0) {
+  const uint32_t nb = *a.nblock;
+  uint32_t claim = 0;
+  if (threadIdx.x == 0) s_item = atomicAdd(&a.queues[0], 1u);
+  __syncthreads();
+  uint32_t idx = s_item;
+  __syncthreads();
+  while (idx < nb) {
+    if (threadIdx.x == 0) claim = atomicAdd(&a.queues[0], 1u);
     const uint32_t i = a.blist[idx];
     if (!block_item<MODE>(a, i, (lds_u32*)pool, wc, (lds_u32*)&s_nused, (lds_u32*)&s_ovf, s_red, &total_acc)) {
       if (threadIdx.x == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = i;
     }
+    if (threadIdx.x == 0) s_item = claim;
+    __syncthreads();
+    idx = s_item;
+    __syncthreads();
+  }
   }
   // phase 2: groups of light endpoints, one per wavefront, private quarter of the pool
-  const uint32_t ng = a.lens[5], ng0 = a.lens[10];
+  if constexpr ((PH & 2) != 0) {
+  const uint32_t ng = *a.ngroups, ng0 = *a.ngroups0;
   lds_u32* tw = (lds_u32*)(pool + wid * LT_WORDS);
   uint32_t wm = LT_WORDS;  // phase 1 left arbitrary words
-  // the next group is claimed before the current one is processed, hiding the queue latency
-  uint32_t next = 0;
-  if (lane == 0) next = atomicAdd(&a.queues[1], 1u);
-  next = __shfl(next, 0, 64);
-  while (next < ng) {
-    const uint32_t g = next;
-    if (lane == 0) next = atomicAdd(&a.queues[1], 1u);
-    const uint32_t gs = a.gstart[g], ge = a.gstart[g + 1];
-    if (!light_group<MODE>(a, gs, ge, g >= ng0, tw, wm, wc, (lds_u64*)s_iacc[wid], (lds_u32*)&s_wovf[wid], &total_acc)) {
-      if (lane == 0) {
-        const uint32_t o = atomicAdd(a.ovf_len, ge - gs);
-        for (uint32_t k = gs; k < ge; k++) a.ovf_list[o + k - gs] = a.wlist[k];
-      }
+  // Software pipeline over groups k, k+1, k+2: while group k runs, the records of k+1 are
+  // loaded (after its pass 1) and the bounds of k+2 (at its end), whose claim was issued at
+  // its start; the dependent loads of a group's setup thus leave the critical path.
+  uint32_t g = 0, gn = 0;
+  if (lane == 0) { g = atomicAdd(&a.queues[1], 1u); gn = atomicAdd(&a.queues[1], 1u); }
+  g = __shfl(g, 0, 64);
+  gn = __shfl(gn, 0, 64);
+  uint32_t gs = 0, ge = 0, ns = 0, ne = 0;
+  uint4 rc = make_uint4(0u, 0u, 0u, 0u);
+  if (g < ng) {
+    gs = a.gstart[g]; ge = a.gstart[g + 1];
+    if (lane < ge - gs) rc = a.lrec[gs + lane];
+  }
+  if (gn < ng) { ns = a.gstart[gn]; ne = a.gstart[gn + 1]; }
+  while (g < ng) {
+    uint32_t claim = 0;
+    if (lane == 0) claim = atomicAdd(&a.queues[1], 1u);
+    uint4 nrc = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t gnn = 0, nns = 0, nne = 0;
+    auto at_b = [&]() {
+      if (gn < ng && lane < ne - ns) nrc = a.lrec[ns + lane];
+    };
+    auto at_c = [&]() {
+      gnn = __shfl(claim, 0, 64);
+      if (gnn < ng) { nns = a.gstart[gnn]; nne = a.gstart[gnn + 1]; }
+    };
+    const uint32_t nk = ge - gs;
+    if (!light_group<MODE>(a, nk, rc, g >= ng0, tw, wm, wc, (lds_u64*)s_iacc[wid], (lds_u32*)&s_wovf[wid], &total_acc,
+                           at_b, at_c)) {
+      uint32_t o = 0;
+      if (lane == 0) o = atomicAdd(a.rerun_len, nk);
+      o = __shfl(o, 0, 64);
+      if (lane < nk) a.rerun_list[o + lane] = rc.x;
     }
-    next = __shfl(next, 0, 64);
+    g = gn; gs = ns; ge = ne; rc = nrc;
+    gn = gnn; ns = nns; ne = nne;
+  }
   }
   if constexpr (MODE == MODE_TOTAL) {
     total_acc = pbf_wave_sum(total_acc);
@@ -856,13 +987,12 @@ __global__ void k_hot_reduce(unsigned long long* __restrict__ hot, unsigned long
 // wavefront (one atomic per class per wave).
 constexpr uint32_t CL_CHUNK = 2048;
 
-__device__ __forceinline__ uint32_t seg_len(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ inpref,
-                                            const uint32_t* __restrict__ lpos, uint32_t i, uint32_t slot) {
-  const uint32_t v = redg[slot] >> 1;
-  return v < i ? inpref[v] : lpos[slot];
-}
+__device__ __forceinline__ uint32_t seg_len(const uint2* __restrict__ sinfo, uint32_t slot) { return sinfo[slot].y; }
 
-// lens: [0] light, [1] heavy, [6] HBM-fallback (the overflow list)
+// lens: [0] light, [1] heavy, [11] HBM fallback.  A light endpoint's table is sized by its
+// distinct-key bound min(T, i); when that exceeds a wavefront table but T itself is small, the
+// table is sized optimistically for T / 2 keys (the endpoint is re-run with a full-size table
+// if it overflows).
 __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, uint32_t cb,
                                                 uint32_t* __restrict__ bound, uint32_t* __restrict__ wlist,
                                                 uint32_t* __restrict__ blist, uint32_t* __restrict__ olist,
@@ -872,17 +1002,20 @@ __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t
   if (valid && T) {
     const uint32_t b = (uint32_t)(T < (uint64_t)i ? T : (uint64_t)i);
     if (b) {
-      const bool light = b <= LIGHT_MAX && T <= LIGHT_WORK && cb < 32u && deg < (1u << cb);
+      const uint32_t est = (b > LIGHT_MAX && T <= LIGHT_WORK) ? min(b, (uint32_t)(T / 2)) : b;
+      const bool light = est <= LIGHT_MAX && T <= LIGHT_WORK && cb < 32u && deg < (1u << cb);
       cls = deg >= WIDE_DEG ? 3u : light ? 1u : 2u;
+      bound[i] = light ? est : b;
+    } else {
+      bound[i] = 0;
     }
-    bound[i] = b;
   }
   const uint64_t m1 = __ballot(cls == 1), m2 = __ballot(cls == 2), m3 = __ballot(cls == 3);
   uint32_t base1 = 0, base2 = 0, base3 = 0;
   if (lane == 0) {
     if (m1) base1 = atomicAdd(&lens[0], (uint32_t)__popcll(m1));
     if (m2) base2 = atomicAdd(&lens[1], (uint32_t)__popcll(m2));
-    if (m3) base3 = atomicAdd(&lens[6], (uint32_t)__popcll(m3));
+    if (m3) base3 = atomicAdd(&lens[11], (uint32_t)__popcll(m3));
   }
   base1 = __shfl(base1, 0, 64);
   base2 = __shfl(base2, 0, 64);
@@ -904,7 +1037,7 @@ struct ClassOut {
 
 // Short lists: classified here.  Long lists: listed in `big` with their chunk counts.
 __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
-                                                  const uint32_t* __restrict__ inpref, const uint32_t* __restrict__ lpos,
+                                                  const uint32_t* __restrict__ inpref, const uint2* __restrict__ sinfo,
                                                   int64_t n, ClassOut co, uint32_t* __restrict__ big,
                                                   uint32_t* __restrict__ big_chunks, unsigned long long* __restrict__ w64) {
   const uint32_t lane = threadIdx.x & 63;
@@ -916,7 +1049,7 @@ __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ r
     uint64_t T = 0;
     const bool small = deg <= 64;
     if (small)
-      for (uint32_t j = 0; j < deg; j++) T += seg_len(redg, inpref, lpos, (uint32_t)i, io + j);
+      for (uint32_t j = 0; j < deg; j++) T += seg_len(sinfo, io + j);
     classify_append((uint32_t)i, i < n && small, T, deg, co.cb, co.bound, co.wlist, co.blist, co.olist, co.lens);
     const uint64_t mb = __ballot(!small);
     uint32_t bb = 0;
@@ -934,7 +1067,7 @@ __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ r
 // One wavefront per chunk of a long list (chunk c of big vertex b, located by a binary search
 // over the scanned chunk counts); partial wedge counts are added into w64.
 __global__ __launch_bounds__(256) void k_wedge_chunks(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
-                                                      const uint32_t* __restrict__ inpref, const uint32_t* __restrict__ lpos,
+                                                      const uint32_t* __restrict__ inpref, const uint2* __restrict__ sinfo,
                                                       const uint32_t* __restrict__ big, const uint32_t* __restrict__ chunk_base,
                                                       const uint32_t* __restrict__ lens, const uint32_t* __restrict__ total_chunks,
                                                       unsigned long long* __restrict__ w64) {
@@ -953,17 +1086,18 @@ __global__ __launch_bounds__(256) void k_wedge_chunks(const uint32_t* __restrict
     uint64_t part = 0;
     uint32_t j = b0 + lane;
     for (; j + 192 < b1; j += 256)
-      part += (uint64_t)seg_len(redg, inpref, lpos, i, io + j) + seg_len(redg, inpref, lpos, i, io + j + 64) +
-              seg_len(redg, inpref, lpos, i, io + j + 128) + seg_len(redg, inpref, lpos, i, io + j + 192);
-    for (; j < b1; j += 64) part += seg_len(redg, inpref, lpos, i, io + j);
+      part += (uint64_t)seg_len(sinfo, io + j) + seg_len(sinfo, io + j + 64) + seg_len(sinfo, io + j + 128) +
+              seg_len(sinfo, io + j + 192);
+    for (; j < b1; j += 64) part += seg_len(sinfo, io + j);
     part = pbf_wave_sum(part);
     if (lane == 0 && part) atomicAdd(&w64[i], part);
   }
 }
 
-// Endpoints with more than SPLIT_DEG neighbours and a dense table (i < D32_MAX) are split
+// Endpoints with more than SPLIT_DEG neighbours and i < SPLIT_IMAX (few keys per chunk of
+// neighbours: latency-bound in one workgroup) are split
 // across workgroups (k_split_*); the other long ones are classified as usual.
-constexpr uint32_t SPLIT_DEG = 1u << 15, SPLIT_CHUNK = 1u << 13;
+constexpr uint32_t SPLIT_DEG = 1u << 12, SPLIT_CHUNK = 1u << 11, SPLIT_IMAX = 2048;
 
 __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ big,
                                                       const uint32_t* __restrict__ lens_in,
@@ -979,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
     const uint32_t i = valid ? big[t] : 0u;
     const uint32_t deg = valid ? roff[i + 1] - roff[i] : 0u;
     const unsigned long long T = valid ? w64[i] : 0ull;
-    const bool sp = valid && T && i > 0 && deg > SPLIT_DEG && deg < WIDE_DEG && i < D32_MAX;
+    const bool sp = valid && T && i > 0 && deg > SPLIT_DEG && deg < WIDE_DEG && i < SPLIT_IMAX;
     classify_append(i, valid && !sp, T, deg, co.cb, co.bound, co.wlist, co.blist, co.olist, co.lens);
     const uint64_t ms = __ballot(sp);
     uint32_t b = 0;
@@ -1010,84 +1144,98 @@ __device__ __forceinline__ uint32_t light_bucket(const uint32_t* __restrict__ ro
   return side * 6u + (11u - (31u - (uint32_t)__clz(light_cap(bound[i]))));
 }
 
-// Adds 1 per active lane to cnt[b] with one atomic per distinct bucket in the wavefront;
-// returns the lane's position (the old value plus its rank among the lanes of its bucket).
-__device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* cnt, uint32_t b, bool act) {
-  uint64_t todo = __ballot(act);
-  uint32_t pos = 0;
-  while (todo) {
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
-    const uint32_t bl = lane_read(b, leader);
-    const uint64_t m = __ballot(act && b == bl);
-    uint32_t base = 0;
-    if (__lane_id() == leader) base = atomicAdd(&cnt[bl], (uint32_t)__popcll(m));
-    base = lane_read(base, leader);
-    if (act && b == bl) pos = base + (uint32_t)__popcll(m & pbf_lanemask_lt());
-    todo &= ~m;
-  }
-  return pos;
-}
+// Two-level counting sort over LG_BLOCKS workgroups (the same items visit the same workgroup in
+// the histogram and the scatter): per-workgroup LDS histograms, one scan per bucket over the
+// workgroups, then a scatter whose positions come from LDS counters seeded with the
+// workgroup's offsets.  Order within a bucket is arbitrary; counts do not depend on it.
+constexpr uint32_t LG_BLOCKS = 256;
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_light_hist(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
                                                     const uint32_t* __restrict__ bound, const uint32_t* __restrict__ wl,
-                                                    const uint32_t* __restrict__ lens, uint32_t* __restrict__ plan) {
+                                                    const uint32_t* __restrict__ lens, uint32_t* __restrict__ bh) {
+  __shared__ uint32_t h[LBUCKETS];
+  if (threadIdx.x < LBUCKETS) h[threadIdx.x] = 0;
+  __syncthreads();
   const uint32_t nl = lens[0];
-  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nl; base += gridDim.x * blockDim.x) {
-    const uint32_t t = base + (threadIdx.x & 63);
-    const bool act = t < nl;
-    const uint32_t b = act ? light_bucket<MODE>(roff, redg, bound, wl[t]) : 0u;
-    wave_bucket_add(plan, b, act);
-  }
+  for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < nl; t += LG_BLOCKS * 256)
+    atomicAdd(&h[light_bucket<MODE>(roff, redg, bound, wl[t])], 1u);
+  __syncthreads();
+  if (threadIdx.x < LBUCKETS) bh[blockIdx.x * LBUCKETS + threadIdx.x] = h[threadIdx.x];
 }
 
-__global__ void k_light_plan(uint32_t* __restrict__ plan, unsigned long long* __restrict__ wbase, uint32_t* __restrict__ ctl,
-                             uint32_t* __restrict__ gstart) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t start = 0, groups = 0;
-  for (uint32_t side = 0; side < 2; side++) {
-    unsigned long long words = 0;
-    for (uint32_t r = 0; r < 6; r++) {
-      const uint32_t b = side * 6 + r;
-      plan[12 + b] = start;
-      start += plan[b];
-      wbase[b] = words;
-      words += (unsigned long long)plan[b] * (LT_WORDS >> r);
-      plan[24 + b] = 0;
+// One workgroup of LG_BLOCKS threads: bucket totals, starts and word bases, the groups of each
+// side, and every workgroup's starting position per bucket (bo).
+__global__ __launch_bounds__(LG_BLOCKS) void k_light_plan(const uint32_t* __restrict__ bh, uint32_t* __restrict__ bo,
+                                                          uint32_t* __restrict__ plan, unsigned long long* __restrict__ wbase,
+                                                          uint32_t* __restrict__ ctl, uint32_t* __restrict__ gstart) {
+  __shared__ uint32_t wsum[LG_BLOCKS / 64];
+  __shared__ uint32_t tot[LBUCKETS];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t excl[LBUCKETS];
+  for (uint32_t q = 0; q < LBUCKETS; q++) {
+    const uint32_t x = bh[t * LBUCKETS + q];
+    const uint32_t inc = wave_incl_add(x);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t k = 0; k < LG_BLOCKS / 64; k++) {
+      if (k < w) before += wsum[k];
+      all += wsum[k];
     }
-    plan[36 + side] = groups;
-    groups += (uint32_t)((words + LT_WORDS - 1) / LT_WORDS);
-    if (side == 0) ctl[10] = groups;
+    excl[q] = before + inc - x;
+    if (t == 0) tot[q] = all;
+    __syncthreads();
   }
-  ctl[5] = groups;
-  gstart[groups] = start;
+  if (t == 0) {
+    uint32_t start = 0, groups = 0;
+    for (uint32_t side = 0; side < 2; side++) {
+      unsigned long long words = 0;
+      for (uint32_t r = 0; r < 6; r++) {
+        const uint32_t b = side * 6 + r;
+        plan[b] = tot[b];
+        plan[12 + b] = start;
+        start += tot[b];
+        wbase[b] = words;
+        words += (unsigned long long)tot[b] * (LT_WORDS >> r);
+      }
+      plan[36 + side] = groups;
+      groups += (uint32_t)((words + LT_WORDS - 1) / LT_WORDS);
+      if (side == 0) ctl[10] = groups;
+    }
+    ctl[5] = groups;
+    gstart[groups] = start;
+  }
+  __syncthreads();
+  for (uint32_t q = 0; q < LBUCKETS; q++) bo[t * LBUCKETS + q] = plan[12 + q] + excl[q];
 }
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_light_scatter(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
                                                        const uint32_t* __restrict__ bound, const uint32_t* __restrict__ wl,
-                                                       const uint32_t* __restrict__ lens, uint32_t* __restrict__ plan,
+                                                       const uint32_t* __restrict__ lens, const uint32_t* __restrict__ bo,
+                                                       const uint32_t* __restrict__ plan,
                                                        const unsigned long long* __restrict__ wbase, uint32_t* __restrict__ out,
-                                                       uint32_t* __restrict__ gstart) {
+                                                       uint4* __restrict__ rec, uint32_t* __restrict__ gstart) {
+  __shared__ uint32_t c[LBUCKETS];
+  if (threadIdx.x < LBUCKETS) c[threadIdx.x] = bo[blockIdx.x * LBUCKETS + threadIdx.x];
+  __syncthreads();
   const uint32_t nl = lens[0];
-  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nl; base += gridDim.x * blockDim.x) {
-    const uint32_t t = base + (threadIdx.x & 63);
-    const bool act = t < nl;
-    const uint32_t i = act ? wl[t] : 0u;
-    const uint32_t b = act ? light_bucket<MODE>(roff, redg, bound, i) : 0u;
-    const uint32_t p = wave_bucket_add(plan + 24, b, act);
-    if (act) {
-      const uint32_t pos = plan[12 + b] + p;
-      out[pos] = i;
-      const unsigned long long prefix = wbase[b] + (unsigned long long)p * (LT_WORDS >> (b % 6u));
-      if ((prefix & (LT_WORDS - 1)) == 0) gstart[plan[36 + b / 6u] + (uint32_t)(prefix / LT_WORDS)] = pos;
-    }
+  for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < nl; t += LG_BLOCKS * 256) {
+    const uint32_t i = wl[t];
+    const uint32_t b = light_bucket<MODE>(roff, redg, bound, i);
+    const uint32_t pos = atomicAdd(&c[b], 1u);
+    out[pos] = i;
+    const uint32_t io = roff[i];
+    rec[pos] = make_uint4(i, io, roff[i + 1] - io, 11u - b % 6u);
+    const unsigned long long prefix = wbase[b] + (unsigned long long)(pos - plan[12 + b]) * (LT_WORDS >> (b % 6u));
+    if ((prefix & (LT_WORDS - 1)) == 0) gstart[plan[36 + b / 6u] + (uint32_t)(prefix / LT_WORDS)] = pos;
   }
 }
 
 // Split endpoints, pass 1: one workgroup per (endpoint, slice of SPLIT_CHUNK neighbours) counts
 // its wedges in a dense LDS table and adds the non-zero counts into the endpoint's dense HBM
-// array G (row of D32_MAX words).
+// array G (row of SPLIT_IMAX words).
 template <int MODE>
 __global__ __launch_bounds__(TPB) void k_split_pass1(CountArgs a, const uint32_t* __restrict__ split,
                                                      const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ ctl,
@@ -1102,7 +1250,7 @@ __global__ __launch_bounds__(TPB) void k_split_pass1(CountArgs a, const uint32_t
   s_acc[wid][lane] = 0;
   if (threadIdx.x == 0) s_ovf = 0;
   WaveCtx wc{(lds_u32*)s_flags[wid], (lds_u64*)s_acc[wid], 0};
-  const Tab t{(lds_u32*)pool, 0, 0, a.cb, nullptr};
+  const Tab t{(lds_u32*)pool, 0, 0, a.cb, nullptr, 0};
   const Track tr{nullptr, (lds_u32*)&s_ovf, 0xFFFFFFFFu, false};
   for (uint32_t item = blockIdx.x; item < tot; item += gridDim.x) {
     uint32_t lo = 0, hi = nsplit;
@@ -1114,7 +1262,7 @@ __global__ __launch_bounds__(TPB) void k_split_pass1(CountArgs a, const uint32_t
     const uint32_t j1 = min(deg, (c + 1) * SPLIT_CHUNK);
     enum_pass<MODE, false, TM_D32>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
     __syncthreads();
-    uint32_t* row = G + (size_t)lo * D32_MAX;
+    uint32_t* row = G + (size_t)lo * SPLIT_IMAX;
     for (uint32_t q = threadIdx.x; q < i; q += TPB)
       if (pool[q]) atomicAdd(&row[q], pool[q]);
     __syncthreads();
@@ -1131,7 +1279,7 @@ __global__ __launch_bounds__(TPB) void k_split_credit(CountArgs a, const uint32_
     const uint32_t i = split[b];
     const bool i_in_S = (a.redg[a.roff[i]] & 1u) == 0u;
     if (!(MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S))) continue;
-    const uint32_t* row = G + (size_t)b * D32_MAX;
+    const uint32_t* row = G + (size_t)b * SPLIT_IMAX;
     unsigned long long sum = 0;
     for (uint32_t q = threadIdx.x; q < i; q += TPB) {
       const uint32_t c = row[q];
@@ -1179,7 +1327,7 @@ __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t
     const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
     const bool i_in_S = (a.redg[io] & 1u) == 0u;
     if (!(MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S))) continue;
-    const Tab t{nullptr, 0, 0, a.cb, G + (size_t)lo * D32_MAX};
+    const Tab t{nullptr, 0, 0, a.cb, G + (size_t)lo * SPLIT_IMAX, 0};
     const uint32_t j1 = min(deg, (c + 1) * SPLIT_CHUNK);
     enum_pass<MODE, true, TM_G32>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
   }
@@ -1194,18 +1342,17 @@ __global__ void k_oidx(const uint32_t* __restrict__ redg, const uint32_t* __rest
     oidx[p] = (redg[p] & 1u) ? rpos[p] : eti[rpos[p]];
 }
 
-// For every slot (r -> y) with y > r: the position of r inside N(y), i.e. the number of
-// entries of N(y) ranked below r (a binary search over the in-prefix of N(y)).  The counting
-// kernel reads it beside the slot, keeping the search off its dependent-load chain.
-__global__ void k_lpos(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
-                       const uint32_t* __restrict__ rown, const uint32_t* __restrict__ inpref, int64_t m2,
-                       uint32_t* __restrict__ lpos) {
+// For every slot (r -> y): roff[y] and the segment length of the wedges (r, y, u2), i.e. the
+// number of entries of N(y) ranked below min(r, y): in(y) when y < r, else the position of r
+// inside N(y) (a binary search over the in-prefix of N(y)).  The counting kernels read both
+// beside the slot, which keeps two dependent loads and the search off their critical path.
+__global__ void k_slotinfo(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
+                           const uint32_t* __restrict__ rown, const uint32_t* __restrict__ inpref, int64_t m2,
+                           uint2* __restrict__ sinfo) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t r = rown[p], y = redg[p] >> 1;
-    if (y > r) {
-      const uint32_t ip = inpref[y];
-      lpos[p] = rank_lower_bound(redg + roff[y], ip < r ? ip : r, r);
-    }
+    const uint32_t yo = roff[y], ip = inpref[y];
+    sinfo[p] = make_uint2(yo, y < r ? ip : rank_lower_bound(redg + yo, ip < r ? ip : r, r));
   }
 }
 
@@ -1241,10 +1388,11 @@ template <int MODE>
 int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lists, uint32_t* ctl) {
   const int64_t n = R.n;
   // ctl: [0] light len, [1] heavy len, [2] big len, [3] heavy queue, [4] light-group queue,
-  //      [5] light groups, [6] HBM-fallback len, [7] total big chunks, [8] split len,
-  //      [9] total split items, [10] light groups on side S, [13] HBM queue
-  // lists: [0, n) light (classification order), [n, 2n) heavy, [2n, 3n) HBM fallback,
-  //        [3n, 4n) light grouped, [4n, 5n + 1) group starts
+  //      [5] light groups, [6] re-run len, [7] total big chunks, [8] split len,
+  //      [9] total split items, [10] light groups on side S, [11] HBM-fallback len,
+  //      [13] HBM queue, [14] zero, [16], [17] re-run queues
+  // lists: [0, n) light (classification order), [n, 2n) heavy, [2n, 3n) re-run,
+  //        [3n, 4n) light grouped, [4n, 5n + 1) group starts, [5n + 1, 6n + 1) HBM fallback
   void *bp2, *up, *pl;
   PBF_CHECK(pbf_scratch(ctx, S_BIG, 4 * (size_t)(3 * n + 64), &bp2));
   PBF_CHECK(pbf_scratch(ctx, S_UB, 8 * (size_t)(n ? n : 1), &up));
@@ -1252,25 +1400,31 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   uint32_t* big_chunks = big + n;
   uint32_t* chunk_base = big + 2 * n;
   unsigned long long* w64 = static_cast<unsigned long long*>(up);
-  hipLaunchKernelGGL(k_lpos, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.rown,
-                     R.inpref, R.m2, const_cast<uint32_t*>(a.lpos));
-  PBF_CHECK(pbf_scratch(ctx, S_LPLAN, 64 * 4 + LBUCKETS * 8, &pl));
+  hipLaunchKernelGGL(k_slotinfo, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.rown,
+                     R.inpref, R.m2, const_cast<uint2*>(a.sinfo));
+  PBF_CHECK(pbf_scratch(ctx, S_LPLAN, 4 * (64 + 2 * LG_BLOCKS * LBUCKETS) + 8 * LBUCKETS, &pl));
   uint32_t* plan = static_cast<uint32_t*>(pl);
-  unsigned long long* wbase = reinterpret_cast<unsigned long long*>(plan + 64);
+  uint32_t* bh = plan + 64;
+  void* lrp;
+  PBF_CHECK(pbf_scratch(ctx, S_LREC, 16 * (size_t)(n ? n : 1), &lrp));
+  uint4* lrec = static_cast<uint4*>(lrp);
+  uint32_t* bo = bh + LG_BLOCKS * LBUCKETS;
+  unsigned long long* wbase = reinterpret_cast<unsigned long long*>(bo + LG_BLOCKS * LBUCKETS);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctl, 0u, (int64_t)64);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, plan, 0u, (int64_t)64);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
-  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, lists + 2 * n, ctl, a.cb};
-  hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.lpos,
+  uint32_t* hbm_list = lists + 5 * n + 1;
+  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb};
+  hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
                      n, co, big, big_chunks, w64);
   PBF_CHECK(pbf_scan_excl_u32(ctx, big_chunks, chunk_base, n, ctl + 7));
-  hipLaunchKernelGGL(k_wedge_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.lpos,
+  hipLaunchKernelGGL(k_wedge_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
                      big, chunk_base, ctl, ctl + 7, w64);
   // split endpoints: list, chunk counts, chunk bases, dense rows
   void *sp, *gp;
-  const int64_t max_split = 2 * g_m2_over(R.m2, SPLIT_DEG) + 8;
+  const int64_t max_split = std::min<int64_t>(SPLIT_IMAX, 2 * g_m2_over(R.m2, SPLIT_DEG) + 8);
   PBF_CHECK(pbf_scratch(ctx, S_SPLIT, 4 * (size_t)(3 * max_split + 64), &sp));
-  PBF_CHECK(pbf_scratch(ctx, S_GROWS, 4 * (size_t)max_split * D32_MAX, &gp));
+  PBF_CHECK(pbf_scratch(ctx, S_GROWS, 4 * (size_t)max_split * SPLIT_IMAX, &gp));
   uint32_t* split = static_cast<uint32_t*>(sp);
   uint32_t* split_chunks = split + max_split;
   uint32_t* split_base = split + 2 * max_split;
@@ -1278,30 +1432,45 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split, 256, 64)), dim3(256), 0, ctx->stream, split_chunks, 0u, max_split);
   hipLaunchKernelGGL(k_classify_big, dim3(64), dim3(256), 0, ctx->stream, R.roff, big, ctl, w64, co, split, split_chunks);
   PBF_CHECK(pbf_scan_excl_u32(ctx, split_chunks, split_base, max_split, ctl + 9));
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split * D32_MAX, 256, 4096)), dim3(256), 0, ctx->stream, G, 0u,
-                     max_split * (int64_t)D32_MAX);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split * SPLIT_IMAX, 256, 4096)), dim3(256), 0, ctx->stream, G, 0u,
+                     max_split * (int64_t)SPLIT_IMAX);
   hipLaunchKernelGGL(k_split_pass1<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G);
   hipLaunchKernelGGL(k_split_credit<MODE>, dim3(256), dim3(TPB), 0, ctx->stream, a, split, ctl, G);
   if (MODE != MODE_TOTAL)
     hipLaunchKernelGGL(k_split_pass2<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G);
   // light endpoints: sort into groups
-  hipLaunchKernelGGL(k_light_hist<MODE>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound,
-                     lists, ctl, plan);
-  hipLaunchKernelGGL(k_light_plan, dim3(1), dim3(64), 0, ctx->stream, plan, wbase, ctl, lists + 4 * n);
-  hipLaunchKernelGGL(k_light_scatter<MODE>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, ctx->stream, R.roff, R.redg,
-                     a.bound, lists, ctl, plan, wbase, lists + 3 * n, lists + 4 * n);
+  hipLaunchKernelGGL(k_light_hist<MODE>, dim3(LG_BLOCKS), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound, lists, ctl,
+                     bh);
+  hipLaunchKernelGGL(k_light_plan, dim3(1), dim3(LG_BLOCKS), 0, ctx->stream, bh, bo, plan, wbase, ctl, lists + 4 * n);
+  hipLaunchKernelGGL(k_light_scatter<MODE>, dim3(LG_BLOCKS), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound, lists, ctl,
+                     bo, plan, wbase, lists + 3 * n, lrec, lists + 4 * n);
   a.wlist = lists + 3 * n;
   a.gstart = lists + 4 * n;
+  a.lrec = lrec;
   a.blist = lists + n;
-  a.lens = ctl;
+  a.nblock = ctl + 1;
+  a.ngroups = ctl + 5;
+  a.ngroups0 = ctl + 10;
   a.queues = ctl + 3;
-  a.ovf_list = lists + 2 * n;
-  a.ovf_len = ctl + 6;
-  hipLaunchKernelGGL(k_count2<MODE>, dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
+  a.ovf_list = hbm_list;
+  a.ovf_len = ctl + 11;
+  a.rerun_list = lists + 2 * n;
+  a.rerun_len = ctl + 6;
+  a.rerun = 0;
+  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
+  hipLaunchKernelGGL((k_count2<MODE, 2>), dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
+  // light endpoints whose optimistic table overflowed: again, one workgroup each, full tables
+  a.blist = lists + 2 * n;
+  a.nblock = ctl + 6;
+  a.ngroups = ctl + 14;
+  a.ngroups0 = ctl + 14;
+  a.queues = ctl + 16;
+  a.rerun = 1;
+  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
   PBF_HIP(hipGetLastError());
   if (MODE != MODE_EDGE && MODE != MODE_TOTAL)
     hipLaunchKernelGGL(k_hot_reduce, dim3(grid_for(HOT_N, 256, 64)), dim3(256), 0, ctx->stream, a.hot, a.B, n);
-  PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 6, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 11, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   const uint32_t novf = (uint32_t)(ctx->pinned[16] & 0xFFFFFFFFu);
   ctx->timing.hub_vertices = novf;
@@ -1312,7 +1481,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
     PBF_CHECK(pbf_scratch(ctx, S_POOL, sizeof(uint32_t) * 2 * (size_t)n * blocks, &pp));
     PBF_HIP(hipMemsetAsync(pp, 0, sizeof(uint32_t) * 2 * (size_t)n * blocks, ctx->stream));
     a.pool = static_cast<uint32_t*>(pp);
-    a.list = lists + 2 * n; a.list_len = ctl + 6; a.queue = ctl + 13;
+    a.list = hbm_list; a.list_len = ctl + 11; a.queue = ctl + 13;
     hipLaunchKernelGGL((k_count_hbm<MODE, HBM_TEAM>), dim3(blocks), dim3(HBM_TEAM), 0, ctx->stream, a);
     PBF_HIP(hipGetLastError());
   }
@@ -1325,16 +1494,16 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
   const int64_t n = R.n, m2 = R.m2;
   void *bp, *lp, *cp, *mp, *hp, *op = nullptr;
   PBF_CHECK(pbf_scratch(ctx, S_BOUND, 4 * (size_t)(n ? n : 1), &bp));
-  PBF_CHECK(pbf_scratch(ctx, S_BINS, 4 * (size_t)(5 * n + 128), &lp));
+  PBF_CHECK(pbf_scratch(ctx, S_BINS, 4 * (size_t)(6 * n + 128), &lp));
   PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
   PBF_CHECK(pbf_scratch(ctx, S_HOT, 8 * (size_t)HOT_N * HOT_R, &hp));
   void* lpp;
-  PBF_CHECK(pbf_scratch(ctx, S_LPOS, 4 * (size_t)(m2 ? m2 : 1), &lpp));
+  PBF_CHECK(pbf_scratch(ctx, S_LPOS, 8 * (size_t)(m2 ? m2 : 1), &lpp));
   uint32_t* lists = static_cast<uint32_t*>(lp);
-  uint32_t* ctl = lists + 5 * n + 64;
+  uint32_t* ctl = lists + 6 * n + 64;
   CountArgs a{};
   a.roff = R.roff; a.redg = R.redg; a.inpref = R.inpref; a.bound = static_cast<uint32_t*>(bp);
-  a.lpos = static_cast<uint32_t*>(lpp);
+  a.sinfo = static_cast<uint2*>(lpp);
   a.n = n;
   a.total = static_cast<unsigned long long*>(mp) + 12;
   a.hot = static_cast<unsigned long long*>(hp);

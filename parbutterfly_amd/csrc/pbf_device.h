@@ -42,9 +42,10 @@ enum ScratchSlot {
   S_RHIST,       // relabel: per-chunk key histograms / offsets
   S_SPLIT,       // split endpoints: list, chunk counts, chunk bases
   S_GROWS,       // split endpoints: dense count rows in HBM
-  S_LPOS,        // per-slot position of the owner inside the neighbour's list
+  S_LPOS,        // per slot (r -> y): roff[y] and the wedge segment length (uint2)
   S_ROWN,        // per-slot owner rank
   S_LPLAN,       // grouping plan of the light endpoints (bucket counts, starts, word bases)
+  S_LREC,        // grouped light endpoints: (i, roff[i], deg, log2 table size)
   S_NUM_SLOTS
 };
 
