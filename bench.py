@@ -12,7 +12,12 @@ over its duration measured with HIP events on the library's stream, against 8 TB
 cpu_baseline: the reference's own BATCHS path (oracle/_ref, compiled from the reference's
 headers) on the same graph, all host cores, median of its timed rounds.
 
+traffic: HBM bytes of one counting phase from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, each
+scaled by the same counter's reading for pbf_copy_device of a known size (--calibrate-copy),
+computed by parbutterfly_amd/profiling.py into profiles/traffic_count_chunglu_2p24.json.
+
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--traffic JSON]
+    python bench.py --calibrate-copy BYTES     # one device copy of BYTES (counter calibration)
 """
 import argparse
 import json
@@ -65,13 +70,24 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_count_chunglu_2p24.json"),
+                    help="JSON with measured HBM bytes per counting phase")
+    ap.add_argument("--calibrate-copy", type=int, default=0, help="run one device copy of this many bytes and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     import parbutterfly_amd as pb
+
+    if args.calibrate_copy:
+        ctx = pb.Context(0)
+        nb = args.calibrate_copy
+        a = pb.DeviceArray(ctx, nb, np.uint8)
+        b = pb.DeviceArray(ctx, nb, np.uint8)
+        b.copy_from(a)
+        print(json.dumps({"calibrate_copy_bytes": nb}), flush=True)
+        return
 
     t0 = time.time()
     g = pb.Graph.chung_lu(CFG["nv"], CFG["nu"], CFG["m"], CFG["gamma"], CFG["gamma"], CFG["seed"])
@@ -132,7 +148,7 @@ def main():
                    "wedges": W, "use_v": use_v, "n_S": ns},
         "phases_ms": {k: round(float(np.mean([p[k] for p in phases])), 4)
                       for k in ("rank_ms", "relabel_ms", "count_ms", "total_ms")},
-        "roofline": {"bound": "hbm", "kernel": "count phase (k_bin + k_count)", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": "counting phase (k_slotinfo ... k_count2 ... k_convert_vert)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "b_alg_bytes": balg, "count_ms": round(count_ms, 4)},
         "host_inclusive_gwedges_s": round(W / h2d_s / 1e9, 3),

@@ -147,6 +147,11 @@ int pbf_device_alloc(pbf_context* ctx, size_t bytes, void** ptr);
 int pbf_device_free(pbf_context* ctx, void* ptr);
 int pbf_copy_to_device(pbf_context* ctx, void* dst, const void* src, size_t bytes);
 int pbf_copy_to_host(pbf_context* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by the library's own streaming kernel (16-byte vector loads and stores
+ * over the whole device); synchronous; dst and src 16-byte aligned (else PBF_ERR_ARG).  It
+ * moves exactly `bytes` in each direction, which makes it the reference for calibrating
+ * hardware traffic counters. */
+int pbf_copy_device(pbf_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* cmpWedgeCounts (butterfly_utils.h:1061-1086) on device or host graphs. */
 int pbf_side(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int* use_v, int64_t* num_wedges);

@@ -111,6 +111,7 @@ _SIGS = {
     "pbf_device_free": (_INT, [_P, _P]),
     "pbf_copy_to_device": (_INT, [_P, _P, _P, ctypes.c_size_t]),
     "pbf_copy_to_host": (_INT, [_P, _P, _P, ctypes.c_size_t]),
+    "pbf_copy_device": (_INT, [_P, _P, _P, ctypes.c_size_t]),
     "pbf_side": (_INT, [_P, _PCSR, _INT, _PI, _PI64]),
     "pbf_rank": (_INT, [_P, _PCSR, _INT, _INT, _P, _P, _PI64]),
     "pbf_count_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
@@ -283,6 +284,12 @@ class DeviceArray:
         if self.nbytes:
             _check(_lib.pbf_copy_to_host(self.ctx._ctx, out.ctypes.data, self.ptr, self.nbytes), "pbf_copy_to_host")
         return out
+
+    def copy_from(self, other):
+        """Device-to-device copy of another DeviceArray of the same size."""
+        assert other.nbytes == self.nbytes
+        if self.nbytes:
+            _check(_lib.pbf_copy_device(self.ctx._ctx, self.ptr, other.ptr, self.nbytes), "pbf_copy_device")
 
     def free(self):
         if self.ptr and self.ctx._ctx:

@@ -1,6 +1,7 @@
 // C ABI of libparbutterfly_amd: context management and the count / rank / peel entry points.
 // Each entry point mirrors a reference function (cited in include/parbutterfly.h) and returns
 // a pbf_status code instead of printing and exiting.
+#include <algorithm>
 #include <cstring>
 
 #include "pbf_device.h"
@@ -230,6 +231,32 @@ extern "C" int pbf_copy_to_device(pbf_context* ctx, void* dst, const void* src, 
   if (!ctx || (bytes && (!dst || !src))) return PBF_ERR_ARG;
   DeviceGuard guard(ctx->device);
   if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  return PBF_OK;
+}
+
+namespace {
+// Streaming copy: 16-byte vectors, grid-stride; the tail (bytes % 16) is copied bytewise.
+__global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16,
+                                              uint8_t* __restrict__ dtail, const uint8_t* __restrict__ stail, size_t ntail) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) dst[k] = src[k];
+  if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+}  // namespace
+
+extern "C" int pbf_copy_device(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return PBF_ERR_ARG;
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  if (bytes) {
+    const size_t n16 = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n16 + 255) / 256 + 1, (size_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, ctx->stream, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), n16, static_cast<uint8_t*>(dst) + n16 * 16,
+                       static_cast<const uint8_t*>(src) + n16 * 16, bytes - n16 * 16);
+    PBF_HIP(hipGetLastError());
+  }
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }
