@@ -183,6 +183,33 @@ int pbf_peel_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use
 int pbf_peel_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
                   int peel_type, const uint64_t* counts, uint64_t* wing);
 
+/* ---------------- batches of graphs (no counterpart in the reference) ----------------
+ * Per-vertex (VERT) and total butterfly counts of `count` independent graphs, each exactly as
+ * pbf_count_vert / pbf_count_total would give them with the graph's own side
+ * (cmpWedgeCounts).  graphs: host array of descriptors whose arrays are in `mem`.
+ * flags: PBF_BATCH_UNION runs one pass of the kernels over the disjoint union of the graphs;
+ * PBF_BATCH_PLAIN runs the per-graph pipeline graph by graph (the baseline);
+ * PBF_BATCH_UNION | PBF_BATCH_CAPTURE (device graphs and outputs only) also captures the pass
+ * in a HIP graph on the context's stream and replays it on later calls with the same graphs,
+ * outputs, rank type and tuning (one captured batch per context).
+ * Outputs (in out_mem): vert_out holds graph g's counts on its side S at
+ * [vert_off[g], vert_off[g+1]) (vert_cap >= sum over graphs of max(nv, nu)); vert_off has
+ * count + 1 entries; totals[g] = butterflies of graph g.  Synchronous. */
+enum { PBF_BATCH_UNION = 0, PBF_BATCH_PLAIN = 1, PBF_BATCH_CAPTURE = 16 };
+int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
+                    int rank_type, int flags, uint64_t* vert_out, int64_t vert_cap,
+                    uint64_t* vert_off, uint64_t* totals, int out_mem);
+/* Releases the context's captured batch graph (pbf_destroy does it too). */
+void pbf_batch_release(pbf_context* ctx);
+/* Autotuner of the counting kernels for a batch: times PBF_BATCH_UNION under each candidate
+ * setting of (light-endpoint key bound, light-endpoint wedge bound, workgroups per CU), keeps
+ * the fastest in the context and returns its index and time.  Arguments as pbf_count_batch. */
+int pbf_batch_autotune(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
+                       int rank_type, uint64_t* vert_out, int64_t vert_cap, uint64_t* vert_off,
+                       uint64_t* totals, int out_mem, int* best, float* best_ms);
+/* Sets the counting kernels' knobs directly (light_max <= 1535, 1 <= blocks_per_cu <= 8). */
+int pbf_set_tuning(pbf_context* ctx, int light_max, int light_work, int blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,6 +120,11 @@ _SIGS = {
     "pbf_edge_to_idxs": (_INT, [_P, _PCSR, _INT, _INT, _P]),
     "pbf_peel_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
     "pbf_peel_edge": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
+    "pbf_count_batch": (_INT, [_P, _I64, _PCSR, _INT, _INT, _INT, _P, _I64, _P, _P, _INT]),
+    "pbf_batch_release": (None, [_P]),
+    "pbf_batch_autotune": (_INT, [_P, _I64, _PCSR, _INT, _INT, _P, _I64, _P, _P, _INT, _PI,
+                                  ctypes.POINTER(ctypes.c_float)]),
+    "pbf_set_tuning": (_INT, [_P, _INT, _INT, _INT]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -443,6 +448,65 @@ class Context:
                                   _enum(PEEL_TYPES, peel_type, "peel type"), self._ptr(counts) if g.m else 1,
                                   self._ptr(wing) if g.m else 1), "pbf_peel_edge")
         return wing
+
+    # ---- batches (no counterpart in the reference) ----
+    BATCH_UNION, BATCH_PLAIN, BATCH_CAPTURE = 0, 1, 16
+
+    def _batch_args(self, graphs):
+        mems = {self._mem(g) for g in graphs}
+        if len(mems) > 1:
+            raise ValueError("a batch is either all host graphs or all device graphs")
+        mem = mems.pop() if mems else MEM_HOST
+        arr = (_CSR * max(len(graphs), 1))()
+        for k, g in enumerate(graphs):
+            arr[k] = g._csr
+        cap = sum(max(g.nv, g.nu) for g in graphs)
+        return mem, arr, cap
+
+    def count_batch(self, graphs, rank_type="ADEG", mode="union", capture=False, outputs=None):
+        """Per-vertex counts and totals of many graphs.  mode: "union" (one pass over the
+        disjoint union) or "plain" (graph by graph).  Host graphs give numpy results
+        (list of per-graph uint64 arrays, uint64 totals); device graphs give
+        (DeviceArray vert, DeviceArray offsets, DeviceArray totals), reusing `outputs` (a
+        previous such triple) when given, which a captured batch needs for its replays."""
+        mem, arr, cap = self._batch_args(graphs)
+        flags = {"union": self.BATCH_UNION, "plain": self.BATCH_PLAIN}[mode] | (self.BATCH_CAPTURE if capture else 0)
+        rt = _enum(RANK_TYPES, rank_type, "rank type")
+        cnt = len(graphs)
+        if mem == MEM_DEVICE:
+            if outputs is None:
+                outputs = (DeviceArray(self, max(cap, 1), np.uint64), DeviceArray(self, cnt + 1, np.uint64),
+                           DeviceArray(self, max(cnt, 1), np.uint64))
+            vout, voff, tot = outputs
+            _check(_lib.pbf_count_batch(self._ctx, cnt, arr, mem, rt, flags, vout.ptr, vout.n, voff.ptr, tot.ptr,
+                                        MEM_DEVICE), "pbf_count_batch")
+            return outputs
+        vout = np.zeros(max(cap, 1), np.uint64)
+        voff = np.zeros(cnt + 1, np.uint64)
+        tot = np.zeros(max(cnt, 1), np.uint64)
+        _check(_lib.pbf_count_batch(self._ctx, cnt, arr, mem, rt, flags, vout.ctypes.data, len(vout), voff.ctypes.data,
+                                    tot.ctypes.data, MEM_HOST), "pbf_count_batch")
+        return [vout[int(voff[k]):int(voff[k + 1])] for k in range(cnt)], tot[:cnt]
+
+    def batch_autotune(self, graphs, rank_type="ADEG"):
+        """Runs the batch autotuner on these graphs; returns (best candidate index, ms)."""
+        mem, arr, cap = self._batch_args(graphs)
+        cnt = len(graphs)
+        if mem == MEM_DEVICE:
+            vout, voff, tot = (DeviceArray(self, max(cap, 1), np.uint64), DeviceArray(self, cnt + 1, np.uint64),
+                               DeviceArray(self, max(cnt, 1), np.uint64))
+            ptrs = (vout.ptr, voff.ptr, tot.ptr)
+        else:
+            vout, voff, tot = np.zeros(max(cap, 1), np.uint64), np.zeros(cnt + 1, np.uint64), np.zeros(max(cnt, 1), np.uint64)
+            ptrs = (vout.ctypes.data, voff.ctypes.data, tot.ctypes.data)
+        best, ms = ctypes.c_int(), ctypes.c_float()
+        _check(_lib.pbf_batch_autotune(self._ctx, cnt, arr, mem, _enum(RANK_TYPES, rank_type, "rank type"), ptrs[0],
+                                       max(cap, 1), ptrs[1], ptrs[2], mem, ctypes.byref(best), ctypes.byref(ms)),
+               "pbf_batch_autotune")
+        return best.value, ms.value
+
+    def set_tuning(self, light_max=1535, light_work=1 << 14, blocks_per_cu=4):
+        _check(_lib.pbf_set_tuning(self._ctx, light_max, light_work, blocks_per_cu), "pbf_set_tuning")
 
     def _as_u64(self, g, counts):
         if isinstance(g, DeviceGraph):

@@ -178,12 +178,14 @@ extern "C" int pbf_trim(pbf_context* ctx) {
   if (!ctx) return PBF_ERR_ARG;
   DeviceGuard guard(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  pbf_batch_release(ctx);  // a captured batch refers to the scratch buffers
   for (int s = 0; s < S_NUM_SLOTS; s++) {
     if (ctx->buf[s]) (void)hipFree(ctx->buf[s]);
     ctx->buf[s] = nullptr;
     ctx->cap[s] = 0;
   }
   ctx->hot_clear = false;
+  ctx->pool_clean = 0;
   return PBF_OK;
 }
 

@@ -1,0 +1,410 @@
+// Batched counting of many independent bipartite graphs (no counterpart in the reference; the
+// per-graph results equal those of Count / CountRank, butterfly_count.h:1561-1665, graph by
+// graph).
+//
+// PBF_BATCH_UNION: the graphs are laid out as one disjoint union -- V ids of all graphs first,
+// then all U ids, each graph keeping its own order -- with a per-vertex flag for the count side
+// S of its graph (cmpWedgeCounts per graph, butterfly_utils.h:1061-1086).  One pass of the
+// ranking, relabel and counting kernels then serves every graph: wedges never cross graphs,
+// the global ranking restricted to a graph is that graph's own ranking (ties by ascending id,
+// V before U), and the persistent work-queue kernels balance the uneven graphs among
+// themselves.  Per-graph outputs are gathered at the end.  With PBF_BATCH_CAPTURE the pass is
+// captured in a HIP graph (no host synchronisation, no allocation, no memset inside) and
+// replayed by identical calls.
+// PBF_BATCH_PLAIN: the same kernels, launched graph by graph from a host loop (the baseline).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pbf_device.h"
+
+namespace {
+struct BGraph {
+  const uint32_t* offV;
+  const uint32_t* edgV;
+  const uint32_t* offU;
+  const uint32_t* edgU;
+  uint32_t nv, nu, m, pad;
+  uint64_t vbase, ubase, ebase;  // union: V ids vbase.., U ids (within the U side) ubase.., edges ebase..
+};
+
+// One workgroup per graph: union offsets and edges (neighbour ids shifted to the union).
+__global__ __launch_bounds__(256) void k_union_build(const BGraph* __restrict__ gs, int64_t count, uint32_t* __restrict__ offV,
+                                                     uint32_t* __restrict__ edgV, uint32_t* __restrict__ offU,
+                                                     uint32_t* __restrict__ edgU, int64_t NV, int64_t NU, int64_t M) {
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const BGraph b = gs[g];
+    const uint32_t eb = (uint32_t)b.ebase, ub = (uint32_t)b.ubase, vb = (uint32_t)b.vbase;
+    for (uint32_t v = threadIdx.x; v < b.nv; v += blockDim.x) offV[vb + v] = eb + b.offV[v];
+    for (uint32_t u = threadIdx.x; u < b.nu; u += blockDim.x) offU[ub + u] = eb + b.offU[u];
+    for (uint32_t e = threadIdx.x; e < b.m; e += blockDim.x) {
+      edgV[eb + e] = ub + b.edgV[e];
+      edgU[eb + e] = vb + b.edgU[e];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    offV[NV] = (uint32_t)M;
+    offU[NU] = (uint32_t)M;
+  }
+}
+
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long x, unsigned long long* sm) {
+  x = pbf_wave_sum(x);
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sm[w] = x;
+  __syncthreads();
+  unsigned long long t = 0;
+  for (uint32_t k = 0; k < (blockDim.x >> 6); k++) t += sm[k];
+  __syncthreads();
+  return t;
+}
+
+// One workgroup per graph: wV, wU (cmpWedgeCounts), the side choice, the per-vertex S flags of
+// the union and |S|.
+__global__ __launch_bounds__(256) void k_batch_side(const BGraph* __restrict__ gs, int64_t count, int64_t NV,
+                                                    uint8_t* __restrict__ sflag, unsigned long long* __restrict__ sn,
+                                                    uint8_t* __restrict__ usev) {
+  __shared__ unsigned long long sm[4];
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const BGraph b = gs[g];
+    unsigned long long wv = 0, wu = 0;
+    for (uint32_t v = threadIdx.x; v < b.nv; v += blockDim.x) {
+      const unsigned long long d = b.offV[v + 1] - b.offV[v];
+      wv += d * (d - (d ? 1 : 0)) / 2;
+    }
+    for (uint32_t u = threadIdx.x; u < b.nu; u += blockDim.x) {
+      const unsigned long long d = b.offU[u + 1] - b.offU[u];
+      wu += d * (d - (d ? 1 : 0)) / 2;
+    }
+    wv = block_sum(wv, sm);
+    wu = block_sum(wu, sm);
+    const bool use_v = wv <= wu;  // ties count on U
+    for (uint32_t v = threadIdx.x; v < b.nv; v += blockDim.x) sflag[b.vbase + v] = use_v ? 0 : 1;
+    for (uint32_t u = threadIdx.x; u < b.nu; u += blockDim.x) sflag[NV + b.ubase + u] = use_v ? 1 : 0;
+    if (threadIdx.x == 0) {
+      sn[g] = use_v ? b.nu : b.nv;
+      usev[g] = use_v ? 1 : 0;
+    }
+  }
+}
+
+// One workgroup per graph: CountRank's conversion (out[x] = B[rank(x)] for x in S) into the
+// graph's output segment, and the graph's total = sum over S / 2.
+__global__ __launch_bounds__(256) void k_batch_out(const BGraph* __restrict__ gs, int64_t count, int64_t NV,
+                                                   const uint8_t* __restrict__ usev, const unsigned long long* __restrict__ voff,
+                                                   const unsigned long long* __restrict__ B, const uint32_t* __restrict__ rank_of,
+                                                   unsigned long long* __restrict__ out, unsigned long long* __restrict__ totals) {
+  __shared__ unsigned long long sm[4];
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const BGraph b = gs[g];
+    const bool use_v = usev[g] != 0;
+    const uint32_t ns = use_v ? b.nu : b.nv;
+    const uint64_t base = use_v ? (uint64_t)NV + b.ubase : b.vbase;
+    unsigned long long* o = out + voff[g];
+    unsigned long long s = 0;
+    for (uint32_t x = threadIdx.x; x < ns; x += blockDim.x) {
+      const unsigned long long c = B[rank_of[base + x]];
+      o[x] = c;
+      s += c;
+    }
+    s = block_sum(s, sm);
+    if (threadIdx.x == 0) totals[g] = s / 2;
+  }
+}
+
+// Plain mode: total of one graph from its per-vertex counts.
+__global__ __launch_bounds__(256) void k_total_of(const unsigned long long* __restrict__ v, int64_t n,
+                                                  unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long sm[4];
+  unsigned long long s = 0;
+  for (int64_t x = threadIdx.x; x < n; x += blockDim.x) s += v[x];
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) *total = s / 2;
+}
+
+struct BatchState {
+  uint64_t sig = 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+uint64_t mix(uint64_t h, uint64_t x) {
+  h ^= x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h;
+}
+
+float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) { (void)hipGetLastError(); return 0; }
+  return ms;
+}
+
+// Everything after the table upload: device work only, no synchronisation (capturable).
+int union_pass(pbf_context* ctx, const BGraph* tab, int64_t count, int64_t NV, int64_t NU, int64_t M, int rank_type,
+               uint32_t* ubuf, uint8_t* sflag, unsigned long long* sn, unsigned long long* voff, uint8_t* usev,
+               unsigned long long* vout, unsigned long long* totals) {
+  DevGraph u;
+  u.nv = NV; u.nu = NU; u.m = M;
+  uint32_t* offV = ubuf;
+  uint32_t* edgV = offV + NV + 1;
+  uint32_t* offU = edgV + M;
+  uint32_t* edgU = offU + NU + 1;
+  u.offV = offV; u.edgV = edgV; u.offU = offU; u.edgU = edgU;
+  const unsigned gb = (unsigned)std::min<int64_t>(count, (int64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_union_build, dim3(gb), dim3(256), 0, ctx->stream, tab, count, offV, edgV, offU, edgU, NV, NU, M);
+  hipLaunchKernelGGL(k_batch_side, dim3(gb), dim3(256), 0, ctx->stream, tab, count, NV, sflag, sn, usev);
+  PBF_HIP(hipGetLastError());
+  RankedGraph R{};
+  PBF_CHECK(pbf_dev_rank(ctx, u, rank_type, &R));
+  PBF_CHECK(pbf_dev_relabel(ctx, u, 0, false, &R, sflag, false));
+  CountOpts opts;
+  opts.sync = false;
+  opts.rank_space = true;
+  unsigned long long* B = nullptr;
+  opts.B_out = &B;
+  PBF_CHECK(pbf_dev_count(ctx, u, 0, R, MODE_VERT, nullptr, nullptr, opts));
+  PBF_CHECK(pbf_scan_excl_u64(ctx, reinterpret_cast<const uint64_t*>(sn), reinterpret_cast<uint64_t*>(voff), count,
+                              reinterpret_cast<uint64_t*>(voff + count)));
+  hipLaunchKernelGGL(k_batch_out, dim3(gb), dim3(256), 0, ctx->stream, tab, count, NV, usev, voff, B, R.rank_of, vout, totals);
+  PBF_HIP(hipGetLastError());
+  return PBF_OK;
+}
+}  // namespace
+
+extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem, int rank_type,
+                               int flags, uint64_t* vert_out, int64_t vert_cap, uint64_t* vert_off, uint64_t* totals,
+                               int out_mem) {
+  if (!ctx || count < 0 || (count > 0 && (!graphs || !vert_out || !vert_off || !totals))) return PBF_ERR_ARG;
+  if (rank_type != PBF_RANK_ADEG && rank_type != PBF_RANK_ACOCORE) return PBF_ERR_UNSUPPORTED;
+  if (mem != PBF_MEM_DEVICE && mem != PBF_MEM_HOST) return PBF_ERR_ARG;
+  if (out_mem != PBF_MEM_DEVICE && out_mem != PBF_MEM_HOST) return PBF_ERR_ARG;
+  const int kind = flags & 15;
+  if (kind != PBF_BATCH_UNION && kind != PBF_BATCH_PLAIN) return PBF_ERR_ARG;
+  const bool capture = (flags & PBF_BATCH_CAPTURE) != 0;
+  if (capture && (kind != PBF_BATCH_UNION || mem != PBF_MEM_DEVICE || out_mem != PBF_MEM_DEVICE)) return PBF_ERR_ARG;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+  if (prev != ctx->device) (void)hipSetDevice(ctx->device);
+  struct Restore {
+    int p, d;
+    ~Restore() { if (p >= 0 && p != d) (void)hipSetDevice(p); }
+  } restore{prev, ctx->device};
+
+  // sizes and union layout (host side)
+  std::vector<BGraph> tab((size_t)count);
+  int64_t NV = 0, NU = 0, M = 0, cap_need = 0;
+  for (int64_t g = 0; g < count; g++) {
+    const pbf_bipartite_csr& d = graphs[g];
+    if (d.nv < 0 || d.nu < 0 || d.m < 0 || !d.offsetsV || !d.offsetsU || (d.m && (!d.edgesV || !d.edgesU)))
+      return PBF_ERR_ARG;
+    if (mem == PBF_MEM_HOST) PBF_CHECK(pbf_validate(&d));
+    tab[g] = BGraph{d.offsetsV, d.edgesV, d.offsetsU, d.edgesU, (uint32_t)d.nv, (uint32_t)d.nu, (uint32_t)d.m, 0,
+                    (uint64_t)NV, (uint64_t)NU, (uint64_t)M};
+    NV += d.nv; NU += d.nu; M += d.m;
+    cap_need += std::max(d.nv, d.nu);
+  }
+  if (NV + NU >= (int64_t(1) << 31) || 2 * M >= (int64_t(1) << 31)) return PBF_ERR_CAPACITY;
+  if (vert_cap < cap_need) return PBF_ERR_ARG;
+  memset(&ctx->timing, 0, sizeof(ctx->timing));
+  hipEvent_t* ev = ctx->ev;
+  if (count == 0) return PBF_OK;
+
+  // output staging for host outputs
+  unsigned long long *vout = reinterpret_cast<unsigned long long*>(vert_out), *voff = nullptr,
+                     *tot = reinterpret_cast<unsigned long long*>(totals);
+  void* op = nullptr;
+  if (out_mem == PBF_MEM_HOST) {
+    PBF_CHECK(pbf_scratch(ctx, S_OUT, 8 * (size_t)(cap_need + 2 * count + 1), &op));
+    vout = static_cast<unsigned long long*>(op);
+    tot = vout + cap_need;
+  }
+
+  if (kind == PBF_BATCH_PLAIN) {
+    std::vector<uint64_t> off((size_t)count + 1, 0);
+    PBF_HIP(hipEventRecord(ev[0], ctx->stream));
+    for (int64_t g = 0; g < count; g++) {
+      const pbf_bipartite_csr& d = graphs[g];
+      DevGraph dg;
+      dg.nv = d.nv; dg.nu = d.nu; dg.m = d.m;
+      if (mem == PBF_MEM_DEVICE) {
+        dg.offV = d.offsetsV; dg.edgV = d.edgesV; dg.offU = d.offsetsU; dg.edgU = d.edgesU;
+      } else {
+        const size_t words = size_t(d.nv + 1) + size_t(d.m) + size_t(d.nu + 1) + size_t(d.m);
+        void* p;
+        PBF_CHECK(pbf_scratch(ctx, S_GRAPH, 4 * words, &p));
+        uint32_t* w = static_cast<uint32_t*>(p);
+        PBF_HIP(hipMemcpyAsync(w, d.offsetsV, 4 * size_t(d.nv + 1), hipMemcpyHostToDevice, ctx->stream));
+        PBF_HIP(hipMemcpyAsync(w + d.nv + 1, d.edgesV, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+        PBF_HIP(hipMemcpyAsync(w + d.nv + 1 + d.m, d.offsetsU, 4 * size_t(d.nu + 1), hipMemcpyHostToDevice, ctx->stream));
+        PBF_HIP(hipMemcpyAsync(w + d.nv + 1 + d.m + d.nu + 1, d.edgesU, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+        dg.offV = w; dg.edgV = w + d.nv + 1; dg.offU = w + d.nv + 1 + d.m; dg.edgU = w + d.nv + 1 + d.m + d.nu + 1;
+      }
+      int use_v;
+      int64_t wsum;
+      PBF_CHECK(pbf_dev_side(ctx, dg, &use_v, &wsum));
+      const int64_t ns = use_v ? d.nu : d.nv;
+      off[g + 1] = off[g] + ns;
+      RankedGraph R{};
+      PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R));
+      PBF_CHECK(pbf_dev_relabel(ctx, dg, use_v, false, &R));
+      PBF_CHECK(pbf_dev_count(ctx, dg, use_v, R, MODE_VERT, reinterpret_cast<uint64_t*>(vout + off[g]), nullptr));
+      hipLaunchKernelGGL(k_total_of, dim3(1), dim3(256), 0, ctx->stream, vout + off[g], ns, tot + g);
+      PBF_HIP(hipGetLastError());
+    }
+    PBF_HIP(hipEventRecord(ev[4], ctx->stream));
+    if (out_mem == PBF_MEM_HOST) {
+      PBF_HIP(hipMemcpyAsync(vert_out, vout, 8 * (size_t)off[count], hipMemcpyDeviceToHost, ctx->stream));
+      PBF_HIP(hipMemcpyAsync(totals, tot, 8 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+      PBF_HIP(hipStreamSynchronize(ctx->stream));
+      memcpy(vert_off, off.data(), 8 * (size_t)(count + 1));
+    } else {
+      PBF_HIP(hipMemcpyAsync(vert_off, off.data(), 8 * (size_t)(count + 1), hipMemcpyHostToDevice, ctx->stream));
+      PBF_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->timing.total_ms = elapsed_ms(ev[0], ev[4]);
+    ctx->timing.count_ms = ctx->timing.total_ms;
+    return PBF_OK;
+  }
+
+  // union: staging of host graphs, the table, the union CSR, flags and per-graph scalars
+  if (mem == PBF_MEM_HOST) {
+    size_t words = 0;
+    for (int64_t g = 0; g < count; g++) words += size_t(tab[g].nv + 1) + size_t(tab[g].nu + 1) + 2 * size_t(tab[g].m);
+    void* sp;
+    PBF_CHECK(pbf_scratch(ctx, S_BSTAGE, 4 * words, &sp));
+    uint32_t* w = static_cast<uint32_t*>(sp);
+    for (int64_t g = 0; g < count; g++) {
+      const pbf_bipartite_csr& d = graphs[g];
+      BGraph& b = tab[g];
+      PBF_HIP(hipMemcpyAsync(w, d.offsetsV, 4 * size_t(d.nv + 1), hipMemcpyHostToDevice, ctx->stream));
+      b.offV = w; w += d.nv + 1;
+      PBF_HIP(hipMemcpyAsync(w, d.edgesV, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      b.edgV = w; w += d.m;
+      PBF_HIP(hipMemcpyAsync(w, d.offsetsU, 4 * size_t(d.nu + 1), hipMemcpyHostToDevice, ctx->stream));
+      b.offU = w; w += d.nu + 1;
+      PBF_HIP(hipMemcpyAsync(w, d.edgesU, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      b.edgU = w; w += d.m;
+    }
+  }
+  void *tp, *up, *fp, *ap;
+  PBF_CHECK(pbf_scratch(ctx, S_BTAB, sizeof(BGraph) * (size_t)count, &tp));
+  PBF_CHECK(pbf_scratch(ctx, S_BUNION, 4 * (size_t)(NV + 1 + NU + 1 + 2 * M), &up));
+  PBF_CHECK(pbf_scratch(ctx, S_BFLAG, (size_t)(NV + NU + 1), &fp));
+  PBF_CHECK(pbf_scratch(ctx, S_BAUX, 8 * (size_t)(3 * count + 2), &ap));
+  unsigned long long* sn = static_cast<unsigned long long*>(ap);
+  unsigned long long* offd = sn + count;  // count + 1 entries
+  uint8_t* usev = reinterpret_cast<uint8_t*>(offd + count + 1);
+  if (out_mem == PBF_MEM_DEVICE) offd = reinterpret_cast<unsigned long long*>(vert_off);
+  BGraph* dtab = static_cast<BGraph*>(tp);
+
+  uint64_t sig = 0;
+  if (capture) {
+    sig = mix(sig, (uint64_t)count);
+    for (const BGraph& b : tab) {
+      sig = mix(sig, reinterpret_cast<uint64_t>(b.offV)); sig = mix(sig, reinterpret_cast<uint64_t>(b.edgV));
+      sig = mix(sig, reinterpret_cast<uint64_t>(b.offU)); sig = mix(sig, reinterpret_cast<uint64_t>(b.edgU));
+      sig = mix(sig, ((uint64_t)b.nv << 32) | b.nu); sig = mix(sig, b.m);
+    }
+    sig = mix(sig, reinterpret_cast<uint64_t>(vout)); sig = mix(sig, reinterpret_cast<uint64_t>(offd));
+    sig = mix(sig, reinterpret_cast<uint64_t>(tot)); sig = mix(sig, (uint64_t)rank_type);
+    sig = mix(sig, ((uint64_t)ctx->tune.light_max << 40) ^ ((uint64_t)ctx->tune.light_work << 8) ^ ctx->tune.blocks_per_cu);
+    sig = mix(sig, reinterpret_cast<uint64_t>(up)); sig = mix(sig, reinterpret_cast<uint64_t>(ctx->buf[S_BINS]));
+    BatchState* st = static_cast<BatchState*>(ctx->batch);
+    if (st && st->exec && st->sig == sig) {
+      PBF_HIP(hipEventRecord(ev[0], ctx->stream));
+      PBF_HIP(hipGraphLaunch(st->exec, ctx->stream));
+      PBF_HIP(hipEventRecord(ev[4], ctx->stream));
+      PBF_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->timing.total_ms = elapsed_ms(ev[0], ev[4]);
+      ctx->timing.count_ms = ctx->timing.total_ms;
+      ctx->timing.wedges = -1;
+      return PBF_OK;
+    }
+  }
+  PBF_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BGraph) * (size_t)count, hipMemcpyHostToDevice, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));  // the table's host copy is released below
+  PBF_HIP(hipEventRecord(ev[0], ctx->stream));
+  PBF_CHECK(union_pass(ctx, dtab, count, NV, NU, M, rank_type, static_cast<uint32_t*>(up), static_cast<uint8_t*>(fp), sn,
+                       offd, usev, vout, tot));
+  PBF_HIP(hipEventRecord(ev[4], ctx->stream));
+  if (out_mem == PBF_MEM_HOST) {
+    std::vector<uint64_t> off((size_t)count + 1);
+    PBF_HIP(hipMemcpyAsync(off.data(), offd, 8 * (size_t)(count + 1), hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    PBF_HIP(hipMemcpyAsync(vert_out, vout, 8 * (size_t)off[count], hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipMemcpyAsync(totals, tot, 8 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(vert_off, off.data(), 8 * (size_t)(count + 1));
+  } else {
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->timing.total_ms = elapsed_ms(ev[0], ev[4]);
+  ctx->timing.count_ms = ctx->timing.total_ms;
+  ctx->timing.wedges = -1;
+  if (capture) {
+    // the eager pass sized every scratch buffer: capture the same pass for replays
+    BatchState* st = static_cast<BatchState*>(ctx->batch);
+    if (!st) { st = new BatchState(); ctx->batch = st; }
+    if (st->exec) { (void)hipGraphExecDestroy(st->exec); st->exec = nullptr; }
+    if (st->graph) { (void)hipGraphDestroy(st->graph); st->graph = nullptr; }
+    PBF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    const int rc = union_pass(ctx, dtab, count, NV, NU, M, rank_type, static_cast<uint32_t*>(up), static_cast<uint8_t*>(fp),
+                              sn, offd, usev, vout, tot);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != PBF_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (ec != hipSuccess) return PBF_ERR_HIP;
+    st->graph = graph;
+    PBF_HIP(hipGraphInstantiate(&st->exec, graph, nullptr, nullptr, 0));
+    st->sig = sig;
+  }
+  return PBF_OK;
+}
+
+extern "C" void pbf_batch_release(pbf_context* ctx) {
+  if (!ctx || !ctx->batch) return;
+  BatchState* st = static_cast<BatchState*>(ctx->batch);
+  if (st->exec) (void)hipGraphExecDestroy(st->exec);
+  if (st->graph) (void)hipGraphDestroy(st->graph);
+  delete st;
+  ctx->batch = nullptr;
+}
+
+// Autotuner: times the union pass under each candidate setting of the counting kernels' knobs
+// and keeps the fastest in the context.
+extern "C" int pbf_batch_autotune(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem, int rank_type,
+                                  uint64_t* vert_out, int64_t vert_cap, uint64_t* vert_off, uint64_t* totals, int out_mem,
+                                  int* best, float* best_ms) {
+  if (!ctx) return PBF_ERR_ARG;
+  static const CountTune cands[] = {
+      {1535, 1u << 14, 4}, {1535, 1u << 14, 2}, {767, 1u << 14, 4}, {767, 1u << 13, 4},
+      {383, 1u << 12, 4},  {1535, 1u << 12, 4}, {767, 1u << 13, 2}, {1535, 1u << 15, 4},
+  };
+  const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
+  int bi = 0;
+  float bms = 1e30f;
+  const CountTune saved = ctx->tune;
+  for (int c = 0; c < nc; c++) {
+    ctx->tune = cands[c];
+    float ms = 0;
+    for (int rep = 0; rep < 2; rep++) {  // first run warms the scratch
+      const int rc = pbf_count_batch(ctx, count, graphs, mem, rank_type, PBF_BATCH_UNION, vert_out, vert_cap, vert_off,
+                                     totals, out_mem);
+      if (rc != PBF_OK) { ctx->tune = saved; return rc; }
+      ms = ctx->timing.total_ms;
+    }
+    if (ms < bms) { bms = ms; bi = c; }
+  }
+  ctx->tune = cands[bi];
+  if (best) *best = bi;
+  if (best_ms) *best_ms = bms;
+  return PBF_OK;
+}
+
+extern "C" int pbf_set_tuning(pbf_context* ctx, int light_max, int light_work, int blocks_per_cu) {
+  if (!ctx || light_max < 0 || light_max > 1535 || light_work < 0 || blocks_per_cu < 1 || blocks_per_cu > 8) return PBF_ERR_ARG;
+  ctx->tune.light_max = (uint32_t)light_max;
+  ctx->tune.light_work = (uint32_t)light_work;
+  ctx->tune.blocks_per_cu = (uint32_t)blocks_per_cu;
+  return PBF_OK;
+}

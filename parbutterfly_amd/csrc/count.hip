@@ -993,17 +993,31 @@ __device__ __forceinline__ uint32_t seg_len(const uint2* __restrict__ sinfo, uin
 // distinct-key bound min(T, i); when that exceeds a wavefront table but T itself is small, the
 // table is sized optimistically for T / 2 keys (the endpoint is re-run with a full-size table
 // if it overflows).
-__device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, uint32_t cb,
-                                                uint32_t* __restrict__ bound, uint32_t* __restrict__ wlist,
-                                                uint32_t* __restrict__ blist, uint32_t* __restrict__ olist,
-                                                uint32_t* __restrict__ lens) {
+struct ClassOut {
+  uint32_t* bound;
+  uint32_t* wlist;
+  uint32_t* blist;
+  uint32_t* olist;
+  uint32_t* lens;
+  uint32_t cb;
+  uint32_t light_max;   // <= LIGHT_MAX
+  uint32_t light_work;
+};
+
+__device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, const ClassOut& co) {
+  uint32_t* __restrict__ bound = co.bound;
+  uint32_t* __restrict__ wlist = co.wlist;
+  uint32_t* __restrict__ blist = co.blist;
+  uint32_t* __restrict__ olist = co.olist;
+  uint32_t* __restrict__ lens = co.lens;
+  const uint32_t cb = co.cb;
   const uint32_t lane = threadIdx.x & 63;
   uint32_t cls = 0;  // 0 none, 1 wavefront, 2 workgroup, 3 HBM
   if (valid && T) {
     const uint32_t b = (uint32_t)(T < (uint64_t)i ? T : (uint64_t)i);
     if (b) {
-      const uint32_t est = (b > LIGHT_MAX && T <= LIGHT_WORK) ? min(b, (uint32_t)(T / 2)) : b;
-      const bool light = est <= LIGHT_MAX && T <= LIGHT_WORK && cb < 32u && deg < (1u << cb);
+      const uint32_t est = (b > co.light_max && T <= co.light_work) ? min(b, (uint32_t)(T / 2)) : b;
+      const bool light = est <= co.light_max && T <= co.light_work && cb < 32u && deg < (1u << cb);
       cls = deg >= WIDE_DEG ? 3u : light ? 1u : 2u;
       bound[i] = light ? est : b;
     } else {
@@ -1026,15 +1040,6 @@ __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t
   if (cls == 3) olist[base3 + __popcll(m3 & lt)] = i;
 }
 
-struct ClassOut {
-  uint32_t* bound;
-  uint32_t* wlist;
-  uint32_t* blist;
-  uint32_t* olist;
-  uint32_t* lens;
-  uint32_t cb;
-};
-
 // Short lists: classified here.  Long lists: listed in `big` with their chunk counts.
 __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
                                                   const uint32_t* __restrict__ inpref, const uint2* __restrict__ sinfo,
@@ -1050,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ r
     const bool small = deg <= 64;
     if (small)
       for (uint32_t j = 0; j < deg; j++) T += seg_len(sinfo, io + j);
-    classify_append((uint32_t)i, i < n && small, T, deg, co.cb, co.bound, co.wlist, co.blist, co.olist, co.lens);
+    classify_append((uint32_t)i, i < n && small, T, deg, co);
     const uint64_t mb = __ballot(!small);
     uint32_t bb = 0;
     if (lane == 0 && mb) bb = atomicAdd(&co.lens[2], (uint32_t)__popcll(mb));
@@ -1114,7 +1119,7 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
     const uint32_t deg = valid ? roff[i + 1] - roff[i] : 0u;
     const unsigned long long T = valid ? w64[i] : 0ull;
     const bool sp = valid && T && i > 0 && deg > SPLIT_DEG && deg < WIDE_DEG && i < SPLIT_IMAX;
-    classify_append(i, valid && !sp, T, deg, co.cb, co.bound, co.wlist, co.blist, co.olist, co.lens);
+    classify_append(i, valid && !sp, T, deg, co);
     const uint64_t ms = __ballot(sp);
     uint32_t b = 0;
     if (lane == 0 && ms) b = atomicAdd(&co.lens[8], (uint32_t)__popcll(ms));
@@ -1380,12 +1385,12 @@ inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
   return (unsigned)b;
 }
 
-constexpr int HBM_TEAM = 256, HBM_BLOCKS = 32;
+constexpr int HBM_TEAM = 256, HBM_BLOCKS = 32, HBM_BLOCKS_NOSYNC = 4;
 
 // Runs the counting phase: classification, the fused kernel, the hub fold and (only if some
 // endpoint needs it) the HBM fallback.  lens/queues are cleared with a fill kernel.
 template <int MODE>
-int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lists, uint32_t* ctl) {
+int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lists, uint32_t* ctl, bool sync) {
   const int64_t n = R.n;
   // ctl: [0] light len, [1] heavy len, [2] big len, [3] heavy queue, [4] light-group queue,
   //      [5] light groups, [6] re-run len, [7] total big chunks, [8] split len,
@@ -1414,7 +1419,9 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, plan, 0u, (int64_t)64);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
   uint32_t* hbm_list = lists + 5 * n + 1;
-  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb};
+  const uint32_t lmax = ctx->tune.light_max < LIGHT_MAX ? ctx->tune.light_max : LIGHT_MAX;
+  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb, lmax, ctx->tune.light_work};
+  const unsigned grid = (unsigned)(ctx->num_cus * (ctx->tune.blocks_per_cu ? ctx->tune.blocks_per_cu : 4));
   hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
                      n, co, big, big_chunks, w64);
   PBF_CHECK(pbf_scan_excl_u32(ctx, big_chunks, chunk_base, n, ctl + 7));
@@ -1457,8 +1464,8 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   a.rerun_list = lists + 2 * n;
   a.rerun_len = ctl + 6;
   a.rerun = 0;
-  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
-  hipLaunchKernelGGL((k_count2<MODE, 2>), dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
+  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
+  hipLaunchKernelGGL((k_count2<MODE, 2>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
   // light endpoints whose optimistic table overflowed: again, one workgroup each, full tables
   a.blist = lists + 2 * n;
   a.nblock = ctl + 6;
@@ -1466,20 +1473,31 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   a.ngroups0 = ctl + 14;
   a.queues = ctl + 16;
   a.rerun = 1;
-  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(ctx->num_cus * 4), dim3(TPB), 0, ctx->stream, a);
+  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
   PBF_HIP(hipGetLastError());
   if (MODE != MODE_EDGE && MODE != MODE_TOTAL)
     hipLaunchKernelGGL(k_hot_reduce, dim3(grid_for(HOT_N, 256, 64)), dim3(256), 0, ctx->stream, a.hot, a.B, n);
-  PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 11, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  PBF_HIP(hipStreamSynchronize(ctx->stream));
-  const uint32_t novf = (uint32_t)(ctx->pinned[16] & 0xFFFFFFFFu);
-  ctx->timing.hub_vertices = novf;
-  if (novf) {
-    // endpoints no LDS table could hold: dense HBM histograms
+  uint32_t novf = 0;
+  int blocks = HBM_BLOCKS_NOSYNC;
+  if (sync) {
+    PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 11, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    novf = (uint32_t)(ctx->pinned[16] & 0xFFFFFFFFu);
+    ctx->timing.hub_vertices = novf;
+    blocks = (int)(novf < (uint32_t)HBM_BLOCKS ? novf : (uint32_t)HBM_BLOCKS);
+  } else {
+    ctx->timing.hub_vertices = -1;  // not read back
+  }
+  if (novf || !sync) {
+    // endpoints no LDS table could hold: dense HBM histograms (every entry is restored to
+    // zero after its endpoint, so the pool is cleared only when first allocated)
     void* pp;
-    const int blocks = (int)(novf < (uint32_t)HBM_BLOCKS ? novf : (uint32_t)HBM_BLOCKS);
-    PBF_CHECK(pbf_scratch(ctx, S_POOL, sizeof(uint32_t) * 2 * (size_t)n * blocks, &pp));
-    PBF_HIP(hipMemsetAsync(pp, 0, sizeof(uint32_t) * 2 * (size_t)n * blocks, ctx->stream));
+    const size_t pool_bytes = sizeof(uint32_t) * 2 * (size_t)n * blocks;
+    PBF_CHECK(pbf_scratch(ctx, S_POOL, pool_bytes, &pp));
+    if (ctx->pool_clean < pool_bytes) {
+      PBF_CHECK(pbf_fill_u32(ctx, static_cast<uint32_t*>(pp), 0u, (int64_t)(pool_bytes / 4)));
+      ctx->pool_clean = pool_bytes;
+    }
     a.pool = static_cast<uint32_t*>(pp);
     a.list = hbm_list; a.list_len = ctl + 11; a.queue = ctl + 13;
     hipLaunchKernelGGL((k_count_hbm<MODE, HBM_TEAM>), dim3(blocks), dim3(HBM_TEAM), 0, ctx->stream, a);
@@ -1490,7 +1508,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
 }  // namespace
 
 int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGraph& R, int mode, uint64_t* out_dev,
-                  const uint32_t* eti) {
+                  const uint32_t* eti, const CountOpts& opts) {
   const int64_t n = R.n, m2 = R.m2;
   void *bp, *lp, *cp, *mp, *hp, *op = nullptr;
   PBF_CHECK(pbf_scratch(ctx, S_BOUND, 4 * (size_t)(n ? n : 1), &bp));
@@ -1521,9 +1539,10 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
     PBF_CHECK(pbf_scratch(ctx, S_COUNTS, 8 * (size_t)(n ? n : 1), &cp));
     a.B = static_cast<unsigned long long*>(cp);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, a.B, 0ull, n);
-    PBF_CHECK(run_count<MODE_VERT>(ctx, R, a, lists, ctl));
+    PBF_CHECK(run_count<MODE_VERT>(ctx, R, a, lists, ctl, opts.sync));
+    if (opts.B_out) *opts.B_out = a.B;
     const int64_t ns = use_v ? g.nu : g.nv, base = use_v ? g.nv : 0;
-    if (ns)
+    if (ns && !opts.rank_space)
       hipLaunchKernelGGL(k_convert_vert, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, ctx->stream, a.B, R.rank_of, base, ns,
                          reinterpret_cast<unsigned long long*>(out_dev));
   } else if (mode == MODE_EDGE) {
@@ -1534,10 +1553,10 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
     a.B = reinterpret_cast<unsigned long long*>(out_dev);
     if (g.m)
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g.m, 256, 8192)), dim3(256), 0, ctx->stream, a.B, 0ull, (int64_t)g.m);
-    PBF_CHECK(run_count<MODE_EDGE>(ctx, R, a, lists, ctl));
+    PBF_CHECK(run_count<MODE_EDGE>(ctx, R, a, lists, ctl, opts.sync));
   } else {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, a.total, 0ull, (int64_t)1);
-    PBF_CHECK(run_count<MODE_TOTAL>(ctx, R, a, lists, ctl));
+    PBF_CHECK(run_count<MODE_TOTAL>(ctx, R, a, lists, ctl, opts.sync));
     PBF_HIP(hipMemcpyAsync(out_dev, a.total, 8, hipMemcpyDeviceToDevice, ctx->stream));
   }
   PBF_HIP(hipGetLastError());

@@ -46,7 +46,19 @@ enum ScratchSlot {
   S_ROWN,        // per-slot owner rank
   S_LPLAN,       // grouping plan of the light endpoints (bucket counts, starts, word bases)
   S_LREC,        // grouped light endpoints: (i, roff[i], deg, log2 table size)
+  S_BSTAGE,      // batch: host graphs staged in HBM
+  S_BTAB,        // batch: per-graph table
+  S_BUNION,      // batch: union CSR
+  S_BFLAG,       // batch: per-vertex side flags
+  S_BAUX,        // batch: per-graph |S|, offsets, sides
   S_NUM_SLOTS
+};
+
+// Runtime knobs of the counting kernels (defaults; the batch autotuner may change them).
+struct CountTune {
+  uint32_t light_max = 1535;     // distinct-key bound of a light endpoint (wavefront table)
+  uint32_t light_work = 1u << 14;  // wedge bound of a light endpoint
+  uint32_t blocks_per_cu = 4;    // workgroups per CU of the persistent counting kernels
 };
 
 struct pbf_context {
@@ -59,6 +71,9 @@ struct pbf_context {
   uint64_t* pinned = nullptr;  // small pinned host buffer (readbacks)
   int num_cus = 256;
   bool hot_clear = false;  // S_HOT has been zeroed
+  size_t pool_clean = 0;   // bytes of S_POOL known to be zero (the HBM kernel leaves them zero)
+  CountTune tune;
+  void* batch = nullptr;   // batch state (batch.hip)
 };
 
 // Returns a device buffer of at least `bytes` for the slot (grow-only; contents undefined).
@@ -95,23 +110,41 @@ struct RankedGraph {
   uint32_t* rpos;     // [2m] position in owner's original CSR (EDGE only, else null)
   uint32_t* inpref;   // [n]  neighbours of lower rank
   uint32_t* rown;     // [2m] owner rank of each slot
-  int64_t wedges;     // W under this ranking
+  int64_t wedges;     // W under this ranking (-1 when not read back)
+  unsigned long long* wedges_dev;  // W on the device
 };
 
 // scan.hip
 int pbf_scan_excl_u32(pbf_context* ctx, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev);
 int pbf_scan_excl_u64(pbf_context* ctx, const uint64_t* in, uint64_t* out, int64_t n, uint64_t* total_dev);
+// Fill kernels (used instead of hipMemsetAsync everywhere a caller may capture the stream).
+int pbf_fill_u32(pbf_context* ctx, uint32_t* p, uint32_t v, int64_t n);
+int pbf_fill_u64(pbf_context* ctx, uint64_t* p, uint64_t v, int64_t n);
 
 // rank.hip
 int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedges);
 int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R);
-int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_pos, RankedGraph* R);
+// sflag (optional): per combined id, 1 when the vertex lies on the count side S (a batch of
+// graphs with their own sides); else S is U when use_v, V otherwise.  sync: read W back into
+// R->wedges (else only R->wedges_dev holds it, and the call never waits for the device).
+int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_pos, RankedGraph* R,
+                    const uint8_t* sflag = nullptr, bool sync = true);
 int pbf_dev_edge_to_idxs(pbf_context* ctx, const DevGraph& g, int use_v, uint32_t* eti);
 
 // count.hip
 enum CountMode { MODE_VERT = 0, MODE_EDGE = 1, MODE_TOTAL = 2 };
+// sync: the HBM fallback is launched only when needed, after reading its list length back;
+// otherwise it is always launched (it exits at once on an empty list) and the call never
+// waits for the device.  rank_space (VERT): counts are left in rank space (*B_out), no
+// conversion to ids.
+struct CountOpts {
+  bool sync = true;
+  bool rank_space = false;
+  unsigned long long** B_out = nullptr;
+};
 int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGraph& R, int mode,
-                  uint64_t* out_dev /* VERT: n_S, EDGE: m, TOTAL: 1 */, const uint32_t* eti);
+                  uint64_t* out_dev /* VERT: n_S, EDGE: m, TOTAL: 1 */, const uint32_t* eti,
+                  const CountOpts& opts = CountOpts());
 
 // peel.hip
 int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint64_t* counts, uint64_t* tip);

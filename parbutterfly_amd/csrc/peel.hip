@@ -471,6 +471,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   void* pp;
   PBF_CHECK(pbf_scratch(ctx, S_POOL, sizeof(uint32_t) * 2 * (size_t)ns * V_HBM_BLOCKS, &pp));
   PBF_HIP(hipMemsetAsync(pp, 0, sizeof(uint32_t) * 2 * (size_t)ns * V_HBM_BLOCKS, ctx->stream));
+  ctx->pool_clean = 0;  // the counting kernels re-clear the pool before their next use
   a.pool = static_cast<uint32_t*>(pp);
   uint32_t* small_list = pb.lists;
   uint32_t* large_list = pb.lists + ns;

@@ -96,7 +96,14 @@ __global__ void k_ranked_degree(DevGraph g, const uint32_t* __restrict__ order, 
 
 // One wavefront per original vertex x: stable counting sort of x's neighbour list by key
 // (descending), written at roff[rank_of[x]].  flag = 1 iff the neighbour lies on S.
-__global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const uint8_t* __restrict__ key,
+// Flag of x's entries: 1 iff the neighbour lies on S, i.e. iff x itself lies on O.
+__device__ __forceinline__ uint32_t entry_flag(int use_v, const uint8_t* __restrict__ sflag, int64_t x, bool xv) {
+  if (sflag) return sflag[x] ? 0u : 1u;
+  return use_v ? (xv ? 1u : 0u) : (xv ? 0u : 1u);
+}
+
+__global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
+                                                 const uint8_t* __restrict__ key,
                                                  const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
                                                  uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
                                                  uint32_t* __restrict__ inpref, uint32_t* __restrict__ rown) {
@@ -112,7 +119,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     if (deg > RL_CHUNK) continue;  // long lists: k_rl_* below
     const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
     const int64_t nbr_shift = xv ? g.nv : 0;  // combined id of neighbour = adj[j] + shift
-    const uint32_t flag = use_v ? (xv ? 1u : 0u) : (xv ? 0u : 1u);
+    const uint32_t flag = entry_flag(use_v, sflag, x, xv);
     const uint32_t r = rank_of[x];
     if (lane < KEYS + 1) { cnt[wid][lane] = 0; run[wid][lane] = 0; }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -245,7 +252,8 @@ __global__ __launch_bounds__(256) void k_rl_offsets(DevGraph g, const uint8_t* _
   }
 }
 
-__global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const uint8_t* __restrict__ key,
+__global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
+                                                    const uint8_t* __restrict__ key,
                                                     const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
                                                     LongLists L, uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
                                                     uint32_t* __restrict__ rown) {
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const
     int64_t sh;
     vertex_list(g, x, &obase, &deg, &adj, &sh);
     const bool xv = x < g.nv;
-    const uint32_t flag = use_v ? (xv ? 1u : 0u) : (xv ? 0u : 1u);
+    const uint32_t flag = entry_flag(use_v, sflag, x, xv);
     const uint32_t rx = rank_of[x];
     const uint32_t rbase = roff[rx];
     if (lane < KEYS) run[wid][lane] = L.hist[(size_t)item * KEYS + lane];
@@ -443,7 +451,7 @@ int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedge
   void* mp;
   PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
   unsigned long long* acc = static_cast<unsigned long long*>(mp);
-  PBF_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(uint64_t), ctx->stream));
+  PBF_CHECK(pbf_fill_u64(ctx, reinterpret_cast<uint64_t*>(acc), 0, 2));
   if (g.nv) hipLaunchKernelGGL(k_side_partial, dim3(grid_for(g.nv, 256, 4096)), dim3(256), 0, ctx->stream, g.offV, g.nv, acc);
   if (g.nu) hipLaunchKernelGGL(k_side_partial, dim3(grid_for(g.nu, 256, 4096)), dim3(256), 0, ctx->stream, g.offU, g.nu, acc + 1);
   PBF_HIP(hipGetLastError());
@@ -483,8 +491,8 @@ int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph
     PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
     uint32_t* lvl_cnt = static_cast<uint32_t*>(mp) + 64;  // 33 level counters (u32 words 64..96)
     uint32_t* state = static_cast<uint32_t*>(mp) + 112;   // k, ntouched
-    PBF_HIP(hipMemsetAsync(removed, 0, 4 * (size_t)n, ctx->stream));
-    PBF_HIP(hipMemsetAsync(lvl_cnt, 0, 4 * 48, ctx->stream));
+    PBF_CHECK(pbf_fill_u32(ctx, removed, 0u, n));
+    PBF_CHECK(pbf_fill_u32(ctx, lvl_cnt, 0u, 48));
     hipLaunchKernelGGL(k_acore_init, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, key, rem, done, lvl_cnt);
     // at most KEYS non-empty levels, and k strictly decreases every round
     for (int round = 0; round < KEYS; round++) {
@@ -508,7 +516,8 @@ int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph
   return PBF_OK;
 }
 
-int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_pos, RankedGraph* R) {
+int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_pos, RankedGraph* R, const uint8_t* sflag,
+                    bool sync) {
   const int64_t n = g.nv + g.nu, m2 = 2 * g.m;
   void *ro, *re, *rpp, *ip, *mp;
   PBF_CHECK(pbf_scratch(ctx, S_ROFF, 4 * (size_t)(n + 1), &ro));
@@ -530,8 +539,8 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
   const uint8_t* key = static_cast<const uint8_t*>(ctx->buf[S_KEY]);
   hipLaunchKernelGGL(k_ranked_degree, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, R->order, R->roff);
   PBF_CHECK(pbf_scan_excl_u32(ctx, R->roff, R->roff, n + 1, nullptr));
-  hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 4, 16384)), dim3(256), 0, ctx->stream, g, use_v, key, R->rank_of, R->roff,
-                     R->redg, R->rpos, R->inpref, R->rown);
+  hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 4, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of,
+                     R->roff, R->redg, R->rpos, R->inpref, R->rown);
   // long lists, chunk-parallel
   {
     const int64_t max_items = 2 * (2 * g.m / RL_CHUNK) + 64;
@@ -549,13 +558,16 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
     LongLists L{llist, lbase, lctl, lctl + 1, static_cast<uint32_t*>(hp2)};
     hipLaunchKernelGGL(k_rl_hist, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, key, L);
     hipLaunchKernelGGL(k_rl_offsets, dim3(grid_for(n, 4, 2048)), dim3(256), 0, ctx->stream, g, key, R->rank_of, L, R->inpref);
-    hipLaunchKernelGGL(k_rl_scatter, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, use_v, key, R->rank_of, R->roff, L,
+    hipLaunchKernelGGL(k_rl_scatter, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of, R->roff, L,
                        R->redg, R->rpos, R->rown);
   }
   unsigned long long* acc = static_cast<unsigned long long*>(mp) + 8;
-  PBF_HIP(hipMemsetAsync(acc, 0, sizeof(uint64_t), ctx->stream));
+  PBF_CHECK(pbf_fill_u64(ctx, reinterpret_cast<uint64_t*>(acc), 0, 1));
   hipLaunchKernelGGL(k_rank_wedges, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, R->roff, R->inpref, n, acc);
   PBF_HIP(hipGetLastError());
+  R->wedges_dev = acc;
+  R->wedges = -1;
+  if (!sync) return PBF_OK;
   PBF_HIP(hipMemcpyAsync(ctx->pinned + 8, acc, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   R->wedges = (int64_t)ctx->pinned[8];

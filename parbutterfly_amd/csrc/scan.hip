@@ -82,9 +82,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const T* in, T* out
 }
 
 template <typename T>
+__global__ void k_fill(T* __restrict__ p, T v, int64_t n) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) p[x] = v;
+}
+
+template <typename T>
+int fill_impl(pbf_context* ctx, T* p, T v, int64_t n) {
+  if (n <= 0) return PBF_OK;
+  int64_t b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)b), dim3(256), 0, ctx->stream, p, v, n);
+  PBF_HIP(hipGetLastError());
+  return PBF_OK;
+}
+
+template <typename T>
 int scan_impl(pbf_context* ctx, const T* in, T* out, int64_t n, T* total_dev) {
   if (n <= 0) {
-    if (total_dev) PBF_HIP(hipMemsetAsync(total_dev, 0, sizeof(T), ctx->stream));
+    if (total_dev) PBF_CHECK(fill_impl<T>(ctx, total_dev, T(0), 1));
     return PBF_OK;
   }
   int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -107,3 +122,5 @@ int pbf_scan_excl_u32(pbf_context* ctx, const uint32_t* in, uint32_t* out, int64
 int pbf_scan_excl_u64(pbf_context* ctx, const uint64_t* in, uint64_t* out, int64_t n, uint64_t* total_dev) {
   return scan_impl<uint64_t>(ctx, in, out, n, total_dev);
 }
+int pbf_fill_u32(pbf_context* ctx, uint32_t* p, uint32_t v, int64_t n) { return fill_impl<uint32_t>(ctx, p, v, n); }
+int pbf_fill_u64(pbf_context* ctx, uint64_t* p, uint64_t v, int64_t n) { return fill_impl<uint64_t>(ctx, p, v, n); }
