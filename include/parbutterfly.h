@@ -160,7 +160,8 @@ int pbf_side(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int* use_v, 
  * plus rankWedgeF (:216-237).  order[r] = combined id of rank r (V ids 0..nv-1, U ids
  * nv..nv+nu-1); rank_of[id] = r; *wedges = W under this order.  Ties inside a key are broken
  * by ascending id (the reference's sampleSort leaves them unspecified).  Any of the output
- * pointers may be NULL.  rank_type: PBF_RANK_ADEG or PBF_RANK_ACOCORE. */
+ * pointers may be NULL.  rank_type: PBF_RANK_ADEG, PBF_RANK_ACOCORE, or PBF_RANK_SIDE (the
+ * count side S first, then O, each by id: side-based wedges, butterfly_count.h:1639-1640). */
 int pbf_rank(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int rank_type,
              uint32_t* order, uint32_t* rank_of, int64_t* wedges);
 

@@ -68,7 +68,9 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-bool rank_supported(int rank_type) { return rank_type == PBF_RANK_ADEG || rank_type == PBF_RANK_ACOCORE; }
+bool rank_supported(int rank_type) {
+  return rank_type == PBF_RANK_ADEG || rank_type == PBF_RANK_ACOCORE || rank_type == PBF_RANK_SIDE;
+}
 bool count_supported(int count_type) { return count_type >= PBF_COUNT_SORT && count_type <= PBF_COUNT_GPU; }
 
 struct DeviceGuard {
@@ -109,7 +111,7 @@ int count_common(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_
   }
   PBF_HIP(hipEventRecord(ev[1], ctx->stream));
   RankedGraph R{};
-  PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R));
+  PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R, use_v));
   PBF_HIP(hipEventRecord(ev[2], ctx->stream));
   PBF_CHECK(pbf_dev_relabel(ctx, dg, use_v, mode == MODE_EDGE, &R));
   PBF_HIP(hipEventRecord(ev[3], ctx->stream));
@@ -286,10 +288,15 @@ extern "C" int pbf_rank(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, i
   DeviceGuard guard(ctx->device);
   DevGraph dg;
   PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  int use_v = 1;
+  if (rank_type == PBF_RANK_SIDE) {
+    int64_t w;
+    PBF_CHECK(pbf_dev_side(ctx, dg, &use_v, &w));
+  }
   RankedGraph R{};
-  PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R));
+  PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R, use_v));
   if (wedges) {
-    PBF_CHECK(pbf_dev_relabel(ctx, dg, 1, false, &R));
+    PBF_CHECK(pbf_dev_relabel(ctx, dg, use_v, false, &R));
     *wedges = R.wedges;
   }
   const size_t nb = 4 * (size_t)R.n;

@@ -155,7 +155,7 @@ int union_pass(pbf_context* ctx, const BGraph* tab, int64_t count, int64_t NV, i
   hipLaunchKernelGGL(k_batch_side, dim3(gb), dim3(256), 0, ctx->stream, tab, count, NV, sflag, sn, usev);
   PBF_HIP(hipGetLastError());
   RankedGraph R{};
-  PBF_CHECK(pbf_dev_rank(ctx, u, rank_type, &R));
+  PBF_CHECK(pbf_dev_rank(ctx, u, rank_type, &R, 0, sflag));
   PBF_CHECK(pbf_dev_relabel(ctx, u, 0, false, &R, sflag, false));
   CountOpts opts;
   opts.sync = false;
@@ -175,7 +175,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
                                int flags, uint64_t* vert_out, int64_t vert_cap, uint64_t* vert_off, uint64_t* totals,
                                int out_mem) {
   if (!ctx || count < 0 || (count > 0 && (!graphs || !vert_out || !vert_off || !totals))) return PBF_ERR_ARG;
-  if (rank_type != PBF_RANK_ADEG && rank_type != PBF_RANK_ACOCORE) return PBF_ERR_UNSUPPORTED;
+  if (rank_type != PBF_RANK_ADEG && rank_type != PBF_RANK_ACOCORE && rank_type != PBF_RANK_SIDE) return PBF_ERR_UNSUPPORTED;
   if (mem != PBF_MEM_DEVICE && mem != PBF_MEM_HOST) return PBF_ERR_ARG;
   if (out_mem != PBF_MEM_DEVICE && out_mem != PBF_MEM_HOST) return PBF_ERR_ARG;
   const int kind = flags & 15;
@@ -245,7 +245,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
       const int64_t ns = use_v ? d.nu : d.nv;
       off[g + 1] = off[g] + ns;
       RankedGraph R{};
-      PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R));
+      PBF_CHECK(pbf_dev_rank(ctx, dg, rank_type, &R, use_v));
       PBF_CHECK(pbf_dev_relabel(ctx, dg, use_v, false, &R));
       PBF_CHECK(pbf_dev_count(ctx, dg, use_v, R, MODE_VERT, reinterpret_cast<uint64_t*>(vout + off[g]), nullptr));
       hipLaunchKernelGGL(k_total_of, dim3(1), dim3(256), 0, ctx->stream, vout + off[g], ns, tot + g);

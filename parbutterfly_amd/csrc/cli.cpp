@@ -8,8 +8,8 @@
 //
 // Count types are the reference's (ASORT ... SERIAL, numeric aliases of :96-107) plus GPU;
 // they are aggregation strategies with identical results, and all of them run the GPU kernels
-// here.  Rank types ADEG and ACOCORE are supported on the GPU (the configurations this build
-// targets); SIDE / DEG / COCORE return PBF_ERR_UNSUPPORTED.  Sparsification (-sparseType) is not
+// here.  Rank types ADEG, ACOCORE and SIDE are supported on the GPU; DEG / COCORE return
+// PBF_ERR_UNSUPPORTED.  Sparsification (-sparseType) is not
 // part of this build and is rejected.  The graph is re-read every round, as the reference does
 // (:331-341).
 #include <chrono>

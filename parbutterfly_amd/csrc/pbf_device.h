@@ -123,7 +123,9 @@ int pbf_fill_u64(pbf_context* ctx, uint64_t* p, uint64_t v, int64_t n);
 
 // rank.hip
 int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedges);
-int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R);
+// SIDE ranking needs the count side: use_v, or the per-vertex flags sflag of a batch.
+int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R, int use_v = 0,
+                 const uint8_t* sflag = nullptr);
 // sflag (optional): per combined id, 1 when the vertex lies on the count side S (a batch of
 // graphs with their own sides); else S is U when use_v, V otherwise.  sync: read W back into
 // R->wedges (else only R->wedges_dev holds it, and the call never waits for the device).

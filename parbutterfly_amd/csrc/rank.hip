@@ -34,6 +34,17 @@ __device__ __forceinline__ uint32_t vertex_degree(const DevGraph& g, int64_t x) 
   return x < g.nv ? g.offV[x + 1] - g.offV[x] : g.offU[x - g.nv + 1] - g.offU[x - g.nv];
 }
 
+// SIDE: every vertex of the count side S before every vertex of O (ties by id), i.e. the
+// wedges are exactly those with both endpoints on S and the centre on O -- the side-based
+// enumeration the reference runs when no ranking is requested (butterfly_count.h:1639-1640).
+__global__ void k_side_key(DevGraph g, int use_v, const uint8_t* __restrict__ sflag, uint8_t* __restrict__ key) {
+  int64_t n = g.nv + g.nu;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    const bool in_s = sflag ? sflag[x] != 0 : (use_v ? x >= g.nv : x < g.nv);
+    key[x] = in_s ? 2 : 1;
+  }
+}
+
 __global__ void k_adeg_key(DevGraph g, uint8_t* __restrict__ key) {
   int64_t n = g.nv + g.nu;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
@@ -464,7 +475,7 @@ int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedge
 }
 
 // Fills R->order / R->rank_of (and the key array in S_KEY) for ADEG or ACOCORE.
-int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R) {
+int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R, int use_v, const uint8_t* sflag) {
   const int64_t n = g.nv + g.nu;
   R->n = n;
   R->m2 = 2 * g.m;
@@ -478,6 +489,8 @@ int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph
   if (n == 0) return PBF_OK;
   if (rank_type == PBF_RANK_ADEG) {
     hipLaunchKernelGGL(k_adeg_key, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, key);
+  } else if (rank_type == PBF_RANK_SIDE) {
+    hipLaunchKernelGGL(k_side_key, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, use_v, sflag, key);
   } else if (rank_type == PBF_RANK_ACOCORE) {
     void *a0, *a1, *a2;
     PBF_CHECK(pbf_scratch(ctx, S_ACORE0, 4 * (size_t)n, &a0));          // rem

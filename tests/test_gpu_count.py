@@ -136,3 +136,22 @@ def test_deterministic_repeats(ctx):
         assert np.array_equal(a, pb.as_u64(ctx.count(g, use_v, "VERT")))
     ea = pb.as_u64(ctx.count(g, use_v, "EDGE"))
     assert np.array_equal(ea, pb.as_u64(ctx.count(g, use_v, "EDGE")))
+
+
+def test_side_ranking(ctx):
+    # SIDE: count side first, then O, each by id; counts equal every other ranking's
+    for tag, g in small_graphs() + [("cl", pb.Graph.chung_lu(3000, 2000, 30000, 2.2, 2.1, 8))]:
+        use_v, _ = ctx.side(g)
+        v = pb.as_u64(ctx.count(g, use_v, "VERT", "GPU", "SIDE"))
+        assert np.array_equal(v, O.count_vert(g, use_v)), tag
+        e = pb.as_u64(ctx.count(g, use_v, "EDGE", "GPU", "SIDE"))
+        assert np.array_equal(e, O.count_edge(g, use_v)), tag
+        assert ctx.count(g, use_v, "TOTAL", "GPU", "SIDE") == O.count_total(g, use_v), tag
+        order, rank_of, w = ctx.rank(g, "SIDE")
+        ids_s = np.arange(g.nv, g.nv + g.nu) if use_v else np.arange(g.nv)
+        ids_o = np.arange(g.nv) if use_v else np.arange(g.nv, g.nv + g.nu)
+        assert np.array_equal(order, np.concatenate([ids_s, ids_o]).astype(np.uint32)), tag
+        # W under SIDE = wedges centred on O with both endpoints on S
+        offs = g.offsetsV if use_v else g.offsetsU
+        d = np.diff(offs).astype(np.int64)
+        assert w == int((d * (d - 1) // 2).sum()), tag
