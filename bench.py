@@ -64,6 +64,69 @@ def cpu_baseline(graph, use_v, rounds=2):
                 use_v=uv, counts=counts)
 
 
+def gen_batch(seeds, workers=16):
+    """Graphs of the mixed-batch distribution (seed per graph), generated in parallel threads
+    (the generator is C code that releases the interpreter lock)."""
+    import parbutterfly_amd as pb
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(pb.Graph.batch_graph, seeds))
+
+
+def batched_bench(pb, ctx, count=4096, first_seed=1000):
+    """Batched workload: `count` graphs (seed first_seed + g; nv, nu log-uniform in [64, 65536],
+    average degree log-uniform in [2, 32], exponent uniform in [2, 3]), all CSRs resident in HBM,
+    per-graph VERT and TOTAL counts into HBM.  graphs/s of the plain per-graph loop, the
+    union pass, the captured union pass, and the captured pass after autotuning; every output
+    compared bit-exactly with the plain loop's."""
+    t0 = time.time()
+    hg = gen_batch([first_seed + g for g in range(count)], workers=min(16, os.cpu_count() or 1))
+    gen_s = time.time() - t0
+    dgs = [g.to_device(ctx) for g in hg]
+    edges = int(sum(g.m for g in hg))
+    print(f"# batch: {count} graphs, {edges} edges, generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    del hg
+    res = {"graphs": count, "edges": edges}
+
+    def run(mode, capture=False, outputs=None, reps=1):
+        best = None
+        for _ in range(reps):
+            ctx.synchronize()
+            t = time.perf_counter()
+            out = ctx.count_batch(dgs, "ADEG", mode=mode, capture=capture, outputs=outputs)
+            dt = time.perf_counter() - t
+            outputs = out
+            best = dt if best is None else min(best, dt)
+        return best, outputs
+
+    def same(a, b):
+        va, oa, ta = (x.numpy() for x in a)
+        vb, ob, tb = (x.numpy() for x in b)
+        return bool(np.array_equal(oa, ob) and np.array_equal(ta[:count], tb[:count]) and
+                    np.array_equal(va[:int(oa[count])], vb[:int(ob[count])]))
+
+    _, ref = run("union")  # warms the scratch
+    t_union, out_u = run("union", reps=2)
+    t_plain, out_p = run("plain")
+    res["plain_graphs_s"] = round(count / t_plain, 1)
+    res["union_graphs_s"] = round(count / t_union, 1)
+    res["bit_exact_union_vs_plain"] = same(out_u, out_p)
+    _, out_c = run("union", capture=True, outputs=out_u)  # eager pass + capture
+    t_cap, out_c = run("union", capture=True, outputs=out_c, reps=3)  # replays
+    res["captured_graphs_s"] = round(count / t_cap, 1)
+    res["bit_exact_captured_vs_plain"] = same(out_c, out_p)
+    best, best_ms = ctx.batch_autotune(dgs)
+    _, out_t = run("union", capture=True, outputs=out_c)
+    t_tuned, out_t = run("union", capture=True, outputs=out_t, reps=3)
+    res["autotuned_config"] = best
+    res["autotuned_graphs_s"] = round(count / t_tuned, 1)
+    res["bit_exact_autotuned_vs_plain"] = same(out_t, out_p)
+    res["speedup_vs_plain"] = round(t_plain / t_tuned, 2)
+    ctx.set_tuning()
+    ctx.trim()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +136,8 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_count_chunglu_2p24.json"),
                     help="JSON with measured HBM bytes per counting phase")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run one device copy of this many bytes and exit")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 4096-graph workload")
+    ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -153,6 +218,10 @@ def main():
                      "traffic": traffic, "b_alg_bytes": balg, "count_ms": round(count_ms, 4)},
         "host_inclusive_gwedges_s": round(W / h2d_s / 1e9, 3),
     }
+    if not args.no_batch:
+        del dg, out
+        ctx.trim()
+        rec["batched"] = batched_bench(pb, ctx, count=args.batch_count)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(g, use_v)
         if cb is not None:
