@@ -211,6 +211,35 @@ int pbf_batch_autotune(pbf_context* ctx, int64_t count, const pbf_bipartite_csr*
 /* Sets the counting kernels' knobs directly (light_max <= 1535, 1 <= blocks_per_cu <= 8). */
 int pbf_set_tuning(pbf_context* ctx, int light_max, int light_work, int blocks_per_cu);
 
+/* ---------------- learned variant selector (no counterpart in the reference) ----------------
+ * Features of each graph of a batch (PBF_MLP_IN floats per graph, row-major): log2 of
+ * nv + 1, nu + 1, m + 1; per side the log2 of the mean degree, of the mean squared degree and of
+ * the maximum degree; log2 of wV + 1 and wU + 1 (cmpWedgeCounts); log2 of W + 1 under SIDE,
+ * ADEG and ACOCORE (exact rank-filtered wedge counts, rankWedgeF butterfly_utils.h:216-237);
+ * and use_v.  One pass over the disjoint union per ranking. */
+#define PBF_MLP_IN 16
+#define PBF_MLP_HIDDEN 64
+#define PBF_MLP_OUT 9
+#define PBF_MLP_PARAMS 6288
+int pbf_batch_features(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
+                       float* features);
+/* The variants the selector scores: variant v = 3 * r + k runs ranking r (PBF_RANK_SIDE,
+ * PBF_RANK_ADEG, PBF_RANK_ACOCORE) with counting knobs k (pbf_variant_tuning). */
+int pbf_variant_tuning(int variant, int* rank_type, int* light_max, int* light_work, int* blocks_per_cu);
+/* MLP 16 -> 64 -> 64 -> 9 (ReLU), FP32 parameters in the layout W1[16][64], b1[64], W2[64][64],
+ * b2[64], W3[64][16] (columns 9..15 zero), b3[16] (entries 9..15 zero).  Products in BF16 on the
+ * matrix cores with FP32 accumulation.  All arrays on the host; synchronous. */
+int pbf_mlp_init(uint64_t seed, float* params);
+/* Adam (lr, beta 0.9 / 0.999) on the mean squared error over n samples X[n][16] -> T[n][9],
+ * minibatches of 128 in a seeded random order each epoch; params updated in place. */
+int pbf_mlp_train(pbf_context* ctx, float* params, const float* X, const float* T, int64_t n, int epochs,
+                  float lr, uint64_t seed, float* final_loss);
+/* Gradient of that loss over n <= 128 samples (same layout as params) and the loss. */
+int pbf_mlp_grad(pbf_context* ctx, const float* params, const float* X, const float* T, int64_t n,
+                 float* grads, float* loss);
+/* Predictions Y[n][9]. */
+int pbf_mlp_predict(pbf_context* ctx, const float* params, const float* X, int64_t n, float* Y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,12 @@ _SIGS = {
     "pbf_batch_autotune": (_INT, [_P, _I64, _PCSR, _INT, _INT, _P, _I64, _P, _P, _INT, _PI,
                                   ctypes.POINTER(ctypes.c_float)]),
     "pbf_set_tuning": (_INT, [_P, _INT, _INT, _INT]),
+    "pbf_batch_features": (_INT, [_P, _I64, _PCSR, _INT, _P]),
+    "pbf_variant_tuning": (_INT, [_INT, _PI, _PI, _PI, _PI]),
+    "pbf_mlp_init": (_INT, [_U64, _P]),
+    "pbf_mlp_train": (_INT, [_P, _P, _P, _P, _I64, _INT, ctypes.c_float, _U64, ctypes.POINTER(ctypes.c_float)]),
+    "pbf_mlp_grad": (_INT, [_P, _P, _P, _P, _I64, _P, ctypes.POINTER(ctypes.c_float)]),
+    "pbf_mlp_predict": (_INT, [_P, _P, _P, _I64, _P]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -508,12 +514,65 @@ class Context:
     def set_tuning(self, light_max=1535, light_work=1 << 14, blocks_per_cu=4):
         _check(_lib.pbf_set_tuning(self._ctx, light_max, light_work, blocks_per_cu), "pbf_set_tuning")
 
+    # ---- learned selector (no counterpart in the reference) ----
+    def batch_features(self, graphs):
+        """float32 [len(graphs), MLP_IN] feature rows (see pbf_batch_features)."""
+        mem, arr, _ = self._batch_args(graphs)
+        F = np.zeros((max(len(graphs), 1), MLP_IN), np.float32)
+        _check(_lib.pbf_batch_features(self._ctx, len(graphs), arr, mem, F.ctypes.data), "pbf_batch_features")
+        return F[:len(graphs)]
+
+    def mlp_train(self, params, X, T, epochs, lr=1e-3, seed=1):
+        """Adam on float32 params (updated in place); returns the last epoch's mean loss."""
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        assert params.dtype == np.float32 and params.size == MLP_PARAMS and params.flags.c_contiguous
+        loss = ctypes.c_float()
+        _check(_lib.pbf_mlp_train(self._ctx, params.ctypes.data, X.ctypes.data, T.ctypes.data, len(X), int(epochs),
+                                  float(lr), int(seed), ctypes.byref(loss)), "pbf_mlp_train")
+        return loss.value
+
+    def mlp_grad(self, params, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        _check(_lib.pbf_mlp_grad(self._ctx, np.ascontiguousarray(params, np.float32).ctypes.data, X.ctypes.data,
+                                 T.ctypes.data, len(X), g.ctypes.data, ctypes.byref(loss)), "pbf_mlp_grad")
+        return g, loss.value
+
+    def mlp_predict(self, params, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((max(len(X), 1), MLP_OUT), np.float32)
+        _check(_lib.pbf_mlp_predict(self._ctx, np.ascontiguousarray(params, np.float32).ctypes.data, X.ctypes.data,
+                                    len(X), Y.ctypes.data), "pbf_mlp_predict")
+        return Y[:len(X)]
+
     def _as_u64(self, g, counts):
         if isinstance(g, DeviceGraph):
             if isinstance(counts, DeviceArray):
                 return counts
             return DeviceArray(self, len(counts), np.uint64, counts)
         return np.ascontiguousarray(as_u64(counts), dtype=np.uint64)
+
+
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 16, 64, 9, 6288
+RANK_NAMES = {v: k for k, v in RANK_TYPES.items()}
+
+
+def mlp_init(seed):
+    """He-uniform float32 parameters of the selector MLP (see pbf_mlp_init)."""
+    p = np.zeros(MLP_PARAMS, np.float32)
+    _check(_lib.pbf_mlp_init(int(seed), p.ctypes.data), "pbf_mlp_init")
+    return p
+
+
+def variant_tuning(v):
+    """(rank type name, light_max, light_work, blocks_per_cu) of selector variant v (0..8)."""
+    r, a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(_lib.pbf_variant_tuning(int(v), ctypes.byref(r), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+           "pbf_variant_tuning")
+    return RANK_NAMES[r.value], a.value, b.value, c.value
 
 
 def as_u64(x):

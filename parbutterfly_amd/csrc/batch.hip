@@ -63,7 +63,7 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long x, un
 // the union and |S|.
 __global__ __launch_bounds__(256) void k_batch_side(const BGraph* __restrict__ gs, int64_t count, int64_t NV,
                                                     uint8_t* __restrict__ sflag, unsigned long long* __restrict__ sn,
-                                                    uint8_t* __restrict__ usev) {
+                                                    uint8_t* __restrict__ usev, unsigned long long* __restrict__ wvu) {
   __shared__ unsigned long long sm[4];
   for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
     const BGraph b = gs[g];
@@ -84,6 +84,7 @@ __global__ __launch_bounds__(256) void k_batch_side(const BGraph* __restrict__ g
     if (threadIdx.x == 0) {
       sn[g] = use_v ? b.nu : b.nv;
       usev[g] = use_v ? 1 : 0;
+      if (wvu) { wvu[2 * g] = wv; wvu[2 * g + 1] = wu; }
     }
   }
 }
@@ -122,6 +123,90 @@ __global__ __launch_bounds__(256) void k_total_of(const unsigned long long* __re
   if (threadIdx.x == 0) *total = s / 2;
 }
 
+// ---- selector features ----
+// in(x) = neighbours of x ranked below x (thread per vertex over the union).
+__global__ __launch_bounds__(256) void k_inpref_union(DevGraph u, const uint32_t* __restrict__ rank_of,
+                                                      uint32_t* __restrict__ in) {
+  const int64_t n = u.nv + u.nu;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    const bool xv = x < u.nv;
+    const uint32_t* off = xv ? u.offV : u.offU;
+    const uint32_t* edg = xv ? u.edgV : u.edgU;
+    const int64_t lx = xv ? x : x - u.nv, shift = xv ? u.nv : 0;
+    const uint32_t r = rank_of[x];
+    uint32_t c = 0;
+    for (uint32_t p = off[lx]; p < off[lx + 1]; p++) c += rank_of[edg[p] + shift] < r ? 1u : 0u;
+    in[x] = c;
+  }
+}
+
+// One workgroup per graph: rankWedgeF summed over the graph's vertices (W under a ranking).
+__global__ __launch_bounds__(256) void k_graph_wedges(const BGraph* __restrict__ gs, int64_t count, int64_t NV,
+                                                      const uint32_t* __restrict__ in, unsigned long long* __restrict__ W) {
+  __shared__ unsigned long long sm[4];
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const BGraph b = gs[g];
+    unsigned long long s = 0;
+    for (uint32_t v = threadIdx.x; v < b.nv + b.nu; v += blockDim.x) {
+      const bool isv = v < b.nv;
+      const uint32_t lv = isv ? v : v - b.nv;
+      const unsigned long long d = isv ? b.offV[lv + 1] - b.offV[lv] : b.offU[lv + 1] - b.offU[lv];
+      const unsigned long long i = in[isv ? b.vbase + lv : NV + b.ubase + lv];
+      s += i * (d - i) + (i ? i * (i - 1) / 2 : 0);
+    }
+    s = block_sum(s, sm);
+    if (threadIdx.x == 0) W[g] = s;
+  }
+}
+
+// One workgroup per graph: the feature row (see pbf_batch_features).
+__global__ __launch_bounds__(256) void k_features(const BGraph* __restrict__ gs, int64_t count,
+                                                  const unsigned long long* __restrict__ wvu, const uint8_t* __restrict__ usev,
+                                                  const unsigned long long* __restrict__ Wad, const unsigned long long* __restrict__ Waco,
+                                                  float* __restrict__ F) {
+  __shared__ unsigned long long sm[4];
+  __shared__ uint32_t mx[2];
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const BGraph b = gs[g];
+    if (threadIdx.x < 2) mx[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long s1 = 0, s2 = 0, t1 = 0, t2 = 0;
+    for (uint32_t v = threadIdx.x; v < b.nv; v += blockDim.x) {
+      const unsigned long long d = b.offV[v + 1] - b.offV[v];
+      s1 += d; s2 += d * d;
+      atomicMax(&mx[0], (uint32_t)d);
+    }
+    for (uint32_t u = threadIdx.x; u < b.nu; u += blockDim.x) {
+      const unsigned long long d = b.offU[u + 1] - b.offU[u];
+      t1 += d; t2 += d * d;
+      atomicMax(&mx[1], (uint32_t)d);
+    }
+    s1 = block_sum(s1, sm); s2 = block_sum(s2, sm); t1 = block_sum(t1, sm); t2 = block_sum(t2, sm);
+    if (threadIdx.x == 0) {
+      float* f = F + g * PBF_MLP_IN;
+      const double nv = b.nv ? b.nv : 1, nu = b.nu ? b.nu : 1;
+      const unsigned long long wv = wvu[2 * g], wu = wvu[2 * g + 1];
+      f[0] = (float)log2(b.nv + 1.0);
+      f[1] = (float)log2(b.nu + 1.0);
+      f[2] = (float)log2(b.m + 1.0);
+      f[3] = (float)log2(s1 / nv + 1.0);
+      f[4] = (float)log2(t1 / nu + 1.0);
+      f[5] = (float)log2(s2 / nv + 1.0);
+      f[6] = (float)log2(t2 / nu + 1.0);
+      f[7] = (float)log2(mx[0] + 1.0);
+      f[8] = (float)log2(mx[1] + 1.0);
+      f[9] = (float)log2((double)wv + 1.0);
+      f[10] = (float)log2((double)wu + 1.0);
+      f[11] = (float)log2((double)(wv < wu ? wv : wu) + 1.0);  // SIDE: wedges centred on O
+      f[12] = (float)log2((double)Wad[g] + 1.0);
+      f[13] = (float)log2((double)Waco[g] + 1.0);
+      f[14] = usev[g] ? 1.f : 0.f;
+      f[15] = (float)log2((double)b.m / (nv * nu) + 1e-9);
+    }
+    __syncthreads();
+  }
+}
+
 struct BatchState {
   uint64_t sig = 0;
   hipGraphExec_t exec = nullptr;
@@ -152,7 +237,8 @@ int union_pass(pbf_context* ctx, const BGraph* tab, int64_t count, int64_t NV, i
   u.offV = offV; u.edgV = edgV; u.offU = offU; u.edgU = edgU;
   const unsigned gb = (unsigned)std::min<int64_t>(count, (int64_t)ctx->num_cus * 16);
   hipLaunchKernelGGL(k_union_build, dim3(gb), dim3(256), 0, ctx->stream, tab, count, offV, edgV, offU, edgU, NV, NU, M);
-  hipLaunchKernelGGL(k_batch_side, dim3(gb), dim3(256), 0, ctx->stream, tab, count, NV, sflag, sn, usev);
+  hipLaunchKernelGGL(k_batch_side, dim3(gb), dim3(256), 0, ctx->stream, tab, count, NV, sflag, sn, usev,
+                     (unsigned long long*)nullptr);
   PBF_HIP(hipGetLastError());
   RankedGraph R{};
   PBF_CHECK(pbf_dev_rank(ctx, u, rank_type, &R, 0, sflag));
@@ -398,6 +484,99 @@ extern "C" int pbf_batch_autotune(pbf_context* ctx, int64_t count, const pbf_bip
   ctx->tune = cands[bi];
   if (best) *best = bi;
   if (best_ms) *best_ms = bms;
+  return PBF_OK;
+}
+
+extern "C" int pbf_variant_tuning(int variant, int* rank_type, int* light_max, int* light_work, int* blocks_per_cu) {
+  static const int ranks[3] = {PBF_RANK_SIDE, PBF_RANK_ADEG, PBF_RANK_ACOCORE};
+  static const int knobs[3][3] = {{1535, 1 << 14, 4}, {767, 1 << 13, 4}, {383, 1 << 12, 2}};
+  if (variant < 0 || variant >= 9) return PBF_ERR_ARG;
+  if (rank_type) *rank_type = ranks[variant / 3];
+  if (light_max) *light_max = knobs[variant % 3][0];
+  if (light_work) *light_work = knobs[variant % 3][1];
+  if (blocks_per_cu) *blocks_per_cu = knobs[variant % 3][2];
+  return PBF_OK;
+}
+
+extern "C" int pbf_batch_features(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem, float* features) {
+  if (!ctx || count < 0 || (count > 0 && (!graphs || !features))) return PBF_ERR_ARG;
+  if (mem != PBF_MEM_DEVICE && mem != PBF_MEM_HOST) return PBF_ERR_ARG;
+  if (count == 0) return PBF_OK;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+  if (prev != ctx->device) (void)hipSetDevice(ctx->device);
+  struct Restore {
+    int p, d;
+    ~Restore() { if (p >= 0 && p != d) (void)hipSetDevice(p); }
+  } restore{prev, ctx->device};
+  std::vector<BGraph> tab((size_t)count);
+  int64_t NV = 0, NU = 0, M = 0;
+  size_t stage_words = 0;
+  for (int64_t g = 0; g < count; g++) {
+    const pbf_bipartite_csr& d = graphs[g];
+    if (d.nv < 0 || d.nu < 0 || d.m < 0 || !d.offsetsV || !d.offsetsU || (d.m && (!d.edgesV || !d.edgesU)))
+      return PBF_ERR_ARG;
+    tab[g] = BGraph{d.offsetsV, d.edgesV, d.offsetsU, d.edgesU, (uint32_t)d.nv, (uint32_t)d.nu, (uint32_t)d.m, 0,
+                    (uint64_t)NV, (uint64_t)NU, (uint64_t)M};
+    NV += d.nv; NU += d.nu; M += d.m;
+    stage_words += size_t(d.nv + 1) + size_t(d.nu + 1) + 2 * size_t(d.m);
+  }
+  if (NV + NU >= (int64_t(1) << 31) || 2 * M >= (int64_t(1) << 31)) return PBF_ERR_CAPACITY;
+  if (mem == PBF_MEM_HOST) {
+    void* sp;
+    PBF_CHECK(pbf_scratch(ctx, S_BSTAGE, 4 * stage_words, &sp));
+    uint32_t* w = static_cast<uint32_t*>(sp);
+    for (int64_t g = 0; g < count; g++) {
+      const pbf_bipartite_csr& d = graphs[g];
+      BGraph& bg = tab[g];
+      PBF_HIP(hipMemcpyAsync(w, d.offsetsV, 4 * size_t(d.nv + 1), hipMemcpyHostToDevice, ctx->stream));
+      bg.offV = w; w += d.nv + 1;
+      PBF_HIP(hipMemcpyAsync(w, d.edgesV, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      bg.edgV = w; w += d.m;
+      PBF_HIP(hipMemcpyAsync(w, d.offsetsU, 4 * size_t(d.nu + 1), hipMemcpyHostToDevice, ctx->stream));
+      bg.offU = w; w += d.nu + 1;
+      PBF_HIP(hipMemcpyAsync(w, d.edgesU, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      bg.edgU = w; w += d.m;
+    }
+  }
+  void *tp, *up, *fp, *ap, *xp;
+  PBF_CHECK(pbf_scratch(ctx, S_BTAB, sizeof(BGraph) * (size_t)count, &tp));
+  PBF_CHECK(pbf_scratch(ctx, S_BUNION, 4 * (size_t)(NV + 1 + NU + 1 + 2 * M), &up));
+  PBF_CHECK(pbf_scratch(ctx, S_BFLAG, (size_t)(NV + NU + 1), &fp));
+  PBF_CHECK(pbf_scratch(ctx, S_BAUX, 8 * (size_t)(3 * count + 2), &ap));
+  PBF_CHECK(pbf_scratch(ctx, S_BFEAT, 8 * (size_t)(4 * count) + 4 * (size_t)count * PBF_MLP_IN + 4 * (size_t)(NV + NU + 1), &xp));
+  BGraph* dtab = static_cast<BGraph*>(tp);
+  unsigned long long* sn = static_cast<unsigned long long*>(ap);
+  uint8_t* usev = reinterpret_cast<uint8_t*>(sn + 2 * count + 1);
+  unsigned long long* wvu = static_cast<unsigned long long*>(xp);
+  unsigned long long* Wad = wvu + 2 * count;
+  unsigned long long* Waco = Wad + count;
+  float* F = reinterpret_cast<float*>(Waco + count);
+  uint32_t* inb = reinterpret_cast<uint32_t*>(F + (size_t)count * PBF_MLP_IN);
+  PBF_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BGraph) * (size_t)count, hipMemcpyHostToDevice, ctx->stream));
+  DevGraph u;
+  u.nv = NV; u.nu = NU; u.m = M;
+  uint32_t* offV = static_cast<uint32_t*>(up);
+  u.offV = offV; u.edgV = offV + NV + 1; u.offU = u.edgV + M; u.edgU = u.offU + NU + 1;
+  const unsigned gb = (unsigned)std::min<int64_t>(count, (int64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_union_build, dim3(gb), dim3(256), 0, ctx->stream, dtab, count, offV, const_cast<uint32_t*>(u.edgV),
+                     const_cast<uint32_t*>(u.offU), const_cast<uint32_t*>(u.edgU), NV, NU, M);
+  hipLaunchKernelGGL(k_batch_side, dim3(gb), dim3(256), 0, ctx->stream, dtab, count, NV, static_cast<uint8_t*>(fp), sn, usev,
+                     wvu);
+  const int64_t n = NV + NU;
+  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256 + 1, 16384);
+  const int ranks[2] = {PBF_RANK_ADEG, PBF_RANK_ACOCORE};
+  unsigned long long* Wout[2] = {Wad, Waco};
+  for (int k = 0; k < 2; k++) {
+    RankedGraph R{};
+    PBF_CHECK(pbf_dev_rank(ctx, u, ranks[k], &R, 0, static_cast<uint8_t*>(fp)));
+    hipLaunchKernelGGL(k_inpref_union, dim3(nb), dim3(256), 0, ctx->stream, u, R.rank_of, inb);
+    hipLaunchKernelGGL(k_graph_wedges, dim3(gb), dim3(256), 0, ctx->stream, dtab, count, NV, inb, Wout[k]);
+  }
+  hipLaunchKernelGGL(k_features, dim3(gb), dim3(256), 0, ctx->stream, dtab, count, wvu, usev, Wad, Waco, F);
+  PBF_HIP(hipGetLastError());
+  PBF_HIP(hipMemcpyAsync(features, F, 4 * (size_t)count * PBF_MLP_IN, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }
 

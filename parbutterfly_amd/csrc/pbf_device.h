@@ -51,6 +51,8 @@ enum ScratchSlot {
   S_BUNION,      // batch: union CSR
   S_BFLAG,       // batch: per-vertex side flags
   S_BAUX,        // batch: per-graph |S|, offsets, sides
+  S_MLP,         // selector MLP: parameters, Adam moments, samples
+  S_BFEAT,       // batch features: per-graph statistics
   S_NUM_SLOTS
 };
 
