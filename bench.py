@@ -123,6 +123,27 @@ def batched_bench(pb, ctx, count=4096, first_seed=1000):
     res["bit_exact_autotuned_vs_plain"] = same(out_t, out_p)
     res["speedup_vs_plain"] = round(t_plain / t_tuned, 2)
     ctx.set_tuning()
+    # batched + learned: features, MLP selection, one union pass per chosen variant
+    from parbutterfly_amd import selector as S
+    if os.path.exists(S.MODEL_PATH):
+        model = S.load()
+        S.run_learned(ctx, model, dgs)  # warms every variant's scratch
+        best_t, parts, pick = None, None, None
+        for _ in range(2):
+            ctx.synchronize()
+            t = time.perf_counter()
+            parts, pick = S.run_learned(ctx, model, dgs)
+            ctx.synchronize()
+            dt = time.perf_counter() - t
+            best_t = dt if best_t is None else min(best_t, dt)
+        verts, tots = S.gather(parts, count)
+        vp, op, tp = (x.numpy() for x in out_p)
+        exact = bool(np.array_equal(tots, tp[:count]) and
+                     all(np.array_equal(verts[k], vp[int(op[k]):int(op[k + 1])]) for k in range(count)))
+        res["learned_graphs_s"] = round(count / best_t, 1)
+        res["learned_speedup_vs_plain"] = round(t_plain / best_t, 2)
+        res["bit_exact_learned_vs_plain"] = exact
+        res["learned_variant_histogram"] = np.bincount(pick, minlength=S.NVAR).tolist()
     ctx.trim()
     return res
 

@@ -85,3 +85,15 @@ def test_autotune_keeps_results(ctx):
     vert, tot = ctx.count_batch(gs, mode="union")
     _check(gs, vert, tot)
     ctx.set_tuning()  # defaults for the tests that follow
+
+
+def test_learned_selection_is_exact(ctx):
+    # any variant choice gives the same counts: an untrained network scatters the graphs over
+    # the variants, and every part must still match the plain loop
+    from parbutterfly_amd import selector as S
+    gs = _mixed(14, 800)
+    model = dict(params=pb.mlp_init(11), fmu=np.zeros(16, np.float32), fsd=np.ones(16, np.float32),
+                 tmu=np.zeros(9, np.float32), tsd=np.ones(9, np.float32))
+    verts, tots, pick = S.count_learned(ctx, model, gs)
+    assert len(set(pick.tolist())) >= 2
+    _check(gs, verts, tots)
