@@ -143,7 +143,12 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
       }
     }
     team_sync<TEAM>();
-    for (uint32_t q = tid; q < sh.nused; q += TEAM) dense[dused[q]] = 0;
+    // leave the whole slice zero, the used list included: a later call with another vertex
+    // count lays its dense counters over this storage
+    for (uint32_t q = tid; q < sh.nused; q += TEAM) {
+      dense[dused[q]] = 0;
+      dused[q] = 0;
+    }
     team_sync<TEAM>();
   }
   if constexpr (MODE == MODE_TOTAL) {
@@ -592,7 +597,10 @@ __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, con
   if (__ballot(sum != 0u) == 0ull) return;
   const uint32_t key = wv.nval ? last : 64u;  // lanes past the end form a trailing run
   const uint32_t x = wave_seg_incl_add(sum, key);
-  if (key < 64u && x && wave_shl1_or(key, ~0u) != key) lds_add(acc + key, (unsigned long long)x);
+  // cross-lane reads run with every lane active: a lane disabled by the branch below would
+  // not supply its key to its left neighbour
+  const uint32_t knext = wave_shl1_or(key, ~0u);
+  if (key < 64u && x && knext != key) lds_add(acc + key, (unsigned long long)x);
 }
 
 // Runs windows wstart, wstart + wstep, ... of a chunk.  The next window is mapped and its
@@ -863,9 +871,12 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
           if constexpr (MODE == MODE_VERT) credit(a, cur[k] >> a.cb, b);
         }
       }
-      const uint32_t k = in ? lane_upper(off_k, nk, q) : 64u;
+      // every lane takes part in the cross-lane steps (see win_credit)
+      const uint32_t kq = lane_upper(off_k, nk, q);
+      const uint32_t k = in ? kq : 64u;
       const uint32_t x = wave_seg_incl_add(sum, k);
-      if (k < 64u && x && wave_shl1_or(k, ~0u) != k) lds_add(iacc + k, (unsigned long long)x);
+      const uint32_t knext = wave_shl1_or(k, ~0u);
+      if (k < 64u && x && knext != k) lds_add(iacc + k, (unsigned long long)x);
     }
     if (words >= wm) wm = 0;  // swept clean
     wave_lds_fence();
@@ -1489,8 +1500,9 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
     ctx->timing.hub_vertices = -1;  // not read back
   }
   if (novf || !sync) {
-    // endpoints no LDS table could hold: dense HBM histograms (every entry is restored to
-    // zero after its endpoint, so the pool is cleared only when first allocated)
+    // endpoints no LDS table could hold: dense HBM histograms (every counter and used-list
+    // entry is restored to zero after its endpoint, so the pool is cleared only when first
+    // allocated)
     void* pp;
     const size_t pool_bytes = sizeof(uint32_t) * 2 * (size_t)n * blocks;
     PBF_CHECK(pbf_scratch(ctx, S_POOL, pool_bytes, &pp));

@@ -155,3 +155,34 @@ def test_side_ranking(ctx):
         offs = g.offsetsV if use_v else g.offsetsU
         d = np.diff(offs).astype(np.int64)
         assert w == int((d * (d - 1) // 2).sum()), tag
+
+
+def _hbm_graph(keys, pad):
+    """S = U (wV <= wU) under SIDE ranking: U vertex `keys + pad` shares the V hub v0 with U
+    vertices 0..keys-1, so as an endpoint it has `keys` distinct keys at a rank above 16384
+    (beyond every LDS table: the HBM-histogram path).  A U hub with keys + 1 leaves in V makes
+    wU > wV."""
+    i = keys + pad
+    v = [0] * (keys + 1)
+    u = list(range(keys)) + [i]
+    leaves = keys + 1
+    v += list(range(1, 1 + leaves))
+    u += [i + 1] * leaves
+    return pb.Graph.from_edges(1 + leaves, i + 2, np.array(v, np.uint32), np.array(u, np.uint32))
+
+
+def test_hbm_fallback_reuse(ctx):
+    # the HBM path must leave its pool slice zero, used lists included: a later call with
+    # another vertex count lays its counters over that storage
+    big, small = _hbm_graph(45000, 5000), _hbm_graph(10000, 10000)
+    for g in (big, small):
+        use_v, _ = ctx.side(g)
+        assert use_v
+    res = []
+    for g in (big, small, big, small, big):
+        v = pb.as_u64(ctx.count(g, True, "VERT", "GPU", "SIDE"))
+        assert ctx.timing()["hub_vertices"] >= 1
+        res.append(v)
+    ob, os_ = O.count_vert(big, True), O.count_vert(small, True)
+    for k, v in enumerate(res):
+        assert np.array_equal(v, ob if k % 2 == 0 else os_), k
