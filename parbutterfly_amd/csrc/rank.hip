@@ -368,8 +368,13 @@ __global__ void k_edge_to_idxs(const uint32_t* __restrict__ offS, const uint32_t
 // k is removed, and each unfinished neighbour below k loses one remaining degree per removed
 // neighbour (getApproxCoCoreRanks, butterfly_utils.h:466-496).  lvl_cnt[l] counts unfinished
 // vertices per level; state[0] holds the current k (or 0xFFFFFFFF when all are finished).
-__global__ void k_acore_init(DevGraph g, uint8_t* __restrict__ lvl, uint32_t* __restrict__ rem,
-                             uint8_t* __restrict__ done, uint32_t* __restrict__ lvl_cnt) {
+// Level counters are aggregated per workgroup in LDS (one global atomic per level and
+// workgroup): a global atomic per vertex on a few dozen addresses serialises.
+__global__ __launch_bounds__(256) void k_acore_init(DevGraph g, uint8_t* __restrict__ lvl, uint32_t* __restrict__ rem,
+                                                    uint8_t* __restrict__ done, uint32_t* __restrict__ lvl_cnt) {
+  __shared__ uint32_t h[KEYS];
+  if (threadIdx.x < KEYS) h[threadIdx.x] = 0;
+  __syncthreads();
   int64_t n = g.nv + g.nu;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
     uint32_t d = vertex_degree(g, x);
@@ -377,8 +382,10 @@ __global__ void k_acore_init(DevGraph g, uint8_t* __restrict__ lvl, uint32_t* __
     uint32_t l = pbf_bitlen(d);
     lvl[x] = (uint8_t)l;
     done[x] = 0;
-    atomicAdd(&lvl_cnt[l], 1u);
+    atomicAdd(&h[l], 1u);
   }
+  __syncthreads();
+  if (threadIdx.x < KEYS && h[threadIdx.x]) atomicAdd(&lvl_cnt[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void k_acore_pick(uint32_t* __restrict__ lvl_cnt, uint32_t* __restrict__ state) {
@@ -397,6 +404,9 @@ __global__ __launch_bounds__(256) void k_acore_remove(DevGraph g, const uint32_t
                                                       uint32_t* __restrict__ ntouched) {
   const uint32_t k = state[0];
   if (k == 0xFFFFFFFFu) return;
+  __shared__ uint32_t nrem;
+  if (threadIdx.x == 0) nrem = 0;
+  __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t n = g.nv + g.nu;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -406,7 +416,7 @@ __global__ __launch_bounds__(256) void k_acore_remove(DevGraph g, const uint32_t
     bool a = x < n && !done[x] && lvl[x] == k;
     uint64_t am = __ballot(a);
     if (a) { done[x] = 1; }
-    if (am && lane == 0) atomicSub(&lvl_cnt[k], (uint32_t)__popcll(am));
+    if (am && lane == 0) atomicAdd(&nrem, (uint32_t)__popcll(am));
     while (am) {
       int src = __ffsll((long long)am) - 1;
       am &= am - 1;
@@ -416,19 +426,34 @@ __global__ __launch_bounds__(256) void k_acore_remove(DevGraph g, const uint32_t
       const uint32_t d = xv ? g.offV[xa + 1] - ob : g.offU[xa - g.nv + 1] - ob;
       const uint32_t* adj = xv ? g.edgV + ob : g.edgU + ob;
       const int64_t sh = xv ? g.nv : 0;
-      for (uint32_t j = lane; j < d; j += 64) {
-        uint32_t y = (uint32_t)(adj[j] + sh);
-        if (lvl[y] < k) {  // unfinished and below k (finished / active vertices have level >= k)
-          if (atomicAdd(&removed[y], 1u) == 0) touched[atomicAdd(ntouched, 1u)] = y;
+      // every lane runs the same iterations: the append below is aggregated per wavefront
+      for (uint32_t j0 = 0; j0 < d; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        uint32_t y = 0;
+        bool fresh = false;
+        if (j < d) {
+          y = (uint32_t)(adj[j] + sh);
+          // unfinished and below k (finished / active vertices have level >= k)
+          fresh = lvl[y] < k && atomicAdd(&removed[y], 1u) == 0;
         }
+        const uint64_t fm = __ballot(fresh);
+        uint32_t b = 0;
+        if (lane == 0 && fm) b = atomicAdd(ntouched, (uint32_t)__popcll(fm));
+        b = __shfl(b, 0, 64);
+        if (fresh) touched[b + __popcll(fm & pbf_lanemask_lt())] = y;
       }
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && nrem) atomicSub(&lvl_cnt[k], nrem);
 }
 
-__global__ void k_acore_update(const uint32_t* __restrict__ touched, const uint32_t* __restrict__ ntouched,
-                               uint8_t* __restrict__ lvl, uint32_t* __restrict__ rem, uint32_t* __restrict__ removed,
-                               uint32_t* __restrict__ lvl_cnt) {
+__global__ __launch_bounds__(256) void k_acore_update(const uint32_t* __restrict__ touched, const uint32_t* __restrict__ ntouched,
+                                                      uint8_t* __restrict__ lvl, uint32_t* __restrict__ rem,
+                                                      uint32_t* __restrict__ removed, uint32_t* __restrict__ lvl_cnt) {
+  __shared__ int h[KEYS];
+  if (threadIdx.x < KEYS) h[threadIdx.x] = 0;
+  __syncthreads();
   uint32_t nt = *ntouched;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     uint32_t y = touched[t];
@@ -438,10 +463,12 @@ __global__ void k_acore_update(const uint32_t* __restrict__ touched, const uint3
     uint32_t ol = lvl[y], nl = pbf_bitlen(r);
     if (nl != ol) {
       lvl[y] = (uint8_t)nl;
-      atomicSub(&lvl_cnt[ol], 1u);
-      atomicAdd(&lvl_cnt[nl], 1u);
+      atomicSub(&h[ol], 1);
+      atomicAdd(&h[nl], 1);
     }
   }
+  __syncthreads();
+  if (threadIdx.x < KEYS && h[threadIdx.x]) atomicAdd(&lvl_cnt[threadIdx.x], (uint32_t)h[threadIdx.x]);
 }
 
 __global__ void k_reset_u32(uint32_t* p) { *p = 0; }
