@@ -148,6 +148,83 @@ def batched_bench(pb, ctx, count=4096, first_seed=1000):
     return res
 
 
+def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000):
+    """Count + peel workload: R-MAT bipartite graph (a, b, c, d) = (0.5, 0.1, 0.1, 0.3), 2^22 x 2^22
+    vertices, 1.2e8 distinct edges, seed 3, ACOCORE ranking, per-edge counts, then the full wing
+    peel.  Properties checked at full size: identical per-edge counts under ADEG and ACOCORE,
+    sum E = 4 TOTAL, wing <= E, and for the top wing level k (and the largest level whose
+    subgraph has at most sub_edges edges) the subgraph H_k = {e : wing(e) >= k} has at least k
+    butterflies on every edge, exactly k on some edge at the top level."""
+    t0 = time.time()
+    g = pb.Graph.rmat(log_n, log_n, m, 0.5, 0.1, 0.1, 0.3, 3)
+    print(f"# rmat: nv={g.nv} nu={g.nu} m={g.m} generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    dg = g.to_device(ctx)
+    use_v, _ = ctx.side(dg)
+    res = {"workload": f"rmat bipartite 2^{log_n} x 2^{log_n}, m={m}, (a,b,c,d)=(.5,.1,.1,.3) seed=3, ACOCORE, "
+                       "per-edge counts + wing peel",
+           "m": int(g.m), "use_v": bool(use_v)}
+    e = pb.DeviceArray(ctx, g.m, np.uint64)
+    ctx.count(dg, use_v, "EDGE", "GPU", "ACOCORE", out=e)  # warms the scratch
+    ph = []
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        ctx.count(dg, use_v, "EDGE", "GPU", "ACOCORE", out=e)
+        ph.append(ctx.timing())
+    ctx.synchronize()
+    step_ms = (time.perf_counter() - t) * 1e3 / 3
+    W = ph[-1]["wedges"]
+    count_ms = float(np.mean([p["count_ms"] for p in ph]))
+    n = g.nv + g.nu
+    balg = 2 * (4 * W + 32 * g.m) + 4 * (n + 1) + 8 * g.m
+    res.update({"wedges": int(W), "edge_count_ms_per_step": round(step_ms, 3),
+                "edge_gwedges_s": round(W / (step_ms * 1e-3) / 1e9, 3),
+                "edge_phases_ms": {k: round(float(np.mean([p[k] for p in ph])), 3)
+                                   for k in ("rank_ms", "relabel_ms", "count_ms", "convert_ms")},
+                "edge_count_roofline_frac": round(balg / (count_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+    E = e.numpy()
+    Ead = ctx.count(dg, use_v, "EDGE", "GPU", "ADEG").numpy()
+    total = ctx.count(dg, use_v, "TOTAL", "GPU", "ACOCORE")
+    res["butterflies"] = int(total)
+    res["edge_counts_equal_across_rankings"] = bool(np.array_equal(E, Ead))
+    res["edge_sum_is_4_total"] = int(E.sum(dtype=np.uint64)) == 4 * int(total)
+    del Ead
+    ctx.synchronize()
+    t = time.perf_counter()
+    wing = ctx.peel_edge(dg, use_v, e)
+    ctx.synchronize()
+    res["peel_s"] = round(time.perf_counter() - t, 3)
+    res["peel_rounds"] = int(ctx.timing()["rounds"])
+    wn = wing.numpy()
+    res["max_wing"] = int(wn.max())
+    res["wing_le_count"] = bool(np.all(wn <= E))
+    print(f"# rmat: count {step_ms:.1f} ms/step, peel {res['peel_s']} s, {res['peel_rounds']} rounds",
+          file=sys.stderr, flush=True)
+    # O-CSR edge list: edge p is (owner, neighbour) of the O side's CSR
+    offO, edgO = (g.offsetsV, g.edgesV) if use_v else (g.offsetsU, g.edgesU)
+    owner = np.repeat(np.arange(len(offO) - 1, dtype=np.uint32), np.diff(offO.astype(np.int64)))
+    vv, uu = (owner, edgO) if use_v else (edgO, owner)
+    levels = np.unique(wn)[::-1]
+    sizes = np.cumsum([int(np.count_nonzero(wn == k)) for k in levels])
+    ks = {int(levels[0])}
+    fit = [int(levels[j]) for j in range(len(levels)) if sizes[j] <= sub_edges]
+    if fit:
+        ks.add(fit[-1])
+    checks = []
+    for k in sorted(ks, reverse=True):
+        sel = wn >= k
+        h = pb.Graph.from_edges(g.nv, g.nu, np.ascontiguousarray(vv[sel]), np.ascontiguousarray(uu[sel]))
+        huv, _ = ctx.side(h)
+        hc = ctx.count(h, huv, "EDGE", "GPU", "ACOCORE")
+        mn = int(hc.min()) if len(hc) else 0
+        ok = mn >= k and (k != int(levels[0]) or mn == k)
+        checks.append({"k": k, "edges": int(sel.sum()), "min_butterflies": mn, "ok": bool(ok)})
+    res["wing_subgraph_checks"] = checks
+    del e, wing, dg
+    ctx.trim()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +236,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run one device copy of this many bytes and exit")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 4096-graph workload")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-rmat", action="store_true", help="skip the R-MAT count + wing-peel workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -243,6 +321,8 @@ def main():
         del dg, out
         ctx.trim()
         rec["batched"] = batched_bench(pb, ctx, count=args.batch_count)
+    if not args.no_rmat:
+        rec["rmat_count_peel"] = rmat_bench(pb, ctx)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(g, use_v)
         if cb is not None:

@@ -89,6 +89,59 @@ __global__ __launch_bounds__(256) void k_batch_side(const BGraph* __restrict__ g
   }
 }
 
+// One workgroup per graph: the graph's ranking in its own block of ranks.  A stable counting
+// sort of the graph's vertices by key (descending; ties in id order, V before U), placed at
+// ranks [base, base + nv + nu), base = the vertices of the graphs before it.  Restricted to
+// the graph this is the order it gets when ranked alone, and lo[r] = base lets the relabel
+// store graph-local ranks, so the counting kernels size and key their tables per graph.
+constexpr uint32_t RKEYS = 33;
+__global__ __launch_bounds__(256) void k_local_rank(const BGraph* __restrict__ gs, int64_t count, int64_t NV,
+                                                    const uint8_t* __restrict__ key, uint32_t* __restrict__ order,
+                                                    uint32_t* __restrict__ rank_of, uint32_t* __restrict__ lo) {
+  __shared__ uint32_t start[RKEYS];
+  __shared__ uint32_t wc[4][RKEYS];
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const BGraph b = gs[g];
+    const uint32_t nv = b.nv, n = b.nv + b.nu;
+    const uint32_t base = (uint32_t)(b.vbase + b.ubase);
+    const uint32_t vb = (uint32_t)b.vbase, ub = (uint32_t)(NV + (int64_t)b.ubase);
+    if (threadIdx.x < RKEYS) start[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) atomicAdd(&start[key[t < nv ? vb + t : ub + (t - nv)]], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t acc = 0;
+      for (int k = (int)RKEYS - 1; k >= 0; k--) { const uint32_t c = start[k]; start[k] = acc; acc += c; }
+    }
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < n; t0 += blockDim.x) {
+      const uint32_t t = t0 + threadIdx.x;
+      const bool act = t < n;
+      const uint32_t x = act ? (t < nv ? vb + t : ub + (t - nv)) : 0u;
+      const uint32_t k = act ? key[x] : 0u;
+      const uint64_t mm = pbf_match_key6(k, __ballot(act));
+      const uint32_t rk = __popcll(mm & pbf_lanemask_lt());
+      if (lane < RKEYS) wc[wid][lane] = 0;
+      __syncthreads();
+      if (act && rk == 0) wc[wid][k] = __popcll(mm);
+      __syncthreads();
+      if (act) {
+        uint32_t before = start[k];
+        for (uint32_t w = 0; w < wid; w++) before += wc[w][k];
+        const uint32_t r = base + before + rk;
+        order[r] = x;
+        rank_of[x] = r;
+        lo[r] = base;
+      }
+      __syncthreads();
+      if (threadIdx.x < RKEYS)
+        start[threadIdx.x] += wc[0][threadIdx.x] + wc[1][threadIdx.x] + wc[2][threadIdx.x] + wc[3][threadIdx.x];
+      __syncthreads();
+    }
+  }
+}
+
 // One workgroup per graph: CountRank's conversion (out[x] = B[rank(x)] for x in S) into the
 // graph's output segment, and the graph's total = sum over S / 2.
 __global__ __launch_bounds__(256) void k_batch_out(const BGraph* __restrict__ gs, int64_t count, int64_t NV,
@@ -225,8 +278,8 @@ float elapsed_ms(hipEvent_t a, hipEvent_t b) {
 }
 
 // Everything after the table upload: device work only, no synchronisation (capturable).
-int union_pass(pbf_context* ctx, const BGraph* tab, int64_t count, int64_t NV, int64_t NU, int64_t M, int rank_type,
-               uint32_t* ubuf, uint8_t* sflag, unsigned long long* sn, unsigned long long* voff, uint8_t* usev,
+int union_pass(pbf_context* ctx, const BGraph* tab, int64_t count, int64_t NV, int64_t NU, int64_t M, int64_t nmax,
+               int rank_type, uint32_t* ubuf, uint8_t* sflag, unsigned long long* sn, unsigned long long* voff, uint8_t* usev,
                unsigned long long* vout, unsigned long long* totals) {
   DevGraph u;
   u.nv = NV; u.nu = NU; u.m = M;
@@ -241,11 +294,17 @@ int union_pass(pbf_context* ctx, const BGraph* tab, int64_t count, int64_t NV, i
                      (unsigned long long*)nullptr);
   PBF_HIP(hipGetLastError());
   RankedGraph R{};
-  PBF_CHECK(pbf_dev_rank(ctx, u, rank_type, &R, 0, sflag));
+  PBF_CHECK(pbf_dev_rank(ctx, u, rank_type, &R, 0, sflag, false));  // keys only
+  void* lp;
+  PBF_CHECK(pbf_scratch(ctx, S_RLO, 4 * (size_t)(NV + NU), &lp));
+  R.lo = static_cast<uint32_t*>(lp);
+  hipLaunchKernelGGL(k_local_rank, dim3(gb), dim3(256), 0, ctx->stream, tab, count, NV,
+                     static_cast<const uint8_t*>(ctx->buf[S_KEY]), R.order, R.rank_of, R.lo);
   PBF_CHECK(pbf_dev_relabel(ctx, u, 0, false, &R, sflag, false));
   CountOpts opts;
   opts.sync = false;
   opts.rank_space = true;
+  opts.key_range = nmax;
   unsigned long long* B = nullptr;
   opts.B_out = &B;
   PBF_CHECK(pbf_dev_count(ctx, u, 0, R, MODE_VERT, nullptr, nullptr, opts));
@@ -278,7 +337,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
 
   // sizes and union layout (host side)
   std::vector<BGraph> tab((size_t)count);
-  int64_t NV = 0, NU = 0, M = 0, cap_need = 0;
+  int64_t NV = 0, NU = 0, M = 0, cap_need = 0, nmax = 0;
   for (int64_t g = 0; g < count; g++) {
     const pbf_bipartite_csr& d = graphs[g];
     if (d.nv < 0 || d.nu < 0 || d.m < 0 || !d.offsetsV || !d.offsetsU || (d.m && (!d.edgesV || !d.edgesU)))
@@ -288,6 +347,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
                     (uint64_t)NV, (uint64_t)NU, (uint64_t)M};
     NV += d.nv; NU += d.nu; M += d.m;
     cap_need += std::max(d.nv, d.nu);
+    nmax = std::max(nmax, d.nv + d.nu);
   }
   if (NV + NU >= (int64_t(1) << 31) || 2 * M >= (int64_t(1) << 31)) return PBF_ERR_CAPACITY;
   if (vert_cap < cap_need) return PBF_ERR_ARG;
@@ -410,7 +470,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
   PBF_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BGraph) * (size_t)count, hipMemcpyHostToDevice, ctx->stream));
   PBF_HIP(hipStreamSynchronize(ctx->stream));  // the table's host copy is released below
   PBF_HIP(hipEventRecord(ev[0], ctx->stream));
-  PBF_CHECK(union_pass(ctx, dtab, count, NV, NU, M, rank_type, static_cast<uint32_t*>(up), static_cast<uint8_t*>(fp), sn,
+  PBF_CHECK(union_pass(ctx, dtab, count, NV, NU, M, nmax, rank_type, static_cast<uint32_t*>(up), static_cast<uint8_t*>(fp), sn,
                        offd, usev, vout, tot));
   PBF_HIP(hipEventRecord(ev[4], ctx->stream));
   if (out_mem == PBF_MEM_HOST) {
@@ -434,7 +494,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
     if (st->exec) { (void)hipGraphExecDestroy(st->exec); st->exec = nullptr; }
     if (st->graph) { (void)hipGraphDestroy(st->graph); st->graph = nullptr; }
     PBF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-    const int rc = union_pass(ctx, dtab, count, NV, NU, M, rank_type, static_cast<uint32_t*>(up), static_cast<uint8_t*>(fp),
+    const int rc = union_pass(ctx, dtab, count, NV, NU, M, nmax, rank_type, static_cast<uint32_t*>(up), static_cast<uint8_t*>(fp),
                               sn, offd, usev, vout, tot);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);

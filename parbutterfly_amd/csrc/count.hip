@@ -57,7 +57,13 @@ struct CountArgs {
   uint32_t rerun;          // this launch re-runs such endpoints with full-size tables
   unsigned long long* hot; // replicated accumulators for ranks < HOT_N (HOT_R replicas each)
   uint32_t cb;             // count bits of a packed 32-bit hash slot (32 - key bits)
+  // A union of graphs with graph-local entries: lo[r] is the first rank of r's graph, every
+  // entry of r's list holds (rank - lo[r]) << 1 | flag, and the tables are keyed by the local
+  // rank, so each graph behaves as if counted alone.  Null for a single graph (lo = 0).
+  const uint32_t* lo;
 };
+
+__device__ __forceinline__ uint32_t lo_of(const CountArgs& a, uint32_t i) { return a.lo ? a.lo[i] : 0u; }
 
 // HBM fallback: dense counters in HBM (pool slice per block), used list in HBM, one workgroup
 // per endpoint.  Only endpoints that no LDS table can hold come here (a workgroup hash that
@@ -79,6 +85,7 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
     if (idx >= nlist) break;
     const uint32_t i = a.list[idx];
     const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
+    const uint32_t lo = lo_of(a, i), il = i - lo;  // entries and keys are local to i's graph
     const bool i_in_S = (a.redg[io] & 1u) == 0u;  // neighbours of i carry flag 1 iff i is on O
     if (tid == 0) sh.nused = 0;
     team_sync<TEAM>();
@@ -89,9 +96,9 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
         uint32_t L = 0, vo = 0, v = 0;
         if (j < deg) {
           v = a.redg[io + j] >> 1;
-          vo = a.roff[v];
-          const uint32_t ip = a.inpref[v];
-          L = (v < i) ? ip : rank_lower_bound(a.redg + vo, ip, i);
+          vo = a.roff[v + lo];
+          const uint32_t ip = a.inpref[v + lo];
+          L = (v < il) ? ip : rank_lower_bound(a.redg + vo, ip, il);
         }
         uint32_t T;
         const uint32_t incl = team_incl_scan<TEAM>(L, sh, &T);
@@ -119,7 +126,7 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
           const unsigned long long s = sh.acc[tid];
           if (s) {
             if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[io + j]], s);
-            else atomicAdd(&a.B[v], s);
+            else atomicAdd(&a.B[v + lo], s);
           }
         }
         team_sync<TEAM>();
@@ -132,7 +139,7 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
           if (c > 1u) {
             const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
             sum += b;
-            if constexpr (MODE == MODE_VERT) atomicAdd(&a.B[key], b);
+            if constexpr (MODE == MODE_VERT) atomicAdd(&a.B[key + lo], b);
           }
         }
         sum = team_sum<TEAM>(sum, sh);
@@ -399,7 +406,7 @@ struct Track {
 // A chunk: up to 64 segments (endpoint i, centre v), one per lane.  Segment s covers the
 // wedge indices [P, P + L) of the chunk, i.e. the entries N(v)[0, L) below min(i, v).
 struct Chunk {
-  uint32_t v;      // centre rank
+  uint32_t v;      // centre rank (global)
   uint32_t slot;   // ranked slot of (i -> v): the edge credited in EDGE mode
   uint32_t L, P;   // segment length and first wedge index
   uint32_t T;      // wedges of the chunk (wave-uniform)
@@ -407,13 +414,13 @@ struct Chunk {
   uint32_t td;     // grouped tables: the endpoint's sub-table (word offset | log2 size << 16)
 };
 
-__device__ __forceinline__ Chunk load_chunk(const CountArgs& a, uint32_t slot, bool valid, uint32_t td) {
+__device__ __forceinline__ Chunk load_chunk(const CountArgs& a, uint32_t slot, bool valid, uint32_t td, uint32_t lo) {
   Chunk ch;
   ch.v = 0; ch.L = 0; ch.slot = slot; ch.td = td;
   uint32_t vo = 0;
   if (valid) {
     const uint2 si = a.sinfo[slot];
-    ch.v = a.redg[slot] >> 1;
+    ch.v = (a.redg[slot] >> 1) + lo;  // global rank of the centre
     vo = si.x;
     ch.L = si.y;
   }
@@ -655,14 +662,15 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
 // long lists (several waves) the waves take different 64-neighbour chunks; otherwise they
 // share the windows of each chunk.  Returns false when a tracked table overflowed.
 template <int MODE, bool PASS2, int TM>
-__device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t jbeg,
-                                          uint32_t jend, uint32_t w, uint32_t nw, WaveCtx& wc, const Track& tr) {
+__device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t lo,
+                                          uint32_t jbeg, uint32_t jend, uint32_t w, uint32_t nw, WaveCtx& wc,
+                                          const Track& tr) {
   const bool split = nw > 1 && jend - jbeg >= 8 * 64;
   const uint32_t cstart = split ? w * 64 : 0u, cstep = split ? nw * 64 : 64u;
   const uint32_t wstart = split ? 0u : w * WIN_SPAN, wstep = split ? WIN_SPAN : nw * WIN_SPAN;
   for (uint32_t c0 = jbeg + cstart; c0 < jend; c0 += cstep) {
     const uint32_t j = c0 + __lane_id();
-    const Chunk ch = load_chunk(a, io + j, j < jend, 0u);
+    const Chunk ch = load_chunk(a, io + j, j < jend, 0u, lo);
     if (!chunk_windows<MODE, PASS2, TM, false>(a, t, ch, wstart, wstep, wc, tr)) return false;
   }
   return true;
@@ -672,7 +680,7 @@ __device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint
 // Cells q = gt, gt + gn, ... of `cells`.
 template <int MODE, int TM>
 __device__ __forceinline__ unsigned long long sweep(const CountArgs& a, const Tab& t, uint32_t cells, uint32_t gt,
-                                                    uint32_t gn) {
+                                                    uint32_t gn, uint32_t lo) {
   unsigned long long sum = 0;
   for (uint32_t q = gt; q < cells; q += gn) {
     uint32_t key, c;
@@ -690,7 +698,7 @@ __device__ __forceinline__ unsigned long long sweep(const CountArgs& a, const Ta
     if (c > 1u) {
       const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
       sum += b;
-      if constexpr (MODE == MODE_VERT) credit(a, key, b);
+      if constexpr (MODE == MODE_VERT) credit(a, key + lo, b);
     }
   }
   return sum;
@@ -699,15 +707,15 @@ __device__ __forceinline__ unsigned long long sweep(const CountArgs& a, const Ta
 // Heavy endpoint, whole workgroup.  Returns false when its hash table overflowed (nothing was
 // credited).
 template <int MODE, int TM>
-__device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t deg, bool i_in_S,
-                              uint32_t cells, WaveCtx& wc, const Track& tr, unsigned long long* red,
+__device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t lo, uint32_t deg,
+                              bool i_in_S, uint32_t cells, WaveCtx& wc, const Track& tr, unsigned long long* red,
                               unsigned long long* total_acc) {
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  enum_pass<MODE, false, TM>(a, t, i, io, 0, deg, w, NWAVE, wc, tr);
+  enum_pass<MODE, false, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr);
   __syncthreads();
   if (lds_load(tr.ovf)) return false;
   if (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S)) {
-    unsigned long long sum = pbf_wave_sum(sweep<MODE, TM>(a, t, cells, tid, TPB));
+    unsigned long long sum = pbf_wave_sum(sweep<MODE, TM>(a, t, cells, tid, TPB, lo));
     if (lane == 0) red[w] = sum;
     __syncthreads();
     if (tid == 0) {
@@ -719,7 +727,8 @@ __device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint
       }
     }
   }
-  if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S)) enum_pass<MODE, true, TM>(a, t, i, io, 0, deg, w, NWAVE, wc, tr);
+  if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S))
+    enum_pass<MODE, true, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr);
   __syncthreads();
   return true;
 }
@@ -730,14 +739,15 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
   const uint32_t tid = threadIdx.x;
   const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
   const bool i_in_S = (a.redg[io] & 1u) == 0u;
+  const uint32_t lo = lo_of(a, i), il = i - lo;  // keys are local ranks below il
   Tab t{pool, 0, 0, a.cb, nullptr, 0};
   Track tr{nused, ovf, 0xFFFFFFFFu, false};
   int tm;
-  uint32_t words, fill = 0, cells = i;
-  if (i < D32_MAX) {
-    tm = TM_D32; words = i;
-  } else if (i < D16_MAX && deg < 65536u) {
-    tm = TM_D16; words = (i + 1) >> 1;
+  uint32_t words, fill = 0, cells = il;
+  if (il < D32_MAX) {
+    tm = TM_D32; words = il;
+  } else if (il < D16_MAX && deg < 65536u) {
+    tm = TM_D16; words = (il + 1) >> 1;
   } else {
     const uint32_t bnd = a.bound[i];
     const bool packed = a.cb < 32u && deg < (1u << a.cb);
@@ -763,10 +773,10 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
   if (tid == 0) { *nused = 0; *ovf = 0; }
   __syncthreads();
   switch (tm) {
-    case TM_D32: return block_item_tm<MODE, TM_D32>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
-    case TM_D16: return block_item_tm<MODE, TM_D16>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
-    case TM_HASH32: return block_item_tm<MODE, TM_HASH32>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
-    default: return block_item_tm<MODE, TM_HASH>(a, t, i, io, deg, i_in_S, cells, wc, tr, red, total_acc);
+    case TM_D32: return block_item_tm<MODE, TM_D32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
+    case TM_D16: return block_item_tm<MODE, TM_D16>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
+    case TM_HASH32: return block_item_tm<MODE, TM_HASH32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
+    default: return block_item_tm<MODE, TM_HASH>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
   }
 }
 
@@ -805,6 +815,7 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
   const bool own = lane < nk;
   const uint32_t i_k = own ? rc.x : 0u, io_k = own ? rc.y : 0u, deg_k = own ? rc.z : 0u;
   const uint32_t lg_k = own ? rc.w : 0u, cap_k = own ? 1u << rc.w : 0u;
+  const uint32_t lo_k = own ? lo_of(a, i_k) : 0u;
   const uint32_t cinc = wave_incl_add(cap_k);
   const uint32_t words = lane_read(cinc, 63);  // <= LT_WORDS by construction of the groups
   const uint32_t off_k = cinc - cap_k;
@@ -829,7 +840,8 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
     const uint32_t slot = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)io_k) + f -
                           (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)dex_k);
     const uint32_t td = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)td_k);
-    return load_chunk(a, slot, f < DS, td);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(kb, (int)lo_k);
+    return load_chunk(a, slot, f < DS, td, lo);
   };
   bool ok = true;
   if (DS <= 64) {
@@ -861,6 +873,10 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
         *(lds_u32x4*)(tw + q) = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
       }
       const uint32_t cur[4] = {f.x, f.y, f.z, f.w};
+      // the 4 words lie in one sub-table (sub-tables are multiples of 64 words); every lane
+      // takes part in the cross-lane steps (see win_credit)
+      const uint32_t kq = lane_upper(off_k, nk, q);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(kq << 2), (int)lo_k);
       uint32_t sum = 0;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -868,11 +884,9 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
         if (c > 1u) {
           const uint32_t b = c * (c - 1u) / 2u;
           sum += b;
-          if constexpr (MODE == MODE_VERT) credit(a, cur[k] >> a.cb, b);
+          if constexpr (MODE == MODE_VERT) credit(a, (cur[k] >> a.cb) + lo, b);
         }
       }
-      // every lane takes part in the cross-lane steps (see win_credit)
-      const uint32_t kq = lane_upper(off_k, nk, q);
       const uint32_t k = in ? kq : 64u;
       const uint32_t x = wave_seg_incl_add(sum, k);
       const uint32_t knext = wave_shl1_or(k, ~0u);
@@ -1013,6 +1027,7 @@ struct ClassOut {
   uint32_t cb;
   uint32_t light_max;   // <= LIGHT_MAX
   uint32_t light_work;
+  const uint32_t* lo;   // graph-local ranks (CountArgs::lo), or null
 };
 
 __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, const ClassOut& co) {
@@ -1025,7 +1040,8 @@ __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t
   const uint32_t lane = threadIdx.x & 63;
   uint32_t cls = 0;  // 0 none, 1 wavefront, 2 workgroup, 3 HBM
   if (valid && T) {
-    const uint32_t b = (uint32_t)(T < (uint64_t)i ? T : (uint64_t)i);
+    const uint32_t il = co.lo ? i - co.lo[i] : i;  // distinct keys are local ranks below il
+    const uint32_t b = (uint32_t)(T < (uint64_t)il ? T : (uint64_t)il);
     if (b) {
       const uint32_t est = (b > co.light_max && T <= co.light_work) ? min(b, (uint32_t)(T / 2)) : b;
       const bool light = est <= co.light_max && T <= co.light_work && cb < 32u && deg < (1u << cb);
@@ -1118,7 +1134,8 @@ constexpr uint32_t SPLIT_DEG = 1u << 12, SPLIT_CHUNK = 1u << 11, SPLIT_IMAX = 20
 __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ big,
                                                       const uint32_t* __restrict__ lens_in,
                                                       const unsigned long long* __restrict__ w64, ClassOut co,
-                                                      uint32_t* __restrict__ split, uint32_t* __restrict__ split_chunks) {
+                                                      uint32_t* __restrict__ split, uint32_t* __restrict__ split_chunks,
+                                                      uint32_t split_cap) {
   const uint32_t nbig = lens_in[2];
   const uint32_t lane = threadIdx.x & 63;
   // every wavefront runs the same number of iterations (the ballots inside need the whole wave)
@@ -1129,14 +1146,18 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
     const uint32_t i = valid ? big[t] : 0u;
     const uint32_t deg = valid ? roff[i + 1] - roff[i] : 0u;
     const unsigned long long T = valid ? w64[i] : 0ull;
-    const bool sp = valid && T && i > 0 && deg > SPLIT_DEG && deg < WIDE_DEG && i < SPLIT_IMAX;
-    classify_append(i, valid && !sp, T, deg, co);
-    const uint64_t ms = __ballot(sp);
+    const uint32_t il = (valid && co.lo) ? i - co.lo[i] : i;
+    const bool cand = valid && T && il > 0 && deg > SPLIT_DEG && deg < WIDE_DEG && il < SPLIT_IMAX;
+    const uint64_t mc = __ballot(cand);
     uint32_t b = 0;
-    if (lane == 0 && ms) b = atomicAdd(&co.lens[8], (uint32_t)__popcll(ms));
+    if (lane == 0 && mc) b = atomicAdd(&co.lens[8], (uint32_t)__popcll(mc));
     b = __shfl(b, 0, 64);
+    const uint32_t slot = b + __popcll(mc & pbf_lanemask_lt());
+    // past the capacity (several graphs of a union, each with its own low ranks) an endpoint
+    // is classified as an ordinary heavy one; the consumers clamp the count to split_cap
+    const bool sp = cand && slot < split_cap;
+    classify_append(i, valid && !sp, T, deg, co);
     if (sp) {
-      const uint32_t slot = b + __popcll(ms & pbf_lanemask_lt());
       split[slot] = i;
       split_chunks[slot] = (deg + SPLIT_CHUNK - 1) / SPLIT_CHUNK;
     }
@@ -1255,12 +1276,12 @@ __global__ __launch_bounds__(256) void k_light_scatter(const uint32_t* __restric
 template <int MODE>
 __global__ __launch_bounds__(TPB) void k_split_pass1(CountArgs a, const uint32_t* __restrict__ split,
                                                      const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ ctl,
-                                                     uint32_t* __restrict__ G) {
+                                                     uint32_t* __restrict__ G, uint32_t split_cap) {
   __shared__ __align__(16) uint32_t pool[POOL_WORDS];
   __shared__ __align__(16) uint32_t s_flags[NWAVE][WIN_SPAN];
   __shared__ unsigned long long s_acc[NWAVE][64];
   __shared__ uint32_t s_ovf;
-  const uint32_t nsplit = ctl[8], tot = ctl[9];
+  const uint32_t nsplit = min(ctl[8], split_cap), tot = ctl[9];
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   *(lds_u32x4*)((lds_u32*)s_flags[wid] + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
   s_acc[wid][lane] = 0;
@@ -1273,13 +1294,14 @@ __global__ __launch_bounds__(TPB) void k_split_pass1(CountArgs a, const uint32_t
     while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (chunk_base[mid] <= item) lo = mid; else hi = mid; }
     const uint32_t i = split[lo], c = item - chunk_base[lo];
     const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
-    for (uint32_t q = threadIdx.x; q < i; q += TPB) pool[q] = 0u;
+    const uint32_t glo = lo_of(a, i), il = i - glo;
+    for (uint32_t q = threadIdx.x; q < il; q += TPB) pool[q] = 0u;
     __syncthreads();
     const uint32_t j1 = min(deg, (c + 1) * SPLIT_CHUNK);
-    enum_pass<MODE, false, TM_D32>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
+    enum_pass<MODE, false, TM_D32>(a, t, i, io, glo, c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
     __syncthreads();
     uint32_t* row = G + (size_t)lo * SPLIT_IMAX;
-    for (uint32_t q = threadIdx.x; q < i; q += TPB)
+    for (uint32_t q = threadIdx.x; q < il; q += TPB)
       if (pool[q]) atomicAdd(&row[q], pool[q]);
     __syncthreads();
   }
@@ -1288,21 +1310,23 @@ __global__ __launch_bounds__(TPB) void k_split_pass1(CountArgs a, const uint32_t
 // Split endpoints: endpoint credits (VERT, endpoint on S) or the total, one workgroup each.
 template <int MODE>
 __global__ __launch_bounds__(TPB) void k_split_credit(CountArgs a, const uint32_t* __restrict__ split,
-                                                      const uint32_t* __restrict__ ctl, const uint32_t* __restrict__ G) {
+                                                      const uint32_t* __restrict__ ctl, const uint32_t* __restrict__ G,
+                                                      uint32_t split_cap) {
   __shared__ unsigned long long red[NWAVE];
-  const uint32_t nsplit = ctl[8];
+  const uint32_t nsplit = min(ctl[8], split_cap);
   for (uint32_t b = blockIdx.x; b < nsplit; b += gridDim.x) {
     const uint32_t i = split[b];
     const bool i_in_S = (a.redg[a.roff[i]] & 1u) == 0u;
     if (!(MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S))) continue;
     const uint32_t* row = G + (size_t)b * SPLIT_IMAX;
+    const uint32_t glo = lo_of(a, i), il = i - glo;
     unsigned long long sum = 0;
-    for (uint32_t q = threadIdx.x; q < i; q += TPB) {
+    for (uint32_t q = threadIdx.x; q < il; q += TPB) {
       const uint32_t c = row[q];
       if (c > 1u) {
         const unsigned long long v = (unsigned long long)c * (c - 1u) / 2u;
         sum += v;
-        if constexpr (MODE == MODE_VERT) credit(a, q, v);
+        if constexpr (MODE == MODE_VERT) credit(a, q + glo, v);
       }
     }
     sum = pbf_wave_sum(sum);
@@ -1324,11 +1348,11 @@ __global__ __launch_bounds__(TPB) void k_split_credit(CountArgs a, const uint32_
 template <int MODE>
 __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t* __restrict__ split,
                                                      const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ ctl,
-                                                     const uint32_t* __restrict__ G) {
+                                                     const uint32_t* __restrict__ G, uint32_t split_cap) {
   __shared__ __align__(16) uint32_t s_flags[NWAVE][WIN_SPAN];
   __shared__ unsigned long long s_acc[NWAVE][64];
   __shared__ uint32_t s_ovf;
-  const uint32_t nsplit = ctl[8], tot = ctl[9];
+  const uint32_t nsplit = min(ctl[8], split_cap), tot = ctl[9];
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   *(lds_u32x4*)((lds_u32*)s_flags[wid] + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
   s_acc[wid][lane] = 0;
@@ -1345,7 +1369,7 @@ __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t
     if (!(MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S))) continue;
     const Tab t{nullptr, 0, 0, a.cb, G + (size_t)lo * SPLIT_IMAX, 0};
     const uint32_t j1 = min(deg, (c + 1) * SPLIT_CHUNK);
-    enum_pass<MODE, true, TM_G32>(a, t, i, io, c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
+    enum_pass<MODE, true, TM_G32>(a, t, i, io, lo_of(a, i), c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
   }
 }
 
@@ -1363,11 +1387,12 @@ __global__ void k_oidx(const uint32_t* __restrict__ redg, const uint32_t* __rest
 // inside N(y) (a binary search over the in-prefix of N(y)).  The counting kernels read both
 // beside the slot, which keeps two dependent loads and the search off their critical path.
 __global__ void k_slotinfo(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
-                           const uint32_t* __restrict__ rown, const uint32_t* __restrict__ inpref, int64_t m2,
-                           uint2* __restrict__ sinfo) {
+                           const uint32_t* __restrict__ rown, const uint32_t* __restrict__ inpref,
+                           const uint32_t* __restrict__ lo, int64_t m2, uint2* __restrict__ sinfo) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = rown[p], y = redg[p] >> 1;
-    const uint32_t yo = roff[y], ip = inpref[y];
+    const uint32_t rg = rown[p], l = lo ? lo[rg] : 0u;
+    const uint32_t r = rg - l, y = redg[p] >> 1;  // both local to the graph of rg
+    const uint32_t yo = roff[y + l], ip = inpref[y + l];
     sinfo[p] = make_uint2(yo, y < r ? ip : rank_lower_bound(redg + yo, ip < r ? ip : r, r));
   }
 }
@@ -1417,7 +1442,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   uint32_t* chunk_base = big + 2 * n;
   unsigned long long* w64 = static_cast<unsigned long long*>(up);
   hipLaunchKernelGGL(k_slotinfo, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.rown,
-                     R.inpref, R.m2, const_cast<uint2*>(a.sinfo));
+                     R.inpref, R.lo, R.m2, const_cast<uint2*>(a.sinfo));
   PBF_CHECK(pbf_scratch(ctx, S_LPLAN, 4 * (64 + 2 * LG_BLOCKS * LBUCKETS) + 8 * LBUCKETS, &pl));
   uint32_t* plan = static_cast<uint32_t*>(pl);
   uint32_t* bh = plan + 64;
@@ -1431,7 +1456,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
   uint32_t* hbm_list = lists + 5 * n + 1;
   const uint32_t lmax = ctx->tune.light_max < LIGHT_MAX ? ctx->tune.light_max : LIGHT_MAX;
-  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb, lmax, ctx->tune.light_work};
+  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb, lmax, ctx->tune.light_work, R.lo};
   const unsigned grid = (unsigned)(ctx->num_cus * (ctx->tune.blocks_per_cu ? ctx->tune.blocks_per_cu : 4));
   hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
                      n, co, big, big_chunks, w64);
@@ -1440,7 +1465,9 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
                      big, chunk_base, ctl, ctl + 7, w64);
   // split endpoints: list, chunk counts, chunk bases, dense rows
   void *sp, *gp;
-  const int64_t max_split = std::min<int64_t>(SPLIT_IMAX, 2 * g_m2_over(R.m2, SPLIT_DEG) + 8);
+  // a single graph has at most SPLIT_IMAX split endpoints (distinct ranks below it); a union
+  // with graph-local ranks may have more, and the capacity then bounds them
+  const int64_t max_split = std::min<int64_t>(R.lo ? 4 * SPLIT_IMAX : SPLIT_IMAX, 2 * g_m2_over(R.m2, SPLIT_DEG) + 8);
   PBF_CHECK(pbf_scratch(ctx, S_SPLIT, 4 * (size_t)(3 * max_split + 64), &sp));
   PBF_CHECK(pbf_scratch(ctx, S_GROWS, 4 * (size_t)max_split * SPLIT_IMAX, &gp));
   uint32_t* split = static_cast<uint32_t*>(sp);
@@ -1448,14 +1475,17 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   uint32_t* split_base = split + 2 * max_split;
   uint32_t* G = static_cast<uint32_t*>(gp);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split, 256, 64)), dim3(256), 0, ctx->stream, split_chunks, 0u, max_split);
-  hipLaunchKernelGGL(k_classify_big, dim3(64), dim3(256), 0, ctx->stream, R.roff, big, ctl, w64, co, split, split_chunks);
+  hipLaunchKernelGGL(k_classify_big, dim3(64), dim3(256), 0, ctx->stream, R.roff, big, ctl, w64, co, split, split_chunks,
+                     (uint32_t)max_split);
   PBF_CHECK(pbf_scan_excl_u32(ctx, split_chunks, split_base, max_split, ctl + 9));
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split * SPLIT_IMAX, 256, 4096)), dim3(256), 0, ctx->stream, G, 0u,
                      max_split * (int64_t)SPLIT_IMAX);
-  hipLaunchKernelGGL(k_split_pass1<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G);
-  hipLaunchKernelGGL(k_split_credit<MODE>, dim3(256), dim3(TPB), 0, ctx->stream, a, split, ctl, G);
+  hipLaunchKernelGGL(k_split_pass1<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G,
+                     (uint32_t)max_split);
+  hipLaunchKernelGGL(k_split_credit<MODE>, dim3(256), dim3(TPB), 0, ctx->stream, a, split, ctl, G, (uint32_t)max_split);
   if (MODE != MODE_TOTAL)
-    hipLaunchKernelGGL(k_split_pass2<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G);
+    hipLaunchKernelGGL(k_split_pass2<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G,
+                       (uint32_t)max_split);
   // light endpoints: sort into groups
   hipLaunchKernelGGL(k_light_hist<MODE>, dim3(LG_BLOCKS), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound, lists, ctl,
                      bh);
@@ -1537,9 +1567,13 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
   a.n = n;
   a.total = static_cast<unsigned long long*>(mp) + 12;
   a.hot = static_cast<unsigned long long*>(hp);
+  a.lo = R.lo;
   {
-    uint32_t kb = 0;  // key bits: n <= 2^kb - 1, so a packed key is never all ones
-    while (kb < 32 && ((uint64_t)1 << kb) - 1 < (uint64_t)n) kb++;
+    // key bits: keys (local ranks) are below key_range <= 2^kb - 1, so a packed key is never
+    // all ones
+    const int64_t key_range = (R.lo && opts.key_range > 0) ? opts.key_range : n;
+    uint32_t kb = 0;
+    while (kb < 32 && ((uint64_t)1 << kb) - 1 < (uint64_t)key_range) kb++;
     a.cb = 32 - kb;
   }
   if (!ctx->hot_clear) {

@@ -53,6 +53,7 @@ enum ScratchSlot {
   S_BAUX,        // batch: per-graph |S|, offsets, sides
   S_MLP,         // selector MLP: parameters, Adam moments, samples
   S_BFEAT,       // batch features: per-graph statistics
+  S_RLO,         // batch: first rank of each rank's graph block (graph-local entries)
   S_NUM_SLOTS
 };
 
@@ -114,6 +115,9 @@ struct RankedGraph {
   uint32_t* rown;     // [2m] owner rank of each slot
   int64_t wedges;     // W under this ranking (-1 when not read back)
   unsigned long long* wedges_dev;  // W on the device
+  // Union of graphs ranked graph by graph (each graph a contiguous block of ranks): lo[r] is
+  // the first rank of r's block and entries hold ranks relative to it.  Null otherwise.
+  uint32_t* lo = nullptr;
 };
 
 // scan.hip
@@ -126,8 +130,9 @@ int pbf_fill_u64(pbf_context* ctx, uint64_t* p, uint64_t v, int64_t n);
 // rank.hip
 int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedges);
 // SIDE ranking needs the count side: use_v, or the per-vertex flags sflag of a batch.
+// sort = false: only the keys (S_KEY) are computed; the caller orders the vertices itself.
 int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R, int use_v = 0,
-                 const uint8_t* sflag = nullptr);
+                 const uint8_t* sflag = nullptr, bool sort = true);
 // sflag (optional): per combined id, 1 when the vertex lies on the count side S (a batch of
 // graphs with their own sides); else S is U when use_v, V otherwise.  sync: read W back into
 // R->wedges (else only R->wedges_dev holds it, and the call never waits for the device).
@@ -145,6 +150,7 @@ struct CountOpts {
   bool sync = true;
   bool rank_space = false;
   unsigned long long** B_out = nullptr;
+  int64_t key_range = 0;  // with graph-local ranks (R.lo): the largest graph's vertex count
 };
 int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGraph& R, int mode,
                   uint64_t* out_dev /* VERT: n_S, EDGE: m, TOTAL: 1 */, const uint32_t* eti,
@@ -160,6 +166,17 @@ __device__ __forceinline__ uint32_t pbf_bitlen(uint32_t d) { return d ? 32u - __
 __device__ __forceinline__ uint64_t pbf_lanemask_lt() {
   const uint32_t lane = __lane_id();
   return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Lanes of the wavefront holding the same 6-bit key (inactive lanes excluded via `active`).
+__device__ __forceinline__ uint64_t pbf_match_key6(uint32_t k, uint64_t active) {
+  uint64_t m = active;
+#pragma unroll
+  for (int b = 0; b < 6; b++) {
+    const uint64_t bal = __ballot((k >> b) & 1u);
+    m &= ((k >> b) & 1u) ? bal : ~bal;
+  }
+  return m;
 }
 
 // Inclusive wave scan (64 lanes).

@@ -63,17 +63,6 @@ __global__ __launch_bounds__(1024) void k_key_hist(const uint8_t* __restrict__ k
   if (threadIdx.x < KEYS) hist[(size_t)(KEYS - 1 - threadIdx.x) * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// Lanes of the wavefront holding the same 6-bit key (inactive lanes excluded via `active`).
-__device__ __forceinline__ uint64_t match_key6(uint32_t k, uint64_t active) {
-  uint64_t m = active;
-#pragma unroll
-  for (int b = 0; b < 6; b++) {
-    uint64_t bal = __ballot((k >> b) & 1u);
-    m &= ((k >> b) & 1u) ? bal : ~bal;
-  }
-  return m;
-}
-
 __global__ __launch_bounds__(1024) void k_key_scatter(const uint8_t* __restrict__ key, int64_t n,
                                                       const uint32_t* __restrict__ hist_scanned,
                                                       uint32_t* __restrict__ order, uint32_t* __restrict__ rank_of) {
@@ -85,7 +74,7 @@ __global__ __launch_bounds__(1024) void k_key_scatter(const uint8_t* __restrict_
   bool act = x < n;
   uint32_t k = act ? key[x] : 0;
   uint64_t active = __ballot(act);
-  uint64_t mm = match_key6(k, active);
+  uint64_t mm = pbf_match_key6(k, active);
   uint32_t rank_in_wave = __popcll(mm & pbf_lanemask_lt());
   if (act && rank_in_wave == 0) wc[wid][k] = __popcll(mm);
   __syncthreads();
@@ -116,8 +105,9 @@ __device__ __forceinline__ uint32_t entry_flag(int use_v, const uint8_t* __restr
 __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
                                                  const uint8_t* __restrict__ key,
                                                  const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
-                                                 uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
-                                                 uint32_t* __restrict__ inpref, uint32_t* __restrict__ rown) {
+                                                 const uint32_t* __restrict__ lo, uint32_t* __restrict__ redg,
+                                                 uint32_t* __restrict__ rpos, uint32_t* __restrict__ inpref,
+                                                 uint32_t* __restrict__ rown) {
   __shared__ uint32_t cnt[4][KEYS + 1];
   __shared__ uint32_t run[4][KEYS + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -132,6 +122,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     const int64_t nbr_shift = xv ? g.nv : 0;  // combined id of neighbour = adj[j] + shift
     const uint32_t flag = entry_flag(use_v, sflag, x, xv);
     const uint32_t r = rank_of[x];
+    const uint32_t lox = lo ? lo[r] : 0u;  // graph-local entries (a union ranked graph by graph)
     if (lane < KEYS + 1) { cnt[wid][lane] = 0; run[wid][lane] = 0; }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -158,12 +149,12 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
       uint32_t y = 0, ky = 0;
       if (act) { y = (uint32_t)(adj[j] + nbr_shift); ky = key[y]; }
       const uint64_t active = __ballot(act);
-      const uint64_t mm = match_key6(ky, active);
+      const uint64_t mm = pbf_match_key6(ky, active);
       const uint32_t rk = __popcll(mm & pbf_lanemask_lt());
       uint32_t slot = 0;
       if (act) {
         slot = rbase + run[wid][ky] + rk;
-        redg[slot] = (rank_of[y] << 1) | flag;
+        redg[slot] = ((rank_of[y] - lox) << 1) | flag;
         rown[slot] = r;
         if (rpos) rpos[slot] = obase + j;
       }
@@ -266,8 +257,8 @@ __global__ __launch_bounds__(256) void k_rl_offsets(DevGraph g, const uint8_t* _
 __global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
                                                     const uint8_t* __restrict__ key,
                                                     const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
-                                                    LongLists L, uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
-                                                    uint32_t* __restrict__ rown) {
+                                                    const uint32_t* __restrict__ lo, LongLists L, uint32_t* __restrict__ redg,
+                                                    uint32_t* __restrict__ rpos, uint32_t* __restrict__ rown) {
   __shared__ uint32_t run[4][KEYS + 1];
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t nl = *L.nlong, tot = *L.nitems;
@@ -281,6 +272,7 @@ __global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const
     const bool xv = x < g.nv;
     const uint32_t flag = entry_flag(use_v, sflag, x, xv);
     const uint32_t rx = rank_of[x];
+    const uint32_t lox = lo ? lo[rx] : 0u;
     const uint32_t rbase = roff[rx];
     if (lane < KEYS) run[wid][lane] = L.hist[(size_t)item * KEYS + lane];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -291,11 +283,11 @@ __global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const
       const bool act = j < j1;
       uint32_t y = 0, ky = 0;
       if (act) { y = (uint32_t)(adj[j] + sh); ky = key[y]; }
-      const uint64_t mm = match_key6(ky, __ballot(act));
+      const uint64_t mm = pbf_match_key6(ky, __ballot(act));
       const uint32_t rk = __popcll(mm & pbf_lanemask_lt());
       if (act) {
         const uint32_t slot = rbase + run[wid][ky] + rk;
-        redg[slot] = (rank_of[y] << 1) | flag;
+        redg[slot] = ((rank_of[y] - lox) << 1) | flag;
         rown[slot] = rx;
         if (rpos) rpos[slot] = obase + j;
       }
@@ -502,7 +494,8 @@ int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedge
 }
 
 // Fills R->order / R->rank_of (and the key array in S_KEY) for ADEG or ACOCORE.
-int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R, int use_v, const uint8_t* sflag) {
+int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph* R, int use_v, const uint8_t* sflag,
+                 bool sort) {
   const int64_t n = g.nv + g.nu;
   R->n = n;
   R->m2 = 2 * g.m;
@@ -546,6 +539,7 @@ int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph
   } else {
     return PBF_ERR_UNSUPPORTED;
   }
+  if (!sort) return PBF_OK;
   const int64_t nb = (n + 1023) / 1024;
   PBF_CHECK(pbf_scratch(ctx, S_HIST, 4 * (size_t)(nb * KEYS), &hp));
   uint32_t* hist = static_cast<uint32_t*>(hp);
@@ -580,7 +574,7 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
   hipLaunchKernelGGL(k_ranked_degree, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, R->order, R->roff);
   PBF_CHECK(pbf_scan_excl_u32(ctx, R->roff, R->roff, n + 1, nullptr));
   hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 4, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of,
-                     R->roff, R->redg, R->rpos, R->inpref, R->rown);
+                     R->roff, R->lo, R->redg, R->rpos, R->inpref, R->rown);
   // long lists, chunk-parallel
   {
     const int64_t max_items = 2 * (2 * g.m / RL_CHUNK) + 64;
@@ -598,8 +592,8 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
     LongLists L{llist, lbase, lctl, lctl + 1, static_cast<uint32_t*>(hp2)};
     hipLaunchKernelGGL(k_rl_hist, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, key, L);
     hipLaunchKernelGGL(k_rl_offsets, dim3(grid_for(n, 4, 2048)), dim3(256), 0, ctx->stream, g, key, R->rank_of, L, R->inpref);
-    hipLaunchKernelGGL(k_rl_scatter, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of, R->roff, L,
-                       R->redg, R->rpos, R->rown);
+    hipLaunchKernelGGL(k_rl_scatter, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of, R->roff,
+                       R->lo, L, R->redg, R->rpos, R->rown);
   }
   unsigned long long* acc = static_cast<unsigned long long*>(mp) + 8;
   PBF_CHECK(pbf_fill_u64(ctx, reinterpret_cast<uint64_t*>(acc), 0, 1));

@@ -41,6 +41,20 @@ def test_union_host(ctx, rank_type):
     _check(gs, vert, tot, rank_type)
 
 
+def test_union_split_capacity(ctx):
+    # 8300 copies of K_{2,4097}: the second V vertex of every copy (degree 4097, graph-local
+    # rank 1) is split across workgroups; there are more such endpoints than the split
+    # capacity (8192), so the rest run as ordinary workgroup endpoints.  Closed form: every V
+    # vertex and every graph has C(4097, 2) butterflies.
+    dg = complete_bipartite(2, 4097).to_device(ctx)
+    k = 8300
+    vout, voff, tot = (x.numpy() for x in ctx.count_batch([dg] * k, "ADEG", mode="union"))
+    c = 4097 * 4096 // 2
+    assert np.array_equal(voff[:k + 1], 2 * np.arange(k + 1, dtype=np.uint64))
+    assert np.all(tot[:k] == c)
+    assert np.all(vout[:2 * k] == c)
+
+
 def test_plain_equals_union(ctx):
     gs = _mixed(12, 200)
     v1, t1 = ctx.count_batch(gs, mode="plain")
