@@ -216,7 +216,7 @@ int pbf_set_tuning(pbf_context* ctx, int light_max, int light_work, int blocks_p
  * nv + 1, nu + 1, m + 1; per side the log2 of the mean degree, of the mean squared degree and of
  * the maximum degree; log2 of wV + 1 and wU + 1 (cmpWedgeCounts); log2 of W + 1 under SIDE,
  * ADEG and ACOCORE (exact rank-filtered wedge counts, rankWedgeF butterfly_utils.h:216-237);
- * and use_v.  One pass over the disjoint union per ranking. */
+ * use_v; and log2 of the density m / (nv nu).  One pass over the disjoint union per ranking. */
 #define PBF_MLP_IN 16
 #define PBF_MLP_HIDDEN 64
 #define PBF_MLP_OUT 9
