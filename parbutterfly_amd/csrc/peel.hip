@@ -55,24 +55,30 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x)
   return x;
 }
 
+// The round kernels visit items ids[0, n) (every item that may still be live), or 0..n-1
+// when ids is null.
 // k = min D over live items (state != 0).
-__global__ void k_min_live(const unsigned long long* __restrict__ D, const uint8_t* __restrict__ state, int64_t n, Ctl c) {
+__global__ void k_min_live(const unsigned long long* __restrict__ D, const uint8_t* __restrict__ state,
+                           const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
   unsigned long long mn = KNONE;
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = ids ? ids[p] : (uint32_t)p;
     if (state[x]) { unsigned long long d = D[x]; mn = d < mn ? d : mn; }
+  }
   mn = wave_min_u64(mn);
   if ((threadIdx.x & 63) == 0 && mn != KNONE) atomicMin(c.k, mn);
 }
 
 // Frontier selection: live items with D == k become current (state 2) and are listed by work
 // class (small / large by `work`); `live` is decremented by the number selected.
-__global__ void k_select(const unsigned long long* __restrict__ D, uint8_t* __restrict__ state, int64_t n,
-                         const uint32_t* __restrict__ work, uint32_t small_max, uint32_t* __restrict__ small_list,
-                         uint32_t* __restrict__ large_list, Ctl c) {
+__global__ void k_select(const unsigned long long* __restrict__ D, uint8_t* __restrict__ state,
+                         const uint32_t* __restrict__ ids, int64_t n, const uint32_t* __restrict__ work,
+                         uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list, Ctl c) {
   const unsigned long long k = *c.k;
   if (k == KNONE) return;
   uint32_t nsel = 0;
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = ids ? ids[p] : (uint32_t)p;
     if (state[x] == 1 && D[x] == k) {
       state[x] = 2;
       nsel++;
@@ -100,10 +106,12 @@ __global__ void k_update(const uint32_t* __restrict__ touched, uint32_t* __restr
 }
 
 // Current items die at the end of the round.
-__global__ void k_retire(uint8_t* __restrict__ state, int64_t n, Ctl c) {
+__global__ void k_retire(uint8_t* __restrict__ state, const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
   if (*c.k == KNONE) return;
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = ids ? ids[p] : (uint32_t)p;
     if (state[x] == 2) state[x] = 0;
+  }
 }
 
 __device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* touched, uint32_t* ntouched) {
@@ -255,13 +263,18 @@ __global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict_
 }
 
 // ---------------- edge peel walk ----------------
+// The walk reads a compacted adjacency: every entry is (neighbour, edge id), edge id = the
+// edge's O-CSR index; entries of dead edges are dropped whenever the live edges fall to half
+// of the entries (the first view is the input CSR itself).
 struct PeelEArgs {
-  const uint32_t* offS;
-  const uint32_t* edgS;
-  const uint32_t* offO;
-  const uint32_t* edgO;
-  const uint32_t* eti;      // S-CSR position -> O-CSR index
-  const uint32_t* owner_o;  // O vertex of each O-CSR index
+  const uint32_t* offS;     // S side: offsets, neighbours (O vertices), edge ids
+  const uint32_t* nbrS;
+  const uint32_t* idS;
+  const uint32_t* offO;     // O side: offsets, neighbours (S vertices), edge ids
+  const uint32_t* nbrO;
+  const uint32_t* idO;
+  const uint32_t* u_of;     // S vertex of each edge id (the input O-CSR)
+  const uint32_t* owner_o;  // O vertex of each edge id
   const uint8_t* state;     // 0 dead, 1 live, 2 peeled this round
   unsigned long long* B;
   const uint32_t* list;
@@ -294,20 +307,21 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     __syncthreads();
     if (idx >= nlist) break;
     const uint32_t e = a.list[idx];
-    const uint32_t v = a.owner_o[e], u = a.edgO[e];
+    const uint32_t v = a.owner_o[e], u = a.u_of[e];
     const uint32_t vo = a.offO[v], vdeg = a.offO[v + 1] - vo;
     const uint32_t uo = a.offS[u], udeg = a.offS[u + 1] - uo;
     const bool cached = udeg <= (uint32_t)UCAP;
     if (cached)
-      for (uint32_t q = tid; q < udeg; q += TEAM) { nu_v[q] = a.edgS[uo + q]; nu_f[q] = a.eti[uo + q]; }
+      for (uint32_t q = tid; q < udeg; q += TEAM) { nu_v[q] = a.nbrS[uo + q]; nu_f[q] = a.idS[uo + q]; }
     __syncthreads();
     for (uint32_t c0 = 0; c0 < vdeg; c0 += TEAM) {
       const uint32_t j = c0 + tid;
-      uint32_t L = 0, so = 0, u2 = 0;
+      uint32_t L = 0, so = 0, u2 = 0, f1 = 0;
       if (j < vdeg) {
         const uint32_t q = vo + j;  // f1 = (v, u2)
-        u2 = a.edgO[q];
-        if (u2 != u && usable(a.state, q, e)) {
+        u2 = a.nbrO[q];
+        f1 = a.idO[q];
+        if (u2 != u && usable(a.state, f1, e)) {
           so = a.offS[u2];
           L = a.offS[u2 + 1] - so;
         }
@@ -322,9 +336,9 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
         const uint32_t s = seg_of<TEAM>(sh.P, w);
         const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
         const uint32_t p2 = sh.vs[s] + kk;
-        const uint32_t v2 = a.edgS[p2];
+        const uint32_t v2 = a.nbrS[p2];
         if (v2 == v) continue;
-        const uint32_t f2 = a.eti[p2];
+        const uint32_t f2 = a.idS[p2];
         if (!usable(a.state, f2, e)) continue;
         // f3 = (v2, u): binary search v2 in N(u)
         uint32_t lo = 0, hi = udeg;
@@ -332,10 +346,10 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
           while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nu_v[mid] < v2) lo = mid + 1; else hi = mid; }
           if (lo == udeg || nu_v[lo] != v2) continue;
         } else {
-          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (a.edgS[uo + mid] < v2) lo = mid + 1; else hi = mid; }
-          if (lo == udeg || a.edgS[uo + lo] != v2) continue;
+          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (a.nbrS[uo + mid] < v2) lo = mid + 1; else hi = mid; }
+          if (lo == udeg || a.nbrS[uo + lo] != v2) continue;
         }
-        const uint32_t f3 = cached ? nu_f[lo] : a.eti[uo + lo];
+        const uint32_t f3 = cached ? nu_f[lo] : a.idS[uo + lo];
         if (!usable(a.state, f3, e)) continue;
         atomicAdd(&sh.acc[s], 1ull);
         if (a.state[f2] == 1) { atomicAdd(&a.B[f2], ~0ull); touch(f2, a.tflag, a.touched, a.ntouched); }
@@ -344,10 +358,9 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
       __syncthreads();
       if (j < vdeg) {
         const unsigned long long cnt = sh.acc[tid];
-        const uint32_t q = vo + j;
-        if (cnt && a.state[q] == 1) {
-          atomicAdd(&a.B[q], (unsigned long long)(-(long long)cnt));
-          touch(q, a.tflag, a.touched, a.ntouched);
+        if (cnt && a.state[f1] == 1) {
+          atomicAdd(&a.B[f1], (unsigned long long)(-(long long)cnt));
+          touch(f1, a.tflag, a.touched, a.ntouched);
         }
       }
       __syncthreads();
@@ -355,23 +368,49 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   }
 }
 
-// work[e] = sum_{u2 in N(v)} deg(u2) for e = (v, u): merge work of the edge's walk (saturating).
+// work[e] = sum_{u2 in N(v)} deg(u2) for e = (v, u): merge work of the edge's walk (saturating),
+// over the current adjacency; owner_o (when given) receives each edge's O vertex.
 __global__ __launch_bounds__(256) void k_peele_work(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ offO,
-                                                    const uint32_t* __restrict__ edgO, int64_t no, uint32_t* __restrict__ work,
-                                                    uint32_t* __restrict__ owner_o) {
+                                                    const uint32_t* __restrict__ nbrO, const uint32_t* __restrict__ idO,
+                                                    int64_t no, uint32_t* __restrict__ work, uint32_t* __restrict__ owner_o) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int64_t v = (int64_t)blockIdx.x * 4 + wid; v < no; v += (int64_t)gridDim.x * 4) {
     uint64_t s = 0;
     const uint32_t b = offO[v], e = offO[v + 1];
     for (uint32_t q = b + lane; q < e; q += 64) {
-      const uint32_t u2 = edgO[q];
+      const uint32_t u2 = nbrO[q];
       s += offS[u2 + 1] - offS[u2];
-      owner_o[q] = (uint32_t)v;
+      if (owner_o) owner_o[idO[q]] = (uint32_t)v;
     }
     s = pbf_wave_sum(s);
     const uint32_t w = (uint32_t)(s < 0xFFFFFFFFull ? s : 0xFFFFFFFFull);
-    for (uint32_t q = b + lane; q < e; q += 64) work[q] = w;
+    for (uint32_t q = b + lane; q < e; q += 64) work[idO[q]] = w;
   }
+}
+
+__global__ void k_iota(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+    p[x] = (uint32_t)x;
+}
+
+// Adjacency compaction: flag[p] = entry p's edge is live (flag[n] = 0), then, from the
+// exclusive scan pos of the flags, the live entries move to pos[p] (order kept) and every
+// offset off[x] becomes pos[off[x]].
+__global__ void k_live_flags(const uint32_t* __restrict__ id, int64_t n, const uint8_t* __restrict__ state,
+                             uint32_t* __restrict__ flag) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p <= n; p += (int64_t)gridDim.x * blockDim.x)
+    flag[p] = (p < n && state[id[p]] == 1) ? 1u : 0u;
+}
+__global__ void k_compact(const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ id,
+                          const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int64_t n,
+                          uint32_t* __restrict__ nbr2, uint32_t* __restrict__ id2) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    if (flag[p]) { nbr2[pos[p]] = nbr[p]; id2[pos[p]] = id[p]; }
+}
+__global__ void k_new_off(const uint32_t* __restrict__ off, int64_t nvert, const uint32_t* __restrict__ pos,
+                          uint32_t* __restrict__ off2) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x <= nvert; x += (int64_t)gridDim.x * blockDim.x)
+    off2[x] = pos[off[x]];
 }
 
 __global__ void k_init_peel(const unsigned long long* __restrict__ counts, unsigned long long* __restrict__ B,
@@ -479,8 +518,8 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   const unsigned sel_grid = grid_for(ns, 256, 4096);
   auto round = [&]() -> int {
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
-    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, ns, pb.c);
-    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, ns, pb.work, small_max,
+    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.c);
+    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.work, small_max,
                        small_list, large_list, pb.c);
     PeelVArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
@@ -494,7 +533,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>), dim3(V_HBM_BLOCKS), dim3(V_LARGE_TEAM), 0, ctx->stream, s);
     hipLaunchKernelGGL(k_update, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
                        pb.c);
-    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, ns, pb.c);
+    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, nullptr, ns, pb.c);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
   };
@@ -514,12 +553,31 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   PBF_CHECK(pbf_scratch(ctx, S_ETI, 4 * (size_t)m, &ep));
   uint32_t* eti = static_cast<uint32_t*>(ep);
   PBF_CHECK(pbf_dev_edge_to_idxs(ctx, g, use_v, eti));
+  const int64_t no = use_v ? g.nv : g.nu, ns = use_v ? g.nu : g.nv;
+  // compaction buffers: two sets of (offO, nbrO, idO, offS, nbrS, idS), plus flags and positions
+  const size_t set_words = size_t(no + 1) + size_t(ns + 1) + 4 * size_t(m);
+  void* cp;
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * (2 * set_words + 2 * size_t(m + 1)), &cp));
+  struct View { uint32_t *offO, *nbrO, *idO, *offS, *nbrS, *idS; };
+  View set[2];
+  uint32_t* w = static_cast<uint32_t*>(cp);
+  for (int k = 0; k < 2; k++) {
+    set[k].offO = w; w += no + 1;
+    set[k].nbrO = w; w += m;
+    set[k].idO = w; w += m;
+    set[k].offS = w; w += ns + 1;
+    set[k].nbrS = w; w += m;
+    set[k].idS = w; w += m;
+  }
+  uint32_t* flag = w;
+  uint32_t* pos = w + m + 1;
+  // the first view is the input CSR, edge ids = O-CSR positions (O side) / eti (S side)
+  View cur{const_cast<uint32_t*>(use_v ? g.offV : g.offU), const_cast<uint32_t*>(use_v ? g.edgV : g.edgU), set[0].idO,
+           const_cast<uint32_t*>(use_v ? g.offU : g.offV), const_cast<uint32_t*>(use_v ? g.edgU : g.edgV), eti};
+  int next = 1;
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(m, 256, 16384)), dim3(256), 0, ctx->stream, set[0].idO, m);
   PeelEArgs a{};
-  a.offS = use_v ? g.offU : g.offV;
-  a.edgS = use_v ? g.edgU : g.edgV;
-  a.offO = use_v ? g.offV : g.offU;
-  a.edgO = use_v ? g.edgV : g.edgU;
-  a.eti = eti;
+  a.u_of = use_v ? g.edgV : g.edgU;
   a.owner_o = pb.owner_o;
   a.state = pb.state;
   a.B = pb.B;
@@ -527,35 +585,82 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   a.touched = pb.touched;
   a.ntouched = pb.c.len + 3;
   a.k = pb.c.k;
-  const int64_t no = use_v ? g.nv : g.nu;
+  auto use_view = [&](const View& v) {
+    a.offO = v.offO; a.nbrO = v.nbrO; a.idO = v.idO;
+    a.offS = v.offS; a.nbrS = v.nbrS; a.idS = v.idS;
+  };
+  use_view(cur);
   hipLaunchKernelGGL(k_init_peel, dim3(grid_for(m, 256, 16384)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const unsigned long long*>(counts), pb.B, pb.D, pb.state, pb.tflag, m);
-  hipLaunchKernelGGL(k_peele_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, a.offS, a.offO, a.edgO, no, pb.work,
-                     pb.owner_o);
+  hipLaunchKernelGGL(k_peele_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, cur.offS, cur.offO, cur.nbrO,
+                     cur.idO, no, pb.work, pb.owner_o);
   PBF_HIP(hipGetLastError());
   const uint32_t small_max = 4096;  // merge work below which one wavefront handles the edge
   uint32_t* small_list = pb.lists;
   uint32_t* large_list = pb.lists + m;
-  const unsigned sel_grid = grid_for(m, 256, 8192);
+  int64_t entries = m;  // entries of the current view: every edge that may still be live
   auto round = [&]() -> int {
+    const unsigned sel_grid = grid_for(entries, 256, 8192);
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
-    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, m, pb.c);
-    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, m, pb.work, small_max, small_list,
-                       large_list, pb.c);
+    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.c);
+    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.work,
+                       small_max, small_list, large_list, pb.c);
     PeelEArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
     hipLaunchKernelGGL((k_peele_walk<E_SMALL_TEAM, E_UCAP>), dim3(ctx->num_cus * 8), dim3(E_SMALL_TEAM), 0, ctx->stream, s);
     s.list = large_list; s.list_len = pb.c.len + 1; s.queue = pb.c.queue + 1;
     hipLaunchKernelGGL((k_peele_walk<E_LARGE_TEAM, E_UCAP>), dim3(ctx->num_cus * 4), dim3(E_LARGE_TEAM), 0, ctx->stream, s);
-    hipLaunchKernelGGL(k_update, dim3(grid_for(m, 256, 4096)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
-                       pb.c);
-    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, m, pb.c);
+    hipLaunchKernelGGL(k_update, dim3(grid_for(entries, 256, 4096)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B,
+                       pb.D, pb.c);
+    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, cur.idO, entries, pb.c);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
   };
+  // drops the dead entries of both sides into the other buffer set, then re-estimates the work
+  auto compact = [&]() -> int {
+    View& nx = set[next];
+    const int64_t sides[2] = {no, ns};
+    for (int side = 0; side < 2; side++) {
+      const uint32_t* off = side ? cur.offS : cur.offO;
+      const uint32_t* nbr = side ? cur.nbrS : cur.nbrO;
+      const uint32_t* id = side ? cur.idS : cur.idO;
+      const unsigned grid = grid_for(entries + 1, 256, 16384);
+      hipLaunchKernelGGL(k_live_flags, dim3(grid), dim3(256), 0, ctx->stream, id, entries, pb.state, flag);
+      PBF_CHECK(pbf_scan_excl_u32(ctx, flag, pos, entries + 1, nullptr));
+      hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, ctx->stream, nbr, id, flag, pos, entries,
+                         side ? nx.nbrS : nx.nbrO, side ? nx.idS : nx.idO);
+      hipLaunchKernelGGL(k_new_off, dim3(grid_for(sides[side] + 1, 256, 16384)), dim3(256), 0, ctx->stream, off,
+                         sides[side], pos, side ? nx.offS : nx.offO);
+    }
+    PBF_HIP(hipGetLastError());
+    cur = nx;
+    next ^= 1;
+    use_view(cur);
+    hipLaunchKernelGGL(k_peele_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, cur.offS, cur.offO, cur.nbrO,
+                       cur.idO, no, pb.work, (uint32_t*)nullptr);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+  };
+  ctx->pinned[33] = (uint64_t)m;
+  PBF_HIP(hipMemcpyAsync(pb.c.live, ctx->pinned + 33, 8, hipMemcpyHostToDevice, ctx->stream));
   int64_t rounds = 0;
-  PBF_CHECK(run_rounds(ctx, pb, m, round, &rounds));
+  uint64_t live = (uint64_t)m;
+  // every round peels at least one live edge, so m rounds always suffice
+  while (rounds <= m) {
+    for (int r = 0; r < ROUND_BATCH; r++) PBF_CHECK(round());
+    rounds += ROUND_BATCH;
+    PBF_HIP(hipMemcpyAsync(ctx->pinned + 32, pb.c.live, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    live = ctx->pinned[32];
+    if (live == 0) break;
+    // the live edges are exactly the state-1 entries after a round; compact at half occupancy
+    if (2 * (int64_t)live < entries) {
+      PBF_CHECK(compact());
+      entries = (int64_t)live;
+    }
+  }
   ctx->timing.rounds = rounds;
+  if (live != 0) return PBF_ERR_DOMAIN;
   PBF_HIP(hipMemcpyAsync(wing, pb.D, 8 * (size_t)m, hipMemcpyDeviceToDevice, ctx->stream));
   return PBF_OK;
 }

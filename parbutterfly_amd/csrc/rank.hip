@@ -389,11 +389,12 @@ __global__ void k_acore_pick(uint32_t* __restrict__ lvl_cnt, uint32_t* __restric
   }
 }
 
-// Remove the active vertices (level == k): one wavefront per vertex walks its neighbours.
+// Remove the active vertices (level == k): one wavefront per vertex walks its neighbours and
+// counts, per unfinished neighbour below k, the removed neighbours (no return value is needed,
+// so the atomics do not wait; a shared append counter would serialise the whole round).
 __global__ __launch_bounds__(256) void k_acore_remove(DevGraph g, const uint32_t* __restrict__ state, uint8_t* __restrict__ lvl,
                                                       uint8_t* __restrict__ done, uint32_t* __restrict__ removed,
-                                                      uint32_t* __restrict__ lvl_cnt, uint32_t* __restrict__ touched,
-                                                      uint32_t* __restrict__ ntouched) {
+                                                      uint32_t* __restrict__ lvl_cnt) {
   const uint32_t k = state[0];
   if (k == 0xFFFFFFFFu) return;
   __shared__ uint32_t nrem;
@@ -418,21 +419,10 @@ __global__ __launch_bounds__(256) void k_acore_remove(DevGraph g, const uint32_t
       const uint32_t d = xv ? g.offV[xa + 1] - ob : g.offU[xa - g.nv + 1] - ob;
       const uint32_t* adj = xv ? g.edgV + ob : g.edgU + ob;
       const int64_t sh = xv ? g.nv : 0;
-      // every lane runs the same iterations: the append below is aggregated per wavefront
-      for (uint32_t j0 = 0; j0 < d; j0 += 64) {
-        const uint32_t j = j0 + lane;
-        uint32_t y = 0;
-        bool fresh = false;
-        if (j < d) {
-          y = (uint32_t)(adj[j] + sh);
-          // unfinished and below k (finished / active vertices have level >= k)
-          fresh = lvl[y] < k && atomicAdd(&removed[y], 1u) == 0;
-        }
-        const uint64_t fm = __ballot(fresh);
-        uint32_t b = 0;
-        if (lane == 0 && fm) b = atomicAdd(ntouched, (uint32_t)__popcll(fm));
-        b = __shfl(b, 0, 64);
-        if (fresh) touched[b + __popcll(fm & pbf_lanemask_lt())] = y;
+      for (uint32_t j = lane; j < d; j += 64) {
+        const uint32_t y = (uint32_t)(adj[j] + sh);
+        // unfinished and below k (finished / active vertices have level >= k)
+        if (lvl[y] < k) atomicAdd(&removed[y], 1u);
       }
     }
   }
@@ -440,16 +430,19 @@ __global__ __launch_bounds__(256) void k_acore_remove(DevGraph g, const uint32_t
   if (threadIdx.x == 0 && nrem) atomicSub(&lvl_cnt[k], nrem);
 }
 
-__global__ __launch_bounds__(256) void k_acore_update(const uint32_t* __restrict__ touched, const uint32_t* __restrict__ ntouched,
-                                                      uint8_t* __restrict__ lvl, uint32_t* __restrict__ rem,
-                                                      uint32_t* __restrict__ removed, uint32_t* __restrict__ lvl_cnt) {
+// Every vertex that lost neighbours this round: remaining degree and level (a scan of the
+// removal counters).
+__global__ __launch_bounds__(256) void k_acore_update(const uint32_t* __restrict__ state, int64_t n, uint8_t* __restrict__ lvl,
+                                                      uint32_t* __restrict__ rem, uint32_t* __restrict__ removed,
+                                                      uint32_t* __restrict__ lvl_cnt) {
+  if (state[0] == 0xFFFFFFFFu) return;
   __shared__ int h[KEYS];
   if (threadIdx.x < KEYS) h[threadIdx.x] = 0;
   __syncthreads();
-  uint32_t nt = *ntouched;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-    uint32_t y = touched[t];
-    uint32_t r = rem[y] - removed[y];
+  for (int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; y < n; y += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t rv = removed[y];
+    if (rv == 0) continue;
+    const uint32_t r = rem[y] - rv;
     removed[y] = 0;
     rem[y] = r;
     uint32_t ol = lvl[y], nl = pbf_bitlen(r);
@@ -462,8 +455,6 @@ __global__ __launch_bounds__(256) void k_acore_update(const uint32_t* __restrict
   __syncthreads();
   if (threadIdx.x < KEYS && h[threadIdx.x]) atomicAdd(&lvl_cnt[threadIdx.x], (uint32_t)h[threadIdx.x]);
 }
-
-__global__ void k_reset_u32(uint32_t* p) { *p = 0; }
 
 __global__ void k_fill_u32b(uint32_t* p, uint32_t v, int64_t n) {
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) p[x] = v;
@@ -514,27 +505,25 @@ int pbf_dev_rank(pbf_context* ctx, const DevGraph& g, int rank_type, RankedGraph
   } else if (rank_type == PBF_RANK_ACOCORE) {
     void *a0, *a1, *a2;
     PBF_CHECK(pbf_scratch(ctx, S_ACORE0, 4 * (size_t)n, &a0));          // rem
-    PBF_CHECK(pbf_scratch(ctx, S_ACORE1, 4 * (size_t)n + 4096, &a1));   // removed | touched? (split below)
-    PBF_CHECK(pbf_scratch(ctx, S_ACORE2, 4 * (size_t)n + (size_t)n + 1024, &a2));
+    PBF_CHECK(pbf_scratch(ctx, S_ACORE1, 4 * (size_t)n, &a1));          // removed neighbours this round
+    PBF_CHECK(pbf_scratch(ctx, S_ACORE2, (size_t)n, &a2));              // finished flags
     uint32_t* rem = static_cast<uint32_t*>(a0);
     uint32_t* removed = static_cast<uint32_t*>(a1);
-    uint32_t* touched = reinterpret_cast<uint32_t*>(a2);
-    uint8_t* done = reinterpret_cast<uint8_t*>(touched + n);
+    uint8_t* done = static_cast<uint8_t*>(a2);
     void* mp;
     PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
     uint32_t* lvl_cnt = static_cast<uint32_t*>(mp) + 64;  // 33 level counters (u32 words 64..96)
-    uint32_t* state = static_cast<uint32_t*>(mp) + 112;   // k, ntouched
+    uint32_t* state = static_cast<uint32_t*>(mp) + 112;   // k
     PBF_CHECK(pbf_fill_u32(ctx, removed, 0u, n));
     PBF_CHECK(pbf_fill_u32(ctx, lvl_cnt, 0u, 48));
     hipLaunchKernelGGL(k_acore_init, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, key, rem, done, lvl_cnt);
     // at most KEYS non-empty levels, and k strictly decreases every round
     for (int round = 0; round < KEYS; round++) {
       hipLaunchKernelGGL(k_acore_pick, dim3(1), dim3(64), 0, ctx->stream, lvl_cnt, state);
-      hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(1), 0, ctx->stream, state + 1);
       hipLaunchKernelGGL(k_acore_remove, dim3(grid_for((n + 63) / 64, 4, 8192)), dim3(256), 0, ctx->stream, g, state, key,
-                         done, removed, lvl_cnt, touched, state + 1);
-      hipLaunchKernelGGL(k_acore_update, dim3(grid_for(n, 256, 2048)), dim3(256), 0, ctx->stream, touched, state + 1, key,
-                         rem, removed, lvl_cnt);
+                         done, removed, lvl_cnt);
+      hipLaunchKernelGGL(k_acore_update, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, state, n, key, rem,
+                         removed, lvl_cnt);
     }
   } else {
     return PBF_ERR_UNSUPPORTED;

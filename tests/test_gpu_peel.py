@@ -52,6 +52,10 @@ def test_peel_chung_lu_hubs(ctx):
     v = pb.as_u64(ctx.count(g, use_v, "VERT"))
     tip = pb.as_u64(ctx.peel(g, use_v, v))
     assert np.array_equal(tip, O.peel_vert(g, use_v, v))
+    # about 800 wing levels: the edge peel compacts its adjacency several times on the way
+    e = pb.as_u64(ctx.count(g, use_v, "EDGE"))
+    wing = pb.as_u64(ctx.peel_edge(g, use_v, e))
+    assert np.array_equal(wing, O.peel_edge(g, use_v, e))
 
 
 def test_peel_device_memory(ctx):
