@@ -23,6 +23,8 @@ using namespace pbf;
 
 namespace {
 constexpr unsigned long long KNONE = ~0ull;
+// work[x] of a peeled item: its walk cost; bit 31 is the side an edge walk starts from
+constexpr uint32_t WORK_MASK = 0x7FFFFFFFu, WORK_SSIDE = 0x80000000u;
 
 inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
   int64_t b = (n + threads - 1) / threads;
@@ -32,18 +34,31 @@ inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
 }
 
 // ---------------- shared round kernels ----------------
-// ctl layout (u64): [0] k, [1] live count; u32 view from word 4: list lengths / queues.
+// ctl layout (u64): [0] k, [1] live count, [2] k of the previous round; u32 view from word 8:
+// list lengths, then queues.
 struct Ctl {
   unsigned long long* k;
   unsigned long long* live;
-  uint32_t* len;    // [0] small, [1] large, [2] ovf, [3] ntouched
+  unsigned long long* kprev;
+  uint32_t* len;    // [0] small, [1] large, [2] ovf, [3] ntouched, [4] previous ntouched, [5] found
   uint32_t* queue;  // [0] small, [1] large, [2] ovf
 };
 
+__global__ void k_ctl_init(Ctl c) {
+  if (threadIdx.x == 0) { *c.k = KNONE; *c.kprev = KNONE; }
+  if (threadIdx.x < 8) c.len[threadIdx.x] = 0;
+}
+
+// The previous round's touched list stays readable until this round's walk appends to it.
 __global__ void k_round_begin(Ctl c) {
-  if (threadIdx.x == 0) *c.k = KNONE;
-  if (threadIdx.x < 4) c.len[threadIdx.x] = 0;
-  if (threadIdx.x < 3) c.queue[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    *c.kprev = *c.k;
+    c.len[4] = c.len[3];
+    c.len[5] = 0;
+    *c.k = KNONE;
+    for (int q = 0; q < 4; q++) c.len[q] = 0;
+    for (int q = 0; q < 3; q++) c.queue[q] = 0;
+  }
 }
 
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
@@ -57,9 +72,41 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x)
 
 // The round kernels visit items ids[0, n) (every item that may still be live), or 0..n-1
 // when ids is null.
-// k = min D over live items (state != 0).
+// Frontier from the previous round's touched items.  After a round at level k every live item
+// has D >= k, and the live items with D == k are all touched ones (an untouched item keeps its
+// D, which was not k or it would have been peeled).  So when a touched item has D == k the new
+// minimum is k again and the frontier is exactly those items; else the full scan below runs.
+__global__ void k_select_touched(const unsigned long long* __restrict__ D, uint8_t* __restrict__ state,
+                                 const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
+                                 uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
+                                 Ctl c) {
+  const unsigned long long k = *c.kprev;
+  if (k == KNONE) return;
+  const uint32_t nt = c.len[4];
+  uint32_t nsel = 0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    const uint32_t x = touched[t];
+    if (state[x] == 1 && D[x] == k) {
+      state[x] = 2;
+      nsel++;
+      const uint32_t w = work[x] & WORK_MASK;
+      if (w == 0) continue;
+      if (w <= small_max) small_list[atomicAdd(&c.len[0], 1u)] = x;
+      else large_list[atomicAdd(&c.len[1], 1u)] = x;
+    }
+  }
+  nsel = pbf_wave_sum(nsel);
+  if ((threadIdx.x & 63) == 0 && nsel) {
+    atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
+    c.len[5] = 1u;
+    *c.k = k;
+  }
+}
+
+// k = min D over live items (state != 0), unless the touched items gave the frontier.
 __global__ void k_min_live(const unsigned long long* __restrict__ D, const uint8_t* __restrict__ state,
                            const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
+  if (c.len[5]) return;
   unsigned long long mn = KNONE;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
@@ -75,14 +122,14 @@ __global__ void k_select(const unsigned long long* __restrict__ D, uint8_t* __re
                          const uint32_t* __restrict__ ids, int64_t n, const uint32_t* __restrict__ work,
                          uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list, Ctl c) {
   const unsigned long long k = *c.k;
-  if (k == KNONE) return;
+  if (k == KNONE || c.len[5]) return;
   uint32_t nsel = 0;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
     if (state[x] == 1 && D[x] == k) {
       state[x] = 2;
       nsel++;
-      const uint32_t w = work[x];
+      const uint32_t w = work[x] & WORK_MASK;
       if (w == 0) continue;
       if (w <= small_max) small_list[atomicAdd(&c.len[0], 1u)] = (uint32_t)x;
       else large_list[atomicAdd(&c.len[1], 1u)] = (uint32_t)x;
@@ -275,6 +322,7 @@ struct PeelEArgs {
   const uint32_t* idO;
   const uint32_t* u_of;     // S vertex of each edge id (the input O-CSR)
   const uint32_t* owner_o;  // O vertex of each edge id
+  const uint32_t* work;     // bit 31: walk from the S end (see k_peele_work)
   const uint8_t* state;     // 0 dead, 1 live, 2 peeled this round
   unsigned long long* B;
   const uint32_t* list;
@@ -292,11 +340,17 @@ __device__ __forceinline__ bool usable(const uint8_t* state, uint32_t f, uint32_
   return s == 1 || (s == 2 && f > e);
 }
 
+// Walk of a peeled edge e = (v, u).  Every butterfly through e is {v, v2} x {u, u2}; it is found
+// either from v (u2 in N(v), v2 in N(u2), test v2 in N(u)) or, symmetrically, from u (v2 in
+// N(u), u2 in N(v2), test u2 in N(v)), whichever end has the smaller two-hop work.  Below, a is
+// the starting end, b the other one: c runs over N(a) \ {b}, d over N(c) \ {a}, and d must lie in
+// N(b) (staged in LDS).  The edge (a, c) loses the butterflies of its segment at once; (c, d)
+// and (b, d) lose one each.
 template <int TEAM, int UCAP>
 __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   __shared__ TeamShared<TEAM> sh;
-  __shared__ uint32_t nu_v[UCAP];    // N(u) (O vertices, ascending)
-  __shared__ uint32_t nu_f[UCAP];    // O-CSR index of (v2, u)
+  __shared__ uint32_t nb_v[UCAP];    // N(b), ascending
+  __shared__ uint32_t nb_f[UCAP];    // edge ids of N(b)
   const uint32_t tid = threadIdx.x;
   if (*a.k == KNONE) return;
   const uint32_t nlist = *a.list_len;
@@ -307,23 +361,32 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     __syncthreads();
     if (idx >= nlist) break;
     const uint32_t e = a.list[idx];
+    const bool from_s = (a.work[e] & WORK_SSIDE) != 0u;
     const uint32_t v = a.owner_o[e], u = a.u_of[e];
-    const uint32_t vo = a.offO[v], vdeg = a.offO[v + 1] - vo;
-    const uint32_t uo = a.offS[u], udeg = a.offS[u + 1] - uo;
-    const bool cached = udeg <= (uint32_t)UCAP;
+    const uint32_t va = from_s ? u : v, vb = from_s ? v : u;
+    // X: lists of a's side; Y: lists of the other side (those of c's and of b)
+    const uint32_t* offX = from_s ? a.offS : a.offO;
+    const uint32_t* nbrX = from_s ? a.nbrS : a.nbrO;
+    const uint32_t* idX = from_s ? a.idS : a.idO;
+    const uint32_t* offY = from_s ? a.offO : a.offS;
+    const uint32_t* nbrY = from_s ? a.nbrO : a.nbrS;
+    const uint32_t* idY = from_s ? a.idO : a.idS;
+    const uint32_t ao = offX[va], adeg = offX[va + 1] - ao;
+    const uint32_t bo = offY[vb], bdeg = offY[vb + 1] - bo;
+    const bool cached = bdeg <= (uint32_t)UCAP;
     if (cached)
-      for (uint32_t q = tid; q < udeg; q += TEAM) { nu_v[q] = a.nbrS[uo + q]; nu_f[q] = a.idS[uo + q]; }
+      for (uint32_t q = tid; q < bdeg; q += TEAM) { nb_v[q] = nbrY[bo + q]; nb_f[q] = idY[bo + q]; }
     __syncthreads();
-    for (uint32_t c0 = 0; c0 < vdeg; c0 += TEAM) {
+    for (uint32_t c0 = 0; c0 < adeg; c0 += TEAM) {
       const uint32_t j = c0 + tid;
-      uint32_t L = 0, so = 0, u2 = 0, f1 = 0;
-      if (j < vdeg) {
-        const uint32_t q = vo + j;  // f1 = (v, u2)
-        u2 = a.nbrO[q];
-        f1 = a.idO[q];
-        if (u2 != u && usable(a.state, f1, e)) {
-          so = a.offS[u2];
-          L = a.offS[u2 + 1] - so;
+      uint32_t L = 0, so = 0, fc = 0;
+      if (j < adeg) {
+        const uint32_t q = ao + j;  // edge (a, c)
+        const uint32_t c = nbrX[q];
+        fc = idX[q];
+        if (c != vb && usable(a.state, fc, e)) {
+          so = offY[c];
+          L = offY[c + 1] - so;
         }
       }
       uint32_t T;
@@ -336,31 +399,31 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
         const uint32_t s = seg_of<TEAM>(sh.P, w);
         const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
         const uint32_t p2 = sh.vs[s] + kk;
-        const uint32_t v2 = a.nbrS[p2];
-        if (v2 == v) continue;
-        const uint32_t f2 = a.idS[p2];
-        if (!usable(a.state, f2, e)) continue;
-        // f3 = (v2, u): binary search v2 in N(u)
-        uint32_t lo = 0, hi = udeg;
+        const uint32_t d = nbrY[p2];
+        if (d == va) continue;
+        const uint32_t fd = idY[p2];
+        if (!usable(a.state, fd, e)) continue;
+        // edge (b, d): search d in N(b)
+        uint32_t lo = 0, hi = bdeg;
         if (cached) {
-          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nu_v[mid] < v2) lo = mid + 1; else hi = mid; }
-          if (lo == udeg || nu_v[lo] != v2) continue;
+          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nb_v[mid] < d) lo = mid + 1; else hi = mid; }
+          if (lo == bdeg || nb_v[lo] != d) continue;
         } else {
-          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (a.nbrS[uo + mid] < v2) lo = mid + 1; else hi = mid; }
-          if (lo == udeg || a.nbrS[uo + lo] != v2) continue;
+          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nbrY[bo + mid] < d) lo = mid + 1; else hi = mid; }
+          if (lo == bdeg || nbrY[bo + lo] != d) continue;
         }
-        const uint32_t f3 = cached ? nu_f[lo] : a.idS[uo + lo];
-        if (!usable(a.state, f3, e)) continue;
+        const uint32_t fb = cached ? nb_f[lo] : idY[bo + lo];
+        if (!usable(a.state, fb, e)) continue;
         atomicAdd(&sh.acc[s], 1ull);
-        if (a.state[f2] == 1) { atomicAdd(&a.B[f2], ~0ull); touch(f2, a.tflag, a.touched, a.ntouched); }
-        if (a.state[f3] == 1) { atomicAdd(&a.B[f3], ~0ull); touch(f3, a.tflag, a.touched, a.ntouched); }
+        if (a.state[fd] == 1) { atomicAdd(&a.B[fd], ~0ull); touch(fd, a.tflag, a.touched, a.ntouched); }
+        if (a.state[fb] == 1) { atomicAdd(&a.B[fb], ~0ull); touch(fb, a.tflag, a.touched, a.ntouched); }
       }
       __syncthreads();
-      if (j < vdeg) {
+      if (j < adeg) {
         const unsigned long long cnt = sh.acc[tid];
-        if (cnt && a.state[f1] == 1) {
-          atomicAdd(&a.B[f1], (unsigned long long)(-(long long)cnt));
-          touch(f1, a.tflag, a.touched, a.ntouched);
+        if (cnt && a.state[fc] == 1) {
+          atomicAdd(&a.B[fc], (unsigned long long)(-(long long)cnt));
+          touch(fc, a.tflag, a.touched, a.ntouched);
         }
       }
       __syncthreads();
@@ -368,23 +431,37 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   }
 }
 
-// work[e] = sum_{u2 in N(v)} deg(u2) for e = (v, u): merge work of the edge's walk (saturating),
-// over the current adjacency; owner_o (when given) receives each edge's O vertex.
-__global__ __launch_bounds__(256) void k_peele_work(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ offO,
-                                                    const uint32_t* __restrict__ nbrO, const uint32_t* __restrict__ idO,
-                                                    int64_t no, uint32_t* __restrict__ work, uint32_t* __restrict__ owner_o) {
+// Two-hop work of every vertex over the current adjacency: sum of the neighbours' degrees.
+__global__ __launch_bounds__(256) void k_hop_work(const uint32_t* __restrict__ off, const uint32_t* __restrict__ nbr,
+                                                  const uint32_t* __restrict__ off_other, int64_t nvert,
+                                                  uint32_t* __restrict__ hop) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int64_t v = (int64_t)blockIdx.x * 4 + wid; v < no; v += (int64_t)gridDim.x * 4) {
+  for (int64_t x = (int64_t)blockIdx.x * 4 + wid; x < nvert; x += (int64_t)gridDim.x * 4) {
     uint64_t s = 0;
-    const uint32_t b = offO[v], e = offO[v + 1];
+    const uint32_t b = off[x], e = off[x + 1];
     for (uint32_t q = b + lane; q < e; q += 64) {
-      const uint32_t u2 = nbrO[q];
-      s += offS[u2 + 1] - offS[u2];
-      if (owner_o) owner_o[idO[q]] = (uint32_t)v;
+      const uint32_t y = nbr[q];
+      s += off_other[y + 1] - off_other[y];
     }
     s = pbf_wave_sum(s);
-    const uint32_t w = (uint32_t)(s < 0xFFFFFFFFull ? s : 0xFFFFFFFFull);
-    for (uint32_t q = b + lane; q < e; q += 64) work[idO[q]] = w;
+    if (lane == 0) hop[x] = (uint32_t)(s < WORK_MASK ? s : WORK_MASK);
+  }
+}
+
+// work[e] for e = (v, u): the smaller of the two ends' two-hop work (saturating), bit 31 set
+// when the walk starts from u; owner_o (when given) receives each edge's O vertex.
+__global__ __launch_bounds__(256) void k_peele_work(const uint32_t* __restrict__ offO, const uint32_t* __restrict__ nbrO,
+                                                    const uint32_t* __restrict__ idO, int64_t no,
+                                                    const uint32_t* __restrict__ hopO, const uint32_t* __restrict__ hopS,
+                                                    uint32_t* __restrict__ work, uint32_t* __restrict__ owner_o) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int64_t v = (int64_t)blockIdx.x * 4 + wid; v < no; v += (int64_t)gridDim.x * 4) {
+    const uint32_t b = offO[v], e = offO[v + 1], hv = hopO[v];
+    for (uint32_t q = b + lane; q < e; q += 64) {
+      const uint32_t hu = hopS[nbrO[q]], id = idO[q];
+      work[id] = hu < hv ? (hu | WORK_SSIDE) : hv;
+      if (owner_o) owner_o[id] = (uint32_t)v;
+    }
   }
 }
 
@@ -458,6 +535,7 @@ int peel_bufs(pbf_context* ctx, int64_t n, PeelBufs* pb) {
   unsigned long long* m64 = static_cast<unsigned long long*>(mp);
   pb->c.k = m64;
   pb->c.live = m64 + 1;
+  pb->c.kprev = m64 + 2;
   pb->c.len = reinterpret_cast<uint32_t*>(m64 + 4);
   pb->c.queue = reinterpret_cast<uint32_t*>(m64 + 8);
   return PBF_OK;
@@ -502,6 +580,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   a.k = pb.c.k;
   hipLaunchKernelGGL(k_init_peel, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const unsigned long long*>(counts), pb.B, pb.D, pb.state, pb.tflag, ns);
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, ctx->stream, pb.c);
   hipLaunchKernelGGL(k_peelv_bound, dim3(grid_for(ns, 4, 16384)), dim3(256), 0, ctx->stream, a.offS, a.edgS, a.offO, ns,
                      pb.work);
   PBF_HIP(hipGetLastError());
@@ -518,6 +597,8 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   const unsigned sel_grid = grid_for(ns, 256, 4096);
   auto round = [&]() -> int {
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
+    hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.state, pb.touched,
+                       pb.work, small_max, small_list, large_list, pb.c);
     hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.c);
     hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.work, small_max,
                        small_list, large_list, pb.c);
@@ -557,7 +638,7 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   // compaction buffers: two sets of (offO, nbrO, idO, offS, nbrS, idS), plus flags and positions
   const size_t set_words = size_t(no + 1) + size_t(ns + 1) + 4 * size_t(m);
   void* cp;
-  PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * (2 * set_words + 2 * size_t(m + 1)), &cp));
+  PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * (2 * set_words + 2 * size_t(m + 1) + size_t(no + ns)), &cp));
   struct View { uint32_t *offO, *nbrO, *idO, *offS, *nbrS, *idS; };
   View set[2];
   uint32_t* w = static_cast<uint32_t*>(cp);
@@ -571,6 +652,8 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   }
   uint32_t* flag = w;
   uint32_t* pos = w + m + 1;
+  uint32_t* hopO = pos + m + 1;  // two-hop work per O vertex, then per S vertex
+  uint32_t* hopS = hopO + no;
   // the first view is the input CSR, edge ids = O-CSR positions (O side) / eti (S side)
   View cur{const_cast<uint32_t*>(use_v ? g.offV : g.offU), const_cast<uint32_t*>(use_v ? g.edgV : g.edgU), set[0].idO,
            const_cast<uint32_t*>(use_v ? g.offU : g.offV), const_cast<uint32_t*>(use_v ? g.edgU : g.edgV), eti};
@@ -579,6 +662,7 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   PeelEArgs a{};
   a.u_of = use_v ? g.edgV : g.edgU;
   a.owner_o = pb.owner_o;
+  a.work = pb.work;
   a.state = pb.state;
   a.B = pb.B;
   a.tflag = pb.tflag;
@@ -592,9 +676,19 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   use_view(cur);
   hipLaunchKernelGGL(k_init_peel, dim3(grid_for(m, 256, 16384)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const unsigned long long*>(counts), pb.B, pb.D, pb.state, pb.tflag, m);
-  hipLaunchKernelGGL(k_peele_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, cur.offS, cur.offO, cur.nbrO,
-                     cur.idO, no, pb.work, pb.owner_o);
-  PBF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, ctx->stream, pb.c);
+  // per-edge walk cost and starting end over the current view
+  auto estimate = [&](uint32_t* owner) -> int {
+    hipLaunchKernelGGL(k_hop_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, cur.offO, cur.nbrO, cur.offS,
+                       no, hopO);
+    hipLaunchKernelGGL(k_hop_work, dim3(grid_for(ns, 4, 16384)), dim3(256), 0, ctx->stream, cur.offS, cur.nbrS, cur.offO,
+                       ns, hopS);
+    hipLaunchKernelGGL(k_peele_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, cur.offO, cur.nbrO, cur.idO,
+                       no, hopO, hopS, pb.work, owner);
+    PBF_HIP(hipGetLastError());
+    return PBF_OK;
+  };
+  PBF_CHECK(estimate(pb.owner_o));
   const uint32_t small_max = 4096;  // merge work below which one wavefront handles the edge
   uint32_t* small_list = pb.lists;
   uint32_t* large_list = pb.lists + m;
@@ -602,6 +696,8 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   auto round = [&]() -> int {
     const unsigned sel_grid = grid_for(entries, 256, 8192);
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
+    hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.state, pb.touched,
+                       pb.work, small_max, small_list, large_list, pb.c);
     hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.c);
     hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.work,
                        small_max, small_list, large_list, pb.c);
@@ -636,10 +732,7 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     cur = nx;
     next ^= 1;
     use_view(cur);
-    hipLaunchKernelGGL(k_peele_work, dim3(grid_for(no, 4, 16384)), dim3(256), 0, ctx->stream, cur.offS, cur.offO, cur.nbrO,
-                       cur.idO, no, pb.work, (uint32_t*)nullptr);
-    PBF_HIP(hipGetLastError());
-    return PBF_OK;
+    return estimate(nullptr);
   };
   ctx->pinned[33] = (uint64_t)m;
   PBF_HIP(hipMemcpyAsync(pb.c.live, ctx->pinned + 33, 8, hipMemcpyHostToDevice, ctx->stream));
