@@ -523,8 +523,8 @@ extern "C" int pbf_batch_autotune(pbf_context* ctx, int64_t count, const pbf_bip
                                   int* best, float* best_ms) {
   if (!ctx) return PBF_ERR_ARG;
   static const CountTune cands[] = {
-      {1535, 1u << 14, 4}, {1535, 1u << 14, 2}, {767, 1u << 14, 4}, {767, 1u << 13, 4},
-      {383, 1u << 12, 4},  {1535, 1u << 12, 4}, {767, 1u << 13, 2}, {1535, 1u << 15, 4},
+      {767, 1u << 14, 4}, {767, 1u << 14, 2}, {1535, 1u << 14, 4}, {767, 1u << 13, 4},
+      {383, 1u << 12, 4}, {511, 1u << 13, 4}, {767, 1u << 13, 2}, {767, 1u << 15, 4},
   };
   const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
   int bi = 0;
@@ -549,7 +549,7 @@ extern "C" int pbf_batch_autotune(pbf_context* ctx, int64_t count, const pbf_bip
 
 extern "C" int pbf_variant_tuning(int variant, int* rank_type, int* light_max, int* light_work, int* blocks_per_cu) {
   static const int ranks[3] = {PBF_RANK_SIDE, PBF_RANK_ADEG, PBF_RANK_ACOCORE};
-  static const int knobs[3][3] = {{1535, 1 << 14, 4}, {767, 1 << 13, 4}, {383, 1 << 12, 2}};
+  static const int knobs[3][3] = {{767, 1 << 14, 4}, {1535, 1 << 14, 4}, {383, 1 << 12, 2}};
   if (variant < 0 || variant >= 9) return PBF_ERR_ARG;
   if (rank_type) *rank_type = ranks[variant / 3];
   if (light_max) *light_max = knobs[variant % 3][0];

@@ -185,6 +185,7 @@ constexpr uint32_t LT_WORDS = POOL_WORDS / NWAVE; // wavefront table: 2048 packe
 constexpr uint32_t LIGHT_MAX = LT_WORDS / 4 * 3 - 1;  // light endpoint: <= 1535 distinct keys ...
 constexpr uint32_t LIGHT_WORK = 1u << 14;         // ... and at most 16384 wedges
 static_assert(LIGHT_MAX + LIGHT_MAX / 3 + 1 <= LT_WORDS, "a light table must fit its wavefront's slots");
+static_assert(NWAVE * LT_WORDS <= POOL_WORDS, "the light tables of a workgroup live in its pool");
 constexpr uint32_t WIDE_DEG = 1u << 24;           // per-window u32 sums need deg(i) < 2^24
 constexpr uint32_t PROBE_MAX = 64;                // probes per insert into a light sub-table
 constexpr uint32_t WIN = 4;                       // wedge indices per lane per window
@@ -1234,7 +1235,7 @@ __global__ __launch_bounds__(LG_BLOCKS) void k_light_plan(const uint32_t* __rest
         plan[12 + b] = start;
         start += tot[b];
         wbase[b] = words;
-        words += (unsigned long long)tot[b] * (LT_WORDS >> r);
+        words += (unsigned long long)tot[b] * (2048u >> r);  // bucket r holds tables of 2^(11 - r) words
       }
       plan[36 + side] = groups;
       groups += (uint32_t)((words + LT_WORDS - 1) / LT_WORDS);
@@ -1265,7 +1266,7 @@ __global__ __launch_bounds__(256) void k_light_scatter(const uint32_t* __restric
     out[pos] = i;
     const uint32_t io = roff[i];
     rec[pos] = make_uint4(i, io, roff[i + 1] - io, 11u - b % 6u);
-    const unsigned long long prefix = wbase[b] + (unsigned long long)(pos - plan[12 + b]) * (LT_WORDS >> (b % 6u));
+    const unsigned long long prefix = wbase[b] + (unsigned long long)(pos - plan[12 + b]) * (2048u >> (b % 6u));
     if ((prefix & (LT_WORDS - 1)) == 0) gstart[plan[36 + b / 6u] + (uint32_t)(prefix / LT_WORDS)] = pos;
   }
 }
