@@ -128,18 +128,19 @@ def evaluate(ctx, model, F, T):
 
 
 def run_learned(ctx, model, graphs):
-    """Batched + learned: features, selection, then one union pass per chosen variant.
-    Returns [(indices, outputs of that pass)] (device outputs for device graphs) and the
-    chosen variant of every graph."""
+    """Batched + learned: features, selection, then one union pass per chosen ranking.
+    The knob half of a variant (table bounds, workgroups per CU) is tuned for one graph run
+    alone, where launch shape dominates; a union pass keeps the context's own knobs (default,
+    or the batch autotuner's).  Returns [(indices, outputs of that pass)] (device outputs for
+    device graphs) and the chosen variant of every graph."""
     F = ctx.batch_features(graphs)
     pick = predict(ctx, model, F).argmin(1)
     parts = []
-    for v in np.unique(pick):
-        idx = np.nonzero(pick == v)[0]
-        rank, knobs = variant_settings(int(v))
-        ctx.set_tuning(**knobs)
+    ranking = pick // 3
+    for r in np.unique(ranking):
+        idx = np.nonzero(ranking == r)[0]
+        rank, _ = variant_settings(int(r) * 3)
         parts.append((idx, ctx.count_batch([graphs[k] for k in idx], rank, mode="union")))
-    ctx.set_tuning()
     return parts, pick
 
 
