@@ -3,8 +3,8 @@
 A small MLP (16 features -> 64 -> 64 -> 9, BF16 on the matrix cores, trained by the library's
 own kernels) predicts the log2 runtime of each of the 9 counting variants (ranking SIDE /
 ADEG / ACOCORE x 3 settings of the counting knobs, see pbf_variant_tuning) from cheap graph
-statistics (pbf_batch_features).  A batch is then split by the predicted best variant and
-every part runs as one disjoint-union pass (pbf_count_batch).  Results are bit-exact whatever
+statistics (pbf_batch_features).  A batch is then split by the ranking of the predicted best
+variant and every part runs as one disjoint-union pass (pbf_count_batch).  Results are bit-exact whatever
 the choice: the variants only change the order of work.
 
     python -m parbutterfly_amd.selector collect --first-seed 1000000 --count 20000 --out DIR
