@@ -208,7 +208,8 @@ void pbf_batch_release(pbf_context* ctx);
 int pbf_batch_autotune(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
                        int rank_type, uint64_t* vert_out, int64_t vert_cap, uint64_t* vert_off,
                        uint64_t* totals, int out_mem, int* best, float* best_ms);
-/* Sets the counting kernels' knobs directly (light_max <= 1535, 1 <= blocks_per_cu <= 8). */
+/* Sets the counting kernels' knobs directly (light_max <= 1535, 0 = the default of each count
+ * mode: 767 per vertex / total, 1535 per edge; 1 <= blocks_per_cu <= 8). */
 int pbf_set_tuning(pbf_context* ctx, int light_max, int light_work, int blocks_per_cu);
 
 /* ---------------- learned variant selector (no counterpart in the reference) ----------------

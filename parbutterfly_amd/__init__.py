@@ -511,7 +511,7 @@ class Context:
                "pbf_batch_autotune")
         return best.value, ms.value
 
-    def set_tuning(self, light_max=767, light_work=1 << 14, blocks_per_cu=4):
+    def set_tuning(self, light_max=0, light_work=1 << 14, blocks_per_cu=4):
         _check(_lib.pbf_set_tuning(self._ctx, light_max, light_work, blocks_per_cu), "pbf_set_tuning")
 
     # ---- learned selector (no counterpart in the reference) ----

@@ -1456,7 +1456,10 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, plan, 0u, (int64_t)64);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
   uint32_t* hbm_list = lists + 5 * n + 1;
-  const uint32_t lmax = ctx->tune.light_max < LIGHT_MAX ? ctx->tune.light_max : LIGHT_MAX;
+  // light-table bound: the context's, or the mode's default (per-edge counting has a second
+  // pass with a 64-bit credit per wedge for every endpoint, and keeps larger tables light)
+  const uint32_t want = ctx->tune.light_max ? ctx->tune.light_max : (MODE == MODE_EDGE ? LIGHT_MAX : 767u);
+  const uint32_t lmax = want < LIGHT_MAX ? want : LIGHT_MAX;
   const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb, lmax, ctx->tune.light_work, R.lo};
   const unsigned grid = (unsigned)(ctx->num_cus * (ctx->tune.blocks_per_cu ? ctx->tune.blocks_per_cu : 4));
   hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,

@@ -59,7 +59,7 @@ enum ScratchSlot {
 
 // Runtime knobs of the counting kernels (defaults; the batch autotuner may change them).
 struct CountTune {
-  uint32_t light_max = 767;      // distinct-key bound of a light endpoint (wavefront table)
+  uint32_t light_max = 0;        // distinct-key bound of a light endpoint (0: the mode's default)
   uint32_t light_work = 1u << 14;  // wedge bound of a light endpoint
   uint32_t blocks_per_cu = 4;    // workgroups per CU of the persistent counting kernels
 };

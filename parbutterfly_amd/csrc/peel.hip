@@ -72,6 +72,19 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x)
 
 // The round kernels visit items ids[0, n) (every item that may still be live), or 0..n-1
 // when ids is null.
+// Appends x to a list when flag is set, one counter atomic per wavefront (the lanes active at
+// the call take part; the lowest of them reserves the room).  A per-lane atomic on one shared
+// counter would serialise every append of the launch.
+__device__ __forceinline__ void wave_append(bool flag, uint32_t x, uint32_t* list, uint32_t* len) {
+  const uint64_t fm = __ballot(flag);
+  if (fm == 0ull) return;
+  const uint32_t leader = (uint32_t)__ffsll((long long)__ballot(1)) - 1u;
+  uint32_t base = 0;
+  if (__lane_id() == leader) base = atomicAdd(len, (uint32_t)__popcll(fm));
+  base = (uint32_t)__shfl((int)base, (int)leader, 64);
+  if (flag) list[base + __popcll(fm & pbf_lanemask_lt())] = x;
+}
+
 // Frontier from the previous round's touched items.  After a round at level k every live item
 // has D >= k, and the live items with D == k are all touched ones (an untouched item keeps its
 // D, which was not k or it would have been peeled).  So when a touched item has D == k the new
@@ -86,14 +99,15 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, uint8
   uint32_t nsel = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const uint32_t x = touched[t];
-    if (state[x] == 1 && D[x] == k) {
+    const bool sel = state[x] == 1 && D[x] == k;
+    uint32_t w = 0;
+    if (sel) {
       state[x] = 2;
       nsel++;
-      const uint32_t w = work[x] & WORK_MASK;
-      if (w == 0) continue;
-      if (w <= small_max) small_list[atomicAdd(&c.len[0], 1u)] = x;
-      else large_list[atomicAdd(&c.len[1], 1u)] = x;
+      w = work[x] & WORK_MASK;
     }
+    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
+    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) {
@@ -126,14 +140,15 @@ __global__ void k_select(const unsigned long long* __restrict__ D, uint8_t* __re
   uint32_t nsel = 0;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
-    if (state[x] == 1 && D[x] == k) {
+    const bool sel = state[x] == 1 && D[x] == k;
+    uint32_t w = 0;
+    if (sel) {
       state[x] = 2;
       nsel++;
-      const uint32_t w = work[x] & WORK_MASK;
-      if (w == 0) continue;
-      if (w <= small_max) small_list[atomicAdd(&c.len[0], 1u)] = (uint32_t)x;
-      else large_list[atomicAdd(&c.len[1], 1u)] = (uint32_t)x;
+      w = work[x] & WORK_MASK;
     }
+    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
+    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
@@ -162,7 +177,7 @@ __global__ void k_retire(uint8_t* __restrict__ state, const uint32_t* __restrict
 }
 
 __device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* touched, uint32_t* ntouched) {
-  if (atomicExch(&tflag[x], 1u) == 0u) touched[atomicAdd(ntouched, 1u)] = x;
+  wave_append(atomicExch(&tflag[x], 1u) == 0u, x, touched, ntouched);
 }
 
 // ---------------- vertex peel walk ----------------
