@@ -33,8 +33,10 @@ struct CountArgs {
   const uint32_t* inpref;
   const uint2* sinfo;      // per slot (i -> v): roff[v] and L, the length of N(v)'s prefix below min(i, v)
   const uint32_t* bound;   // distinct-u2 bound per endpoint: min(wedges, i)
-  const uint32_t* oidx;    // EDGE: O-CSR index of each ranked slot
-  unsigned long long* B;   // VERT: counts in rank space; EDGE: counts in O-CSR order
+  // VERT: counts in rank space.  EDGE: counts per ranked slot; an edge's two slots are summed
+  // into its O-CSR position at the end (k_edge_out_*).  Slot indexing keeps the credits of a
+  // segment, consecutive entries of N(v), on neighbouring addresses.
+  unsigned long long* B;
   unsigned long long* total;
   const uint32_t* list;
   const uint32_t* list_len;
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
             const uint32_t c = dense[u2];
             if (c > 1u) {
               atomicAdd(&sh.acc[s], (unsigned long long)(c - 1u));
-              if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[p]], (unsigned long long)(c - 1u));
+              if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[p], (unsigned long long)(c - 1u));
             }
           }
         }
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
         if (pass == 2 && j < deg) {
           const unsigned long long s = sh.acc[tid];
           if (s) {
-            if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[io + j]], s);
+            if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[io + j], s);
             else atomicAdd(&a.B[v + lo], s);
           }
         }
@@ -592,7 +594,7 @@ __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, con
       const uint32_t c = cnt[k];
       if (c > 1u) {
         val[k] = c - 1u;
-        if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[wv.pos[k]]], (unsigned long long)(c - 1u));
+        if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[wv.pos[k]], (unsigned long long)(c - 1u));
       }
     }
   }
@@ -651,7 +653,7 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
     const unsigned long long mine = wc.acc[lane];
     if (mine) {
       wc.acc[lane] = 0;
-      if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[a.oidx[ch.slot]], mine);
+      if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[ch.slot], mine);
       else credit(a, ch.v, mine);
     }
     wave_lds_fence();
@@ -1377,6 +1379,20 @@ __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t
 // O-CSR index of every ranked slot: the payload is an O-CSR position when the slot's owner
 // is on O (flag 1: neighbour on S), else an S-CSR position mapped through eti
 // (CountERank's conversion, butterfly_ecount.h:1723-1741).
+// Per-edge output from the per-slot counts: every edge has one slot owned by its S end (flag 0)
+// and one owned by its O end (flag 1); the first pass stores, the second adds, each writing
+// every edge exactly once (no atomics).
+__global__ void k_edge_out_s(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ oidx,
+                             const unsigned long long* __restrict__ Bs, int64_t m2, unsigned long long* __restrict__ out) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x)
+    if ((redg[p] & 1u) == 0u) out[oidx[p]] = Bs[p];
+}
+__global__ void k_edge_out_o(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ oidx,
+                             const unsigned long long* __restrict__ Bs, int64_t m2, unsigned long long* __restrict__ out) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x)
+    if (redg[p] & 1u) out[oidx[p]] += Bs[p];
+}
+
 __global__ void k_oidx(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ rpos, const uint32_t* __restrict__ eti,
                        int64_t m2, uint32_t* __restrict__ oidx) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x)
@@ -1599,11 +1615,15 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
     PBF_CHECK(pbf_scratch(ctx, S_OIDX, 4 * (size_t)(m2 ? m2 : 1), &op));
     uint32_t* oidx = static_cast<uint32_t*>(op);
     if (m2) hipLaunchKernelGGL(k_oidx, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, R.rpos, eti, m2, oidx);
-    a.oidx = oidx;
-    a.B = reinterpret_cast<unsigned long long*>(out_dev);
-    if (g.m)
-      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g.m, 256, 8192)), dim3(256), 0, ctx->stream, a.B, 0ull, (int64_t)g.m);
+    PBF_CHECK(pbf_scratch(ctx, S_COUNTS, 8 * (size_t)(m2 ? m2 : 1), &cp));
+    a.B = static_cast<unsigned long long*>(cp);
+    if (m2) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(m2, 256, 8192)), dim3(256), 0, ctx->stream, a.B, 0ull, m2);
     PBF_CHECK(run_count<MODE_EDGE>(ctx, R, a, lists, ctl, opts.sync));
+    if (m2) {
+      unsigned long long* out = reinterpret_cast<unsigned long long*>(out_dev);
+      hipLaunchKernelGGL(k_edge_out_s, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, oidx, a.B, m2, out);
+      hipLaunchKernelGGL(k_edge_out_o, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, oidx, a.B, m2, out);
+    }
   } else {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, a.total, 0ull, (int64_t)1);
     PBF_CHECK(run_count<MODE_TOTAL>(ctx, R, a, lists, ctl, opts.sync));
