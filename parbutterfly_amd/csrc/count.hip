@@ -483,7 +483,10 @@ __device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint
     if constexpr (MULTI) wv.td[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(s << 2), (int)ch.td);
   }
 #pragma unroll
-  for (uint32_t k = 0; k < WIN; k++) wv.ent[k] = k < wv.nval ? a.redg[wv.pos[k]] : 0u;
+  // unconditional loads (lanes past the end read slot 0, and their entries are never used): a
+  // fixed count of loads in flight lets the consumer wait for this window's loads only, not for
+  // the next window's, which are issued before this one is aggregated
+  for (uint32_t k = 0; k < WIN; k++) wv.ent[k] = a.redg[k < wv.nval ? wv.pos[k] : 0u];
 }
 
 // The sub-table of a grouped endpoint.
@@ -625,6 +628,10 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
   Win cur, nxt;
   uint32_t b0 = wstart;
   win_map<MULTI>(a, wc.flags, b0, ch, tag, cur);
+  // the first window's loads complete before the loop: with no load of `cur` pending on entry,
+  // the waits inside the loop cover only what they must (otherwise they also wait for the next
+  // window's loads, issued just before, and the pipelining is lost).  0x0F70: vmcnt(0).
+  __builtin_amdgcn_s_waitcnt(0x0F70);
   while (true) {
     const uint32_t nb = b0 + wstep;
     const bool more = nb < ch.T;
