@@ -290,6 +290,26 @@ def main():
     gpu_counts = pb.as_u64(out)
     assert np.array_equal(gpu_counts, pb.as_u64(host_counts)), "device and host paths disagree"
 
+    # per-edge counts on the same graph (same ranking, same W): B_alg(EDGE) = 2 B_pass + 4 (n + 1) + 8 m
+    eout = pb.DeviceArray(ctx, g.m, np.uint64)
+    ctx.count(dg, use_v, "EDGE", "GPU", "ADEG", out=eout)
+    ctx.synchronize()
+    eph = []
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ctx.count(dg, use_v, "EDGE", "GPU", "ADEG", out=eout)
+        eph.append(ctx.timing())
+    ctx.synchronize()
+    edge_ms = (time.perf_counter() - t0) * 1e3 / 5
+    edge_count_ms = float(np.mean([p["count_ms"] for p in eph]))
+    edge_balg = 2 * (4 * W + 32 * g.m) + 4 * (n + 1) + 8 * g.m
+    esum = int(pb.as_u64(eout).sum(dtype=np.uint64))
+    edge = {"gwedges_s": round(W / (edge_ms * 1e-3) / 1e9, 3), "ms_per_step": round(edge_ms, 4),
+            "count_ms": round(edge_count_ms, 4), "b_alg_bytes": edge_balg,
+            "roofline_frac": round(edge_balg / (edge_count_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "sum_is_4_total": esum == 2 * int(gpu_counts.sum(dtype=np.uint64))}
+    del eout
+
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
@@ -316,6 +336,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "b_alg_bytes": balg, "count_ms": round(count_ms, 4)},
         "host_inclusive_gwedges_s": round(W / h2d_s / 1e9, 3),
+        "per_edge": edge,
     }
     if not args.no_batch:
         del dg, out

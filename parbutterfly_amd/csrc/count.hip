@@ -628,10 +628,11 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
   Win cur, nxt;
   uint32_t b0 = wstart;
   win_map<MULTI>(a, wc.flags, b0, ch, tag, cur);
-  // the first window's loads complete before the loop: with no load of `cur` pending on entry,
-  // the waits inside the loop cover only what they must (otherwise they also wait for the next
-  // window's loads, issued just before, and the pipelining is lost).  0x0F70: vmcnt(0).
-  __builtin_amdgcn_s_waitcnt(0x0F70);
+  // the first window's loads and lane exchanges complete before the loop: with nothing of
+  // `cur` pending on entry, the waits inside the loop cover only what they must (otherwise
+  // they also wait for the next window's loads and exchanges, issued just before, and the
+  // pipelining is lost).  0x0070: vmcnt(0) lgkmcnt(0).
+  __builtin_amdgcn_s_waitcnt(0x0070);
   while (true) {
     const uint32_t nb = b0 + wstep;
     const bool more = nb < ch.T;
