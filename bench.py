@@ -148,6 +148,29 @@ def batched_bench(pb, ctx, count=4096, first_seed=1000):
     return res
 
 
+def edge_bench(pb, ctx, dg, use_v, W, m, n, vert_sum, steps=5):
+    """Per-edge counts of the headline graph: throughput, roofline fraction of the counting phase,
+    and sum E = 4 TOTAL = 2 sum_S B (the per-vertex run's sum)."""
+    eout = pb.DeviceArray(ctx, m, np.uint64)
+    ctx.count(dg, use_v, "EDGE", "GPU", "ADEG", out=eout)
+    ctx.synchronize()
+    eph = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.count(dg, use_v, "EDGE", "GPU", "ADEG", out=eout)
+        eph.append(ctx.timing())
+    ctx.synchronize()
+    edge_ms = (time.perf_counter() - t0) * 1e3 / steps
+    edge_count_ms = float(np.mean([p["count_ms"] for p in eph]))
+    balg = 2 * (4 * W + 32 * m) + 4 * (n + 1) + 8 * m
+    esum = int(pb.as_u64(eout).sum(dtype=np.uint64))
+    del eout
+    return {"gwedges_s": round(W / (edge_ms * 1e-3) / 1e9, 3), "ms_per_step": round(edge_ms, 4),
+            "count_ms": round(edge_count_ms, 4), "b_alg_bytes": balg,
+            "roofline_frac": round(balg / (edge_count_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "sum_is_4_total": esum == 2 * vert_sum}
+
+
 def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000):
     """Count + peel workload: R-MAT bipartite graph (a, b, c, d) = (0.5, 0.1, 0.1, 0.3), 2^22 x 2^22
     vertices, 1.2e8 distinct edges, seed 3, ACOCORE ranking, per-edge counts, then the full wing
@@ -237,6 +260,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 4096-graph workload")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--no-rmat", action="store_true", help="skip the R-MAT count + wing-peel workload")
+    ap.add_argument("--no-edge", action="store_true", help="skip the per-edge counts on the Chung-Lu graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -291,24 +315,9 @@ def main():
     assert np.array_equal(gpu_counts, pb.as_u64(host_counts)), "device and host paths disagree"
 
     # per-edge counts on the same graph (same ranking, same W): B_alg(EDGE) = 2 B_pass + 4 (n + 1) + 8 m
-    eout = pb.DeviceArray(ctx, g.m, np.uint64)
-    ctx.count(dg, use_v, "EDGE", "GPU", "ADEG", out=eout)
-    ctx.synchronize()
-    eph = []
-    t0 = time.perf_counter()
-    for _ in range(5):
-        ctx.count(dg, use_v, "EDGE", "GPU", "ADEG", out=eout)
-        eph.append(ctx.timing())
-    ctx.synchronize()
-    edge_ms = (time.perf_counter() - t0) * 1e3 / 5
-    edge_count_ms = float(np.mean([p["count_ms"] for p in eph]))
-    edge_balg = 2 * (4 * W + 32 * g.m) + 4 * (n + 1) + 8 * g.m
-    esum = int(pb.as_u64(eout).sum(dtype=np.uint64))
-    edge = {"gwedges_s": round(W / (edge_ms * 1e-3) / 1e9, 3), "ms_per_step": round(edge_ms, 4),
-            "count_ms": round(edge_count_ms, 4), "b_alg_bytes": edge_balg,
-            "roofline_frac": round(edge_balg / (edge_count_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "sum_is_4_total": esum == 2 * int(gpu_counts.sum(dtype=np.uint64))}
-    del eout
+    edge = None
+    if not args.no_edge:
+        edge = edge_bench(pb, ctx, dg, use_v, W, g.m, n, int(gpu_counts.sum(dtype=np.uint64)))
 
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
