@@ -518,7 +518,10 @@ __global__ void k_init_peel(const unsigned long long* __restrict__ counts, unsig
 }
 
 constexpr int V_SMALL_TEAM = 64, V_SMALL_CAP = 1024, V_LARGE_TEAM = 256, V_LARGE_CAP = 8192, V_HBM_BLOCKS = 32;
-constexpr int E_SMALL_TEAM = 64, E_LARGE_TEAM = 256, E_UCAP = 2048;
+// Edge walks are latency bound: the wavefront teams keep a small LDS copy of N(b) (longer lists
+// are searched in place) so that 8 teams fit per SIMD.
+constexpr int E_SMALL_TEAM = 64, E_SMALL_UCAP = 256, E_SMALL_PER_CU = 32;
+constexpr int E_LARGE_TEAM = 256, E_UCAP = 2048, E_LARGE_PER_CU = 7;
 constexpr int ROUND_BATCH = 16;
 
 // Peel-state buffers common to both peels.
@@ -704,7 +707,7 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     return PBF_OK;
   };
   PBF_CHECK(estimate(pb.owner_o));
-  const uint32_t small_max = 4096;  // merge work below which one wavefront handles the edge
+  const uint32_t small_max = 1024;  // walk work up to which one wavefront handles the edge
   uint32_t* small_list = pb.lists;
   uint32_t* large_list = pb.lists + m;
   int64_t entries = m;  // entries of the current view: every edge that may still be live
@@ -718,9 +721,11 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
                        small_max, small_list, large_list, pb.c);
     PeelEArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
-    hipLaunchKernelGGL((k_peele_walk<E_SMALL_TEAM, E_UCAP>), dim3(ctx->num_cus * 8), dim3(E_SMALL_TEAM), 0, ctx->stream, s);
+    hipLaunchKernelGGL((k_peele_walk<E_SMALL_TEAM, E_SMALL_UCAP>), dim3(ctx->num_cus * E_SMALL_PER_CU), dim3(E_SMALL_TEAM), 0,
+                       ctx->stream, s);
     s.list = large_list; s.list_len = pb.c.len + 1; s.queue = pb.c.queue + 1;
-    hipLaunchKernelGGL((k_peele_walk<E_LARGE_TEAM, E_UCAP>), dim3(ctx->num_cus * 4), dim3(E_LARGE_TEAM), 0, ctx->stream, s);
+    hipLaunchKernelGGL((k_peele_walk<E_LARGE_TEAM, E_UCAP>), dim3(ctx->num_cus * E_LARGE_PER_CU), dim3(E_LARGE_TEAM), 0,
+                       ctx->stream, s);
     hipLaunchKernelGGL(k_update, dim3(grid_for(entries, 256, 4096)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B,
                        pb.D, pb.c);
     hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, cur.idO, entries, pb.c);
