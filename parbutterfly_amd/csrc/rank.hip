@@ -18,6 +18,7 @@
 namespace {
 constexpr int KEYS = 33;  // key values 0..32
 constexpr uint32_t RL_CHUNK = 1024;  // lists longer than this are relabelled in chunks
+constexpr uint32_t RL_SMALL = 16;    // lists up to this long are relabelled by one thread each
 
 __global__ void k_side_partial(const uint32_t* __restrict__ off, int64_t n, unsigned long long* acc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     const bool xv = x < g.nv;
     const uint32_t obase = xv ? g.offV[x] : g.offU[x - g.nv];
     const uint32_t deg = xv ? g.offV[x + 1] - obase : g.offU[x - g.nv + 1] - obase;
-    if (deg > RL_CHUNK) continue;  // long lists: k_rl_* below
+    if (deg <= RL_SMALL || deg > RL_CHUNK) continue;  // short lists: k_relabel_small; long: k_rl_*
     const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
     const int64_t nbr_shift = xv ? g.nv : 0;  // combined id of neighbour = adj[j] + shift
     const uint32_t flag = entry_flag(use_v, sflag, x, xv);
@@ -163,6 +164,55 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
       if (act && rk == 0) run[wid][ky] += __popcll(mm);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+// Short lists (at most RL_SMALL entries), one thread each: the same stable sort by key, with
+// the position of entry j counted directly (entries of a higher key, then earlier entries of
+// the same key), so a wavefront serves 64 vertices instead of one.
+__global__ __launch_bounds__(256) void k_relabel_small(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
+                                                       const uint8_t* __restrict__ key, const uint32_t* __restrict__ rank_of,
+                                                       const uint32_t* __restrict__ roff, const uint32_t* __restrict__ lo,
+                                                       uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
+                                                       uint32_t* __restrict__ inpref, uint32_t* __restrict__ rown) {
+  const int64_t n = g.nv + g.nu;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    const bool xv = x < g.nv;
+    const uint32_t obase = xv ? g.offV[x] : g.offU[x - g.nv];
+    const uint32_t deg = (xv ? g.offV[x + 1] : g.offU[x - g.nv + 1]) - obase;
+    if (deg > RL_SMALL) continue;
+    const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
+    const int64_t sh = xv ? g.nv : 0;
+    const uint32_t flag = entry_flag(use_v, sflag, x, xv);
+    const uint32_t r = rank_of[x];
+    const uint32_t lox = lo ? lo[r] : 0u;
+    const uint32_t kself = key[x];
+    uint32_t y[RL_SMALL], ky[RL_SMALL];
+#pragma unroll
+    for (uint32_t j = 0; j < RL_SMALL; j++) {
+      y[j] = j < deg ? (uint32_t)(adj[j] + sh) : 0u;
+      ky[j] = j < deg ? (uint32_t)key[y[j]] : 0u;
+    }
+    // in(r): neighbours of strictly higher key, plus same-key neighbours when x is in U
+    uint32_t in = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < RL_SMALL; j++)
+      in += (j < deg && (ky[j] > kself || (!xv && ky[j] == kself))) ? 1u : 0u;
+    inpref[r] = in;
+    const uint32_t rbase = roff[r];
+#pragma unroll
+    for (uint32_t j = 0; j < RL_SMALL; j++) {
+      if (j < deg) {
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < RL_SMALL; k++)
+          pos += (k < deg && (ky[k] > ky[j] || (k < j && ky[k] == ky[j]))) ? 1u : 0u;
+        const uint32_t slot = rbase + pos;
+        redg[slot] = ((rank_of[y[j]] - lox) << 1) | flag;
+        rown[slot] = r;
+        if (rpos) rpos[slot] = obase + j;
+      }
     }
   }
 }
@@ -562,6 +612,8 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
   const uint8_t* key = static_cast<const uint8_t*>(ctx->buf[S_KEY]);
   hipLaunchKernelGGL(k_ranked_degree, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, R->order, R->roff);
   PBF_CHECK(pbf_scan_excl_u32(ctx, R->roff, R->roff, n + 1, nullptr));
+  hipLaunchKernelGGL(k_relabel_small, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, key,
+                     R->rank_of, R->roff, R->lo, R->redg, R->rpos, R->inpref, R->rown);
   hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 4, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of,
                      R->roff, R->lo, R->redg, R->rpos, R->inpref, R->rown);
   // long lists, chunk-parallel
