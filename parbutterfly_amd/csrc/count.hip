@@ -1083,16 +1083,37 @@ __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ r
                                                   const uint32_t* __restrict__ inpref, const uint2* __restrict__ sinfo,
                                                   int64_t n, ClassOut co, uint32_t* __restrict__ big,
                                                   uint32_t* __restrict__ big_chunks, unsigned long long* __restrict__ w64) {
-  const uint32_t lane = threadIdx.x & 63;
+  __shared__ unsigned long long s_T[4][64];
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; base < n; base += nwaves * 64) {
+  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wid) * 64; base < n; base += nwaves * 64) {
     const int64_t i = base + lane;
     uint32_t io = 0, deg = 0;
     if (i < n) { io = roff[i]; deg = roff[i + 1] - io; }
-    uint64_t T = 0;
     const bool small = deg <= 64;
-    if (small)
-      for (uint32_t j = 0; j < deg; j++) T += seg_len(sinfo, io + j);
+    // T of the short lists, summed by the whole wavefront over their concatenated slots (at most
+    // 4096): lane l reads slot q = q0 + l of the concatenation, whose owner k is found among the
+    // lanes' starts, so the reads are coalesced and every lane stays busy
+    const uint32_t d = (i < n && small) ? deg : 0u;
+    const uint32_t cinc = wave_incl_add(d);
+    const uint32_t C = lane_read(cinc, 63), c = cinc - d;
+    s_T[wid][lane] = 0;
+    wave_lds_fence();
+    for (uint32_t q0 = 0; q0 < C; q0 += 64) {
+      const uint32_t q = q0 + lane;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t bit = 32; bit; bit >>= 1) {  // largest k with c_k <= q
+        const uint32_t cand = k + bit;
+        const uint32_t cc = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((cand & 63u) << 2), (int)c);
+        if (cand < 64u && cc <= q) k = cand;
+      }
+      const uint32_t ck = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(k << 2), (int)c);
+      const uint32_t iok = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(k << 2), (int)io);
+      if (q < C) atomicAdd(&s_T[wid][k], (unsigned long long)seg_len(sinfo, iok + (q - ck)));
+    }
+    wave_lds_fence();
+    const uint64_t T = s_T[wid][lane];
     classify_append((uint32_t)i, i < n && small, T, deg, co);
     const uint64_t mb = __ballot(!small);
     uint32_t bb = 0;
