@@ -1432,13 +1432,21 @@ __global__ void k_oidx(const uint32_t* __restrict__ redg, const uint32_t* __rest
 // number of entries of N(y) ranked below min(r, y): in(y) when y < r, else the position of r
 // inside N(y) (a binary search over the in-prefix of N(y)).  The counting kernels read both
 // beside the slot, which keeps two dependent loads and the search off their critical path.
-__global__ void k_slotinfo(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
-                           const uint32_t* __restrict__ rown, const uint32_t* __restrict__ inpref,
-                           const uint32_t* __restrict__ lo, int64_t m2, uint2* __restrict__ sinfo) {
+// rinfo[r] = (roff[r], in(r)): one 8-byte gather per slot instead of two 4-byte ones.
+__global__ void k_rinfo(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ inpref, int64_t n,
+                        uint2* __restrict__ rinfo) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    rinfo[r] = make_uint2(roff[r], inpref[r]);
+}
+
+__global__ void k_slotinfo(const uint2* __restrict__ rinfo, const uint32_t* __restrict__ redg,
+                           const uint32_t* __restrict__ rown, const uint32_t* __restrict__ lo, int64_t m2,
+                           uint2* __restrict__ sinfo) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t rg = rown[p], l = lo ? lo[rg] : 0u;
     const uint32_t r = rg - l, y = redg[p] >> 1;  // both local to the graph of rg
-    const uint32_t yo = roff[y + l], ip = inpref[y + l];
+    const uint2 yi = rinfo[y + l];
+    const uint32_t yo = yi.x, ip = yi.y;
     sinfo[p] = make_uint2(yo, y < r ? ip : rank_lower_bound(redg + yo, ip < r ? ip : r, r));
   }
 }
@@ -1487,8 +1495,14 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   uint32_t* big_chunks = big + n;
   uint32_t* chunk_base = big + 2 * n;
   unsigned long long* w64 = static_cast<unsigned long long*>(up);
-  hipLaunchKernelGGL(k_slotinfo, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.rown,
-                     R.inpref, R.lo, R.m2, const_cast<uint2*>(a.sinfo));
+  {
+    void* rip;
+    PBF_CHECK(pbf_scratch(ctx, S_RINFO, 8 * (size_t)(n ? n : 1), &rip));
+    uint2* rinfo = static_cast<uint2*>(rip);
+    hipLaunchKernelGGL(k_rinfo, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.inpref, n, rinfo);
+    hipLaunchKernelGGL(k_slotinfo, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, rinfo, R.redg, R.rown, R.lo,
+                       R.m2, const_cast<uint2*>(a.sinfo));
+  }
   PBF_CHECK(pbf_scratch(ctx, S_LPLAN, 4 * (64 + 2 * LG_BLOCKS * LBUCKETS) + 8 * LBUCKETS, &pl));
   uint32_t* plan = static_cast<uint32_t*>(pl);
   uint32_t* bh = plan + 64;

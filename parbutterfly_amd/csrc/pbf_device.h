@@ -54,6 +54,7 @@ enum ScratchSlot {
   S_MLP,         // selector MLP: parameters, Adam moments, samples
   S_BFEAT,       // batch features: per-graph statistics
   S_RLO,         // batch: first rank of each rank's graph block (graph-local entries)
+  S_RINFO,       // per rank: (roff, in) packed for the slot-info gathers
   S_NUM_SLOTS
 };
 

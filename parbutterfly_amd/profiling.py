@@ -18,10 +18,10 @@ import os
 from collections import defaultdict
 
 # kernels launched between the counting phase's start and end events (count.hip)
-COUNT_PHASE = ("k_slotinfo", "k_fill_u32", "k_fill_u64", "k_classify", "k_classify_big", "k_wedge_chunks",
+COUNT_PHASE = ("k_rinfo", "k_slotinfo", "k_fill_u32", "k_fill_u64", "k_classify", "k_classify_big", "k_wedge_chunks",
                "k_scan_tiles", "k_tile_sums", "k_scan_sums", "k_light_hist", "k_light_plan", "k_light_scatter",
                "k_split_pass1", "k_split_credit", "k_split_pass2", "k_count2", "k_count_hbm", "k_hot_reduce",
-               "k_convert_vert", "k_oidx")
+               "k_convert_vert", "k_oidx", "k_edge_out_s", "k_edge_out_o")
 
 
 def _short(name):
