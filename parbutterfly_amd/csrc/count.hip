@@ -791,9 +791,9 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
   }
 }
 
-// Table size of a light endpoint with distinct-key bound b (load <= 3/4).
+// Table size of a light endpoint with distinct-key bound b (load <= 3/4), at least 16 slots.
 __device__ __forceinline__ uint32_t light_cap(uint32_t b) {
-  uint32_t cap = 64;
+  uint32_t cap = 16;
   while (cap < b + b / 3 + 1) cap <<= 1;
   return cap;
 }
@@ -872,7 +872,7 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
   }
   if (ok && (MODE == MODE_TOTAL || (MODE == MODE_VERT && !o_side))) {
     // endpoint credits: each lane sweeps 4 words of one sub-table per step (sub-tables are
-    // multiples of 64 words) and clears them; per-endpoint sums are reduced with a segmented
+    // multiples of 16 words) and clears them; per-endpoint sums are reduced with a segmented
     // scan (an endpoint's sum of C(c,2) is at most T^2 / 2 < 2^32) into iacc
     const uint32_t cmask = (1u << a.cb) - 1u;
     for (uint32_t q0 = 0; q0 < words; q0 += WIN_SPAN) {
@@ -884,7 +884,7 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
         *(lds_u32x4*)(tw + q) = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
       }
       const uint32_t cur[4] = {f.x, f.y, f.z, f.w};
-      // the 4 words lie in one sub-table (sub-tables are multiples of 64 words); every lane
+      // the 4 words lie in one sub-table (sub-tables are multiples of 16 words); every lane
       // takes part in the cross-lane steps (see win_credit)
       const uint32_t kq = lane_upper(off_k, nk, q);
       const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(kq << 2), (int)lo_k);
@@ -1197,20 +1197,26 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
 }
 
 // ---------------- grouping of the light endpoints ----------------
-// Light endpoints are sorted by (side, table size descending) with a counting sort over 12
-// buckets: bucket = side * 6 + (11 - log2 cap), side 1 when a VERT endpoint lies on O (its
-// groups run pass 2 instead of the sweep).  Within a side the tables then tile consecutive
-// LT_WORDS-word groups exactly: the word prefix of every endpoint is a multiple of its own
-// power-of-two size, so a group starts wherever the prefix is a multiple of LT_WORDS.
-// plan (u32): [0, 12) bucket counts, [12, 24) bucket starts, [24, 36) scatter counters,
-// [36, 38) first group of each side; wbase (u64): word prefix of each bucket within its side.
-constexpr uint32_t LBUCKETS = 12;
+// Light endpoints are sorted by (side, table size descending) with a counting sort over 16
+// buckets: bucket = side * 8 + (11 - log2 cap) for caps 2048 .. 16, side 1 when a VERT endpoint
+// lies on O (its groups run pass 2 instead of the sweep).  Within a side the tables then tile
+// consecutive LT_WORDS-word groups: grouping counts every table as max(cap, 64) words (so a
+// group holds at most 32 endpoints, one record per lane of its first half), and since these
+// sizes are powers of two in decreasing order, the prefix of every endpoint is a multiple of its
+// own size and a group starts wherever the prefix is a multiple of LT_WORDS.  Inside a group
+// the tables are laid out by their real sizes (light_group), which only packs them tighter.
+// plan (u32): [0, 16) bucket counts, [16, 32) bucket starts, [32, 34) first group of each side;
+// wbase (u64): word prefix of each bucket within its side.
+constexpr uint32_t LB_SIDE = 8, LBUCKETS = 2 * LB_SIDE;
+constexpr uint32_t PLAN_START = LBUCKETS, PLAN_GROUP0 = 2 * LBUCKETS;
+// grouping words of a table of bucket r (2^(11 - r) slots): never below 64
+__device__ __forceinline__ uint32_t group_words(uint32_t r) { return 2048u >> (r < 5u ? r : 5u); }
 
 template <int MODE>
 __device__ __forceinline__ uint32_t light_bucket(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ redg,
                                                  const uint32_t* __restrict__ bound, uint32_t i) {
   const uint32_t side = (MODE == MODE_VERT && (redg[roff[i]] & 1u)) ? 1u : 0u;
-  return side * 6u + (11u - (31u - (uint32_t)__clz(light_cap(bound[i]))));
+  return side * LB_SIDE + (11u - (31u - (uint32_t)__clz(light_cap(bound[i]))));
 }
 
 // Two-level counting sort over LG_BLOCKS workgroups (the same items visit the same workgroup in
@@ -1260,15 +1266,15 @@ __global__ __launch_bounds__(LG_BLOCKS) void k_light_plan(const uint32_t* __rest
     uint32_t start = 0, groups = 0;
     for (uint32_t side = 0; side < 2; side++) {
       unsigned long long words = 0;
-      for (uint32_t r = 0; r < 6; r++) {
-        const uint32_t b = side * 6 + r;
+      for (uint32_t r = 0; r < LB_SIDE; r++) {
+        const uint32_t b = side * LB_SIDE + r;
         plan[b] = tot[b];
-        plan[12 + b] = start;
+        plan[PLAN_START + b] = start;
         start += tot[b];
         wbase[b] = words;
-        words += (unsigned long long)tot[b] * (2048u >> r);  // bucket r holds tables of 2^(11 - r) words
+        words += (unsigned long long)tot[b] * group_words(r);
       }
-      plan[36 + side] = groups;
+      plan[PLAN_GROUP0 + side] = groups;
       groups += (uint32_t)((words + LT_WORDS - 1) / LT_WORDS);
       if (side == 0) ctl[10] = groups;
     }
@@ -1276,7 +1282,7 @@ __global__ __launch_bounds__(LG_BLOCKS) void k_light_plan(const uint32_t* __rest
     gstart[groups] = start;
   }
   __syncthreads();
-  for (uint32_t q = 0; q < LBUCKETS; q++) bo[t * LBUCKETS + q] = plan[12 + q] + excl[q];
+  for (uint32_t q = 0; q < LBUCKETS; q++) bo[t * LBUCKETS + q] = plan[PLAN_START + q] + excl[q];
 }
 
 template <int MODE>
@@ -1296,9 +1302,10 @@ __global__ __launch_bounds__(256) void k_light_scatter(const uint32_t* __restric
     const uint32_t pos = atomicAdd(&c[b], 1u);
     out[pos] = i;
     const uint32_t io = roff[i];
-    rec[pos] = make_uint4(i, io, roff[i + 1] - io, 11u - b % 6u);
-    const unsigned long long prefix = wbase[b] + (unsigned long long)(pos - plan[12 + b]) * (2048u >> (b % 6u));
-    if ((prefix & (LT_WORDS - 1)) == 0) gstart[plan[36 + b / 6u] + (uint32_t)(prefix / LT_WORDS)] = pos;
+    rec[pos] = make_uint4(i, io, roff[i + 1] - io, 11u - b % LB_SIDE);
+    const unsigned long long prefix =
+        wbase[b] + (unsigned long long)(pos - plan[PLAN_START + b]) * group_words(b % LB_SIDE);
+    if ((prefix & (LT_WORDS - 1)) == 0) gstart[plan[PLAN_GROUP0 + b / LB_SIDE] + (uint32_t)(prefix / LT_WORDS)] = pos;
   }
 }
 
