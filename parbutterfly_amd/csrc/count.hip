@@ -43,8 +43,9 @@ struct CountArgs {
   uint32_t* queue;         // work counter
   uint32_t* ovf_list;
   uint32_t* ovf_len;
-  uint32_t* pool;          // T_HBM: per-block dense counters [n] followed by used list [n]
+  uint32_t* pool;          // T_HBM: per-block dense counters [kn] followed by used list [kn]
   int64_t n;
+  int64_t kn;              // key range: n, or the largest graph of a union with graph-local ranks
   // fused kernel (k_count2): two work lists and their queues
   const uint32_t* blist;   // endpoints processed by a whole workgroup
   const uint32_t* wlist;   // light endpoints, grouped (see light_group)
@@ -75,8 +76,8 @@ __global__ __launch_bounds__(TEAM) void k_count_hbm(CountArgs a) {
   __shared__ TeamShared<TEAM> sh;
   const uint32_t tid = threadIdx.x;
   const uint32_t nlist = *a.list_len;
-  uint32_t* dense = a.pool + (size_t)blockIdx.x * (size_t)a.n * 2;
-  uint32_t* dused = dense + a.n;
+  uint32_t* dense = a.pool + (size_t)blockIdx.x * (size_t)a.kn * 2;
+  uint32_t* dused = dense + a.kn;
   unsigned long long block_total = 0;
 
   while (true) {
@@ -1589,13 +1590,17 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   if (MODE != MODE_EDGE && MODE != MODE_TOTAL)
     hipLaunchKernelGGL(k_hot_reduce, dim3(grid_for(HOT_N, 256, 64)), dim3(256), 0, ctx->stream, a.hot, a.B, n);
   uint32_t novf = 0;
-  int blocks = HBM_BLOCKS_NOSYNC;
+  // each block needs 2 kn words of dense counters: with small key ranges (a union of small
+  // graphs) a 256 MB pool holds many more blocks
+  const int64_t fit = std::min<int64_t>((int64_t)ctx->num_cus * 4, ((int64_t)256 << 20) / (8 * std::max<int64_t>(a.kn, 1)));
+  int blocks = (int)std::max<int64_t>(HBM_BLOCKS_NOSYNC, fit);
   if (sync) {
     PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 11, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     PBF_HIP(hipStreamSynchronize(ctx->stream));
     novf = (uint32_t)(ctx->pinned[16] & 0xFFFFFFFFu);
     ctx->timing.hub_vertices = novf;
-    blocks = (int)(novf < (uint32_t)HBM_BLOCKS ? novf : (uint32_t)HBM_BLOCKS);
+    const int64_t most = std::max<int64_t>(HBM_BLOCKS, fit);
+    blocks = (int)std::min<int64_t>((int64_t)novf, most);
   } else {
     ctx->timing.hub_vertices = -1;  // not read back
   }
@@ -1604,7 +1609,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
     // entry is restored to zero after its endpoint, so the pool is cleared only when first
     // allocated)
     void* pp;
-    const size_t pool_bytes = sizeof(uint32_t) * 2 * (size_t)n * blocks;
+    const size_t pool_bytes = sizeof(uint32_t) * 2 * (size_t)a.kn * blocks;
     PBF_CHECK(pbf_scratch(ctx, S_POOL, pool_bytes, &pp));
     if (ctx->pool_clean < pool_bytes) {
       PBF_CHECK(pbf_fill_u32(ctx, static_cast<uint32_t*>(pp), 0u, (int64_t)(pool_bytes / 4)));
@@ -1642,6 +1647,7 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
     // key bits: keys (local ranks) are below key_range <= 2^kb - 1, so a packed key is never
     // all ones
     const int64_t key_range = (R.lo && opts.key_range > 0) ? opts.key_range : n;
+    a.kn = key_range > 0 ? key_range : 1;
     uint32_t kb = 0;
     while (kb < 32 && ((uint64_t)1 << kb) - 1 < (uint64_t)key_range) kb++;
     a.cb = 32 - kb;
