@@ -89,7 +89,8 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t x, uint32_t* lis
 // has D >= k, and the live items with D == k are all touched ones (an untouched item keeps its
 // D, which was not k or it would have been peeled).  So when a touched item has D == k the new
 // minimum is k again and the frontier is exactly those items; else the full scan below runs.
-__global__ void k_select_touched(const unsigned long long* __restrict__ D, uint8_t* __restrict__ state,
+__global__ void k_select_touched(const unsigned long long* __restrict__ D, const unsigned long long* __restrict__ B,
+                                 uint8_t* __restrict__ state,
                                  const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
                                  uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
                                  Ctl c) {
@@ -104,7 +105,8 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, uint8
     if (sel) {
       state[x] = 2;
       nsel++;
-      w = work[x] & WORK_MASK;
+      // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
+      w = B[x] ? (work[x] & WORK_MASK) : 0u;
     }
     wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
     wave_append(sel && w > small_max, x, large_list, &c.len[1]);
@@ -132,7 +134,8 @@ __global__ void k_min_live(const unsigned long long* __restrict__ D, const uint8
 
 // Frontier selection: live items with D == k become current (state 2) and are listed by work
 // class (small / large by `work`); `live` is decremented by the number selected.
-__global__ void k_select(const unsigned long long* __restrict__ D, uint8_t* __restrict__ state,
+__global__ void k_select(const unsigned long long* __restrict__ D, const unsigned long long* __restrict__ B,
+                         uint8_t* __restrict__ state,
                          const uint32_t* __restrict__ ids, int64_t n, const uint32_t* __restrict__ work,
                          uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list, Ctl c) {
   const unsigned long long k = *c.k;
@@ -145,7 +148,8 @@ __global__ void k_select(const unsigned long long* __restrict__ D, uint8_t* __re
     if (sel) {
       state[x] = 2;
       nsel++;
-      w = work[x] & WORK_MASK;
+      // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
+      w = B[x] ? (work[x] & WORK_MASK) : 0u;
     }
     wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
     wave_append(sel && w > small_max, x, large_list, &c.len[1]);
@@ -615,10 +619,10 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   const unsigned sel_grid = grid_for(ns, 256, 4096);
   auto round = [&]() -> int {
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
-    hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.state, pb.touched,
+    hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
                        pb.work, small_max, small_list, large_list, pb.c);
     hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.c);
-    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.work, small_max,
+    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, nullptr, ns, pb.work, small_max,
                        small_list, large_list, pb.c);
     PeelVArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
@@ -714,10 +718,10 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   auto round = [&]() -> int {
     const unsigned sel_grid = grid_for(entries, 256, 8192);
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
-    hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.state, pb.touched,
+    hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
                        pb.work, small_max, small_list, large_list, pb.c);
     hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.c);
-    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.work,
+    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, cur.idO, entries, pb.work,
                        small_max, small_list, large_list, pb.c);
     PeelEArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
