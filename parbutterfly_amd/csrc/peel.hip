@@ -395,6 +395,11 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     const bool cached = bdeg <= (uint32_t)UCAP;
     if (cached)
       for (uint32_t q = tid; q < bdeg; q += TEAM) { nb_v[q] = nbrY[bo + q]; nb_f[q] = idY[bo + q]; }
+    // B[e] counts exactly the butterflies through e whose four edges are live or peeled in
+    // this round: once the walk has met that many, nothing is left to find
+    const unsigned long long target = a.B[e];
+    volatile uint32_t* found = &sh.nused;
+    if (tid == 0) *found = 0;
     __syncthreads();
     for (uint32_t c0 = 0; c0 < adeg; c0 += TEAM) {
       const uint32_t j = c0 + tid;
@@ -415,13 +420,15 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
       sh.acc[tid] = 0;
       __syncthreads();
       for (uint32_t w = tid; w < T; w += TEAM) {
+        if (*found >= target) break;
         const uint32_t s = seg_of<TEAM>(sh.P, w);
         const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
         const uint32_t p2 = sh.vs[s] + kk;
         const uint32_t d = nbrY[p2];
         if (d == va) continue;
         const uint32_t fd = idY[p2];
-        if (!usable(a.state, fd, e)) continue;
+        const uint8_t sd = a.state[fd];
+        if (sd == 0) continue;
         // edge (b, d): search d in N(b)
         uint32_t lo = 0, hi = bdeg;
         if (cached) {
@@ -432,7 +439,10 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
           if (lo == bdeg || nbrY[bo + lo] != d) continue;
         }
         const uint32_t fb = cached ? nb_f[lo] : idY[bo + lo];
-        if (!usable(a.state, fb, e)) continue;
+        const uint8_t sb = a.state[fb];
+        if (sb == 0) continue;
+        atomicAdd((uint32_t*)found, 1u);  // a butterfly through e
+        if (!(sd == 1 || fd > e) || !(sb == 1 || fb > e)) continue;  // charged by a smaller peeled edge
         atomicAdd(&sh.acc[s], 1ull);
         if (a.state[fd] == 1) { atomicAdd(&a.B[fd], ~0ull); touch(fd, a.tflag, a.touched, a.ntouched); }
         if (a.state[fb] == 1) { atomicAdd(&a.B[fb], ~0ull); touch(fb, a.tflag, a.touched, a.ntouched); }
@@ -446,7 +456,9 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
         }
       }
       __syncthreads();
+      if (*found >= target) break;  // uniform: read by every thread after the barrier
     }
+    __syncthreads();  // the next item resets the counter
   }
 }
 
