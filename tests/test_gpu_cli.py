@@ -1,0 +1,53 @@
+"""The command-line driver (BipartiteButterflies.C's flags and printed lines) on the GPU, against
+the totals the compiled reference wrote into the golden fixtures."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import parbutterfly_amd as pb
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CLI = os.path.join(os.path.dirname(pb.library_path()), "bin", "bipartite_butterflies")
+GOLD = os.path.join(HERE, "golden")
+
+
+def _run(*args):
+    print("cli", args, flush=True)
+    r = subprocess.run([CLI, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.mark.parametrize("name", ["uniform_1000x1000_m10000_s1", "rmat_8x8_m3000_s3", "k_4_5"])
+def test_cli_totals_match_reference(name):
+    adj = os.path.join(GOLD, name + ".adj")
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    total = int(z["total_SERIAL_ADEG"][0])
+    # per vertex with tip peeling, numeric aliases (-t 13 = GPU count, -tp 6 = GPU peel)
+    rc, out, err = _run("-t", "13", "-w", "4", "-e", "0", "-tp", "6", "-r", "1", adj)  # -w 4: ADEG
+    assert rc == 0, err
+    got = [int(x) for x in re.findall(r"number of butterflies: (\d+)", out)]
+    assert got == [total, total], out  # the reference reads and counts rounds + 1 times
+    assert "GPU Peel" in out
+    # per edge, string flags, ACOCORE, wing peeling
+    rc, out, err = _run("-countType", "GPU", "-rankType", "ACOCORE", "-per", "EDGE", "-peelType", "GPU", "-r", "0", adj)
+    assert rc == 0, err
+    assert [int(x) for x in re.findall(r"number of edge butterflies: (\d+)", out)] == [total], out
+    # total only, a CPU count type name (every count type runs the GPU kernels here)
+    rc, out, err = _run("-countType", "BATCHS", "-rankType", "SIDE", "-per", "TOTAL", "-r", "0", adj)
+    assert rc == 0, err
+    assert [int(x) for x in re.findall(r"number of butterflies: (\d+)", out)] == [total], out
+
+
+def test_cli_errors():
+    adj = os.path.join(GOLD, "k_4_5.adj")
+    rc, _, err = _run("-rankType", "DEG", "-r", "0", adj)
+    assert rc != 0 and "unsupported" in err.lower()
+    rc, _, err = _run("-sparseType", "ESPARSE", "-r", "0", adj)
+    assert rc != 0 and "sparsification" in err
+    rc, _, err = _run("-r", "0", os.path.join(GOLD, "no_such_file.adj"))
+    assert rc != 0
