@@ -25,6 +25,71 @@ int check_desc(const pbf_bipartite_csr* g) {
   return PBF_OK;
 }
 
+constexpr size_t STAGE_BYTES = size_t(8) << 20, STAGE_MIN = size_t(1) << 20;
+
+int stage_init(pbf_context* ctx) {
+  for (int k = 0; k < 2; k++) {
+    if (!ctx->stage[k]) PBF_HIP(hipHostMalloc(&ctx->stage[k], STAGE_BYTES, hipHostMallocDefault));
+    if (!ctx->stage_ev[k]) PBF_HIP(hipEventCreateWithFlags(&ctx->stage_ev[k], hipEventDisableTiming));
+  }
+  return PBF_OK;
+}
+
+int stage_wait(pbf_context* ctx, int k) {
+  if (ctx->stage_busy[k]) PBF_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+  ctx->stage_busy[k] = false;
+  return PBF_OK;
+}
+}  // namespace
+
+int pbf_h2d(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes < STAGE_MIN) {
+    if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return PBF_OK;
+  }
+  PBF_CHECK(stage_init(ctx));
+  int k = 0;
+  for (size_t off = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
+    const size_t n = std::min(STAGE_BYTES, bytes - off);
+    PBF_CHECK(stage_wait(ctx, k));  // this buffer's previous DMA is done
+    memcpy(ctx->stage[k], static_cast<const char*>(src) + off, n);
+    PBF_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, ctx->stage[k], n, hipMemcpyHostToDevice, ctx->stream));
+    PBF_HIP(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+    ctx->stage_busy[k] = true;
+  }
+  return PBF_OK;
+}
+
+int pbf_d2h(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes < STAGE_MIN) {
+    if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    return PBF_OK;
+  }
+  PBF_CHECK(stage_init(ctx));
+  for (int k = 0; k < 2; k++) PBF_CHECK(stage_wait(ctx, k));
+  const size_t nch = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+  auto issue = [&](size_t c) -> int {
+    const int k = (int)(c & 1);
+    const size_t off = c * STAGE_BYTES, n = std::min(STAGE_BYTES, bytes - off);
+    PBF_HIP(hipMemcpyAsync(ctx->stage[k], static_cast<const char*>(src) + off, n, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+    ctx->stage_busy[k] = true;
+    return PBF_OK;
+  };
+  PBF_CHECK(issue(0));
+  for (size_t c = 0; c < nch; c++) {
+    if (c + 1 < nch) PBF_CHECK(issue(c + 1));  // its buffer was emptied in the previous step
+    const int k = (int)(c & 1);
+    PBF_CHECK(stage_wait(ctx, k));
+    const size_t off = c * STAGE_BYTES, n = std::min(STAGE_BYTES, bytes - off);
+    memcpy(static_cast<char*>(dst) + off, ctx->stage[k], n);
+  }
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  return PBF_OK;
+}
+
+namespace {
 // Resolves the graph to device pointers, uploading a host graph into the context.
 int resolve_graph(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, DevGraph* dg) {
   PBF_CHECK(check_desc(g));
@@ -39,10 +104,10 @@ int resolve_graph(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, DevGrap
   void* p;
   PBF_CHECK(pbf_scratch(ctx, S_GRAPH, 4 * words, &p));
   uint32_t* d = static_cast<uint32_t*>(p);
-  PBF_HIP(hipMemcpyAsync(d, g->offsetsV, 4 * size_t(g->nv + 1), hipMemcpyHostToDevice, ctx->stream));
-  PBF_HIP(hipMemcpyAsync(d + g->nv + 1, g->edgesV, 4 * size_t(g->m), hipMemcpyHostToDevice, ctx->stream));
-  PBF_HIP(hipMemcpyAsync(d + g->nv + 1 + g->m, g->offsetsU, 4 * size_t(g->nu + 1), hipMemcpyHostToDevice, ctx->stream));
-  PBF_HIP(hipMemcpyAsync(d + g->nv + 1 + g->m + g->nu + 1, g->edgesU, 4 * size_t(g->m), hipMemcpyHostToDevice, ctx->stream));
+  PBF_CHECK(pbf_h2d(ctx, d, g->offsetsV, 4 * size_t(g->nv + 1)));
+  PBF_CHECK(pbf_h2d(ctx, d + g->nv + 1, g->edgesV, 4 * size_t(g->m)));
+  PBF_CHECK(pbf_h2d(ctx, d + g->nv + 1 + g->m, g->offsetsU, 4 * size_t(g->nu + 1)));
+  PBF_CHECK(pbf_h2d(ctx, d + g->nv + 1 + g->m + g->nu + 1, g->edgesU, 4 * size_t(g->m)));
   dg->offV = d;
   dg->edgV = d + g->nv + 1;
   dg->offU = d + g->nv + 1 + g->m;
@@ -57,7 +122,7 @@ int out_buffer(pbf_context* ctx, int mem, void* user, size_t bytes, void** dev) 
 }
 
 int finish_output(pbf_context* ctx, int mem, void* user, const void* dev, size_t bytes) {
-  if (mem == PBF_MEM_HOST && bytes) PBF_HIP(hipMemcpyAsync(user, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (mem == PBF_MEM_HOST) return pbf_d2h(ctx, user, dev, bytes);
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }
@@ -196,6 +261,10 @@ extern "C" int pbf_destroy(pbf_context* ctx) {
   DeviceGuard guard(ctx->device);
   pbf_trim(ctx);
   for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; k++) {
+    if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
+    if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
+  }
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -234,7 +303,7 @@ extern "C" int pbf_device_free(pbf_context* ctx, void* ptr) {
 extern "C" int pbf_copy_to_device(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx || (bytes && (!dst || !src))) return PBF_ERR_ARG;
   DeviceGuard guard(ctx->device);
-  if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (bytes) PBF_CHECK(pbf_h2d(ctx, dst, src, bytes));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }
@@ -268,7 +337,7 @@ extern "C" int pbf_copy_device(pbf_context* ctx, void* dst, const void* src, siz
 extern "C" int pbf_copy_to_host(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx || (bytes && (!dst || !src))) return PBF_ERR_ARG;
   DeviceGuard guard(ctx->device);
-  if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (bytes) PBF_CHECK(pbf_d2h(ctx, dst, src, bytes));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }
@@ -349,7 +418,7 @@ int peel_common(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v
   if (mem == PBF_MEM_HOST) {
     void* cp;
     PBF_CHECK(pbf_scratch(ctx, S_COUNTS, bytes ? bytes : 8, &cp));
-    if (bytes) PBF_HIP(hipMemcpyAsync(cp, counts, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (bytes) PBF_CHECK(pbf_h2d(ctx, cp, counts, bytes));
     cdev = static_cast<const uint64_t*>(cp);
   }
   PBF_CHECK(out_buffer(ctx, mem, out, bytes ? bytes : 8, &odev));

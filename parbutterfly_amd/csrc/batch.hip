@@ -379,10 +379,10 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
         void* p;
         PBF_CHECK(pbf_scratch(ctx, S_GRAPH, 4 * words, &p));
         uint32_t* w = static_cast<uint32_t*>(p);
-        PBF_HIP(hipMemcpyAsync(w, d.offsetsV, 4 * size_t(d.nv + 1), hipMemcpyHostToDevice, ctx->stream));
-        PBF_HIP(hipMemcpyAsync(w + d.nv + 1, d.edgesV, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
-        PBF_HIP(hipMemcpyAsync(w + d.nv + 1 + d.m, d.offsetsU, 4 * size_t(d.nu + 1), hipMemcpyHostToDevice, ctx->stream));
-        PBF_HIP(hipMemcpyAsync(w + d.nv + 1 + d.m + d.nu + 1, d.edgesU, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+        PBF_CHECK(pbf_h2d(ctx, w, d.offsetsV, 4 * size_t(d.nv + 1)));
+        PBF_CHECK(pbf_h2d(ctx, w + d.nv + 1, d.edgesV, 4 * size_t(d.m)));
+        PBF_CHECK(pbf_h2d(ctx, w + d.nv + 1 + d.m, d.offsetsU, 4 * size_t(d.nu + 1)));
+        PBF_CHECK(pbf_h2d(ctx, w + d.nv + 1 + d.m + d.nu + 1, d.edgesU, 4 * size_t(d.m)));
         dg.offV = w; dg.edgV = w + d.nv + 1; dg.offU = w + d.nv + 1 + d.m; dg.edgU = w + d.nv + 1 + d.m + d.nu + 1;
       }
       int use_v;
@@ -399,8 +399,8 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
     }
     PBF_HIP(hipEventRecord(ev[4], ctx->stream));
     if (out_mem == PBF_MEM_HOST) {
-      PBF_HIP(hipMemcpyAsync(vert_out, vout, 8 * (size_t)off[count], hipMemcpyDeviceToHost, ctx->stream));
-      PBF_HIP(hipMemcpyAsync(totals, tot, 8 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+      PBF_CHECK(pbf_d2h(ctx, vert_out, vout, 8 * (size_t)off[count]));
+      PBF_CHECK(pbf_d2h(ctx, totals, tot, 8 * (size_t)count));
       PBF_HIP(hipStreamSynchronize(ctx->stream));
       memcpy(vert_off, off.data(), 8 * (size_t)(count + 1));
     } else {
@@ -422,13 +422,13 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
     for (int64_t g = 0; g < count; g++) {
       const pbf_bipartite_csr& d = graphs[g];
       BGraph& b = tab[g];
-      PBF_HIP(hipMemcpyAsync(w, d.offsetsV, 4 * size_t(d.nv + 1), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.offsetsV, 4 * size_t(d.nv + 1)));
       b.offV = w; w += d.nv + 1;
-      PBF_HIP(hipMemcpyAsync(w, d.edgesV, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.edgesV, 4 * size_t(d.m)));
       b.edgV = w; w += d.m;
-      PBF_HIP(hipMemcpyAsync(w, d.offsetsU, 4 * size_t(d.nu + 1), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.offsetsU, 4 * size_t(d.nu + 1)));
       b.offU = w; w += d.nu + 1;
-      PBF_HIP(hipMemcpyAsync(w, d.edgesU, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.edgesU, 4 * size_t(d.m)));
       b.edgU = w; w += d.m;
     }
   }
@@ -477,8 +477,8 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
     std::vector<uint64_t> off((size_t)count + 1);
     PBF_HIP(hipMemcpyAsync(off.data(), offd, 8 * (size_t)(count + 1), hipMemcpyDeviceToHost, ctx->stream));
     PBF_HIP(hipStreamSynchronize(ctx->stream));
-    PBF_HIP(hipMemcpyAsync(vert_out, vout, 8 * (size_t)off[count], hipMemcpyDeviceToHost, ctx->stream));
-    PBF_HIP(hipMemcpyAsync(totals, tot, 8 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_CHECK(pbf_d2h(ctx, vert_out, vout, 8 * (size_t)off[count]));
+    PBF_CHECK(pbf_d2h(ctx, totals, tot, 8 * (size_t)count));
     PBF_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(vert_off, off.data(), 8 * (size_t)(count + 1));
   } else {
@@ -589,13 +589,13 @@ extern "C" int pbf_batch_features(pbf_context* ctx, int64_t count, const pbf_bip
     for (int64_t g = 0; g < count; g++) {
       const pbf_bipartite_csr& d = graphs[g];
       BGraph& bg = tab[g];
-      PBF_HIP(hipMemcpyAsync(w, d.offsetsV, 4 * size_t(d.nv + 1), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.offsetsV, 4 * size_t(d.nv + 1)));
       bg.offV = w; w += d.nv + 1;
-      PBF_HIP(hipMemcpyAsync(w, d.edgesV, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.edgesV, 4 * size_t(d.m)));
       bg.edgV = w; w += d.m;
-      PBF_HIP(hipMemcpyAsync(w, d.offsetsU, 4 * size_t(d.nu + 1), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.offsetsU, 4 * size_t(d.nu + 1)));
       bg.offU = w; w += d.nu + 1;
-      PBF_HIP(hipMemcpyAsync(w, d.edgesU, 4 * size_t(d.m), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, w, d.edgesU, 4 * size_t(d.m)));
       bg.edgU = w; w += d.m;
     }
   }

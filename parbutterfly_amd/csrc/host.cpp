@@ -14,6 +14,8 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <omp.h>
+
 #include "parbutterfly.h"
 #include "pbf_internal.h"
 
@@ -108,14 +110,30 @@ extern "C" int pbf_graph_from_edges(int64_t nv, int64_t nu, int64_t ne, const ui
 // (butterfly_utils.h:1013-1051); the algorithms additionally rely on sorted, duplicate-free
 // lists (every "else break" in the wedge loops) and on the two CSRs describing the same edges.
 // ------------------------------------------------------------------------------------------
+// Lowest index x in [0, n) with bad(x), or n; checked in parallel, so the error reported is the
+// one a serial scan would meet first.
+template <class F>
+static int64_t first_bad(int64_t n, F bad) {
+  int64_t first = n;
+#pragma omp parallel for schedule(static, 4096) reduction(min : first)
+  for (int64_t x = 0; x < n; x++)
+    if (x < first && bad(x)) first = x;
+  return first;
+}
+
 static int validate_side(int64_t n, int64_t other, int64_t m, const uint32_t* off, const uint32_t* edg) {
   if (off[0] != 0 || (int64_t)off[n] != m) return PBF_ERR_FORMAT;
-  for (int64_t x = 0; x < n; x++) {
-    if (off[x + 1] < off[x]) return PBF_ERR_UNSORTED;
-    for (uint32_t p = off[x]; p < off[x + 1]; p++) {
-      if ((int64_t)edg[p] >= other) return PBF_ERR_RANGE;
-      if (p > off[x] && edg[p] <= edg[p - 1]) return PBF_ERR_UNSORTED;
-    }
+  // offsets first, so that the list scan below reads only inside [0, m)
+  if (first_bad(n, [&](int64_t x) { return off[x + 1] < off[x]; }) < n) return PBF_ERR_UNSORTED;
+  int64_t x = first_bad(n, [&](int64_t x) {
+    for (uint32_t p = off[x]; p < off[x + 1]; p++)
+      if ((int64_t)edg[p] >= other || (p > off[x] && edg[p] <= edg[p - 1])) return true;
+    return false;
+  });
+  if (x == n) return PBF_OK;
+  for (uint32_t p = off[x]; p < off[x + 1]; p++) {
+    if ((int64_t)edg[p] >= other) return PBF_ERR_RANGE;
+    if (p > off[x] && edg[p] <= edg[p - 1]) return PBF_ERR_UNSORTED;
   }
   return PBF_OK;
 }
@@ -129,16 +147,33 @@ extern "C" int pbf_validate(const pbf_bipartite_csr* g) {
   if (st) return st;
   st = validate_side(g->nu, g->nv, g->m, g->offsetsU, g->edgesU);
   if (st) return st;
-  // The U lists must be the transpose of the V lists: walking V lists in order visits each
-  // U list's entries in ascending V order.
-  std::vector<uint32_t> cur(g->offsetsU, g->offsetsU + g->nu);
-  for (int64_t v = 0; v < g->nv; v++)
-    for (uint32_t p = g->offsetsV[v]; p < g->offsetsV[v + 1]; p++) {
-      uint32_t u = g->edgesV[p];
-      if (cur[u] >= g->offsetsU[u + 1] || g->edgesU[cur[u]] != (uint32_t)v) return PBF_ERR_FORMAT;
-      cur[u]++;
-    }
-  return PBF_OK;
+  // The U lists must be the transpose of the V lists: walking the V lists in order visits each
+  // U list's entries in ascending V order. Each thread owns a range of U ids (cut at edge
+  // boundaries) and walks all V lists, following only the entries in its range.
+  const uint32_t *offU = g->offsetsU, *edgU = g->edgesU, *offV = g->offsetsV, *edgV = g->edgesV;
+  int ok = 1;
+#pragma omp parallel reduction(& : ok)
+  {
+    int nt = omp_get_num_threads(), t = omp_get_thread_num();
+    int64_t m = g->m;
+    // thread t owns the U ids whose lists start in edge range [m t / nt, m (t + 1) / nt)
+    auto cut = [&](int k) -> uint32_t {
+      if (k == 0) return 0;
+      if (k == nt) return (uint32_t)g->nu;
+      return (uint32_t)(std::lower_bound(offU, offU + g->nu, (uint32_t)(m * k / nt)) - offU);
+    };
+    uint32_t lo = cut(t), hi = cut(t + 1);
+    std::vector<uint32_t> cur(offU + lo, offU + hi);
+    for (int64_t v = 0; v < g->nv && ok; v++)
+      for (uint32_t p = offV[v]; p < offV[v + 1]; p++) {
+        uint32_t u = edgV[p];
+        if (u < lo || u >= hi) continue;
+        uint32_t& c = cur[u - lo];
+        if (c >= offU[u + 1] || edgU[c] != (uint32_t)v) { ok = 0; break; }
+        c++;
+      }
+  }
+  return ok ? PBF_OK : PBF_ERR_FORMAT;
 }
 
 extern "C" int pbf_side_host(const pbf_bipartite_csr* g, int* use_v, int64_t* num_wedges) {

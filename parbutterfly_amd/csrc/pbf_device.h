@@ -78,6 +78,11 @@ struct pbf_context {
   size_t pool_clean = 0;   // bytes of S_POOL known to be zero (the HBM kernel leaves them zero)
   CountTune tune;
   void* batch = nullptr;   // batch state (batch.hip)
+  // pinned staging for host transfers (pbf_h2d / pbf_d2h): two buffers in turn, so the host
+  // copy of one chunk overlaps the DMA of the other
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_busy[2] = {false, false};
 };
 
 // Returns a device buffer of at least `bytes` for the slot (grow-only; contents undefined).
@@ -120,6 +125,12 @@ struct RankedGraph {
   // the first rank of r's block and entries hold ranks relative to it.  Null otherwise.
   uint32_t* lo = nullptr;
 };
+
+// api.hip: host <-> device copies on the context's stream through pinned staging (large
+// transfers run at DMA speed instead of the pageable path's).  pbf_h2d returns once the host
+// source has been read; pbf_d2h returns once the host destination is written.
+int pbf_h2d(pbf_context* ctx, void* dst, const void* src, size_t bytes);
+int pbf_d2h(pbf_context* ctx, void* dst, const void* src, size_t bytes);
 
 // scan.hip
 int pbf_scan_excl_u32(pbf_context* ctx, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev);

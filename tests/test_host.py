@@ -129,3 +129,40 @@ def test_side_host_matches_definition():
     dv, du = np.diff(g.offsetsV).astype(np.int64), np.diff(g.offsetsU).astype(np.int64)
     wv, wu = int((dv * (dv - 1) // 2).sum()), int((du * (du - 1) // 2).sum())
     assert g.side() == (wv <= wu, min(wv, wu))
+
+
+def test_validate_error_codes():
+    """pbf_validate on well-formed and broken CSR pairs; each broken case holds one fault, placed
+    at several positions so that every thread's share of the parallel checks meets one."""
+    g = pb.Graph.uniform(3000, 2000, 40000, 11)
+    parts = lambda: [g.offsetsV.copy(), g.edgesV.copy(), g.offsetsU.copy(), g.edgesU.copy()]
+    assert g.validate() == 0
+    for pos in (0, 7919, 20000, 39999):
+        # a V entry replaced by a neighbour absent from the list: the U lists are no transpose
+        oV, eV, oU, eU = parts()
+        v = int(np.searchsorted(oV, pos, side="right") - 1)
+        lst = set(eV[oV[v]:oV[v + 1]].tolist())
+        lo = int(eV[pos - 1]) + 1 if pos > oV[v] else 0
+        hi = int(eV[pos + 1]) if pos + 1 < oV[v + 1] else g.nu
+        free = [u for u in range(lo, hi) if u not in lst]
+        if free:
+            eV[pos] = free[0]
+            assert pb.Graph(g.nv, g.nu, oV, eV, oU, eU).validate() == 2, pos  # PBF_ERR_FORMAT
+        # out-of-range neighbour
+        oV, eV, oU, eU = parts()
+        eU[pos % g.m] = g.nv
+        assert pb.Graph(g.nv, g.nu, oV, eV, oU, eU).validate() == 3, pos  # PBF_ERR_RANGE
+        # two neighbours of one list swapped
+        oV, eV, oU, eU = parts()
+        x = int(np.searchsorted(oU, pos, side="right") - 1)
+        if oU[x + 1] - oU[x] >= 2:
+            a = int(oU[x])
+            eU[a], eU[a + 1] = eU[a + 1], eU[a]
+            assert pb.Graph(g.nv, g.nu, oV, eV, oU, eU).validate() == 4, pos  # PBF_ERR_UNSORTED
+    # decreasing offsets, and an offset array that does not end at m
+    oV, eV, oU, eU = parts()
+    oV[1500] = oV[1501] + 1
+    assert pb.Graph(g.nv, g.nu, oV, eV, oU, eU).validate() == 4
+    oV, eV, oU, eU = parts()
+    oU[-1] -= 1
+    assert pb.Graph(g.nv, g.nu, oV, eV, oU, eU).validate() == 2
