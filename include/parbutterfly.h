@@ -85,6 +85,7 @@ typedef struct {
   int64_t wedges;     /* W = rank-filtered wedges processed (rankWedgeF) */
   int64_t rounds;     /* peeling rounds */
   int64_t hub_vertices;   /* endpoints that took the HBM-histogram path */
+  double sparsify_ms;     /* sparsification (pbf_sparsify / pbf_count_total_sparse) */
 } pbf_timing;
 
 const char* pbf_status_str(int status);
@@ -183,6 +184,30 @@ int pbf_peel_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use
 /* PeelE (butterfly_epeel.h:519): wing numbers from per-edge counts (m entries, O-CSR order). */
 int pbf_peel_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
                   int peel_type, const uint64_t* counts, uint64_t* wing);
+
+/* ---------------- approximate totals by sparsification ----------------
+ * SparseType (butterfly_utils.h:28) with the driver's numeric -s values
+ * (BipartiteButterflies.C:186-191). */
+enum { PBF_SPARSE_NONE = 0, PBF_SPARSE_COLOR = 1, PBF_SPARSE_EDGE = 2 };
+/* eSparseBipartite (butterfly_utils.h:796-898) for PBF_SPARSE_EDGE: edge (v, u) stays when its
+ * hashed colour among denom colours is 0.  clrSparseBipartite (:909-996) for PBF_SPARSE_COLOR:
+ * every vertex gets a hashed colour and the edges whose ends share a colour stay.  Both then
+ * drop the vertices left without edges and renumber each side in id order (delZeroDeg,
+ * :744-785).  1 <= denom < 2^32, 0 <= seed < 2^56 (else PBF_ERR_ARG).  The output arrays are
+ * the caller's, in `mem`, with room for the input's sizes (nv + 1, m, nu + 1, m); the sizes of
+ * the sparsified graph go to host *nv_out, *nu_out, *m_out.  Synchronous. */
+int pbf_sparsify(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int sparse_type, int64_t denom,
+                 int64_t seed, int64_t* nv_out, int64_t* nu_out, int64_t* m_out, uint32_t* offsetsV,
+                 uint32_t* edgesV, uint32_t* offsetsU, uint32_t* edgesU);
+/* CountTotal on the sparsified graph (its side chosen by cmpWedgeCounts on that graph), scaled
+ * as the driver scales it (BipartiteButterflies.C:213-214): *approx = total * denom^3 for
+ * PBF_SPARSE_COLOR, total * denom^4 for PBF_SPARSE_EDGE, the product taken in double precision
+ * and truncated to int64 as there (PBF_ERR_DOMAIN when it reaches 2^63).  *sparse_total (may
+ * be NULL) = the exact count of the sparsified graph.  PBF_SPARSE_NONE counts the graph
+ * itself.  Results are host values; synchronous. */
+int pbf_count_total_sparse(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int sparse_type,
+                           int64_t denom, int64_t seed, int count_type, int rank_type, int64_t* approx,
+                           uint64_t* sparse_total);
 
 /* ---------------- batches of graphs (no counterpart in the reference) ----------------
  * Per-vertex (VERT) and total butterfly counts of `count` independent graphs, each exactly as

@@ -596,3 +596,115 @@ void orc_brute_peel_edge(const orc_graph* g, int use_v, uint64_t* wing) {
   }
   free(eti); free(owner_o); free(alive); free(B);
 }
+
+/* ---------------- sparsification (approximate TOTAL) ---------------- */
+
+/* hashInt(ulong) (ligra/utils.h:389-397).  Which edges a seed keeps is defined by this mix, so
+ * its steps and constants are the reference's. */
+static uint64_t ref_hash64(uint64_t a) {
+  a = (a + 0x7ed55d166bef7a1dull) + (a << 12);
+  a = (a ^ 0xc761c23c510fa2ddull) ^ (a >> 9);
+  a = (a + 0x165667b183a9c0e1ull) + (a << 59);
+  a = (a + 0xd3a2646cab3487e3ull) ^ (a << 49);
+  a = (a + 0xfd7046c5ef9ab54cull) + (a << 3);
+  a = (a ^ 0xb55a4f090dd4a67bull) ^ (a >> 32);
+  return a;
+}
+uint64_t orc_hash64(uint64_t a) { return ref_hash64(a); }
+
+/* delZeroDeg (butterfly_utils.h:744-785) applied to the kept entries: vertices with a kept
+ * entry are renumbered in id order on each side; lists keep their order. */
+static void del_zero_deg(const orc_graph* g, const uint8_t* keepV, const uint8_t* keepU, int64_t* nv_out,
+                         int64_t* nu_out, int64_t* m_out, uint32_t* offV, uint32_t* edgV, uint32_t* offU,
+                         uint32_t* edgU) {
+  uint32_t* idV = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(g->nv + 1));
+  uint32_t* idU = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(g->nu + 1));
+  int64_t nv2 = 0, nu2 = 0;
+  for (int64_t v = 0; v < g->nv; v++) {
+    int any = 0;
+    for (uint32_t p = g->offV[v]; p < g->offV[v + 1]; p++) any |= keepV[p];
+    idV[v] = any ? (uint32_t)nv2++ : 0xFFFFFFFFu;
+  }
+  for (int64_t u = 0; u < g->nu; u++) {
+    int any = 0;
+    for (uint32_t p = g->offU[u]; p < g->offU[u + 1]; p++) any |= keepU[p];
+    idU[u] = any ? (uint32_t)nu2++ : 0xFFFFFFFFu;
+  }
+  int64_t mv = 0, mu = 0;
+  for (int64_t v = 0; v < g->nv; v++) {
+    if (idV[v] == 0xFFFFFFFFu) continue;
+    offV[idV[v]] = (uint32_t)mv;
+    for (uint32_t p = g->offV[v]; p < g->offV[v + 1]; p++)
+      if (keepV[p]) edgV[mv++] = idU[g->edgV[p]];
+  }
+  offV[nv2] = (uint32_t)mv;
+  for (int64_t u = 0; u < g->nu; u++) {
+    if (idU[u] == 0xFFFFFFFFu) continue;
+    offU[idU[u]] = (uint32_t)mu;
+    for (uint32_t p = g->offU[u]; p < g->offU[u + 1]; p++)
+      if (keepU[p]) edgU[mu++] = idV[g->edgU[p]];
+  }
+  offU[nu2] = (uint32_t)mu;
+  *nv_out = nv2;
+  *nu_out = nu2;
+  *m_out = mv == mu ? mv : -1;
+  free(idV);
+  free(idU);
+}
+
+/* eSparseBipartite (butterfly_utils.h:796-898), type 2, and clrSparseBipartite (:909-996),
+ * type 1, each followed by delZeroDeg.  numColors = max(1, (uint32) denom).
+ *   ESPARSE: slot (i, u) of V's CSR gets colour hashInt(seed + concat) % numColors with
+ *     concat = (i + 0x9e3779b9 + (seed << 6) + (seed >> 2))
+ *              ^ ((uint32)(u + 0x9e3779b9) + (seed << 6) + (seed >> 2))   (:802-810);
+ *     each U slot (i, v) copies the colour of the V slot holding i, found by binary search in
+ *     N(v) (:813-825); entries of colour 0 stay.
+ *   CLRSPARSE: colour(x) = hashInt(seed + x) % numColors per side (:911-914); an entry stays
+ *     when its two ends have the same colour (:921-937).
+ * Output arrays as large as the input's; returns 0, or -1 on a bad type. */
+int orc_sparsify(const orc_graph* g, int type, int64_t denom, int64_t seed, int64_t* nv_out, int64_t* nu_out,
+                 int64_t* m_out, uint32_t* offV, uint32_t* edgV, uint32_t* offU, uint32_t* edgU) {
+  if (type != 1 && type != 2) return -1;
+  const uint64_t ncol = (uint32_t)denom > 1u ? (uint32_t)denom : 1u;
+  const uint64_t s = (uint64_t)seed, salt = (s << 6) + (s >> 2);
+  size_t mm = (size_t)(g->m ? g->m : 1);
+  uint8_t* keepV = (uint8_t*)malloc(mm);
+  uint8_t* keepU = (uint8_t*)malloc(mm);
+  if (type == 2) {
+    uint32_t* colV = (uint32_t*)malloc(sizeof(uint32_t) * mm);
+    for (int64_t i = 0; i < g->nv; i++)
+      for (uint32_t p = g->offV[i]; p < g->offV[i + 1]; p++) {
+        uint32_t u = g->edgV[p];
+        uint64_t concat = (uint64_t)i + 0x9e3779b9ull + salt;
+        concat ^= (uint64_t)(uint32_t)(u + 0x9e3779b9u) + salt;
+        colV[p] = (uint32_t)(ref_hash64(s + concat) % ncol);
+        keepV[p] = colV[p] == 0;
+      }
+    for (int64_t i = 0; i < g->nu; i++)
+      for (uint32_t q = g->offU[i]; q < g->offU[i + 1]; q++) {
+        uint32_t v = g->edgU[q];
+        uint32_t lo = g->offV[v], hi = g->offV[v + 1];
+        while (lo < hi) {  /* first position in N(v) holding a value >= i */
+          uint32_t mid = lo + (hi - lo) / 2;
+          if (g->edgV[mid] < (uint32_t)i) lo = mid + 1; else hi = mid;
+        }
+        keepU[q] = colV[lo] == 0;
+      }
+    free(colV);
+  } else {
+    for (int64_t v = 0; v < g->nv; v++) {
+      uint64_t cv = ref_hash64(s + (uint64_t)v) % ncol;
+      for (uint32_t p = g->offV[v]; p < g->offV[v + 1]; p++)
+        keepV[p] = ref_hash64(s + g->edgV[p]) % ncol == cv;
+    }
+    for (int64_t u = 0; u < g->nu; u++) {
+      uint64_t cu = ref_hash64(s + (uint64_t)u) % ncol;
+      for (uint32_t p = g->offU[u]; p < g->offU[u + 1]; p++)
+        keepU[p] = ref_hash64(s + g->edgU[p]) % ncol == cu;
+    }
+  }
+  del_zero_deg(g, keepV, keepU, nv_out, nu_out, m_out, offV, edgV, offU, edgU);
+  free(keepV);
+  free(keepU);
+  return 0;
+}

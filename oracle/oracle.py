@@ -57,6 +57,9 @@ def lib():
             "orc_brute_edge": (None, [PG, INT, P]),
             "orc_brute_peel_vert": (None, [PG, INT, P]),
             "orc_brute_peel_edge": (None, [PG, INT, P]),
+            "orc_sparsify": (INT, [PG, INT, I64, I64, ctypes.POINTER(I64), ctypes.POINTER(I64),
+                                   ctypes.POINTER(I64), P, P, P, P]),
+            "orc_hash64": (ctypes.c_uint64, [ctypes.c_uint64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -193,6 +196,47 @@ def brute_peel_edge(graph, use_v):
     out = np.zeros(max(graph.m, 1), np.uint64)
     lib().orc_brute_peel_edge(ctypes.byref(g), int(use_v), out.ctypes.data)
     return out[:graph.m]
+
+
+class CSR:
+    """A plain graph record (nv, nu, m and the four uint32 arrays)."""
+
+    def __init__(self, nv, nu, offsetsV, edgesV, offsetsU, edgesU):
+        self.nv, self.nu = int(nv), int(nu)
+        self.offsetsV, self.edgesV = np.asarray(offsetsV, np.uint32), np.asarray(edgesV, np.uint32)
+        self.offsetsU, self.edgesU = np.asarray(offsetsU, np.uint32), np.asarray(edgesU, np.uint32)
+        self.m = int(self.edgesV.shape[0])
+
+
+SPARSE = {"COLOR": 1, "EDGE": 2}
+
+
+def sparsify(graph, sparse_type="EDGE", denom=25, seed=0):
+    """eSparseBipartite / clrSparseBipartite + delZeroDeg; returns a CSR."""
+    g = _g(graph)
+    m = max(graph.m, 1)
+    oV, eV = np.zeros(graph.nv + 1, np.uint32), np.zeros(m, np.uint32)
+    oU, eU = np.zeros(graph.nu + 1, np.uint32), np.zeros(m, np.uint32)
+    nv, nu, mo = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    assert lib().orc_sparsify(ctypes.byref(g), SPARSE[sparse_type], int(denom), int(seed), ctypes.byref(nv),
+                              ctypes.byref(nu), ctypes.byref(mo), oV.ctypes.data, eV.ctypes.data, oU.ctypes.data,
+                              eU.ctypes.data) == 0
+    assert mo.value >= 0, "V and U sides kept different edge counts"
+    return CSR(nv.value, nu.value, oV[:nv.value + 1], eV[:mo.value], oU[:nu.value + 1], eU[:mo.value])
+
+
+def hash64(a):
+    """hashInt(ulong) of ligra/utils.h:389-397."""
+    return int(lib().orc_hash64(int(a) & (2 ** 64 - 1)))
+
+
+def approx_total(graph, sparse_type="EDGE", denom=25, seed=0, rank_type="ADEG"):
+    """(scaled total, exact total of the sparsified graph), scaled as the driver does
+    (BipartiteButterflies.C:213-214): the long total times pow(denom, 3 or 4), truncated."""
+    h = sparsify(graph, sparse_type, denom, seed)
+    t = count_total(h, side(h)[0], rank_type) if h.m else 0
+    k = 3 if sparse_type == "COLOR" else 4
+    return int(float(t) * float(denom) ** k), t
 
 
 # ---------------- compiled reference (oracle/_ref) ----------------

@@ -17,7 +17,9 @@
 // Usage:
 //   ref_harness <op> <graph> <out.bin> [-countType T] [-rankType R] [-peelType P]
 //               [-rounds N] [-a max_array_size] [-m max_wedges]
-//   op    : side | rank | vert | edge | total | peelv | peele  (rank: counts = order, peel = {W})
+//   op    : side | rank | vert | edge | total | peelv | peele | sparse
+//           (rank: counts = order, peel = {W}; sparse: see below, options -sparseType EDGE|COLOR,
+//            -d denom, -seed seed)
 //   graph : AdjacencyHypergraph text file (readBipartite) or a PBFCSR1 binary file
 // Output file layout (little endian):
 //   int64 use_v, int64 side_wedges, int64 n_counts, int64 counts[n_counts],
@@ -118,6 +120,9 @@ int main(int argc, char** argv) {
   long rounds = 1;
   long max_array_size = 23090996160L;  // Compute's default -a (BipartiteButterflies.C:175)
   long max_wedges = 2577500000L;       // Compute's default -m (BipartiteButterflies.C:174)
+  std::string sparse_type = "EDGE";   // op sparse: EDGE (eSparseBipartite) or COLOR
+  long denom = 25;                     // Compute's default -d (BipartiteButterflies.C:176)
+  long seed = 0;
   for (int i = 4; i + 1 < argc; i += 2) {
     std::string k = argv[i], v = argv[i + 1];
     if (k == "-countType") ct = v;
@@ -126,6 +131,9 @@ int main(int argc, char** argv) {
     else if (k == "-rounds") rounds = atol(v.c_str());
     else if (k == "-a") max_array_size = atol(v.c_str());
     else if (k == "-m") max_wedges = atol(v.c_str());
+    else if (k == "-sparseType") sparse_type = v;
+    else if (k == "-d") denom = atol(v.c_str());
+    else if (k == "-seed") seed = atol(v.c_str());
   }
   CountType ty = parse_count(ct);
   RankType tw = parse_rank(rt);
@@ -203,6 +211,26 @@ int main(int argc, char** argv) {
       }
       free(eti);
       free(b);
+    } else if (op == "sparse") {
+      // eSparseBipartite / clrSparseBipartite (butterfly_utils.h:796-996, with delZeroDeg
+      // :744-785), then CountTotal of the sparsified graph and the CLI's scaling
+      // (BipartiteButterflies.C:213-214).  counts = {nv, nu, m, offsetsV[nv+1], edgesV[m],
+      // offsetsU[nu+1], edgesU[m]} of the sparsified graph; peel = {total, scaled total}.
+      double t0 = now_s();
+      bipartiteCSR H = sparse_type == "COLOR" ? clrSparseBipartite(G, denom, seed) : eSparseBipartite(G, denom, seed);
+      secs.push_back(now_s() - t0);
+      counts.push_back(H.nv); counts.push_back(H.nu); counts.push_back(H.numEdges);
+      for (long i = 0; i <= H.nv; i++) counts.push_back(H.offsetsV[i]);
+      for (long i = 0; i < H.numEdges; i++) counts.push_back(H.edgesV[i]);
+      for (long i = 0; i <= H.nu; i++) counts.push_back(H.offsetsU[i]);
+      for (long i = 0; i < H.numEdges; i++) counts.push_back(H.edgesU[i]);
+      tuple<bool, long> huv = cmpWedgeCounts(H);
+      long nb = H.numEdges ? CountTotal(H, get<0>(huv), get<1>(huv), max_wedges, max_array_size, ty, tw) : 0;
+      peel.push_back(nb);
+      if (sparse_type == "COLOR") nb *= pow(denom, 3);
+      else nb *= pow(denom, 4);
+      peel.push_back(nb);
+      H.del();
     } else {
       fprintf(stderr, "unknown op %s\n", op.c_str());
       return 2;

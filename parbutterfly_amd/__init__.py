@@ -41,6 +41,7 @@ COUNT_TYPES = {"SORT": 0, "ASORT": 1, "HASH": 2, "AHASH": 3, "HIST": 4, "AHIST":
 RANK_TYPES = {"SIDE": 0, "COCORE": 1, "ACOCORE": 2, "DEG": 3, "ADEG": 4}
 PEEL_TYPES = {"SORT": 0, "HASH": 1, "HIST": 2, "BATCHS": 3, "BATCHWA": 4, "GPU": 5}
 PER_TYPES = {"VERT": 0, "EDGE": 1, "TOTAL": 2}
+SPARSE_TYPES = {"NONE": 0, "COLOR": 1, "EDGE": 2}  # SparseType, butterfly_utils.h:28
 MEM_HOST, MEM_DEVICE = 0, 1
 
 STATUS = {0: "PBF_OK", 1: "PBF_ERR_ARG", 2: "PBF_ERR_FORMAT", 3: "PBF_ERR_RANGE", 4: "PBF_ERR_UNSORTED",
@@ -69,7 +70,8 @@ class _Graph(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("rank_ms", ctypes.c_double), ("relabel_ms", ctypes.c_double), ("count_ms", ctypes.c_double),
                 ("convert_ms", ctypes.c_double), ("peel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("wedges", ctypes.c_int64), ("rounds", ctypes.c_int64), ("hub_vertices", ctypes.c_int64)]
+                ("wedges", ctypes.c_int64), ("rounds", ctypes.c_int64), ("hub_vertices", ctypes.c_int64),
+                ("sparsify_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -120,6 +122,9 @@ _SIGS = {
     "pbf_edge_to_idxs": (_INT, [_P, _PCSR, _INT, _INT, _P]),
     "pbf_peel_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
     "pbf_peel_edge": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
+    "pbf_sparsify": (_INT, [_P, _PCSR, _INT, _INT, _I64, _I64, _PI64, _PI64, _PI64, _P, _P, _P, _P]),
+    "pbf_count_total_sparse": (_INT, [_P, _PCSR, _INT, _INT, _I64, _I64, _INT, _INT, _PI64,
+                                      ctypes.POINTER(ctypes.c_uint64)]),
     "pbf_count_batch": (_INT, [_P, _I64, _PCSR, _INT, _INT, _INT, _P, _I64, _P, _P, _INT]),
     "pbf_batch_release": (None, [_P]),
     "pbf_batch_autotune": (_INT, [_P, _I64, _PCSR, _INT, _INT, _P, _I64, _P, _P, _INT, _PI,
@@ -334,6 +339,15 @@ class DeviceGraph:
     def n(self):
         return self.nv + self.nu
 
+    @classmethod
+    def _from_arrays(cls, ctx, nv, nu, m, offsetsV, edgesV, offsetsU, edgesU):
+        """Wraps device arrays (possibly longer than nv + 1 / m / nu + 1 / m) as a graph."""
+        self = cls.__new__(cls)
+        self.nv, self.nu, self.m, self.ctx = int(nv), int(nu), int(m), ctx
+        self.offsetsV, self.edgesV, self.offsetsU, self.edgesU = offsetsV, edgesV, offsetsU, edgesU
+        self._csr = _CSR(self.nv, self.nu, self.m, offsetsV.ptr, edgesV.ptr, offsetsU.ptr, edgesU.ptr)
+        return self
+
 
 class Context:
     """A GPU context (device, stream, cached scratch).  One per host thread."""
@@ -454,6 +468,31 @@ class Context:
                                   _enum(PEEL_TYPES, peel_type, "peel type"), self._ptr(counts) if g.m else 1,
                                   self._ptr(wing) if g.m else 1), "pbf_peel_edge")
         return wing
+
+    # ---- approximate totals by sparsification ----
+    def sparsify(self, g, sparse_type="EDGE", denom=25, seed=0):
+        """eSparseBipartite / clrSparseBipartite + delZeroDeg: a Graph for a host graph, a
+        DeviceGraph for a device graph."""
+        st = _enum(SPARSE_TYPES, sparse_type, "sparse type")
+        bufs = [self._alloc_out(g, n, np.uint32) for n in (g.nv + 1, max(g.m, 1), g.nu + 1, max(g.m, 1))]
+        nv, nu, m = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib.pbf_sparsify(self._ctx, ctypes.byref(g._csr), self._mem(g), st, int(denom), int(seed),
+                                 ctypes.byref(nv), ctypes.byref(nu), ctypes.byref(m), *[self._ptr(b) for b in bufs]),
+               "pbf_sparsify")
+        nv, nu, m = nv.value, nu.value, m.value
+        if isinstance(g, DeviceGraph):
+            return DeviceGraph._from_arrays(self, nv, nu, m, *bufs)
+        return Graph(nv, nu, bufs[0][:nv + 1], bufs[1][:m], bufs[2][:nu + 1], bufs[3][:m])
+
+    def count_total_sparse(self, g, sparse_type="EDGE", denom=25, seed=0, count_type="GPU", rank_type="ADEG"):
+        """CountTotal of the sparsified graph with the driver's scaling: (approx, sparse_total)."""
+        approx, tot = ctypes.c_int64(), ctypes.c_uint64()
+        _check(_lib.pbf_count_total_sparse(self._ctx, ctypes.byref(g._csr), self._mem(g),
+                                           _enum(SPARSE_TYPES, sparse_type, "sparse type"), int(denom), int(seed),
+                                           _enum(COUNT_TYPES, count_type, "count type"),
+                                           _enum(RANK_TYPES, rank_type, "rank type"), ctypes.byref(approx),
+                                           ctypes.byref(tot)), "pbf_count_total_sparse")
+        return approx.value, tot.value
 
     # ---- batches (no counterpart in the reference) ----
     BATCH_UNION, BATCH_PLAIN, BATCH_CAPTURE = 0, 1, 16

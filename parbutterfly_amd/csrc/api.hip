@@ -443,3 +443,94 @@ extern "C" int pbf_peel_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int m
                              const uint64_t* counts, uint64_t* wing) {
   return peel_common(ctx, g, mem, use_v, peel_type, counts, wing, true);
 }
+
+extern "C" int pbf_sparsify(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int sparse_type, int64_t denom,
+                            int64_t seed, int64_t* nv_out, int64_t* nu_out, int64_t* m_out, uint32_t* offsetsV,
+                            uint32_t* edgesV, uint32_t* offsetsU, uint32_t* edgesU) {
+  if (!ctx || !nv_out || !nu_out || !m_out || !offsetsV || !offsetsU) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  DevGraph dg, sg;
+  PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  if (dg.m > 0 && (!edgesV || !edgesU)) return PBF_ERR_ARG;
+  uint32_t *oV = offsetsV, *eV = edgesV, *oU = offsetsU, *eU = edgesU;
+  if (mem == PBF_MEM_HOST) {
+    void* p;
+    PBF_CHECK(pbf_scratch(ctx, S_SPARSE1, 4 * (size_t)(dg.nv + 1 + dg.m + dg.nu + 1 + dg.m), &p));
+    oV = static_cast<uint32_t*>(p);
+    eV = oV + dg.nv + 1;
+    oU = eV + dg.m;
+    eU = oU + dg.nu + 1;
+  }
+  memset(&ctx->timing, 0, sizeof(ctx->timing));
+  PBF_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  PBF_CHECK(pbf_dev_sparsify(ctx, dg, sparse_type, denom, seed, &sg, oV, eV, oU, eU));
+  PBF_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  if (mem == PBF_MEM_HOST) {
+    PBF_CHECK(pbf_d2h(ctx, offsetsV, oV, 4 * (size_t)(sg.nv + 1)));
+    PBF_CHECK(pbf_d2h(ctx, edgesV, eV, 4 * (size_t)sg.m));
+    PBF_CHECK(pbf_d2h(ctx, offsetsU, oU, 4 * (size_t)(sg.nu + 1)));
+    PBF_CHECK(pbf_d2h(ctx, edgesU, eU, 4 * (size_t)sg.m));
+  }
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->timing.sparsify_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+  ctx->timing.total_ms = ctx->timing.sparsify_ms;
+  *nv_out = sg.nv;
+  *nu_out = sg.nu;
+  *m_out = sg.m;
+  return PBF_OK;
+}
+
+extern "C" int pbf_count_total_sparse(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int sparse_type,
+                                      int64_t denom, int64_t seed, int count_type, int rank_type, int64_t* approx,
+                                      uint64_t* sparse_total) {
+  if (!ctx || !approx) return PBF_ERR_ARG;
+  if (sparse_type < PBF_SPARSE_NONE || sparse_type > PBF_SPARSE_EDGE) return PBF_ERR_ARG;
+  if (!count_supported(count_type)) return PBF_ERR_ARG;
+  if (!rank_supported(rank_type)) return PBF_ERR_UNSUPPORTED;
+  DeviceGuard guard(ctx->device);
+  DevGraph dg, sg;
+  PBF_CHECK(resolve_graph(ctx, g, mem, &dg));
+  memset(&ctx->timing, 0, sizeof(ctx->timing));
+  hipEvent_t* ev = ctx->ev;
+  PBF_HIP(hipEventRecord(ev[0], ctx->stream));
+  if (sparse_type == PBF_SPARSE_NONE) {
+    sg = dg;
+  } else {
+    void* p;
+    PBF_CHECK(pbf_scratch(ctx, S_SPARSE1, 4 * (size_t)(dg.nv + 1 + dg.m + dg.nu + 1 + dg.m), &p));
+    uint32_t* oV = static_cast<uint32_t*>(p);
+    PBF_CHECK(pbf_dev_sparsify(ctx, dg, sparse_type, denom, seed, &sg, oV, oV + dg.nv + 1, oV + dg.nv + 1 + dg.m,
+                               oV + dg.nv + 1 + dg.m + dg.nu + 1));
+  }
+  PBF_HIP(hipEventRecord(ev[1], ctx->stream));
+  uint64_t total = 0;
+  int64_t wedges = 0;
+  RankedGraph R{};
+  if (sg.m > 0) {
+    int use_v = 0;
+    PBF_CHECK(pbf_dev_side(ctx, sg, &use_v, &wedges));
+    PBF_HIP(hipEventRecord(ev[2], ctx->stream));
+    PBF_CHECK(pbf_dev_rank(ctx, sg, rank_type, &R, use_v));
+    PBF_HIP(hipEventRecord(ev[3], ctx->stream));
+    PBF_CHECK(pbf_dev_relabel(ctx, sg, use_v, false, &R));
+    PBF_HIP(hipEventRecord(ev[4], ctx->stream));
+    void* out_dev;
+    PBF_CHECK(pbf_scratch(ctx, S_OUT, 8, &out_dev));
+    PBF_CHECK(pbf_dev_count(ctx, sg, use_v, R, MODE_TOTAL, static_cast<uint64_t*>(out_dev), nullptr));
+    PBF_HIP(hipEventRecord(ev[5], ctx->stream));
+    PBF_HIP(hipMemcpyAsync(ctx->pinned + 50, out_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    total = ctx->pinned[50];
+    ctx->timing.rank_ms = elapsed(ev[2], ev[3]);
+    ctx->timing.relabel_ms = elapsed(ev[3], ev[4]);
+    ctx->timing.count_ms = elapsed(ev[4], ev[5]);
+    ctx->timing.total_ms = elapsed(ev[0], ev[5]);
+    ctx->timing.wedges = R.wedges;
+  } else {
+    PBF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->timing.total_ms = elapsed(ev[0], ev[1]);
+  }
+  ctx->timing.sparsify_ms = sparse_type == PBF_SPARSE_NONE ? 0.0 : elapsed(ev[0], ev[1]);
+  if (sparse_total) *sparse_total = total;
+  return pbf_sparse_scale(total, sparse_type, denom, approx);
+}

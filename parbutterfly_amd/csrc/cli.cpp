@@ -4,15 +4,18 @@
 //
 //   bipartite_butterflies [-countType T | -t N] [-rankType R | -w N] [-peelType P | -tp N]
 //                         [-per VERT|EDGE|TOTAL | -e N | -total] [-r rounds] [-m max_wedges]
-//                         [-a max_array_size] [-nopeel] [-device D] <inFile>
+//                         [-a max_array_size] [-nopeel] [-sparseType EDGE|COLOR | -s N] [-d denom]
+//                         [-device D] <inFile>
 //
 // Count types are the reference's (ASORT ... SERIAL, numeric aliases of :96-107) plus GPU;
 // they are aggregation strategies with identical results, and all of them run the GPU kernels
 // here.  Rank types ADEG, ACOCORE and SIDE are supported on the GPU; DEG / COCORE return
-// PBF_ERR_UNSUPPORTED.  Sparsification (-sparseType) is not
-// part of this build and is rejected.  The graph is re-read every round, as the reference does
-// (:331-341).
+// PBF_ERR_UNSUPPORTED.  The graph is re-read every round, as the reference does (:331-341);
+// with -sparseType it is sparsified (seed 0) on the GPU after each read (:332-337), the count
+// becomes TOTAL and the printed total is scaled by denom^3 (COLOR) or denom^4 (EDGE)
+// (:193, :213-214).
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -67,6 +70,25 @@ const char* peel_label(int t) {
     case PBF_PEEL_GPU: return "GPU Peel";
   }
   return "";
+}
+
+// SparseType from -s / -sparseType (:177-192).
+int sparse_type(const Args& P) {
+  const long s = P.num("-s", NONE);
+  if (s == NONE) {
+    const std::string t = P.str("-sparseType", "");
+    return t == "EDGE" ? PBF_SPARSE_EDGE : t == "COLOR" ? PBF_SPARSE_COLOR : PBF_SPARSE_NONE;
+  }
+  return s == 1 ? PBF_SPARSE_COLOR : s == 2 ? PBF_SPARSE_EDGE : PBF_SPARSE_NONE;
+}
+
+// num_butterflies *= pow(denom, 3 or 4) on a long (:213-214).
+int scale_total(uint64_t total, int sparse, long denom, int64_t* out) {
+  if (sparse == PBF_SPARSE_NONE) { *out = (int64_t)total; return 0; }
+  const double s = (double)(int64_t)total * std::pow((double)denom, sparse == PBF_SPARSE_COLOR ? 3.0 : 4.0);
+  if (!(s < 9223372036854775808.0)) return PBF_ERR_DOMAIN;
+  *out = (int64_t)s;
+  return 0;
 }
 
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
@@ -137,10 +159,9 @@ int compute(pbf_context* ctx, const pbf_bipartite_csr& g, const Args& P) {
   } else {
     per = P.has("-total") ? PBF_PER_TOTAL : (te == 0 ? PBF_PER_VERT : PBF_PER_EDGE);
   }
-  if (P.num("-s", 0) != 0 || !P.str("-sparseType", "").empty()) {
-    fprintf(stderr, "sparsification (-sparseType) is not supported by this build\n");
-    return PBF_ERR_UNSUPPORTED;
-  }
+  const int sparse = sparse_type(P);
+  const long denom = P.num("-d", 25);
+  if (sparse != PBF_SPARSE_NONE) per = PBF_PER_TOTAL;
   if (per == PBF_PER_TOTAL) nopeel = true;
 
   double t0 = now();
@@ -156,7 +177,10 @@ int compute(pbf_context* ctx, const pbf_bipartite_csr& g, const Args& P) {
     st = pbf_count_total(ctx, &g, PBF_MEM_HOST, use_v, ty, tw, &total);
     if (st) return fail("pbf_count_total", st);
     printf("%s : %.3g\n", count_label(ty), now() - t0);
-    printf("number of butterflies: %llu\n", (unsigned long long)total);
+    int64_t scaled = 0;
+    st = scale_total(total, sparse, denom, &scaled);
+    if (st) return fail("scaling", st);
+    printf("number of butterflies: %lld\n", (long long)scaled);
     return 0;
   }
   if (per == PBF_PER_VERT) {
@@ -213,6 +237,21 @@ int main(int argc, char** argv) {
     pbf_graph g;
     st = pbf_read_bipartite(file, &g);
     if (st) { rc = fail("readBipartite", st); break; }
+    const int sparse = sparse_type(P);
+    if (sparse != PBF_SPARSE_NONE) {
+      pbf_graph h;
+      st = pbf_graph_alloc(g.csr.nv, g.csr.nu, g.csr.m, &h);
+      if (!st) {
+        int64_t nv = 0, nu = 0, m = 0;
+        st = pbf_sparsify(ctx, &g.csr, PBF_MEM_HOST, sparse, P.num("-d", 25), 0, &nv, &nu, &m,
+                          const_cast<uint32_t*>(h.csr.offsetsV), const_cast<uint32_t*>(h.csr.edgesV),
+                          const_cast<uint32_t*>(h.csr.offsetsU), const_cast<uint32_t*>(h.csr.edgesU));
+        h.csr.nv = nv; h.csr.nu = nu; h.csr.m = m;  // prefixes of the input-sized arrays
+      }
+      pbf_graph_free(&g);
+      if (st) { pbf_graph_free(&h); rc = fail("sparsify", st); break; }
+      g = h;
+    }
     rc = compute(ctx, g.csr, P);
     pbf_graph_free(&g);
   }

@@ -55,6 +55,8 @@ enum ScratchSlot {
   S_BFEAT,       // batch features: per-graph statistics
   S_RLO,         // batch: first rank of each rank's graph block (graph-local entries)
   S_RINFO,       // per rank: (roff, in) packed for the slot-info gathers
+  S_SPARSE0,     // sparsification: scanned slot flags, new ids, tile owners
+  S_SPARSE1,     // sparsification: the sparsified graph
   S_NUM_SLOTS
 };
 
@@ -171,6 +173,13 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
 // peel.hip
 int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint64_t* counts, uint64_t* tip);
 int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint64_t* counts, uint64_t* wing);
+
+// sparse.hip: the sparsified graph of g (eSparseBipartite / clrSparseBipartite + delZeroDeg)
+// into the given device arrays (room for g's sizes); *out gets its sizes and pointers.
+int pbf_dev_sparsify(pbf_context* ctx, const DevGraph& g, int sparse_type, int64_t denom, int64_t seed, DevGraph* out,
+                     uint32_t* offV2, uint32_t* edgV2, uint32_t* offU2, uint32_t* edgU2);
+// The driver's scaling of a sparsified total (BipartiteButterflies.C:213-214).
+int pbf_sparse_scale(uint64_t total, int sparse_type, int64_t denom, int64_t* approx);
 
 // ---------------- device helpers ----------------
 __device__ __forceinline__ uint32_t pbf_bitlen(uint32_t d) { return d ? 32u - __clz(d) : 0u; }

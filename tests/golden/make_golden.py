@@ -2,11 +2,12 @@
 
 Run in a container that has the reference tree (oracle/Makefile builds oracle/_ref/ref_harness
 from the reference's own headers):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--sparse-only]
 For every graph below it stores the AdjacencyHypergraph input (<name>.adj) and the reference's
 outputs (<name>.npz): cmpWedgeCounts, per-vertex counts (SERIAL and BATCHS), per-edge counts,
 totals, tip numbers (PBATCHS), wing numbers (PBATCHS), and the reference's ADEG / ACOCORE orders
-with the wedge count W each gives.  The reference prints only totals (BipartiteButterflies.C:
+with the wedge count W each gives; sparse_cases.npz holds the graphs that the reference's
+sparsifiers make of them (make_sparse).  The reference prints only totals (BipartiteButterflies.C:
 215, 234, 262); the harness dumps the arrays that Compute discards.
 """
 import os
@@ -34,11 +35,38 @@ def graphs():
     ]
 
 
+SPARSE_CASES = [("EDGE", 2, 5), ("EDGE", 3, 1), ("EDGE", 25, 7), ("COLOR", 2, 5), ("COLOR", 3, 1), ("COLOR", 25, 7),
+                ("EDGE", 4, 123456789), ("COLOR", 4, 123456789)]
+
+
+def make_sparse(tmp):
+    """sparse_cases.npz: for every graph and (type, denom, seed) of SPARSE_CASES the graph that
+    eSparseBipartite / clrSparseBipartite (with delZeroDeg) make of it, and CountTotal of that
+    graph with the driver's scaling."""
+    out_bin = os.path.join(tmp, "s.bin")
+    rec = {}
+    for name, _ in graphs():
+        adj = os.path.join(HERE, name + ".adj")
+        for st, d, seed in SPARSE_CASES:
+            _, _, c, p, _ = O.run_ref("sparse", adj, out_bin, "BATCHS", "ADEG",
+                                      extra=("-sparseType", st, "-d", str(d), "-seed", str(seed)))
+            nv, nu, m = (int(x) for x in c[:3])
+            k = f"{name}__{st}_{d}_{seed}__"
+            rec[k + "sizes"] = np.array([nv, nu, m], np.int64)
+            rec[k + "csr"] = c[3:].astype(np.uint32)  # offsetsV, edgesV, offsetsU, edgesU
+            rec[k + "totals"] = p  # total of the sparsified graph, scaled total
+    np.savez_compressed(os.path.join(HERE, "sparse_cases.npz"), **rec)
+    print("sparse_cases", len(rec) // 3)
+
+
 def main():
     O.build()
     if not O.ref_available():
         raise SystemExit("oracle/_ref/ref_harness is not built (needs the reference tree)")
     tmp = tempfile.mkdtemp()
+    make_sparse(tmp)
+    if "--sparse-only" in sys.argv:
+        return
     out_bin = os.path.join(tmp, "o.bin")
     for name, g in graphs():
         adj = os.path.join(HERE, name + ".adj")

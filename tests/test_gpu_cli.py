@@ -47,7 +47,24 @@ def test_cli_errors():
     adj = os.path.join(GOLD, "k_4_5.adj")
     rc, _, err = _run("-rankType", "DEG", "-r", "0", adj)
     assert rc != 0 and "unsupported" in err.lower()
-    rc, _, err = _run("-sparseType", "ESPARSE", "-r", "0", adj)
-    assert rc != 0 and "sparsification" in err
+    # an unknown -sparseType string means no sparsification (BipartiteButterflies.C:180-183)
+    rc, out, err = _run("-sparseType", "ESPARSE", "-r", "0", adj)
+    assert rc == 0 and "number of butterflies: 60" in out, err
     rc, _, err = _run("-r", "0", os.path.join(GOLD, "no_such_file.adj"))
     assert rc != 0
+
+
+@pytest.mark.parametrize("flags,st,d", [(("-sparseType", "EDGE", "-d", "3"), "EDGE", 3),
+                                        (("-s", "1", "-d", "2"), "COLOR", 2),
+                                        (("-sparseType", "COLOR"), "COLOR", 25)])
+def test_cli_sparsified_total(flags, st, d):
+    # the driver sparsifies with seed 0 after each read and prints the scaled TOTAL
+    # (BipartiteButterflies.C:193, :213-214, :332-337); the oracle is pinned to the reference's
+    # sparsifiers by tests/test_oracle_sparse.py
+    from oracle import oracle as O
+    name = "chunglu_2000x500_m12000_g21_s2"
+    adj = os.path.join(GOLD, name + ".adj")
+    want = O.approx_total(pb.Graph.read(adj), st, d, 0)[0]
+    rc, out, err = _run(*flags, "-per", "VERT", "-r", "1", adj)
+    assert rc == 0, err
+    assert [int(x) for x in re.findall(r"number of butterflies: (-?\d+)", out)] == [want, want], out

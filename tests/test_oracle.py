@@ -19,7 +19,8 @@ from oracle import oracle as O
 from tests.graphs import complete_bipartite, small_graphs
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+CASES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+               if os.path.exists(p[:-4] + ".adj"))
 
 
 def _load(name):
