@@ -171,6 +171,36 @@ def edge_bench(pb, ctx, dg, use_v, W, m, n, vert_sum, steps=5):
             "sum_is_4_total": esum == 2 * vert_sum}
 
 
+def sparse_bench(pb, ctx, dg, exact_total, denom=25, seed=0, reps=5):
+    """Approximate TOTAL by sparsification (the driver's -sparseType EDGE / COLOR, -d 25, seed 0)
+    on the headline graph in HBM: sparsify + side + rank + relabel + TOTAL count per call.  The
+    sparsification passes are HBM streams; their algorithmic bytes are one read of both input
+    CSRs and one write of both sparsified CSRs."""
+    out = {}
+    for st in ("EDGE", "COLOR"):
+        h = ctx.sparsify(dg, st, denom, seed)
+        nv2, nu2, m2 = h.nv, h.nu, h.m
+        del h
+        ctx.count_total_sparse(dg, st, denom, seed)
+        ctx.synchronize()
+        ph = []
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            approx, exact = ctx.count_total_sparse(dg, st, denom, seed)
+            ph.append(ctx.timing())
+        call_ms = (time.perf_counter() - t0) * 1e3 / reps
+        sp_ms = float(np.mean([p["sparsify_ms"] for p in ph]))
+        balg = 4 * ((dg.nv + 1) + 2 * dg.m + (dg.nu + 1)) + 4 * ((nv2 + 1) + 2 * m2 + (nu2 + 1))
+        out[st] = {"denom": denom, "seed": seed, "sparsified": {"nv": nv2, "nu": nu2, "m": m2},
+                   "approx_total": approx, "sparse_total": exact, "exact_total": exact_total,
+                   "rel_error": round((approx - exact_total) / exact_total, 5) if exact_total else None,
+                   "ms_per_call": round(call_ms, 4), "sparsify_ms": round(sp_ms, 4),
+                   "count_ms": round(float(np.mean([p["count_ms"] for p in ph])), 4),
+                   "sparsify_b_alg_bytes": balg,
+                   "sparsify_roofline_frac": round(balg / (sp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000):
     """Count + peel workload: R-MAT bipartite graph (a, b, c, d) = (0.5, 0.1, 0.1, 0.3), 2^22 x 2^22
     vertices, 1.2e8 distinct edges, seed 3, ACOCORE ranking, per-edge counts, then the full wing
@@ -261,6 +291,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--no-rmat", action="store_true", help="skip the R-MAT count + wing-peel workload")
     ap.add_argument("--no-edge", action="store_true", help="skip the per-edge counts on the Chung-Lu graph")
+    ap.add_argument("--no-sparse", action="store_true", help="skip the sparsified (approximate) totals")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -324,6 +355,10 @@ def main():
     if not args.no_edge:
         edge = edge_bench(pb, ctx, dg, use_v, W, g.m, n, int(gpu_counts.sum(dtype=np.uint64)))
 
+    sparse = None
+    if not args.no_sparse:
+        sparse = sparse_bench(pb, ctx, dg, int(gpu_counts.sum(dtype=np.uint64)) // 2)
+
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
@@ -351,6 +386,7 @@ def main():
                      "traffic": traffic, "b_alg_bytes": balg, "count_ms": round(count_ms, 4)},
         "host_inclusive_gwedges_s": round(W / h2d_s / 1e9, 3),
         "per_edge": edge,
+        "sparse_total": sparse,
     }
     if not args.no_batch:
         del dg, out

@@ -39,7 +39,18 @@ struct Rule {
   uint32_t ncol;  // max(1, denom) as the reference's uintE
   uint64_t seed;
   uint64_t salt;  // (seed << 6) + (seed >> 2)
+  uint64_t inv;   // floor((2^64 - 1) / ncol)
 };
+
+// a % r.ncol without a 64-bit division: q = mulhi(a, inv) satisfies a/d - 3 < q <= a/d, so
+// a - q d lies in [0, 3d) and two conditional subtractions finish it.
+__device__ __forceinline__ uint64_t mod_ncol(const Rule& r, uint64_t a) {
+  const uint64_t q = __umul64hi(a, r.inv);
+  uint64_t m = a - q * r.ncol;
+  if (m >= r.ncol) m -= r.ncol;
+  if (m >= r.ncol) m -= r.ncol;
+  return m;
+}
 
 // Keep edge (v in V, u in U)?
 //  EDGE  (butterfly_utils.h:802-810): concat = (v + 0x9e3779b9 + salt) ^ (u + 0x9e3779b9 + salt),
@@ -52,9 +63,9 @@ __device__ __forceinline__ bool keep_edge(const Rule& r, uint32_t v, uint32_t u)
   if (r.type == PBF_SPARSE_EDGE) {
     const uint64_t a = (uint64_t)v + 0x9e3779b9ull + r.salt;
     const uint64_t b = (uint64_t)(uint32_t)(u + 0x9e3779b9u) + r.salt;
-    return ref_hash64(r.seed + (a ^ b)) % r.ncol == 0;
+    return mod_ncol(r, ref_hash64(r.seed + (a ^ b))) == 0;
   }
-  return ref_hash64(r.seed + v) % r.ncol == ref_hash64(r.seed + u) % r.ncol;
+  return mod_ncol(r, ref_hash64(r.seed + v)) == mod_ncol(r, ref_hash64(r.seed + u));
 }
 
 // Last x in [lo, hi] with off[x] <= p, given off[lo] <= p.
@@ -144,6 +155,7 @@ int pbf_dev_sparsify(pbf_context* ctx, const DevGraph& g, int sparse_type, int64
   r.ncol = (uint32_t)denom;
   r.seed = (uint64_t)seed;
   r.salt = ((uint64_t)seed << 6) + ((uint64_t)seed >> 2);
+  r.inv = ~0ull / r.ncol;
   const int64_t m = g.m, ntiles = (m + SP_TILE - 1) / SP_TILE;
   const int64_t words = 2 * (m + 1) + (g.nv + 1) + (g.nu + 1) + 2 * (ntiles + 1);
   void* sp;
