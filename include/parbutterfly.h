@@ -154,6 +154,13 @@ int pbf_copy_to_host(pbf_context* ctx, void* dst, const void* src, size_t bytes)
  * hardware traffic counters. */
 int pbf_copy_device(pbf_context* ctx, void* dst, const void* src, size_t bytes);
 
+/* readBipartite (butterfly_utils.h:1008-1058) with its output in HBM: the file is copied to the
+ * context's device as is and tokenised, parsed (text) and validated there, with the error codes
+ * of pbf_read_bipartite (the first error in file order, then pbf_validate's order).  out's
+ * arrays point into one device allocation returned in *storage (release it with
+ * pbf_device_free).  Synchronous. */
+int pbf_read_bipartite_device(pbf_context* ctx, const char* path, pbf_bipartite_csr* out, void** storage);
+
 /* cmpWedgeCounts (butterfly_utils.h:1061-1086) on device or host graphs. */
 int pbf_side(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int* use_v, int64_t* num_wedges);
 

@@ -122,6 +122,7 @@ _SIGS = {
     "pbf_edge_to_idxs": (_INT, [_P, _PCSR, _INT, _INT, _P]),
     "pbf_peel_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
     "pbf_peel_edge": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P, _P]),
+    "pbf_read_bipartite_device": (_INT, [_P, ctypes.c_char_p, _PCSR, ctypes.POINTER(_P)]),
     "pbf_sparsify": (_INT, [_P, _PCSR, _INT, _INT, _I64, _I64, _PI64, _PI64, _PI64, _P, _P, _P, _P]),
     "pbf_count_total_sparse": (_INT, [_P, _PCSR, _INT, _INT, _I64, _I64, _INT, _INT, _PI64,
                                       ctypes.POINTER(ctypes.c_uint64)]),
@@ -295,6 +296,16 @@ class DeviceArray:
             if self.nbytes:
                 _check(_lib.pbf_copy_to_device(ctx._ctx, self.ptr, h.ctypes.data, self.nbytes), "pbf_copy_to_device")
 
+    @classmethod
+    def _view(cls, ctx, ptr, n, dtype, base):
+        """n elements at device address ptr inside `base` (a DeviceArray that owns the memory)."""
+        self = cls.__new__(cls)
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = ptr
+        self._base = base
+        return self
+
     def numpy(self):
         out = np.empty(self.n, dtype=self.dtype)
         if self.nbytes:
@@ -308,6 +319,9 @@ class DeviceArray:
             _check(_lib.pbf_copy_device(self.ctx._ctx, self.ptr, other.ptr, self.nbytes), "pbf_copy_device")
 
     def free(self):
+        if getattr(self, "_base", None) is not None:  # a view: the base owns the memory
+            self.ptr = None
+            return
         if self.ptr and self.ctx._ctx:
             _lib.pbf_device_free(self.ctx._ctx, self.ptr)
         self.ptr = None
@@ -338,6 +352,20 @@ class DeviceGraph:
     @property
     def n(self):
         return self.nv + self.nu
+
+    @classmethod
+    def read(cls, ctx, path):
+        """readBipartite straight into HBM (text parsed and everything validated on the GPU)."""
+        csr, blk = _CSR(), ctypes.c_void_p()
+        _check(_lib.pbf_read_bipartite_device(ctx._ctx, os.fsencode(path), ctypes.byref(csr), ctypes.byref(blk)),
+               "pbf_read_bipartite_device")
+        base = DeviceArray._view(ctx, blk.value, 0, np.uint8, None)
+        base._base = None  # owns the block
+        arrs = [DeviceArray._view(ctx, getattr(csr, k), n, np.uint32, base) for k, n in
+                (("offsetsV", csr.nv + 1), ("edgesV", csr.m), ("offsetsU", csr.nu + 1), ("edgesU", csr.m))]
+        g = cls._from_arrays(ctx, csr.nv, csr.nu, csr.m, *arrs)
+        g._block = base
+        return g
 
     @classmethod
     def _from_arrays(cls, ctx, nv, nu, m, offsetsV, edgesV, offsetsU, edgesU):

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libparbutterfly_amd.so")
 CLI = os.path.join(HERE, "bin", "bipartite_butterflies")
 
-HIP_SOURCES = ["scan.hip", "rank.hip", "count.hip", "peel.hip", "batch.hip", "mlp.hip", "sparse.hip", "api.hip"]
+HIP_SOURCES = ["scan.hip", "rank.hip", "count.hip", "peel.hip", "batch.hip", "mlp.hip", "sparse.hip", "textio.hip", "api.hip"]
 HOST_SOURCES = ["host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -214,14 +214,17 @@ struct Mapped {
   }
 };
 
+// The whitespace of the reference's tokeniser (isSpace, ligra/IO.h:75-84).
+inline bool is_space(char c) { return c == ' ' || c == '\n' || c == '\r' || c == '\t' || c == 0; }
+
 struct Tok {
   const char* p;
   const char* e;
   bool next_word(const char** ws, const char** we) {
-    while (p < e && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) p++;
+    while (p < e && is_space(*p)) p++;
     if (p >= e) return false;
     *ws = p;
-    while (p < e && !(*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) p++;
+    while (p < e && !is_space(*p)) p++;
     *we = p;
     return true;
   }

@@ -57,6 +57,9 @@ enum ScratchSlot {
   S_RINFO,       // per rank: (roff, in) packed for the slot-info gathers
   S_SPARSE0,     // sparsification: scanned slot flags, new ids, tile owners
   S_SPARSE1,     // sparsification: the sparsified graph
+  S_TEXT0,       // device reader: the file's bytes, tile token counts
+  S_TEXT1,       // device reader: list-start flags, tile owners
+  S_TEXT2,       // device reader: first-error key
   S_NUM_SLOTS
 };
 

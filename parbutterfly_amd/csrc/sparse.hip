@@ -3,7 +3,7 @@
 // with the driver's scaling (apps/hyper/BipartiteButterflies.C:213-214).
 //
 // Each side of the graph is handled as a stream of list entries (slots):
-//   k_sp_bounds   owner vertex of the first slot of every 2048-slot tile (binary search);
+//   k_tile_owner  owner vertex of the first slot of every 2048-slot tile (binary search);
 //   k_sp_mark     keep flag of every slot (the tile's offsets staged in LDS for the owner
 //                 search; the colour is recomputed from the ids, no gathers);
 //   scan          exclusive scan of the flags in place: pos[p] = new position of slot p,
@@ -15,7 +15,7 @@
 // Every pass streams its arrays once (the edge pass also gathers the other side's new ids).
 #include <cmath>
 
-#include "pbf_device.h"
+#include "owner.h"
 
 namespace {
 
@@ -68,25 +68,6 @@ __device__ __forceinline__ bool keep_edge(const Rule& r, uint32_t v, uint32_t u)
   return mod_ncol(r, ref_hash64(r.seed + v)) == mod_ncol(r, ref_hash64(r.seed + u));
 }
 
-// Last x in [lo, hi] with off[x] <= p, given off[lo] <= p.
-__device__ __forceinline__ uint32_t owner_in(const uint32_t* off, uint32_t lo, uint32_t hi, uint32_t p) {
-  while (lo < hi) {
-    const uint32_t mid = lo + ((hi - lo + 1) >> 1);
-    if (off[mid] <= p) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// xb[t] = owner of slot t * SP_TILE for every tile t < ntiles; xb[ntiles] = owner of slot m - 1.
-__global__ void k_sp_bounds(const uint32_t* __restrict__ off, int64_t n, int64_t m, int64_t ntiles,
-                            uint32_t* __restrict__ xb) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t p = (uint32_t)(t < ntiles ? t * SP_TILE : m - 1);
-    xb[t] = owner_in(off, 0, (uint32_t)(n - 1), p);
-  }
-}
-
 // flag[p] = 1 when slot p's edge is kept (p < m); flag[m] = 0.  One tile of SP_TILE slots per
 // block; the owners of the tile lie in [xb[t], xb[t+1]].
 template <bool VSIDE>
@@ -95,18 +76,14 @@ __global__ __launch_bounds__(SP_THREADS) void k_sp_mark(const uint32_t* __restri
                                                         uint32_t* __restrict__ flag) {
   __shared__ uint32_t s_off[SP_LDS];
   const int64_t t = blockIdx.x;
-  const uint32_t x0 = xb[t], x1 = xb[t + 1];
-  const uint32_t span = x1 - x0 + 1;  // owners x0 .. x1
-  const bool staged = span <= (uint32_t)SP_LDS;
-  if (staged)
-    for (uint32_t k = threadIdx.x; k < span; k += SP_THREADS) s_off[k] = off[x0 + k];
-  __syncthreads();
+  TileOwners own;
+  own.stage(off, xb, t, s_off, SP_LDS);
   const int64_t base = t * SP_TILE;
 #pragma unroll
   for (int k = 0; k < SP_PER; k++) {
     const int64_t p = base + (int64_t)k * SP_THREADS + threadIdx.x;
     if (p < m) {
-      const uint32_t x = staged ? x0 + owner_in(s_off, 0, span - 1, (uint32_t)p) : owner_in(off, x0, x1, (uint32_t)p);
+      const uint32_t x = own.owner(off, s_off, (uint32_t)p);
       const uint32_t y = edg[p];
       flag[p] = (VSIDE ? keep_edge(r, x, y) : keep_edge(r, y, x)) ? 1u : 0u;
     }
@@ -174,7 +151,7 @@ int pbf_dev_sparsify(pbf_context* ctx, const DevGraph& g, int sparse_type, int64
   uint32_t* xbs[2] = {xbV, xbU};
   for (int s = 0; s < 2; s++) {
     if (m > 0) {
-      hipLaunchKernelGGL(k_sp_bounds, dim3(sp_grid(ntiles + 1, 1024)), dim3(256), 0, ctx->stream, offs[s], n_side[s], m,
+      hipLaunchKernelGGL(k_tile_owner<SP_TILE>, dim3(sp_grid(ntiles + 1, 1024)), dim3(256), 0, ctx->stream, offs[s], n_side[s], m,
                          ntiles, xbs[s]);
       if (s == 0)
         hipLaunchKernelGGL(k_sp_mark<true>, dim3((unsigned)ntiles), dim3(SP_THREADS), 0, ctx->stream, offs[s], edgs[s], m, r,
