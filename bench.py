@@ -201,6 +201,37 @@ def sparse_bench(pb, ctx, dg, exact_total, denom=25, seed=0, reps=5):
     return out
 
 
+def read_bench(pb, ctx, g, reps=3):
+    """readBipartite of the headline graph from a file in the page cache: the host reader (parse
+    on one core, then the CSR copied to HBM by the first count) against the device reader (file
+    bytes copied to HBM, tokenised, parsed and validated there), for the AdjacencyHypergraph text
+    and the binary CSR.  Medians of `reps` reads; the device reads are checked against the host
+    read."""
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for kind, path, write in (("text", os.path.join(td, "g.adj"), g.write),
+                                  ("binary", os.path.join(td, "g.bcsr"), g.write_binary)):
+            write(path)
+            size = os.path.getsize(path)
+            th, td_ = [], []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                h = pb.Graph.read(path)
+                th.append(time.perf_counter() - t0)
+                del h
+                t0 = time.perf_counter()
+                d = pb.DeviceGraph.read(ctx, path)
+                td_.append(time.perf_counter() - t0)
+            same = all(np.array_equal(getattr(g, k), getattr(d, k).numpy())
+                       for k in ("offsetsV", "edgesV", "offsetsU", "edgesU"))
+            del d
+            hs, ds = sorted(th)[reps // 2], sorted(td_)[reps // 2]
+            out[kind] = {"bytes": size, "host_read_s": round(hs, 4), "device_read_s": round(ds, 4),
+                         "device_read_gb_s": round(size / ds / 1e9, 2), "speedup": round(hs / ds, 1),
+                         "same_as_host": bool(same)}
+    return out
+
+
 def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000):
     """Count + peel workload: R-MAT bipartite graph (a, b, c, d) = (0.5, 0.1, 0.1, 0.3), 2^22 x 2^22
     vertices, 1.2e8 distinct edges, seed 3, ACOCORE ranking, per-edge counts, then the full wing
@@ -292,6 +323,7 @@ def main():
     ap.add_argument("--no-rmat", action="store_true", help="skip the R-MAT count + wing-peel workload")
     ap.add_argument("--no-edge", action="store_true", help="skip the per-edge counts on the Chung-Lu graph")
     ap.add_argument("--no-sparse", action="store_true", help="skip the sparsified (approximate) totals")
+    ap.add_argument("--no-read", action="store_true", help="skip the file-reader timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -359,6 +391,10 @@ def main():
     if not args.no_sparse:
         sparse = sparse_bench(pb, ctx, dg, int(gpu_counts.sum(dtype=np.uint64)) // 2)
 
+    reads = None
+    if not args.no_read:
+        reads = read_bench(pb, ctx, g)
+
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
@@ -387,6 +423,7 @@ def main():
         "host_inclusive_gwedges_s": round(W / h2d_s / 1e9, 3),
         "per_edge": edge,
         "sparse_total": sparse,
+        "read_path": reads,
     }
     if not args.no_batch:
         del dg, out

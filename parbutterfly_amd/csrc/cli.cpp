@@ -5,12 +5,13 @@
 //   bipartite_butterflies [-countType T | -t N] [-rankType R | -w N] [-peelType P | -tp N]
 //                         [-per VERT|EDGE|TOTAL | -e N | -total] [-r rounds] [-m max_wedges]
 //                         [-a max_array_size] [-nopeel] [-sparseType EDGE|COLOR | -s N] [-d denom]
-//                         [-device D] <inFile>
+//                         [-device D] [-hostRead] <inFile>
 //
 // Count types are the reference's (ASORT ... SERIAL, numeric aliases of :96-107) plus GPU;
 // they are aggregation strategies with identical results, and all of them run the GPU kernels
 // here.  Rank types ADEG, ACOCORE and SIDE are supported on the GPU; DEG / COCORE return
-// PBF_ERR_UNSUPPORTED.  The graph is re-read every round, as the reference does (:331-341);
+// PBF_ERR_UNSUPPORTED.  The graph is re-read every round, as the reference does (:331-341),
+// straight into HBM (pbf_read_bipartite_device; -hostRead reads it into host memory instead);
 // with -sparseType it is sparsified (seed 0) on the GPU after each read (:332-337), the count
 // becomes TOTAL and the printed total is scaled by denom^3 (COLOR) or denom^4 (EDGE)
 // (:193, :213-214).
@@ -98,7 +99,21 @@ int fail(const char* what, int st) {
   return st ? st : 1;
 }
 
-int compute(pbf_context* ctx, const pbf_bipartite_csr& g, const Args& P) {
+// A count / peel output in the graph's memory space (mem), readable on the host after fetch().
+struct OutBuf {
+  pbf_context* ctx;
+  int mem;
+  size_t n;
+  std::vector<uint64_t> host;
+  void* dev = nullptr;
+  OutBuf(pbf_context* c, int m, size_t count) : ctx(c), mem(m), n(count), host(count ? count : 1) {}
+  int init() { return mem == PBF_MEM_DEVICE ? pbf_device_alloc(ctx, 8 * (n ? n : 1), &dev) : PBF_OK; }
+  uint64_t* ptr() { return mem == PBF_MEM_DEVICE ? static_cast<uint64_t*>(dev) : host.data(); }
+  int fetch() { return mem == PBF_MEM_DEVICE ? pbf_copy_to_host(ctx, host.data(), dev, 8 * (n ? n : 1)) : PBF_OK; }
+  ~OutBuf() { if (dev) pbf_device_free(ctx, dev); }
+};
+
+int compute(pbf_context* ctx, const pbf_bipartite_csr& g, int mem, const Args& P) {
   // count type (:79-109)
   long tl = P.num("-t", NONE);
   std::string ts = P.str("-countType", "");
@@ -167,16 +182,19 @@ int compute(pbf_context* ctx, const pbf_bipartite_csr& g, const Args& P) {
   double t0 = now();
   int use_v = 0;
   int64_t nw = 0;
-  int st = pbf_side(ctx, &g, PBF_MEM_HOST, &use_v, &nw);
+  int st = pbf_side(ctx, &g, mem, &use_v, &nw);
   if (st) return fail("pbf_side", st);
   printf("preprocess (wedge counts) : %.3g\n", now() - t0);
 
   if (per == PBF_PER_TOTAL) {
-    uint64_t total = 0;
+    OutBuf t(ctx, mem, 1);
+    if ((st = t.init())) return fail("pbf_device_alloc", st);
     t0 = now();
-    st = pbf_count_total(ctx, &g, PBF_MEM_HOST, use_v, ty, tw, &total);
+    st = pbf_count_total(ctx, &g, mem, use_v, ty, tw, t.ptr());
     if (st) return fail("pbf_count_total", st);
     printf("%s : %.3g\n", count_label(ty), now() - t0);
+    if ((st = t.fetch())) return fail("pbf_copy_to_host", st);
+    const uint64_t total = t.host[0];
     int64_t scaled = 0;
     st = scale_total(total, sparse, denom, &scaled);
     if (st) return fail("scaling", st);
@@ -185,35 +203,41 @@ int compute(pbf_context* ctx, const pbf_bipartite_csr& g, const Args& P) {
   }
   if (per == PBF_PER_VERT) {
     const int64_t ns = use_v ? g.nu : g.nv;
-    std::vector<uint64_t> b((size_t)ns);
+    OutBuf b(ctx, mem, (size_t)ns);
+    if ((st = b.init())) return fail("pbf_device_alloc", st);
     t0 = now();
-    st = pbf_count_vert(ctx, &g, PBF_MEM_HOST, use_v, ty, tw, b.data());
+    st = pbf_count_vert(ctx, &g, mem, use_v, ty, tw, b.ptr());
     if (st) return fail("pbf_count_vert", st);
     printf("%s : %.3g\n", count_label(ty), now() - t0);
+    if ((st = b.fetch())) return fail("pbf_copy_to_host", st);
     unsigned long long sum = 0;
-    for (auto x : b) sum += x;
+    for (int64_t i = 0; i < ns; i++) sum += b.host[i];
     printf("number of butterflies: %llu\n", sum / 2);
     if (!nopeel) {
-      std::vector<uint64_t> tip((size_t)ns);
+      OutBuf tip(ctx, mem, (size_t)ns);
+      if ((st = tip.init())) return fail("pbf_device_alloc", st);
       t0 = now();
-      st = pbf_peel_vert(ctx, &g, PBF_MEM_HOST, use_v, tp, b.data(), tip.data());
+      st = pbf_peel_vert(ctx, &g, mem, use_v, tp, b.ptr(), tip.ptr());
       if (st) return fail("pbf_peel_vert", st);
       printf("%s : %.3g\n", peel_label(tp), now() - t0);
     }
     return 0;
   }
-  std::vector<uint64_t> e((size_t)g.m);
+  OutBuf e(ctx, mem, (size_t)g.m);
+  if ((st = e.init())) return fail("pbf_device_alloc", st);
   t0 = now();
-  st = pbf_count_edge(ctx, &g, PBF_MEM_HOST, use_v, ty, tw, e.data());
+  st = pbf_count_edge(ctx, &g, mem, use_v, ty, tw, e.ptr());
   if (st) return fail("pbf_count_edge", st);
   printf("%s : %.3g\n", count_label(ty), now() - t0);
+  if ((st = e.fetch())) return fail("pbf_copy_to_host", st);
   unsigned long long sum = 0;
-  for (auto x : e) sum += x;
+  for (int64_t i = 0; i < g.m; i++) sum += e.host[i];
   printf("number of edge butterflies: %llu\n", sum / 4);
   if (!nopeel) {
-    std::vector<uint64_t> wing((size_t)g.m);
+    OutBuf wing(ctx, mem, (size_t)g.m);
+    if ((st = wing.init())) return fail("pbf_device_alloc", st);
     t0 = now();
-    st = pbf_peel_edge(ctx, &g, PBF_MEM_HOST, use_v, tp, e.data(), wing.data());
+    st = pbf_peel_edge(ctx, &g, mem, use_v, tp, e.ptr(), wing.ptr());
     if (st) return fail("pbf_peel_edge", st);
     printf("%s : %.3g\n", peel_label(tp), now() - t0);
   }
@@ -233,27 +257,42 @@ int main(int argc, char** argv) {
   int st = pbf_create(&ctx, (int)P.num("-device", 0));
   if (st) return fail("pbf_create", st);
   int rc = 0;
+  const bool host_read = P.has("-hostRead");
+  const int mem = host_read ? PBF_MEM_HOST : PBF_MEM_DEVICE;
+  const int sparse = sparse_type(P);
   for (long r = 0; r < rounds + 1 && rc == 0; r++) {
-    pbf_graph g;
-    st = pbf_read_bipartite(file, &g);
+    // the graph: in host memory (pbf_read_bipartite) or read straight into HBM
+    pbf_graph hg{};
+    pbf_bipartite_csr g{};
+    void* blk = nullptr;
+    st = host_read ? pbf_read_bipartite(file, &hg) : pbf_read_bipartite_device(ctx, file, &g, &blk);
     if (st) { rc = fail("readBipartite", st); break; }
-    const int sparse = sparse_type(P);
+    if (host_read) g = hg.csr;
     if (sparse != PBF_SPARSE_NONE) {
-      pbf_graph h;
-      st = pbf_graph_alloc(g.csr.nv, g.csr.nu, g.csr.m, &h);
+      // sparsified with seed 0 after each read (:332-337), into arrays of the input's sizes
+      const size_t words = size_t(g.nv + 1) + size_t(g.m) + size_t(g.nu + 1) + size_t(g.m);
+      pbf_graph hs{};
+      void* bs = nullptr;
+      st = host_read ? pbf_graph_alloc(g.nv, g.nu, g.m, &hs) : pbf_device_alloc(ctx, 4 * words, &bs);
+      pbf_bipartite_csr s{};
       if (!st) {
-        int64_t nv = 0, nu = 0, m = 0;
-        st = pbf_sparsify(ctx, &g.csr, PBF_MEM_HOST, sparse, P.num("-d", 25), 0, &nv, &nu, &m,
-                          const_cast<uint32_t*>(h.csr.offsetsV), const_cast<uint32_t*>(h.csr.edgesV),
-                          const_cast<uint32_t*>(h.csr.offsetsU), const_cast<uint32_t*>(h.csr.edgesU));
-        h.csr.nv = nv; h.csr.nu = nu; h.csr.m = m;  // prefixes of the input-sized arrays
+        uint32_t* w = host_read ? const_cast<uint32_t*>(hs.csr.offsetsV) : static_cast<uint32_t*>(bs);
+        uint32_t *oV = w, *eV = oV + g.nv + 1, *oU = eV + g.m, *eU = oU + g.nu + 1;
+        st = pbf_sparsify(ctx, &g, mem, sparse, P.num("-d", 25), 0, &s.nv, &s.nu, &s.m, oV, eV, oU, eU);
+        s.offsetsV = oV; s.edgesV = eV; s.offsetsU = oU; s.edgesU = eU;  // prefixes of those arrays
       }
-      pbf_graph_free(&g);
-      if (st) { pbf_graph_free(&h); rc = fail("sparsify", st); break; }
-      g = h;
+      if (host_read) pbf_graph_free(&hg); else pbf_device_free(ctx, blk);
+      hg = hs;
+      blk = bs;
+      g = s;
+      if (st) {
+        if (host_read) pbf_graph_free(&hg); else pbf_device_free(ctx, blk);
+        rc = fail("sparsify", st);
+        break;
+      }
     }
-    rc = compute(ctx, g.csr, P);
-    pbf_graph_free(&g);
+    rc = compute(ctx, g, mem, P);
+    if (host_read) pbf_graph_free(&hg); else pbf_device_free(ctx, blk);
   }
   pbf_destroy(ctx);
   return rc;

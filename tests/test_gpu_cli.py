@@ -41,6 +41,11 @@ def test_cli_totals_match_reference(name):
     rc, out, err = _run("-countType", "BATCHS", "-rankType", "SIDE", "-per", "TOTAL", "-r", "0", adj)
     assert rc == 0, err
     assert [int(x) for x in re.findall(r"number of butterflies: (\d+)", out)] == [total], out
+    # the graph read into host memory instead of straight into HBM
+    for per, pat, div in (("VERT", r"number of butterflies: (\d+)", 1), ("EDGE", r"number of edge butterflies: (\d+)", 1)):
+        rc, out, err = _run("-hostRead", "-per", per, "-peelType", "GPU", "-r", "0", adj)
+        assert rc == 0, err
+        assert [int(x) for x in re.findall(pat, out)] == [total], out
 
 
 def test_cli_errors():
