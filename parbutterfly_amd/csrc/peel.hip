@@ -218,6 +218,8 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
   const unsigned long long k = *a.k;
   if (k == KNONE) return;
   const uint32_t nlist = *a.list_len;
+  // the first nlist workgroups drain the queue: the others leave without touching its counter
+  if (blockIdx.x >= nlist) return;
   uint32_t* dense = nullptr;
   uint32_t* dused = nullptr;
   if constexpr (KIND == T_HBM) {
@@ -373,6 +375,8 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   const uint32_t tid = threadIdx.x;
   if (*a.k == KNONE) return;
   const uint32_t nlist = *a.list_len;
+  // the first nlist workgroups drain the queue: the others leave without touching its counter
+  if (blockIdx.x >= nlist) return;
   while (true) {
     if (tid == 0) sh.item = atomicAdd(a.queue, 1u);
     __syncthreads();
