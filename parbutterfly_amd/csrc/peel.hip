@@ -372,6 +372,10 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   __shared__ TeamShared<TEAM> sh;
   __shared__ uint32_t nb_v[UCAP];    // N(b), ascending
   __shared__ uint32_t nb_f[UCAP];    // edge ids of N(b)
+  // membership filter of N(b): one bit per hashed id (no false negatives), so most tested
+  // vertices are rejected by one LDS read before any search or state load
+  constexpr int BM_LOG = UCAP >= 2048 ? 13 : 11;
+  __shared__ uint32_t nb_bits[(1 << BM_LOG) / 32];
   const uint32_t tid = threadIdx.x;
   if (*a.k == KNONE) return;
   const uint32_t nlist = *a.list_len;
@@ -397,8 +401,17 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     const uint32_t ao = offX[va], adeg = offX[va + 1] - ao;
     const uint32_t bo = offY[vb], bdeg = offY[vb + 1] - bo;
     const bool cached = bdeg <= (uint32_t)UCAP;
-    if (cached)
-      for (uint32_t q = tid; q < bdeg; q += TEAM) { nb_v[q] = nbrY[bo + q]; nb_f[q] = idY[bo + q]; }
+    if (cached) {
+      for (uint32_t q = tid; q < (1u << BM_LOG) / 32; q += TEAM) nb_bits[q] = 0;
+      __syncthreads();
+      for (uint32_t q = tid; q < bdeg; q += TEAM) {
+        const uint32_t y = nbrY[bo + q];
+        nb_v[q] = y;
+        nb_f[q] = idY[bo + q];
+        const uint32_t h = (y * 0x9E3779B1u) >> (32 - BM_LOG);
+        atomicOr(&nb_bits[h >> 5], 1u << (h & 31));
+      }
+    }
     // B[e] counts exactly the butterflies through e whose four edges are live or peeled in
     // this round: once the walk has met that many, nothing is left to find
     const unsigned long long target = a.B[e];
@@ -430,15 +443,21 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
         const uint32_t p2 = sh.vs[s] + kk;
         const uint32_t d = nbrY[p2];
         if (d == va) continue;
-        const uint32_t fd = idY[p2];
-        const uint8_t sd = a.state[fd];
-        if (sd == 0) continue;
-        // edge (b, d): search d in N(b)
-        uint32_t lo = 0, hi = bdeg;
+        // edge (b, d): d in N(b)?  And edge (c, d) still there?
+        uint32_t lo = 0, hi = bdeg, fd;
+        uint8_t sd;
         if (cached) {
+          const uint32_t h = (d * 0x9E3779B1u) >> (32 - BM_LOG);
+          if (!((nb_bits[h >> 5] >> (h & 31)) & 1u)) continue;
           while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nb_v[mid] < d) lo = mid + 1; else hi = mid; }
           if (lo == bdeg || nb_v[lo] != d) continue;
+          fd = idY[p2];
+          sd = a.state[fd];
+          if (sd == 0) continue;
         } else {
+          fd = idY[p2];
+          sd = a.state[fd];
+          if (sd == 0) continue;
           while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nbrY[bo + mid] < d) lo = mid + 1; else hi = mid; }
           if (lo == bdeg || nbrY[bo + lo] != d) continue;
         }
