@@ -40,8 +40,21 @@ struct Ctl {
   unsigned long long* k;
   unsigned long long* live;
   unsigned long long* kprev;
-  uint32_t* len;    // [0] small, [1] large, [2] ovf, [3] ntouched, [4] previous ntouched, [5] found
-  uint32_t* queue;  // [0] small, [1] large, [2] ovf
+  uint32_t* len;    // [0] small, [1] large, [2] ovf, [3] ntouched, [4] previous ntouched, [5] found,
+                    // [6] split (reservations; the list holds at most its capacity)
+  uint32_t* queue;  // [0] small, [1] large, [2] ovf, [3..5] split walk counters,
+                    // [8..8+VS_MAX_ROWS) used-key counts of the split rows; all reset per round
+};
+constexpr int VS_MAX_ROWS = 32, CTL_QUEUES = 8 + VS_MAX_ROWS;
+
+// The split class of the vertex peel: frontier items with at least min_paths two-hop paths, or
+// more possible distinct two-hop neighbours (work) than max_work, are walked by many
+// workgroups at once (k_vsplit_*), up to cap of them per round.  paths == null: no split
+// class (edge peel).
+struct SplitSel {
+  const uint32_t* paths;
+  uint32_t min_paths, max_work, cap;
+  uint32_t* list;
 };
 
 __global__ void k_ctl_init(Ctl c) {
@@ -57,7 +70,8 @@ __global__ void k_round_begin(Ctl c) {
     c.len[5] = 0;
     *c.k = KNONE;
     for (int q = 0; q < 4; q++) c.len[q] = 0;
-    for (int q = 0; q < 3; q++) c.queue[q] = 0;
+    c.len[6] = 0;
+    for (int q = 0; q < CTL_QUEUES; q++) c.queue[q] = 0;
   }
 }
 
@@ -85,6 +99,22 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t x, uint32_t* lis
   if (flag) list[base + __popcll(fm & pbf_lanemask_lt())] = x;
 }
 
+// wave_append into a list of capacity cap: true for the lanes whose x found room (the counter
+// still counts every request).
+__device__ __forceinline__ bool wave_append_cap(bool flag, uint32_t x, uint32_t* list, uint32_t* len, uint32_t cap) {
+  const uint64_t fm = __ballot(flag);
+  if (fm == 0ull) return false;
+  const uint32_t leader = (uint32_t)__ffsll((long long)__ballot(1)) - 1u;
+  uint32_t base = 0;
+  if (__lane_id() == leader) base = atomicAdd(len, (uint32_t)__popcll(fm));
+  base = (uint32_t)__shfl((int)base, (int)leader, 64);
+  if (!flag) return false;
+  const uint32_t pos = base + __popcll(fm & pbf_lanemask_lt());
+  if (pos >= cap) return false;
+  list[pos] = x;
+  return true;
+}
+
 // Frontier from the previous round's touched items.  After a round at level k every live item
 // has D >= k, and the live items with D == k are all touched ones (an untouched item keeps its
 // D, which was not k or it would have been peeled).  So when a touched item has D == k the new
@@ -93,7 +123,7 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
                                  uint8_t* __restrict__ state,
                                  const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
                                  uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
-                                 Ctl c) {
+                                 SplitSel ss, Ctl c) {
   const unsigned long long k = *c.kprev;
   if (k == KNONE) return;
   const uint32_t nt = c.len[4];
@@ -108,8 +138,11 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
       // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
       w = B[x] ? (work[x] & WORK_MASK) : 0u;
     }
-    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
-    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
+    bool sp = false;
+    if (ss.paths)
+      sp = wave_append_cap(sel && w != 0 && (w > ss.max_work || ss.paths[x] >= ss.min_paths), x, ss.list, &c.len[6], ss.cap);
+    wave_append(sel && !sp && w != 0 && w <= small_max, x, small_list, &c.len[0]);
+    wave_append(sel && !sp && w > small_max, x, large_list, &c.len[1]);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) {
@@ -137,7 +170,8 @@ __global__ void k_min_live(const unsigned long long* __restrict__ D, const uint8
 __global__ void k_select(const unsigned long long* __restrict__ D, const unsigned long long* __restrict__ B,
                          uint8_t* __restrict__ state,
                          const uint32_t* __restrict__ ids, int64_t n, const uint32_t* __restrict__ work,
-                         uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list, Ctl c) {
+                         uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
+                         SplitSel ss, Ctl c) {
   const unsigned long long k = *c.k;
   if (k == KNONE || c.len[5]) return;
   uint32_t nsel = 0;
@@ -151,8 +185,11 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
       // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
       w = B[x] ? (work[x] & WORK_MASK) : 0u;
     }
-    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
-    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
+    bool sp = false;
+    if (ss.paths)
+      sp = wave_append_cap(sel && w != 0 && (w > ss.max_work || ss.paths[x] >= ss.min_paths), x, ss.list, &c.len[6], ss.cap);
+    wave_append(sel && !sp && w != 0 && w <= small_max, x, small_list, &c.len[0]);
+    wave_append(sel && !sp && w > small_max, x, large_list, &c.len[1]);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
@@ -315,9 +352,11 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
   }
 }
 
-// bound[u] = min(sum_{v in N(u)} deg(v), ns): distinct two-hop neighbours of u.
+// bound[u] = min(sum_{v in N(u)} deg(v), ns): distinct two-hop neighbours of u; paths[u] = the
+// sum itself (two-hop paths, saturating at 2^31 - 1).
 __global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ edgS,
-                                                     const uint32_t* __restrict__ offO, int64_t ns, uint32_t* __restrict__ bound) {
+                                                     const uint32_t* __restrict__ offO, int64_t ns, uint32_t* __restrict__ bound,
+                                                     uint32_t* __restrict__ paths) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int64_t u = (int64_t)blockIdx.x * 4 + wid; u < ns; u += (int64_t)gridDim.x * 4) {
     uint64_t s = 0;
@@ -326,7 +365,169 @@ __global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict_
       s += offO[v + 1] - offO[v];
     }
     s = pbf_wave_sum(s);
-    if (lane == 0) bound[u] = (uint32_t)(s < (uint64_t)ns ? s : (uint64_t)ns);
+    if (lane == 0) {
+      bound[u] = (uint32_t)(s < (uint64_t)ns ? s : (uint64_t)ns);
+      paths[u] = (uint32_t)(s < (uint64_t)WORK_MASK ? s : (uint64_t)WORK_MASK);
+    }
+  }
+}
+
+// ---------------- vertex peel: walks split over many workgroups ----------------
+// A frontier vertex u with many two-hop paths (the late rounds of a power-law graph peel one
+// such vertex at a time) is walked by the whole device: its centres v in N(u) are cut into
+// chunks of VS_CHUNK, a chunk's paths into units of VS_PCH, and workgroups take units; the
+// counts c(u, u2) go to a dense row of u in HBM (global atomics), the keys met to the row's
+// used list; k_vsplit_credit then subtracts C(c, 2) from every live u2 and clears the row.
+constexpr uint32_t VS_CHUNK = 256, VS_PCH = 4096, VS_MIN_PATHS = 1u << 15;
+
+struct VSplitArgs {
+  const uint32_t* offS;
+  const uint32_t* edgS;
+  const uint32_t* offO;
+  const uint32_t* edgO;
+  int64_t ns;
+  const uint32_t* list;     // split items, row r = list[r]
+  const uint32_t* len;      // reservations of the list (rows = min(*len, cap))
+  uint32_t cap;
+  uint32_t* rows;           // per row: dense counters [ns], zero between rounds
+  uint32_t* used;           // per row: keys with a non-zero count [ns]
+  uint32_t* nused;          // per row (reset every round with the round's counters)
+  unsigned long long* sub;  // remaining units: (row << 48) | (chunk << 24) | unit
+  uint32_t sub_cap;
+  uint32_t* ctr;            // [0] chunk queue, [1] sub-units listed, [2] sub-unit queue
+  unsigned long long* B;
+  const unsigned long long* D;
+  uint32_t* tflag;
+  uint32_t* touched;
+  uint32_t* ntouched;
+  const unsigned long long* k;
+};
+
+// Values another workgroup or an earlier launch wrote (list entries, counters) are read with
+// device-scope atomic loads: vector loads that go to the device-coherent level, never a stale
+// cached copy.
+__device__ __forceinline__ uint32_t vs_load(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long vs_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t vs_nrows(const VSplitArgs& a) {
+  const uint32_t n = vs_load(a.len);
+  return n < a.cap ? n : a.cap;
+}
+
+// Row r's chunk starts: every thread computes the prefix of chunk counts over the rows (at
+// most cap rows, read from L2).  Returns the row holding global chunk g and the chunk in it.
+__device__ __forceinline__ void vs_locate(const VSplitArgs& a, uint32_t nrows, uint32_t g, uint32_t* row, uint32_t* chunk) {
+  uint32_t before = 0;
+  for (uint32_t r = 0; r < nrows; r++) {
+    const uint32_t u = vs_load(a.list + r);
+    const uint32_t nch = (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
+    if (g < before + nch) { *row = r; *chunk = g - before; return; }
+    before += nch;
+  }
+  *row = nrows; *chunk = 0;
+}
+
+// Paths of unit s of chunk c of row r, counted into the row.  The chunk's centres are scanned
+// into sh (one per thread); returns the chunk's path total.
+__device__ uint64_t vs_unit(const VSplitArgs& a, TeamShared<256>& sh, uint32_t r, uint32_t c, uint32_t s) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t u = vs_load(a.list + r);
+  const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
+  const uint32_t j = c * VS_CHUNK + tid;
+  uint32_t L = 0, vo = 0;
+  if (j < deg) {
+    const uint32_t v = a.edgS[uo + j];
+    vo = a.offO[v];
+    L = a.offO[v + 1] - vo;
+  }
+  uint32_t T;
+  const uint32_t incl = team_incl_scan<256>(L, sh, &T);
+  sh.P[tid] = incl;
+  sh.vs[tid] = vo;
+  __syncthreads();
+  uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+  uint32_t* used = a.used + (size_t)r * (size_t)a.ns;
+  const uint64_t w0 = (uint64_t)s * VS_PCH;
+  const uint32_t wend = (uint32_t)(w0 + VS_PCH < T ? w0 + VS_PCH : T);
+  for (uint32_t w = (uint32_t)w0 + tid; w < wend; w += 256) {
+    const uint32_t sg = seg_of<256>(sh.P, w);
+    const uint32_t kk = w - (sg ? sh.P[sg - 1] : 0u);
+    const uint32_t u2 = a.edgO[sh.vs[sg] + kk];
+    const bool fresh = u2 != u && atomicAdd(&row[u2], 1u) == 0u;
+    wave_append(fresh, u2, used, &a.nused[r]);
+  }
+  __syncthreads();
+  return T;
+}
+
+// Phase 1: workgroups take whole chunks; a chunk with more than VS_PCH paths keeps its first
+// unit and lists the others for phase 2.  The list has room for every unit of a round: a row
+// has at most m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
+__global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
+  __shared__ TeamShared<256> sh;
+  if (*a.k == KNONE) return;
+  const uint32_t nrows = vs_nrows(a);
+  if (nrows == 0) return;
+  uint32_t total = 0;
+  for (uint32_t r = 0; r < nrows; r++) {
+    const uint32_t u = vs_load(a.list + r);
+    total += (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
+  }
+  if (blockIdx.x >= total) return;
+  while (true) {
+    if (threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[0], 1u);
+    __syncthreads();
+    const uint32_t g = sh.item;
+    __syncthreads();
+    if (g >= total) break;
+    uint32_t r, c;
+    vs_locate(a, nrows, g, &r, &c);
+    const uint64_t T = vs_unit(a, sh, r, c, 0);
+    const uint32_t nunits = (uint32_t)((T + VS_PCH - 1) / VS_PCH);
+    for (uint32_t s = 1 + threadIdx.x; s < nunits; s += 256) {
+      const uint32_t pos = atomicAdd(&a.ctr[1], 1u);
+      if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 48) | ((unsigned long long)c << 24) | s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
+  __shared__ TeamShared<256> sh;
+  if (*a.k == KNONE) return;
+  const uint32_t listed = vs_load(a.ctr + 1);
+  const uint32_t n = listed < a.sub_cap ? listed : a.sub_cap;
+  if (blockIdx.x >= n) return;
+  while (true) {
+    if (threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[2], 1u);
+    __syncthreads();
+    const uint32_t q = sh.item;
+    __syncthreads();
+    if (q >= n) break;
+    const unsigned long long e = vs_load(a.sub + q);
+    vs_unit(a, sh, (uint32_t)(e >> 48), (uint32_t)(e >> 24) & 0xFFFFFFu, (uint32_t)e & 0xFFFFFFu);
+  }
+}
+
+// C(c, 2) off every live u2 of every row, rows cleared.
+__global__ __launch_bounds__(256) void k_vsplit_credit(VSplitArgs a) {
+  const unsigned long long k = *a.k;
+  if (k == KNONE) return;
+  const uint32_t nrows = vs_nrows(a);
+  for (uint32_t r = 0; r < nrows; r++) {
+    uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+    const uint32_t* used = a.used + (size_t)r * (size_t)a.ns;
+    const uint32_t n = vs_load(a.nused + r);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+      const uint32_t key = vs_load(used + q);
+      const uint32_t c = atomicExch(&row[key], 0u);
+      const bool hit = c > 1u && a.D[key] > k;
+      if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
+      wave_append(hit && atomicExch(&a.tflag[key], 1u) == 0u, key, a.touched, a.ntouched);
+    }
   }
 }
 
@@ -638,8 +839,9 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   hipLaunchKernelGGL(k_init_peel, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const unsigned long long*>(counts), pb.B, pb.D, pb.state, pb.tflag, ns);
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, ctx->stream, pb.c);
+  uint32_t* paths = pb.owner_o;  // the vertex peel has no edge owners
   hipLaunchKernelGGL(k_peelv_bound, dim3(grid_for(ns, 4, 16384)), dim3(256), 0, ctx->stream, a.offS, a.edgS, a.offO, ns,
-                     pb.work);
+                     pb.work, paths);
   PBF_HIP(hipGetLastError());
   // HBM fallback pool, only when some vertex could overflow the largest LDS table
   const uint32_t small_max = (V_SMALL_CAP / 4) * 3 - V_SMALL_TEAM;
@@ -652,13 +854,40 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   uint32_t* large_list = pb.lists + ns;
   uint32_t* ovf_list = pb.lists + 2 * ns;
   const unsigned sel_grid = grid_for(ns, 256, 4096);
+  // split walks: up to VS rows per round, each a dense row and a used list of ns words
+  const int64_t no = use_v ? g.nv : g.nu;
+  const uint32_t vrows = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(VS_MAX_ROWS, (int64_t(1) << 31) / (8 * ns)));
+  const uint32_t sub_cap = vrows * (uint32_t)(g.m / VS_PCH + no / VS_CHUNK + 2);
+  VSplitArgs vs{};
+  {
+    const size_t words = 2 * (size_t)vrows * (size_t)ns + (size_t)vrows;
+    void* sp;
+    PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words + 8 * (size_t)sub_cap + 64, &sp));
+    uint32_t* w = static_cast<uint32_t*>(sp);
+    vs.rows = w;
+    vs.used = w + (size_t)vrows * ns;
+    uint32_t* vlist = vs.used + (size_t)vrows * ns;
+    vs.sub = reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(vlist + vrows + 2) & ~uintptr_t(7));
+    // the dense rows start at zero; every round's credit pass leaves them so
+    PBF_CHECK(pbf_fill_u32(ctx, vs.rows, 0u, (int64_t)vrows * ns));
+    vs.list = vlist;
+    vs.nused = pb.c.queue + 8;
+  }
+  vs.offS = a.offS; vs.edgS = a.edgS; vs.offO = a.offO; vs.edgO = a.edgO; vs.ns = ns;
+  vs.len = pb.c.len + 6;
+  vs.cap = vrows;
+  vs.sub_cap = sub_cap;
+  vs.ctr = pb.c.queue + 3;
+  vs.B = pb.B; vs.D = pb.D; vs.tflag = pb.tflag; vs.touched = pb.touched; vs.ntouched = pb.c.len + 3; vs.k = pb.c.k;
+  // beyond the large LDS table's fill limit a walk would overflow into the one-workgroup HBM path
+  const SplitSel ss{paths, VS_MIN_PATHS, (V_LARGE_CAP / 4) * 3 - V_LARGE_TEAM, vrows, const_cast<uint32_t*>(vs.list)};
   auto round = [&]() -> int {
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
     hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
-                       pb.work, small_max, small_list, large_list, pb.c);
+                       pb.work, small_max, small_list, large_list, ss, pb.c);
     hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.c);
     hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, nullptr, ns, pb.work, small_max,
-                       small_list, large_list, pb.c);
+                       small_list, large_list, ss, pb.c);
     PeelVArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
     s.ovf_list = ovf_list; s.ovf_len = pb.c.len + 2;
@@ -669,6 +898,9 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
                        ctx->stream, s);
     s.list = ovf_list; s.list_len = pb.c.len + 2; s.queue = pb.c.queue + 2;
     hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>), dim3(V_HBM_BLOCKS), dim3(V_LARGE_TEAM), 0, ctx->stream, s);
+    hipLaunchKernelGGL(k_vsplit_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
+    hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
+    hipLaunchKernelGGL(k_vsplit_credit, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_update, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
                        pb.c);
     hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, nullptr, ns, pb.c);
@@ -754,10 +986,10 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     const unsigned sel_grid = grid_for(entries, 256, 8192);
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
     hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
-                       pb.work, small_max, small_list, large_list, pb.c);
+                       pb.work, small_max, small_list, large_list, SplitSel{}, pb.c);
     hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.c);
     hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, cur.idO, entries, pb.work,
-                       small_max, small_list, large_list, pb.c);
+                       small_max, small_list, large_list, SplitSel{}, pb.c);
     PeelEArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
     hipLaunchKernelGGL((k_peele_walk<E_SMALL_TEAM, E_SMALL_UCAP>), dim3(ctx->num_cus * E_SMALL_PER_CU), dim3(E_SMALL_TEAM), 0,

@@ -65,3 +65,27 @@ def test_peel_device_memory(ctx):
     e = ctx.count(dg, use_v, "EDGE", "GPU", "ACOCORE")
     wing = ctx.peel_edge(dg, use_v, e)
     assert np.array_equal(pb.as_u64(wing), O.peel_edge(g, use_v, pb.as_u64(e)))
+
+
+def test_tip_peel_split_walks(ctx):
+    # every S vertex has more than 65536 two-hop paths, so the rounds walk their frontier
+    # vertices across many workgroups (dense rows in HBM); one per round in the uniform graph
+    g = pb.Graph.uniform(1000, 300, 200000, 4)
+    use_v, _ = ctx.side(g)
+    assert use_v
+    v = O.count_vert(g, use_v)
+    tip = pb.as_u64(ctx.peel(g, use_v, v))
+    assert np.array_equal(tip, O.peel_vert(g, use_v, v))
+    # K_{100,1500}: all 100 S vertices peel in one round, beyond the split rows of a round
+    k = complete_bipartite(100, 1500)
+    use_v, _ = ctx.side(k)
+    assert not use_v
+    v = pb.as_u64(ctx.count(k, use_v, "VERT"))
+    tip = pb.as_u64(ctx.peel(k, use_v, v))
+    assert (tip == 99 * (1500 * 1499 // 2)).all()
+    # a power-law graph: hubs split, the rest on the LDS walks
+    h = pb.Graph.chung_lu(20000, 2000, 200000, 2.1, 2.1, 6)
+    use_v, _ = ctx.side(h)
+    v = O.count_vert(h, use_v)
+    tip = pb.as_u64(ctx.peel(h, use_v, v))
+    assert np.array_equal(tip, O.peel_vert(h, use_v, v))
