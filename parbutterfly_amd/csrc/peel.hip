@@ -153,16 +153,23 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
 }
 
 // k = min D over live items (state != 0), unless the touched items gave the frontier.
-__global__ void k_min_live(const unsigned long long* __restrict__ D, const uint8_t* __restrict__ state,
-                           const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
+__global__ __launch_bounds__(256) void k_min_live(const unsigned long long* __restrict__ D, const uint8_t* __restrict__ state,
+                                                  const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
+  __shared__ unsigned long long s_min[4];
   if (c.len[5]) return;
   unsigned long long mn = KNONE;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
     if (state[x]) { unsigned long long d = D[x]; mn = d < mn ? d : mn; }
   }
+  // one atomic per workgroup: per-wavefront atomics on the one word serialise the launch
   mn = wave_min_u64(mn);
-  if ((threadIdx.x & 63) == 0 && mn != KNONE) atomicMin(c.k, mn);
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = mn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) mn = s_min[w] < mn ? s_min[w] : mn;
+    if (mn != KNONE) atomicMin(c.k, mn);
+  }
 }
 
 // Frontier selection: live items with D == k become current (state 2) and are listed by work
@@ -378,7 +385,7 @@ __global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict_
 // chunks of VS_CHUNK, a chunk's paths into units of VS_PCH, and workgroups take units; the
 // counts c(u, u2) go to a dense row of u in HBM (global atomics), the keys met to the row's
 // used list; k_vsplit_credit then subtracts C(c, 2) from every live u2 and clears the row.
-constexpr uint32_t VS_CHUNK = 256, VS_PCH = 4096, VS_MIN_PATHS = 1u << 15;
+constexpr uint32_t VS_CHUNK = 256, VS_PCH = 1024, VS_MIN_PATHS = 1u << 15;
 
 struct VSplitArgs {
   const uint32_t* offS;
@@ -453,13 +460,24 @@ __device__ uint64_t vs_unit(const VSplitArgs& a, TeamShared<256>& sh, uint32_t r
   uint32_t* used = a.used + (size_t)r * (size_t)a.ns;
   const uint64_t w0 = (uint64_t)s * VS_PCH;
   const uint32_t wend = (uint32_t)(w0 + VS_PCH < T ? w0 + VS_PCH : T);
-  for (uint32_t w = (uint32_t)w0 + tid; w < wend; w += 256) {
-    const uint32_t sg = seg_of<256>(sh.P, w);
-    const uint32_t kk = w - (sg ? sh.P[sg - 1] : 0u);
-    const uint32_t u2 = a.edgO[sh.vs[sg] + kk];
-    const bool fresh = u2 != u && atomicAdd(&row[u2], 1u) == 0u;
-    wave_append(fresh, u2, used, &a.nused[r]);
+  // VS_PCH / 256 paths per thread, all loads and counter updates issued before any result is
+  // used (the updates return the old count: a key's first path appends it to the used list)
+  constexpr uint32_t PER = VS_PCH / 256;
+  uint32_t key[PER], old[PER];
+#pragma unroll
+  for (uint32_t q = 0; q < PER; q++) {
+    const uint32_t w = (uint32_t)w0 + q * 256 + tid;
+    key[q] = u;
+    if (w < wend) {
+      const uint32_t sg = seg_of<256>(sh.P, w);
+      const uint32_t kk = w - (sg ? sh.P[sg - 1] : 0u);
+      key[q] = a.edgO[sh.vs[sg] + kk];
+    }
   }
+#pragma unroll
+  for (uint32_t q = 0; q < PER; q++) old[q] = key[q] != u ? atomicAdd(&row[key[q]], 1u) : 1u;
+#pragma unroll
+  for (uint32_t q = 0; q < PER; q++) wave_append(old[q] == 0u, key[q], used, &a.nused[r]);
   __syncthreads();
   return T;
 }
@@ -885,7 +903,8 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
     hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
                        pb.work, small_max, small_list, large_list, ss, pb.c);
-    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns, pb.c);
+    hipLaunchKernelGGL(k_min_live, dim3(grid_for(ns, 1024, 2048)), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns,
+                       pb.c);
     hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, nullptr, ns, pb.work, small_max,
                        small_list, large_list, ss, pb.c);
     PeelVArgs s = a;
@@ -987,7 +1006,8 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
     hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
                        pb.work, small_max, small_list, large_list, SplitSel{}, pb.c);
-    hipLaunchKernelGGL(k_min_live, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO, entries, pb.c);
+    hipLaunchKernelGGL(k_min_live, dim3(grid_for(entries, 1024, 2048)), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO,
+                       entries, pb.c);
     hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, cur.idO, entries, pb.work,
                        small_max, small_list, large_list, SplitSel{}, pb.c);
     PeelEArgs s = a;
