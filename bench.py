@@ -171,6 +171,20 @@ def edge_bench(pb, ctx, dg, use_v, W, m, n, vert_sum, steps=5):
             "sum_is_4_total": esum == 2 * vert_sum}
 
 
+def tip_bench(pb, ctx, dg, use_v, counts):
+    """Tip numbers of the headline graph from its per-vertex counts (Peel, the CLI's default
+    after VERT counts): one call, timed with the library's events and the host clock."""
+    t0 = time.perf_counter()
+    tip = ctx.peel(dg, use_v, counts)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    tm = ctx.timing()
+    T = tip.numpy()
+    B = pb.as_u64(counts)
+    return {"peel_s": round(wall, 3), "peel_ms_events": round(tm["peel_ms"], 1), "rounds": int(tm["rounds"]),
+            "levels": int(len(np.unique(T))), "max_tip": int(T.max()), "tip_le_count": bool(np.all(T <= B))}
+
+
 def sparse_bench(pb, ctx, dg, exact_total, denom=25, seed=0, reps=5):
     """Approximate TOTAL by sparsification (the driver's -sparseType EDGE / COLOR, -d 25, seed 0)
     on the headline graph in HBM: sparsify + side + rank + relabel + TOTAL count per call.  The
@@ -323,6 +337,7 @@ def main():
     ap.add_argument("--no-rmat", action="store_true", help="skip the R-MAT count + wing-peel workload")
     ap.add_argument("--no-edge", action="store_true", help="skip the per-edge counts on the Chung-Lu graph")
     ap.add_argument("--no-sparse", action="store_true", help="skip the sparsified (approximate) totals")
+    ap.add_argument("--no-tip", action="store_true", help="skip the tip peel of the Chung-Lu graph")
     ap.add_argument("--no-read", action="store_true", help="skip the file-reader timings")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -387,6 +402,11 @@ def main():
     if not args.no_edge:
         edge = edge_bench(pb, ctx, dg, use_v, W, g.m, n, int(gpu_counts.sum(dtype=np.uint64)))
 
+    tipres = None
+    if not args.no_tip:
+        tipres = tip_bench(pb, ctx, dg, use_v, out)
+        print(f"# tip peel {tipres['peel_s']} s, {tipres['rounds']} rounds", file=sys.stderr, flush=True)
+
     sparse = None
     if not args.no_sparse:
         sparse = sparse_bench(pb, ctx, dg, int(gpu_counts.sum(dtype=np.uint64)) // 2)
@@ -422,6 +442,7 @@ def main():
                      "traffic": traffic, "b_alg_bytes": balg, "count_ms": round(count_ms, 4)},
         "host_inclusive_gwedges_s": round(W / h2d_s / 1e9, 3),
         "per_edge": edge,
+        "tip_peel": tipres,
         "sparse_total": sparse,
         "read_path": reads,
     }

@@ -110,3 +110,32 @@ def test_rmat_2p22_count_and_wing_peel(ctx):
         assert int(hc.min()) >= k, f"wing level {k}"
         if k == int(levels[0]):
             assert int(hc.min()) == k
+
+
+def test_chunglu_2p24_tip_peel(ctx):
+    # tip numbers of the headline graph (about 2e5 levels, the late ones one hub each, walked
+    # across the device); checked by the defining property of tip numbers: in the subgraph of
+    # the S vertices with tip >= k (and all of O) every such vertex lies in at least k
+    # butterflies, and at the top level some vertex in exactly k
+    g = pb.Graph.chung_lu(1 << 20, 1 << 18, 1 << 24, 2.1, 2.1, 2)
+    dg = g.to_device(ctx)
+    use_v, _ = ctx.side(dg)
+    b = ctx.count(dg, use_v, "VERT", "GPU", "ADEG")
+    tip = ctx.peel(dg, use_v, b).numpy()
+    B = pb.as_u64(b)
+    assert np.all(tip <= B)
+    offS, edgS = (g.offsetsU, g.edgesU) if use_v else (g.offsetsV, g.edgesV)
+    owner = np.repeat(np.arange(len(offS) - 1, dtype=np.uint32), np.diff(offS.astype(np.int64)))
+    levels = np.unique(tip)[::-1]
+    sizes = np.cumsum([int(np.count_nonzero(tip == k)) for k in levels[:4000]])
+    ks = [int(levels[0])] + [int(levels[j]) for j in (len(sizes) // 4, len(sizes) - 1)]
+    for k in ks:
+        keep = tip[owner] >= k
+        s_ids, o_ids = owner[keep], edgS[keep].astype(np.uint32)
+        vv, uu = (o_ids, s_ids) if use_v else (s_ids, o_ids)
+        h = pb.Graph.from_edges(g.nv, g.nu, np.ascontiguousarray(vv), np.ascontiguousarray(uu))
+        hc = pb.as_u64(ctx.count(h, use_v, "VERT", "GPU", "ADEG"))
+        sel = tip >= k
+        assert int(hc[sel].min()) >= k, f"tip level {k}"
+        if k == int(levels[0]):
+            assert int(hc[sel].min()) == k
