@@ -26,6 +26,17 @@ constexpr unsigned long long KNONE = ~0ull;
 // work[x] of a peeled item: its walk cost; bit 31 is the side an edge walk starts from
 constexpr uint32_t WORK_MASK = 0x7FFFFFFFu, WORK_SSIDE = 0x80000000u;
 
+// Control words and list entries written by earlier launches (or other workgroups) are read
+// with device-scope atomic loads, which are vector loads served by the device-coherent level:
+// a plain load of a uniform address compiles to a scalar load, and such loads were seen to
+// return values older than the previous launch's writes.
+__device__ __forceinline__ uint32_t dev_ld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long dev_ld(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -65,8 +76,8 @@ __global__ void k_ctl_init(Ctl c) {
 // The previous round's touched list stays readable until this round's walk appends to it.
 __global__ void k_round_begin(Ctl c) {
   if (threadIdx.x == 0) {
-    *c.kprev = *c.k;
-    c.len[4] = c.len[3];
+    *c.kprev = dev_ld(c.k);
+    c.len[4] = dev_ld(c.len + 3);
     c.len[5] = 0;
     *c.k = KNONE;
     for (int q = 0; q < 4; q++) c.len[q] = 0;
@@ -124,9 +135,9 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
                                  const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
                                  uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
                                  SplitSel ss, Ctl c) {
-  const unsigned long long k = *c.kprev;
+  const unsigned long long k = dev_ld(c.kprev);
   if (k == KNONE) return;
-  const uint32_t nt = c.len[4];
+  const uint32_t nt = dev_ld(c.len + 4);
   uint32_t nsel = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const uint32_t x = touched[t];
@@ -156,7 +167,7 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
 __global__ __launch_bounds__(256) void k_min_live(const unsigned long long* __restrict__ D, const uint8_t* __restrict__ state,
                                                   const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
   __shared__ unsigned long long s_min[4];
-  if (c.len[5]) return;
+  if (dev_ld(c.len + 5)) return;
   unsigned long long mn = KNONE;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
@@ -179,8 +190,8 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
                          const uint32_t* __restrict__ ids, int64_t n, const uint32_t* __restrict__ work,
                          uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
                          SplitSel ss, Ctl c) {
-  const unsigned long long k = *c.k;
-  if (k == KNONE || c.len[5]) return;
+  const unsigned long long k = dev_ld(c.k);
+  if (k == KNONE || dev_ld(c.len + 5)) return;
   uint32_t nsel = 0;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
@@ -205,8 +216,8 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
 // Touched live items: D = max(B, k) (updateBuckets, butterfly_putils.h:1036-1043).
 __global__ void k_update(const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
                          const unsigned long long* __restrict__ B, unsigned long long* __restrict__ D, Ctl c) {
-  const unsigned long long k = *c.k;
-  const uint32_t nt = c.len[3];
+  const unsigned long long k = dev_ld(c.k);
+  const uint32_t nt = dev_ld(c.len + 3);
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const uint32_t x = touched[t];
     tflag[x] = 0;
@@ -217,7 +228,7 @@ __global__ void k_update(const uint32_t* __restrict__ touched, uint32_t* __restr
 
 // Current items die at the end of the round.
 __global__ void k_retire(uint8_t* __restrict__ state, const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
-  if (*c.k == KNONE) return;
+  if (dev_ld(c.k) == KNONE) return;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
     if (state[x] == 2) state[x] = 0;
@@ -259,9 +270,9 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
   __shared__ uint32_t lcnts[KIND == T_LDS ? CAP : 1];
   __shared__ uint32_t lused[(KIND == T_LDS && USED) ? CAP : 1];
   const uint32_t tid = threadIdx.x;
-  const unsigned long long k = *a.k;
+  const unsigned long long k = dev_ld(a.k);
   if (k == KNONE) return;
-  const uint32_t nlist = *a.list_len;
+  const uint32_t nlist = dev_ld(a.list_len);
   // the first nlist workgroups drain the queue: the others leave without touching its counter
   if (blockIdx.x >= nlist) return;
   uint32_t* dense = nullptr;
@@ -279,11 +290,11 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
     const uint32_t idx = sh.item;
     __syncthreads();
     if (idx >= nlist) break;
-    const uint32_t u = a.list[idx];
+    const uint32_t u = dev_ld(a.list + idx);
     const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
     LdsTable t;
     if constexpr (KIND == T_LDS) {
-      const uint32_t bnd = a.bound[u];
+      const uint32_t bnd = dev_ld(a.bound + u);
       uint32_t cap = CAP;
       if constexpr (!USED) {
         uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
@@ -410,18 +421,8 @@ struct VSplitArgs {
   const unsigned long long* k;
 };
 
-// Values another workgroup or an earlier launch wrote (list entries, counters) are read with
-// device-scope atomic loads: vector loads that go to the device-coherent level, never a stale
-// cached copy.
-__device__ __forceinline__ uint32_t vs_load(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long vs_load(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ uint32_t vs_nrows(const VSplitArgs& a) {
-  const uint32_t n = vs_load(a.len);
+  const uint32_t n = dev_ld(a.len);
   return n < a.cap ? n : a.cap;
 }
 
@@ -430,7 +431,7 @@ __device__ __forceinline__ uint32_t vs_nrows(const VSplitArgs& a) {
 __device__ __forceinline__ void vs_locate(const VSplitArgs& a, uint32_t nrows, uint32_t g, uint32_t* row, uint32_t* chunk) {
   uint32_t before = 0;
   for (uint32_t r = 0; r < nrows; r++) {
-    const uint32_t u = vs_load(a.list + r);
+    const uint32_t u = dev_ld(a.list + r);
     const uint32_t nch = (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
     if (g < before + nch) { *row = r; *chunk = g - before; return; }
     before += nch;
@@ -442,7 +443,7 @@ __device__ __forceinline__ void vs_locate(const VSplitArgs& a, uint32_t nrows, u
 // into sh (one per thread); returns the chunk's path total.
 __device__ uint64_t vs_unit(const VSplitArgs& a, TeamShared<256>& sh, uint32_t r, uint32_t c, uint32_t s) {
   const uint32_t tid = threadIdx.x;
-  const uint32_t u = vs_load(a.list + r);
+  const uint32_t u = dev_ld(a.list + r);
   const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
   const uint32_t j = c * VS_CHUNK + tid;
   uint32_t L = 0, vo = 0;
@@ -487,12 +488,12 @@ __device__ uint64_t vs_unit(const VSplitArgs& a, TeamShared<256>& sh, uint32_t r
 // has at most m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
 __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
   __shared__ TeamShared<256> sh;
-  if (*a.k == KNONE) return;
+  if (dev_ld(a.k) == KNONE) return;
   const uint32_t nrows = vs_nrows(a);
   if (nrows == 0) return;
   uint32_t total = 0;
   for (uint32_t r = 0; r < nrows; r++) {
-    const uint32_t u = vs_load(a.list + r);
+    const uint32_t u = dev_ld(a.list + r);
     total += (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
   }
   if (blockIdx.x >= total) return;
@@ -515,8 +516,8 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
 
 __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
   __shared__ TeamShared<256> sh;
-  if (*a.k == KNONE) return;
-  const uint32_t listed = vs_load(a.ctr + 1);
+  if (dev_ld(a.k) == KNONE) return;
+  const uint32_t listed = dev_ld(a.ctr + 1);
   const uint32_t n = listed < a.sub_cap ? listed : a.sub_cap;
   if (blockIdx.x >= n) return;
   while (true) {
@@ -525,22 +526,22 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
     const uint32_t q = sh.item;
     __syncthreads();
     if (q >= n) break;
-    const unsigned long long e = vs_load(a.sub + q);
+    const unsigned long long e = dev_ld(a.sub + q);
     vs_unit(a, sh, (uint32_t)(e >> 48), (uint32_t)(e >> 24) & 0xFFFFFFu, (uint32_t)e & 0xFFFFFFu);
   }
 }
 
 // C(c, 2) off every live u2 of every row, rows cleared.
 __global__ __launch_bounds__(256) void k_vsplit_credit(VSplitArgs a) {
-  const unsigned long long k = *a.k;
+  const unsigned long long k = dev_ld(a.k);
   if (k == KNONE) return;
   const uint32_t nrows = vs_nrows(a);
   for (uint32_t r = 0; r < nrows; r++) {
     uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
     const uint32_t* used = a.used + (size_t)r * (size_t)a.ns;
-    const uint32_t n = vs_load(a.nused + r);
+    const uint32_t n = dev_ld(a.nused + r);
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-      const uint32_t key = vs_load(used + q);
+      const uint32_t key = dev_ld(used + q);
       const uint32_t c = atomicExch(&row[key], 0u);
       const bool hit = c > 1u && a.D[key] > k;
       if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
@@ -596,8 +597,8 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   constexpr int BM_LOG = UCAP >= 2048 ? 13 : 11;
   __shared__ uint32_t nb_bits[(1 << BM_LOG) / 32];
   const uint32_t tid = threadIdx.x;
-  if (*a.k == KNONE) return;
-  const uint32_t nlist = *a.list_len;
+  if (dev_ld(a.k) == KNONE) return;
+  const uint32_t nlist = dev_ld(a.list_len);
   // the first nlist workgroups drain the queue: the others leave without touching its counter
   if (blockIdx.x >= nlist) return;
   while (true) {
@@ -606,7 +607,7 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     const uint32_t idx = sh.item;
     __syncthreads();
     if (idx >= nlist) break;
-    const uint32_t e = a.list[idx];
+    const uint32_t e = dev_ld(a.list + idx);
     const bool from_s = (a.work[e] & WORK_SSIDE) != 0u;
     const uint32_t v = a.owner_o[e], u = a.u_of[e];
     const uint32_t va = from_s ? u : v, vb = from_s ? v : u;
@@ -633,7 +634,7 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     }
     // B[e] counts exactly the butterflies through e whose four edges are live or peeled in
     // this round: once the walk has met that many, nothing is left to find
-    const unsigned long long target = a.B[e];
+    const unsigned long long target = dev_ld(a.B + e);
     volatile uint32_t* found = &sh.nused;
     if (tid == 0) *found = 0;
     __syncthreads();
