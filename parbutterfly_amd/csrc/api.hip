@@ -25,7 +25,7 @@ int check_desc(const pbf_bipartite_csr* g) {
   return PBF_OK;
 }
 
-constexpr size_t STAGE_BYTES = size_t(8) << 20, STAGE_MIN = size_t(1) << 20;
+constexpr size_t STAGE_BYTES = size_t(8) << 20;
 
 int stage_init(pbf_context* ctx) {
   for (int k = 0; k < 2; k++) {
@@ -43,10 +43,7 @@ int stage_wait(pbf_context* ctx, int k) {
 }  // namespace
 
 int pbf_h2d(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
-  if (bytes < STAGE_MIN) {
-    if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    return PBF_OK;
-  }
+  if (bytes == 0) return PBF_OK;
   PBF_CHECK(stage_init(ctx));
   int k = 0;
   for (size_t off = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
@@ -61,8 +58,7 @@ int pbf_h2d(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
 }
 
 int pbf_d2h(pbf_context* ctx, void* dst, const void* src, size_t bytes) {
-  if (bytes < STAGE_MIN) {
-    if (bytes) PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (bytes == 0) {
     PBF_HIP(hipStreamSynchronize(ctx->stream));
     return PBF_OK;
   }
@@ -183,8 +179,8 @@ int count_common(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_
   PBF_CHECK(pbf_dev_count(ctx, dg, use_v, R, mode, static_cast<uint64_t*>(out_dev), eti));
   PBF_HIP(hipEventRecord(ev[4], ctx->stream));
   if (mode == MODE_TOTAL) {
-    PBF_HIP(hipMemcpyAsync(out_user, out_dev, 8, mem == PBF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                           ctx->stream));
+    if (mem == PBF_MEM_DEVICE) PBF_HIP(hipMemcpyAsync(out_user, out_dev, 8, hipMemcpyDeviceToDevice, ctx->stream));
+    else PBF_CHECK(pbf_d2h(ctx, out_user, out_dev, 8));
     PBF_HIP(hipStreamSynchronize(ctx->stream));
   } else {
     PBF_CHECK(finish_output(ctx, mem, out_user, out_dev, out_bytes));
@@ -369,9 +365,13 @@ extern "C" int pbf_rank(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, i
     *wedges = R.wedges;
   }
   const size_t nb = 4 * (size_t)R.n;
-  hipMemcpyKind kind = mem == PBF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  if (order && nb) PBF_HIP(hipMemcpyAsync(order, R.order, nb, kind, ctx->stream));
-  if (rank_of && nb) PBF_HIP(hipMemcpyAsync(rank_of, R.rank_of, nb, kind, ctx->stream));
+  if (mem == PBF_MEM_DEVICE) {
+    if (order && nb) PBF_HIP(hipMemcpyAsync(order, R.order, nb, hipMemcpyDeviceToDevice, ctx->stream));
+    if (rank_of && nb) PBF_HIP(hipMemcpyAsync(rank_of, R.rank_of, nb, hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    if (order) PBF_CHECK(pbf_d2h(ctx, order, R.order, nb));
+    if (rank_of) PBF_CHECK(pbf_d2h(ctx, rank_of, R.rank_of, nb));
+  }
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }

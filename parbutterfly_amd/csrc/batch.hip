@@ -404,7 +404,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
       PBF_HIP(hipStreamSynchronize(ctx->stream));
       memcpy(vert_off, off.data(), 8 * (size_t)(count + 1));
     } else {
-      PBF_HIP(hipMemcpyAsync(vert_off, off.data(), 8 * (size_t)(count + 1), hipMemcpyHostToDevice, ctx->stream));
+      PBF_CHECK(pbf_h2d(ctx, vert_off, off.data(), 8 * (size_t)(count + 1)));
       PBF_HIP(hipStreamSynchronize(ctx->stream));
     }
     ctx->timing.total_ms = elapsed_ms(ev[0], ev[4]);
@@ -467,7 +467,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
       return PBF_OK;
     }
   }
-  PBF_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BGraph) * (size_t)count, hipMemcpyHostToDevice, ctx->stream));
+  PBF_CHECK(pbf_h2d(ctx, dtab, tab.data(), sizeof(BGraph) * (size_t)count));
   PBF_HIP(hipStreamSynchronize(ctx->stream));  // the table's host copy is released below
   PBF_HIP(hipEventRecord(ev[0], ctx->stream));
   PBF_CHECK(union_pass(ctx, dtab, count, NV, NU, M, nmax, rank_type, static_cast<uint32_t*>(up), static_cast<uint8_t*>(fp), sn,
@@ -475,7 +475,7 @@ extern "C" int pbf_count_batch(pbf_context* ctx, int64_t count, const pbf_bipart
   PBF_HIP(hipEventRecord(ev[4], ctx->stream));
   if (out_mem == PBF_MEM_HOST) {
     std::vector<uint64_t> off((size_t)count + 1);
-    PBF_HIP(hipMemcpyAsync(off.data(), offd, 8 * (size_t)(count + 1), hipMemcpyDeviceToHost, ctx->stream));
+    PBF_CHECK(pbf_d2h(ctx, off.data(), offd, 8 * (size_t)(count + 1)));
     PBF_HIP(hipStreamSynchronize(ctx->stream));
     PBF_CHECK(pbf_d2h(ctx, vert_out, vout, 8 * (size_t)off[count]));
     PBF_CHECK(pbf_d2h(ctx, totals, tot, 8 * (size_t)count));
@@ -613,7 +613,7 @@ extern "C" int pbf_batch_features(pbf_context* ctx, int64_t count, const pbf_bip
   unsigned long long* Waco = Wad + count;
   float* F = reinterpret_cast<float*>(Waco + count);
   uint32_t* inb = reinterpret_cast<uint32_t*>(F + (size_t)count * PBF_MLP_IN);
-  PBF_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BGraph) * (size_t)count, hipMemcpyHostToDevice, ctx->stream));
+  PBF_CHECK(pbf_h2d(ctx, dtab, tab.data(), sizeof(BGraph) * (size_t)count));
   DevGraph u;
   u.nv = NV; u.nu = NU; u.m = M;
   uint32_t* offV = static_cast<uint32_t*>(up);
@@ -635,7 +635,7 @@ extern "C" int pbf_batch_features(pbf_context* ctx, int64_t count, const pbf_bip
   }
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(256), 0, ctx->stream, dtab, count, wvu, usev, Wad, Waco, F);
   PBF_HIP(hipGetLastError());
-  PBF_HIP(hipMemcpyAsync(features, F, 4 * (size_t)count * PBF_MLP_IN, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_CHECK(pbf_d2h(ctx, features, F, 4 * (size_t)count * PBF_MLP_IN));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }

@@ -300,23 +300,23 @@ extern "C" int pbf_mlp_train(pbf_context* ctx, float* params, const float* X, co
   // permutations of all epochs, uploaded per epoch
   std::vector<int> perm((size_t)n);
   for (int64_t k = 0; k < n; k++) perm[k] = (int)k;
-  PBF_HIP(hipMemcpyAsync(b.p, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, ctx->stream));
+  PBF_CHECK(pbf_h2d(ctx, b.p, params, sizeof(float) * NPARAM));
   PBF_CHECK(pbf_fill_u32(ctx, reinterpret_cast<uint32_t*>(b.m), 0u, 2 * NPARAM));
-  PBF_HIP(hipMemcpyAsync(b.x, X, sizeof(float) * (size_t)n * IN, hipMemcpyHostToDevice, ctx->stream));
-  PBF_HIP(hipMemcpyAsync(b.t, T, sizeof(float) * (size_t)n * OUT, hipMemcpyHostToDevice, ctx->stream));
+  PBF_CHECK(pbf_h2d(ctx, b.x, X, sizeof(float) * (size_t)n * IN));
+  PBF_CHECK(pbf_h2d(ctx, b.t, T, sizeof(float) * (size_t)n * OUT));
   uint64_t st = seed;
   const int64_t steps = (n + MB - 1) / MB;
   for (int e = 0; e < epochs; e++) {
     for (int64_t k = n - 1; k > 0; k--) std::swap(perm[k], perm[splitmix(st) % (uint64_t)(k + 1)]);
-    PBF_HIP(hipMemcpyAsync(b.idx, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    PBF_CHECK(pbf_h2d(ctx, b.idx, perm.data(), sizeof(int) * (size_t)n));
     hipLaunchKernelGGL(k_mlp_train<true>, dim3(1), dim3(NT), 0, ctx->stream, b.p, b.m, b.v, b.x, b.t, b.idx, n, steps, lr,
                        (int64_t)e * steps, nullptr, b.l);
     PBF_HIP(hipGetLastError());
     PBF_HIP(hipStreamSynchronize(ctx->stream));  // the permutation buffer is reused next epoch
   }
-  PBF_HIP(hipMemcpyAsync(params, b.p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_CHECK(pbf_d2h(ctx, params, b.p, sizeof(float) * NPARAM));
   float l = 0;
-  PBF_HIP(hipMemcpyAsync(&l, b.l, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  PBF_CHECK(pbf_d2h(ctx, &l, b.l, sizeof(float)));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   if (final_loss) *final_loss = l;
   return PBF_OK;
@@ -327,15 +327,15 @@ extern "C" int pbf_mlp_grad(pbf_context* ctx, const float* params, const float* 
   if (!ctx || !params || !X || !T || !grads || n <= 0 || n > MB) return PBF_ERR_ARG;
   DevBufs b;
   PBF_CHECK(bufs(ctx, MB, &b));
-  PBF_HIP(hipMemcpyAsync(b.p, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, ctx->stream));
-  PBF_HIP(hipMemcpyAsync(b.x, X, sizeof(float) * (size_t)n * IN, hipMemcpyHostToDevice, ctx->stream));
-  PBF_HIP(hipMemcpyAsync(b.t, T, sizeof(float) * (size_t)n * OUT, hipMemcpyHostToDevice, ctx->stream));
+  PBF_CHECK(pbf_h2d(ctx, b.p, params, sizeof(float) * NPARAM));
+  PBF_CHECK(pbf_h2d(ctx, b.x, X, sizeof(float) * (size_t)n * IN));
+  PBF_CHECK(pbf_h2d(ctx, b.t, T, sizeof(float) * (size_t)n * OUT));
   hipLaunchKernelGGL(k_mlp_train<false>, dim3(1), dim3(NT), 0, ctx->stream, b.p, b.m, b.v, b.x, b.t, nullptr, n, (int64_t)1,
                      0.f, (int64_t)0, b.g, b.l);
   PBF_HIP(hipGetLastError());
-  PBF_HIP(hipMemcpyAsync(grads, b.g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_CHECK(pbf_d2h(ctx, grads, b.g, sizeof(float) * NPARAM));
   float l = 0;
-  PBF_HIP(hipMemcpyAsync(&l, b.l, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  PBF_CHECK(pbf_d2h(ctx, &l, b.l, sizeof(float)));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   if (loss) *loss = l;
   return PBF_OK;
@@ -346,12 +346,12 @@ extern "C" int pbf_mlp_predict(pbf_context* ctx, const float* params, const floa
   if (n == 0) return PBF_OK;
   DevBufs b;
   PBF_CHECK(bufs(ctx, n, &b));
-  PBF_HIP(hipMemcpyAsync(b.p, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, ctx->stream));
-  PBF_HIP(hipMemcpyAsync(b.x, X, sizeof(float) * (size_t)n * IN, hipMemcpyHostToDevice, ctx->stream));
+  PBF_CHECK(pbf_h2d(ctx, b.p, params, sizeof(float) * NPARAM));
+  PBF_CHECK(pbf_h2d(ctx, b.x, X, sizeof(float) * (size_t)n * IN));
   const unsigned blocks = (unsigned)std::min<int64_t>((n + MB - 1) / MB, (int64_t)ctx->num_cus * 4);
   hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(NT), 0, ctx->stream, b.p, b.x, n, b.t);
   PBF_HIP(hipGetLastError());
-  PBF_HIP(hipMemcpyAsync(Y, b.t, sizeof(float) * (size_t)n * OUT, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_CHECK(pbf_d2h(ctx, Y, b.t, sizeof(float) * (size_t)n * OUT));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
   return PBF_OK;
 }

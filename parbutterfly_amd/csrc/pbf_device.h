@@ -131,9 +131,10 @@ struct RankedGraph {
   uint32_t* lo = nullptr;
 };
 
-// api.hip: host <-> device copies on the context's stream through pinned staging (large
-// transfers run at DMA speed instead of the pageable path's).  pbf_h2d returns once the host
-// source has been read; pbf_d2h returns once the host destination is written.
+// api.hip: host <-> device copies on the context's stream, always through the context's pinned
+// staging buffers (host memory is never handed to the runtime's pageable-memory copy path;
+// large transfers run at DMA speed).  pbf_h2d returns once the host source has been read;
+// pbf_d2h returns once the host destination is written.
 int pbf_h2d(pbf_context* ctx, void* dst, const void* src, size_t bytes);
 int pbf_d2h(pbf_context* ctx, void* dst, const void* src, size_t bytes);
 
