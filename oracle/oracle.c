@@ -17,6 +17,10 @@
  *                         compiled reference's SERIAL and BATCHS modes.
  *   orc_peel_vert/edge  - textbook one-at-a-time peeling (orc_brute_peel_*), K_{a,b} closed
  *                         forms, golden vectors from the compiled reference's PBATCHS mode.
+ *   orc_sparsify        - the graphs the compiled reference's eSparseBipartite /
+ *                         clrSparseBipartite (+ delZeroDeg) write for 40 (graph, sparsifier,
+ *                         denom, seed) cases, array for array; one colour keeps every edge;
+ *                         an independent numpy restatement of the colour rule.
  *
  * Graph convention (as the reference's bipartiteCSR, butterfly_utils.h:184-197): V has nv
  * vertices, U has nu; offsetsV[nv+1]/edgesV[m] list U-neighbours of V vertices, offsetsU[nu+1]/
