@@ -53,10 +53,9 @@ struct Ctl {
   unsigned long long* kprev;
   uint32_t* len;    // [0] small, [1] large, [2] ovf, [3] ntouched, [4] previous ntouched, [5] found,
                     // [6] split (reservations; the list holds at most its capacity)
-  uint32_t* queue;  // [0] small, [1] large, [2] ovf, [3..5] split walk counters,
-                    // [8..8+VS_MAX_ROWS) used-key counts of the split rows; all reset per round
+  uint32_t* queue;  // [0] small, [1] large, [2] ovf, [3..5] split walk counters; all reset per round
 };
-constexpr int VS_MAX_ROWS = 32, CTL_QUEUES = 8 + VS_MAX_ROWS;
+constexpr int VS_MAX_ROWS = 32, CTL_QUEUES = 8;
 
 // The split class of the vertex peel: frontier items with at least min_paths two-hop paths, or
 // more possible distinct two-hop neighbours (work) than max_work, are walked by many
@@ -394,8 +393,9 @@ __global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict_
 // A frontier vertex u with many two-hop paths (the late rounds of a power-law graph peel one
 // such vertex at a time) is walked by the whole device: its centres v in N(u) are cut into
 // chunks of VS_CHUNK, a chunk's paths into units of VS_PCH, and workgroups take units; the
-// counts c(u, u2) go to a dense row of u in HBM (global atomics), the keys met to the row's
-// used list; k_vsplit_credit then subtracts C(c, 2) from every live u2 and clears the row.
+// counts c(u, u2) go to a dense row of u in HBM (global atomics whose results are not waited
+// for), and every counted key sets the flag of its block of 64 keys; k_vsplit_credit then
+// sweeps the flagged blocks, subtracts C(c, 2) from every live u2 and clears rows and flags.
 constexpr uint32_t VS_CHUNK = 256, VS_PCH = 1024, VS_MIN_PATHS = 1u << 15;
 
 struct VSplitArgs {
@@ -408,11 +408,12 @@ struct VSplitArgs {
   const uint32_t* len;      // reservations of the list (rows = min(*len, cap))
   uint32_t cap;
   uint32_t* rows;           // per row: dense counters [ns], zero between rounds
-  uint32_t* used;           // per row: keys with a non-zero count [ns]
-  uint32_t* nused;          // per row (reset every round with the round's counters)
-  unsigned long long* sub;  // remaining units: (row << 48) | (chunk << 24) | unit
+  uint32_t* flags;          // per row: one word per block of 64 keys [nblk], zero between rounds
+  uint32_t nblk;
+  unsigned long long* sub;  // units after each chunk's first: (row << 48) | (chunk << 24) | unit,
+                            // the units of one chunk consecutive
   uint32_t sub_cap;
-  uint32_t* ctr;            // [0] chunk queue, [1] sub-units listed, [2] sub-unit queue
+  uint32_t* ctr;            // [1] units listed
   unsigned long long* B;
   const unsigned long long* D;
   uint32_t* tflag;
@@ -439,11 +440,11 @@ __device__ __forceinline__ void vs_locate(const VSplitArgs& a, uint32_t nrows, u
   *row = nrows; *chunk = 0;
 }
 
-// Paths of unit s of chunk c of row r, counted into the row.  The chunk's centres are scanned
-// into sh (one per thread); returns the chunk's path total.
-__device__ uint64_t vs_unit(const VSplitArgs& a, TeamShared<256>& sh, uint32_t r, uint32_t c, uint32_t s) {
+// Chunk c of u: the inclusive prefix of its centres' path counts into sh.P and their O-list
+// starts into sh.vs (one centre per thread).  Returns the chunk's path total, which is below
+// 2^32: the centres are distinct, so their paths are distinct edges.
+__device__ uint32_t vs_chunk(const VSplitArgs& a, TeamShared<256>& sh, uint32_t u, uint32_t c) {
   const uint32_t tid = threadIdx.x;
-  const uint32_t u = dev_ld(a.list + r);
   const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
   const uint32_t j = c * VS_CHUNK + tid;
   uint32_t L = 0, vo = 0;
@@ -457,14 +458,21 @@ __device__ uint64_t vs_unit(const VSplitArgs& a, TeamShared<256>& sh, uint32_t r
   sh.P[tid] = incl;
   sh.vs[tid] = vo;
   __syncthreads();
+  return T;
+}
+
+// Paths [s * VS_PCH, min((s + 1) * VS_PCH, T)) of the chunk in sh, counted into row r of u:
+// VS_PCH / 256 paths per thread, all key loads issued before the counter updates, which
+// return nothing (a key's block flag is set by a plain store of 1 instead).
+__device__ __forceinline__ void vs_count(const VSplitArgs& a, const TeamShared<256>& sh, uint32_t r, uint32_t u,
+                                         uint32_t s, uint32_t T) {
+  const uint32_t tid = threadIdx.x;
   uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
-  uint32_t* used = a.used + (size_t)r * (size_t)a.ns;
+  uint32_t* flag = a.flags + (size_t)r * a.nblk;
   const uint64_t w0 = (uint64_t)s * VS_PCH;
   const uint32_t wend = (uint32_t)(w0 + VS_PCH < T ? w0 + VS_PCH : T);
-  // VS_PCH / 256 paths per thread, all loads and counter updates issued before any result is
-  // used (the updates return the old count: a key's first path appends it to the used list)
   constexpr uint32_t PER = VS_PCH / 256;
-  uint32_t key[PER], old[PER];
+  uint32_t key[PER];
 #pragma unroll
   for (uint32_t q = 0; q < PER; q++) {
     const uint32_t w = (uint32_t)w0 + q * 256 + tid;
@@ -476,16 +484,18 @@ __device__ uint64_t vs_unit(const VSplitArgs& a, TeamShared<256>& sh, uint32_t r
     }
   }
 #pragma unroll
-  for (uint32_t q = 0; q < PER; q++) old[q] = key[q] != u ? atomicAdd(&row[key[q]], 1u) : 1u;
-#pragma unroll
-  for (uint32_t q = 0; q < PER; q++) wave_append(old[q] == 0u, key[q], used, &a.nused[r]);
-  __syncthreads();
-  return T;
+  for (uint32_t q = 0; q < PER; q++) {
+    if (key[q] != u) {
+      atomicAdd(&row[key[q]], 1u);
+      flag[key[q] >> 6] = 1u;
+    }
+  }
 }
 
-// Phase 1: workgroups take whole chunks; a chunk with more than VS_PCH paths keeps its first
-// unit and lists the others for phase 2.  The list has room for every unit of a round: a row
-// has at most m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
+// Phase 1: workgroups take whole chunks (chunk g, g + grid, ...) and count their first unit;
+// a chunk with more than VS_PCH paths reserves consecutive room for its other units and lists
+// them for phase 2.  The list has room for every unit of a round: a row has at most
+// m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
 __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
   __shared__ TeamShared<256> sh;
   if (dev_ld(a.k) == KNONE) return;
@@ -496,57 +506,75 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
     const uint32_t u = dev_ld(a.list + r);
     total += (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
   }
-  if (blockIdx.x >= total) return;
-  while (true) {
-    if (threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[0], 1u);
-    __syncthreads();
-    const uint32_t g = sh.item;
-    __syncthreads();
-    if (g >= total) break;
+  for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
     uint32_t r, c;
     vs_locate(a, nrows, g, &r, &c);
-    const uint64_t T = vs_unit(a, sh, r, c, 0);
-    const uint32_t nunits = (uint32_t)((T + VS_PCH - 1) / VS_PCH);
+    const uint32_t u = dev_ld(a.list + r);
+    const uint32_t T = vs_chunk(a, sh, u, c);
+    const uint32_t nunits = (uint32_t)(((uint64_t)T + VS_PCH - 1) / VS_PCH);
+    if (nunits > 1 && threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[1], nunits - 1);
+    vs_count(a, sh, r, u, 0, T);
+    __syncthreads();
+    const uint32_t base = sh.item;
     for (uint32_t s = 1 + threadIdx.x; s < nunits; s += 256) {
-      const uint32_t pos = atomicAdd(&a.ctr[1], 1u);
+      const uint32_t pos = base + s - 1;
       if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 48) | ((unsigned long long)c << 24) | s;
     }
   }
 }
 
+// Phase 2: workgroup b takes the listed units [b * per, (b + 1) * per); consecutive units of
+// one chunk reuse the chunk's prefix already in LDS, and the next list entry is loaded while
+// the current unit runs.
 __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
   __shared__ TeamShared<256> sh;
   if (dev_ld(a.k) == KNONE) return;
   const uint32_t listed = dev_ld(a.ctr + 1);
   const uint32_t n = listed < a.sub_cap ? listed : a.sub_cap;
-  if (blockIdx.x >= n) return;
-  while (true) {
-    if (threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[2], 1u);
-    __syncthreads();
-    const uint32_t q = sh.item;
-    __syncthreads();
-    if (q >= n) break;
-    const unsigned long long e = dev_ld(a.sub + q);
-    vs_unit(a, sh, (uint32_t)(e >> 48), (uint32_t)(e >> 24) & 0xFFFFFFu, (uint32_t)e & 0xFFFFFFu);
+  const uint32_t per = (uint32_t)(((uint64_t)n + gridDim.x - 1) / gridDim.x);
+  const uint64_t q0 = (uint64_t)blockIdx.x * per;
+  const uint64_t q1 = q0 + per < n ? q0 + per : n;
+  if (q0 >= q1) return;
+  uint32_t cr = ~0u, cc = ~0u, u = 0, T = 0;
+  unsigned long long e = dev_ld(a.sub + q0);
+  for (uint64_t q = q0; q < q1; q++) {
+    const unsigned long long en = q + 1 < q1 ? dev_ld(a.sub + q + 1) : 0ull;
+    const uint32_t r = (uint32_t)(e >> 48), c = (uint32_t)(e >> 24) & 0xFFFFFFu;
+    if (r != cr || c != cc) {
+      u = dev_ld(a.list + r);
+      T = vs_chunk(a, sh, u, c);
+      cr = r;
+      cc = c;
+    }
+    vs_count(a, sh, r, u, (uint32_t)e & 0xFFFFFFu, T);
+    e = en;
   }
 }
 
-// C(c, 2) off every live u2 of every row, rows cleared.
+// The flagged blocks of every row, one wavefront per block: C(c, 2) off every live u2, the
+// counters and the flag cleared.
 __global__ __launch_bounds__(256) void k_vsplit_credit(VSplitArgs a) {
   const unsigned long long k = dev_ld(a.k);
   if (k == KNONE) return;
   const uint32_t nrows = vs_nrows(a);
-  for (uint32_t r = 0; r < nrows; r++) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t nb = (uint64_t)nrows * a.nblk;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / 64u);
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64u; t < nb; t += nwaves) {
+    uint32_t* fl = a.flags + t;  // row-major: t = r * nblk + block
+    if (dev_ld(fl) == 0u) continue;
+    const uint32_t r = (uint32_t)(t / a.nblk);
+    const uint32_t key = (uint32_t)(t - (uint64_t)r * a.nblk) * 64u + lane;
     uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
-    const uint32_t* used = a.used + (size_t)r * (size_t)a.ns;
-    const uint32_t n = dev_ld(a.nused + r);
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-      const uint32_t key = dev_ld(used + q);
-      const uint32_t c = atomicExch(&row[key], 0u);
-      const bool hit = c > 1u && a.D[key] > k;
-      if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
-      wave_append(hit && atomicExch(&a.tflag[key], 1u) == 0u, key, a.touched, a.ntouched);
+    uint32_t c = 0;
+    if (key < (uint64_t)a.ns) {
+      c = dev_ld(row + key);
+      if (c != 0u) __hip_atomic_store(row + key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    const bool hit = c > 1u && a.D[key] > k;
+    if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
+    wave_append(hit && atomicExch(&a.tflag[key], 1u) == 0u, key, a.touched, a.ntouched);
+    if (lane == 0) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -873,24 +901,25 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   uint32_t* large_list = pb.lists + ns;
   uint32_t* ovf_list = pb.lists + 2 * ns;
   const unsigned sel_grid = grid_for(ns, 256, 4096);
-  // split walks: up to VS rows per round, each a dense row and a used list of ns words
+  // split walks: up to vrows rows per round, each a dense row of ns words and its block flags
   const int64_t no = use_v ? g.nv : g.nu;
   const uint32_t vrows = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(VS_MAX_ROWS, (int64_t(1) << 31) / (8 * ns)));
   const uint32_t sub_cap = vrows * (uint32_t)(g.m / VS_PCH + no / VS_CHUNK + 2);
+  const uint32_t nblk = (uint32_t)((ns + 63) / 64);
   VSplitArgs vs{};
   {
-    const size_t words = 2 * (size_t)vrows * (size_t)ns + (size_t)vrows;
+    const size_t words = (size_t)vrows * ((size_t)ns + nblk) + (size_t)vrows;
     void* sp;
     PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words + 8 * (size_t)sub_cap + 64, &sp));
     uint32_t* w = static_cast<uint32_t*>(sp);
     vs.rows = w;
-    vs.used = w + (size_t)vrows * ns;
-    uint32_t* vlist = vs.used + (size_t)vrows * ns;
+    vs.flags = w + (size_t)vrows * ns;
+    uint32_t* vlist = vs.flags + (size_t)vrows * nblk;
     vs.sub = reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(vlist + vrows + 2) & ~uintptr_t(7));
-    // the dense rows start at zero; every round's credit pass leaves them so
-    PBF_CHECK(pbf_fill_u32(ctx, vs.rows, 0u, (int64_t)vrows * ns));
+    // rows and flags start at zero; every round's credit pass leaves them so
+    PBF_CHECK(pbf_fill_u32(ctx, vs.rows, 0u, (int64_t)vrows * ((int64_t)ns + nblk)));
     vs.list = vlist;
-    vs.nused = pb.c.queue + 8;
+    vs.nblk = nblk;
   }
   vs.offS = a.offS; vs.edgS = a.edgS; vs.offO = a.offO; vs.edgO = a.edgO; vs.ns = ns;
   vs.len = pb.c.len + 6;
