@@ -551,30 +551,39 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
   }
 }
 
-// The flagged blocks of every row, one wavefront per block: C(c, 2) off every live u2, the
-// counters and the flag cleared.
+// One wavefront per block of 64 keys, over all rows at once: lane r < cap reads row r's flag
+// of the block (rows past the round's count are never flagged), every lane its key's D and
+// tflag, all before any of them is used; then C(c_r, 2) summed over the flagged rows comes
+// off B of every live key, and the counters and flags are cleared.  Each key has one lane in
+// the launch, so tflag (set by this round's walks, or here) needs no exchange.
 __global__ __launch_bounds__(256) void k_vsplit_credit(VSplitArgs a) {
-  const unsigned long long k = dev_ld(a.k);
-  if (k == KNONE) return;
-  const uint32_t nrows = vs_nrows(a);
   const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t nb = (uint64_t)nrows * a.nblk;
   const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / 64u);
-  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64u; t < nb; t += nwaves) {
-    uint32_t* fl = a.flags + t;  // row-major: t = r * nblk + block
-    if (dev_ld(fl) == 0u) continue;
-    const uint32_t r = (uint32_t)(t / a.nblk);
-    const uint32_t key = (uint32_t)(t - (uint64_t)r * a.nblk) * 64u + lane;
-    uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
-    uint32_t c = 0;
-    if (key < (uint64_t)a.ns) {
-      c = dev_ld(row + key);
+  for (uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64u; b < a.nblk; b += nwaves) {
+    const unsigned long long k = dev_ld(a.k);
+    uint32_t* fl = a.flags + (size_t)lane * a.nblk + b;
+    const uint32_t f = lane < a.cap ? dev_ld(fl) : 0u;
+    const uint32_t key = (uint32_t)b * 64u + lane;
+    const bool in = key < (uint64_t)a.ns;
+    const unsigned long long d = in ? a.D[key] : 0ull;
+    const uint32_t tf = in ? a.tflag[key] : 1u;
+    uint64_t fm = __ballot(f != 0u);
+    if (fm == 0ull) continue;
+    if (f != 0u) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long sub = 0;
+    while (fm) {
+      const uint32_t r = (uint32_t)__ffsll((long long)fm) - 1u;
+      fm &= fm - 1ull;
+      uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+      const uint32_t c = in ? dev_ld(row + key) : 0u;
       if (c != 0u) __hip_atomic_store(row + key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sub += (unsigned long long)c * (c - (c != 0u)) / 2u;
     }
-    const bool hit = c > 1u && a.D[key] > k;
-    if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
-    wave_append(hit && atomicExch(&a.tflag[key], 1u) == 0u, key, a.touched, a.ntouched);
-    if (lane == 0) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool hit = k != KNONE && sub != 0ull && d > k;
+    if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)sub));
+    const bool add = hit && tf == 0u;
+    if (add) a.tflag[key] = 1u;
+    wave_append(add, key, a.touched, a.ntouched);
   }
 }
 
@@ -921,6 +930,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     vs.list = vlist;
     vs.nblk = nblk;
   }
+  const unsigned credit_grid = grid_for(nblk, 4, ctx->num_cus * 8);  // a wavefront per block
   vs.offS = a.offS; vs.edgS = a.edgS; vs.offO = a.offO; vs.edgO = a.edgO; vs.ns = ns;
   vs.len = pb.c.len + 6;
   vs.cap = vrows;
@@ -949,7 +959,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>), dim3(V_HBM_BLOCKS), dim3(V_LARGE_TEAM), 0, ctx->stream, s);
     hipLaunchKernelGGL(k_vsplit_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
-    hipLaunchKernelGGL(k_vsplit_credit, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, vs);
+    hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_update, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
                        pb.c);
     hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, nullptr, ns, pb.c);
