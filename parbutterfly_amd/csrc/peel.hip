@@ -555,35 +555,56 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
 // of the block (rows past the round's count are never flagged), every lane its key's D and
 // tflag, all before any of them is used; then C(c_r, 2) summed over the flagged rows comes
 // off B of every live key, and the counters and flags are cleared.  Each key has one lane in
-// the launch, so tflag (set by this round's walks, or here) needs no exchange.
-__global__ __launch_bounds__(256) void k_vsplit_credit(VSplitArgs a) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / 64u);
-  for (uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64u; b < a.nblk; b += nwaves) {
-    const unsigned long long k = dev_ld(a.k);
-    uint32_t* fl = a.flags + (size_t)lane * a.nblk + b;
-    const uint32_t f = lane < a.cap ? dev_ld(fl) : 0u;
+// the launch, so tflag (set by this round's walks, or here) needs no exchange.  The touched
+// keys of a workgroup's 16 blocks take one counter atomic (mid-run rounds touch most of the
+// side, and per-wavefront atomics on the one counter serialise).
+constexpr uint32_t VS_CREDIT_WAVES = 16;
+__global__ __launch_bounds__(64 * VS_CREDIT_WAVES) void k_vsplit_credit(VSplitArgs a) {
+  __shared__ uint32_t s_cnt[VS_CREDIT_WAVES];
+  __shared__ uint32_t s_base;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  const unsigned long long k = dev_ld(a.k);
+  for (uint64_t b0 = (uint64_t)blockIdx.x * VS_CREDIT_WAVES; b0 < a.nblk; b0 += (uint64_t)gridDim.x * VS_CREDIT_WAVES) {
+    const uint64_t b = b0 + wid;
     const uint32_t key = (uint32_t)b * 64u + lane;
-    const bool in = key < (uint64_t)a.ns;
-    const unsigned long long d = in ? a.D[key] : 0ull;
-    const uint32_t tf = in ? a.tflag[key] : 1u;
-    uint64_t fm = __ballot(f != 0u);
-    if (fm == 0ull) continue;
-    if (f != 0u) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long sub = 0;
-    while (fm) {
-      const uint32_t r = (uint32_t)__ffsll((long long)fm) - 1u;
-      fm &= fm - 1ull;
-      uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
-      const uint32_t c = in ? dev_ld(row + key) : 0u;
-      if (c != 0u) __hip_atomic_store(row + key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      sub += (unsigned long long)c * (c - (c != 0u)) / 2u;
+    bool add = false;
+    if (b < a.nblk) {
+      uint32_t* fl = a.flags + (size_t)lane * a.nblk + b;
+      const uint32_t f = lane < a.cap ? dev_ld(fl) : 0u;
+      const bool in = key < (uint64_t)a.ns;
+      const unsigned long long d = in ? a.D[key] : 0ull;
+      const uint32_t tf = in ? a.tflag[key] : 1u;
+      uint64_t fm = __ballot(f != 0u);
+      if (fm != 0ull) {
+        if (f != 0u) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long sub = 0;
+        while (fm) {
+          const uint32_t r = (uint32_t)__ffsll((long long)fm) - 1u;
+          fm &= fm - 1ull;
+          uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+          const uint32_t c = in ? dev_ld(row + key) : 0u;
+          if (c != 0u) __hip_atomic_store(row + key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          sub += (unsigned long long)c * (c - (c != 0u)) / 2u;
+        }
+        const bool hit = k != KNONE && sub != 0ull && d > k;
+        if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)sub));
+        add = hit && tf == 0u;
+        if (add) a.tflag[key] = 1u;
+      }
     }
-    const bool hit = k != KNONE && sub != 0ull && d > k;
-    if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)sub));
-    const bool add = hit && tf == 0u;
-    if (add) a.tflag[key] = 1u;
-    wave_append(add, key, a.touched, a.ntouched);
+    const uint64_t am = __ballot(add);
+    if (lane == 0) s_cnt[wid] = (uint32_t)__popcll(am);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (uint32_t w = 0; w < VS_CREDIT_WAVES; w++) t += s_cnt[w];
+      s_base = t ? atomicAdd(a.ntouched, t) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = s_base;
+    for (uint32_t w = 0; w < wid; w++) off += s_cnt[w];
+    if (add) a.touched[off + __popcll(am & pbf_lanemask_lt())] = key;
+    __syncthreads();
   }
 }
 
@@ -930,7 +951,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     vs.list = vlist;
     vs.nblk = nblk;
   }
-  const unsigned credit_grid = grid_for(nblk, 4, ctx->num_cus * 8);  // a wavefront per block
+  const unsigned credit_grid = grid_for(nblk, VS_CREDIT_WAVES, ctx->num_cus * 2);  // a wavefront per block
   vs.offS = a.offS; vs.edgS = a.edgS; vs.offO = a.offO; vs.edgO = a.edgO; vs.ns = ns;
   vs.len = pb.c.len + 6;
   vs.cap = vrows;
@@ -959,7 +980,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>), dim3(V_HBM_BLOCKS), dim3(V_LARGE_TEAM), 0, ctx->stream, s);
     hipLaunchKernelGGL(k_vsplit_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
-    hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(256), 0, ctx->stream, vs);
+    hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(64 * VS_CREDIT_WAVES), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_update, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
                        pb.c);
     hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, nullptr, ns, pb.c);
