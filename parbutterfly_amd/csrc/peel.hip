@@ -213,8 +213,11 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
 }
 
 // Touched live items: D = max(B, k) (updateBuckets, butterfly_putils.h:1036-1043).
-__global__ void k_update(const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
-                         const unsigned long long* __restrict__ B, unsigned long long* __restrict__ D, Ctl c) {
+// End of a round, one launch for two independent passes: touched items get D[x] = max(B[x], k)
+// (and lose their flag), current items (state 2) die.  A touched item is live, never current.
+__global__ void k_update_retire(const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
+                                const unsigned long long* __restrict__ B, unsigned long long* __restrict__ D,
+                                uint8_t* __restrict__ state, const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
   const unsigned long long k = dev_ld(c.k);
   const uint32_t nt = dev_ld(c.len + 3);
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
@@ -223,11 +226,7 @@ __global__ void k_update(const uint32_t* __restrict__ touched, uint32_t* __restr
     const unsigned long long b = B[x];
     D[x] = b > k ? b : k;
   }
-}
-
-// Current items die at the end of the round.
-__global__ void k_retire(uint8_t* __restrict__ state, const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
-  if (dev_ld(c.k) == KNONE) return;
+  if (k == KNONE) return;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
     if (state[x] == 2) state[x] = 0;
@@ -981,9 +980,8 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL(k_vsplit_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(64 * VS_CREDIT_WAVES), 0, ctx->stream, vs);
-    hipLaunchKernelGGL(k_update, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B, pb.D,
-                       pb.c);
-    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, nullptr, ns, pb.c);
+    hipLaunchKernelGGL(k_update_retire, dim3(std::max(grid_for(ns, 256, 2048), sel_grid)), dim3(256), 0, ctx->stream,
+                       pb.touched, pb.tflag, pb.B, pb.D, pb.state, nullptr, ns, pb.c);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
   };
@@ -1078,9 +1076,8 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     s.list = large_list; s.list_len = pb.c.len + 1; s.queue = pb.c.queue + 1;
     hipLaunchKernelGGL((k_peele_walk<E_LARGE_TEAM, E_UCAP>), dim3(ctx->num_cus * E_LARGE_PER_CU), dim3(E_LARGE_TEAM), 0,
                        ctx->stream, s);
-    hipLaunchKernelGGL(k_update, dim3(grid_for(entries, 256, 4096)), dim3(256), 0, ctx->stream, pb.touched, pb.tflag, pb.B,
-                       pb.D, pb.c);
-    hipLaunchKernelGGL(k_retire, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.state, cur.idO, entries, pb.c);
+    hipLaunchKernelGGL(k_update_retire, dim3(std::max(grid_for(entries, 256, 4096), sel_grid)), dim3(256), 0,
+                       ctx->stream, pb.touched, pb.tflag, pb.B, pb.D, pb.state, cur.idO, entries, pb.c);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
   };
