@@ -875,22 +875,38 @@ int peel_bufs(pbf_context* ctx, int64_t n, PeelBufs* pb) {
   return PBF_OK;
 }
 
-// Runs rounds in batches until no live item remains; returns the number of rounds.
+// Runs rounds in batches until no live item remains; returns the number of rounds up to the
+// batch after which none remained.  One batch is queued ahead of the one whose live count the
+// host waits for (each batch copies the count to its own pinned word and records its own
+// event), so the device does not idle while the host checks; rounds queued after the last
+// item has gone find no minimum and do nothing.
 template <class RoundF>
 int run_rounds(pbf_context* ctx, PeelBufs& pb, int64_t n, RoundF round, int64_t* rounds_out) {
   ctx->pinned[33] = (uint64_t)n;
   PBF_HIP(hipMemcpyAsync(pb.c.live, ctx->pinned + 33, 8, hipMemcpyHostToDevice, ctx->stream));
-  int64_t rounds = 0;
   // every round peels at least one live item, so n rounds always suffice
-  while (rounds <= n) {
+  const int64_t max_batches = n / ROUND_BATCH + 1;
+  auto issue = [&](int64_t b) -> int {
     for (int r = 0; r < ROUND_BATCH; r++) PBF_CHECK(round());
-    rounds += ROUND_BATCH;
-    PBF_HIP(hipMemcpyAsync(ctx->pinned + 32, pb.c.live, 8, hipMemcpyDeviceToHost, ctx->stream));
-    PBF_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->pinned[32] == 0) break;
+    PBF_HIP(hipMemcpyAsync(ctx->pinned + 40 + (b & 1), pb.c.live, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PBF_HIP(hipEventRecord(ctx->ev[6 + (b & 1)], ctx->stream));
+    return PBF_OK;
+  };
+  PBF_CHECK(issue(0));
+  int64_t issued = 1;
+  int st = PBF_ERR_DOMAIN;
+  for (int64_t b = 0; b < max_batches; b++) {
+    if (issued == b + 1 && issued < max_batches) PBF_CHECK(issue(issued++));
+    PBF_HIP(hipEventSynchronize(ctx->ev[6 + (b & 1)]));
+    if (ctx->pinned[40 + (b & 1)] == 0) {
+      *rounds_out = (b + 1) * ROUND_BATCH;
+      st = PBF_OK;
+      break;
+    }
   }
-  *rounds_out = rounds;
-  return ctx->pinned[32] == 0 ? PBF_OK : PBF_ERR_DOMAIN;
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  if (st != PBF_OK) *rounds_out = issued * ROUND_BATCH;
+  return st;
 }
 }  // namespace
 
