@@ -409,9 +409,11 @@ struct VSplitArgs {
   uint32_t* rows;           // per row: dense counters [ns], zero between rounds
   uint32_t* flags;          // per row: one word per block of 64 keys [nblk], zero between rounds
   uint32_t nblk;
-  unsigned long long* sub;  // units after each chunk's first: (row << 48) | (chunk << 24) | unit,
+  unsigned long long* sub;  // units after each chunk's first: (row << 59) | (global chunk << 22) | unit,
                             // the units of one chunk consecutive
   uint32_t sub_cap;
+  uint2* pre;               // per global chunk g < pre_cap: the chunk's (prefix, O-list start) pairs
+  uint32_t pre_cap;
   uint32_t* ctr;            // [1] units listed
   unsigned long long* B;
   const unsigned long long* D;
@@ -442,7 +444,7 @@ __device__ __forceinline__ void vs_locate(const VSplitArgs& a, uint32_t nrows, u
 // Chunk c of u: the inclusive prefix of its centres' path counts into sh.P and their O-list
 // starts into sh.vs (one centre per thread).  Returns the chunk's path total, which is below
 // 2^32: the centres are distinct, so their paths are distinct edges.
-__device__ uint32_t vs_chunk(const VSplitArgs& a, TeamShared<256>& sh, uint32_t u, uint32_t c) {
+__device__ uint32_t vs_chunk(const VSplitArgs& a, TeamShared<256>& sh, uint32_t u, uint32_t c, uint2* keep) {
   const uint32_t tid = threadIdx.x;
   const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
   const uint32_t j = c * VS_CHUNK + tid;
@@ -456,8 +458,21 @@ __device__ uint32_t vs_chunk(const VSplitArgs& a, TeamShared<256>& sh, uint32_t 
   const uint32_t incl = team_incl_scan<256>(L, sh, &T);
   sh.P[tid] = incl;
   sh.vs[tid] = vo;
+  if (keep) keep[tid] = make_uint2(incl, vo);
   __syncthreads();
   return T;
+}
+
+// The stored prefix of global chunk g (written by k_vsplit_chunks) into sh; returns its total.
+__device__ uint32_t vs_chunk_stored(const VSplitArgs& a, TeamShared<256>& sh, uint32_t g) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(a.pre + (size_t)g * VS_CHUNK + tid);
+  const uint32_t incl = dev_ld(q), vo = dev_ld(q + 1);
+  __syncthreads();  // every thread is done with the previous chunk
+  sh.P[tid] = incl;
+  sh.vs[tid] = vo;
+  __syncthreads();
+  return sh.P[VS_CHUNK - 1];
 }
 
 // Paths [s * VS_PCH, min((s + 1) * VS_PCH, T)) of the chunk in sh, counted into row r of u:
@@ -509,7 +524,7 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
     uint32_t r, c;
     vs_locate(a, nrows, g, &r, &c);
     const uint32_t u = dev_ld(a.list + r);
-    const uint32_t T = vs_chunk(a, sh, u, c);
+    const uint32_t T = vs_chunk(a, sh, u, c, g < a.pre_cap ? a.pre + (size_t)g * VS_CHUNK : nullptr);
     const uint32_t nunits = (uint32_t)(((uint64_t)T + VS_PCH - 1) / VS_PCH);
     if (nunits > 1 && threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[1], nunits - 1);
     vs_count(a, sh, r, u, 0, T);
@@ -517,14 +532,15 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
     const uint32_t base = sh.item;
     for (uint32_t s = 1 + threadIdx.x; s < nunits; s += 256) {
       const uint32_t pos = base + s - 1;
-      if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 48) | ((unsigned long long)c << 24) | s;
+      if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 59) | ((unsigned long long)g << 22) | s;
     }
   }
 }
 
 // Phase 2: workgroup b takes the listed units [b * per, (b + 1) * per); consecutive units of
-// one chunk reuse the chunk's prefix already in LDS, and the next list entry is loaded while
-// the current unit runs.
+// one chunk reuse the chunk's prefix already in LDS, which is read from the copy phase 1 kept
+// (the first pre_cap chunks of a round) or else scanned again; the next list entry is loaded
+// while the current unit runs.
 __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
   __shared__ TeamShared<256> sh;
   if (dev_ld(a.k) == KNONE) return;
@@ -534,18 +550,24 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
   const uint64_t q0 = (uint64_t)blockIdx.x * per;
   const uint64_t q1 = q0 + per < n ? q0 + per : n;
   if (q0 >= q1) return;
-  uint32_t cr = ~0u, cc = ~0u, u = 0, T = 0;
+  uint32_t cr = ~0u, cg = ~0u, u = 0, T = 0;
   unsigned long long e = dev_ld(a.sub + q0);
   for (uint64_t q = q0; q < q1; q++) {
     const unsigned long long en = q + 1 < q1 ? dev_ld(a.sub + q + 1) : 0ull;
-    const uint32_t r = (uint32_t)(e >> 48), c = (uint32_t)(e >> 24) & 0xFFFFFFu;
-    if (r != cr || c != cc) {
+    const uint32_t r = (uint32_t)(e >> 59), g = (uint32_t)(e >> 22) & 0xFFFFFFFFu;
+    if (r != cr || g != cg) {
       u = dev_ld(a.list + r);
-      T = vs_chunk(a, sh, u, c);
+      if (g < a.pre_cap) {
+        T = vs_chunk_stored(a, sh, g);
+      } else {
+        uint32_t r2, c;
+        vs_locate(a, vs_nrows(a), g, &r2, &c);
+        T = vs_chunk(a, sh, u, c, nullptr);
+      }
       cr = r;
-      cc = c;
+      cg = g;
     }
-    vs_count(a, sh, r, u, (uint32_t)e & 0xFFFFFFu, T);
+    vs_count(a, sh, r, u, (uint32_t)e & 0x3FFFFFu, T);
     e = en;
   }
 }
@@ -951,11 +973,13 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   const uint32_t vrows = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(VS_MAX_ROWS, (int64_t(1) << 31) / (8 * ns)));
   const uint32_t sub_cap = vrows * (uint32_t)(g.m / VS_PCH + no / VS_CHUNK + 2);
   const uint32_t nblk = (uint32_t)((ns + 63) / 64);
+  // kept chunk prefixes: every chunk of a round up to 2^16 of them (2 KB each)
+  const uint32_t pre_cap = (uint32_t)std::min<int64_t>((int64_t)vrows * (no / VS_CHUNK + 2), int64_t(1) << 16);
   VSplitArgs vs{};
   {
     const size_t words = (size_t)vrows * ((size_t)ns + nblk) + (size_t)vrows;
     void* sp;
-    PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words + 8 * (size_t)sub_cap + 64, &sp));
+    PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words + 8 * (size_t)sub_cap + 8 * (size_t)pre_cap * VS_CHUNK + 64, &sp));
     uint32_t* w = static_cast<uint32_t*>(sp);
     vs.rows = w;
     vs.flags = w + (size_t)vrows * ns;
@@ -965,6 +989,8 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     PBF_CHECK(pbf_fill_u32(ctx, vs.rows, 0u, (int64_t)vrows * ((int64_t)ns + nblk)));
     vs.list = vlist;
     vs.nblk = nblk;
+    vs.pre = reinterpret_cast<uint2*>(vs.sub + sub_cap);
+    vs.pre_cap = pre_cap;
   }
   const unsigned credit_grid = grid_for(nblk, VS_CREDIT_WAVES, ctx->num_cus * 2);  // a wavefront per block
   vs.offS = a.offS; vs.edgS = a.edgS; vs.offO = a.offO; vs.edgO = a.edgO; vs.ns = ns;
