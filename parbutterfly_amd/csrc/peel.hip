@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict_
 // counts c(u, u2) go to a dense row of u in HBM (global atomics whose results are not waited
 // for), and every counted key sets the flag of its block of 64 keys; k_vsplit_credit then
 // sweeps the flagged blocks, subtracts C(c, 2) from every live u2 and clears rows and flags.
-constexpr uint32_t VS_CHUNK = 256, VS_PCH = 1024, VS_MIN_PATHS = 1u << 15;
+constexpr uint32_t VS_CHUNK = 256, VS_PCH = 256, VS_MIN_PATHS = 1u << 15;
 
 struct VSplitArgs {
   const uint32_t* offS;
@@ -409,7 +409,7 @@ struct VSplitArgs {
   uint32_t* rows;           // per row: dense counters [ns], zero between rounds
   uint32_t* flags;          // per row: one word per block of 64 keys [nblk], zero between rounds
   uint32_t nblk;
-  unsigned long long* sub;  // units after each chunk's first: (row << 59) | (global chunk << 22) | unit,
+  unsigned long long* sub;  // units after each chunk's first: (row << 59) | (global chunk << 27) | unit,
                             // the units of one chunk consecutive
   uint32_t sub_cap;
   uint2* pre;               // per global chunk g < pre_cap: the chunk's (prefix, O-list start) pairs
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
     const uint32_t base = sh.item;
     for (uint32_t s = 1 + threadIdx.x; s < nunits; s += 256) {
       const uint32_t pos = base + s - 1;
-      if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 59) | ((unsigned long long)g << 22) | s;
+      if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 59) | ((unsigned long long)g << 27) | s;
     }
   }
 }
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
   unsigned long long e = dev_ld(a.sub + q0);
   for (uint64_t q = q0; q < q1; q++) {
     const unsigned long long en = q + 1 < q1 ? dev_ld(a.sub + q + 1) : 0ull;
-    const uint32_t r = (uint32_t)(e >> 59), g = (uint32_t)(e >> 22) & 0xFFFFFFFFu;
+    const uint32_t r = (uint32_t)(e >> 59), g = (uint32_t)(e >> 27) & 0xFFFFFFFFu;
     if (r != cr || g != cg) {
       u = dev_ld(a.list + r);
       if (g < a.pre_cap) {
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
       cr = r;
       cg = g;
     }
-    vs_count(a, sh, r, u, (uint32_t)e & 0x3FFFFFu, T);
+    vs_count(a, sh, r, u, (uint32_t)e & 0x7FFFFFFu, T);
     e = en;
   }
 }
