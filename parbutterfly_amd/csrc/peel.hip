@@ -134,13 +134,16 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
                                  const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
                                  uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
                                  SplitSel ss, Ctl c) {
+  // control words, state and D are loaded before any of them is tested
   const unsigned long long k = dev_ld(c.kprev);
-  if (k == KNONE) return;
   const uint32_t nt = dev_ld(c.len + 4);
+  if (k == KNONE) return;
   uint32_t nsel = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const uint32_t x = touched[t];
-    const bool sel = state[x] == 1 && D[x] == k;
+    const uint8_t st = state[x];
+    const unsigned long long d = D[x];
+    const bool sel = st == 1 && d == k;
     uint32_t w = 0;
     if (sel) {
       state[x] = 2;
@@ -170,7 +173,9 @@ __global__ __launch_bounds__(256) void k_min_live(const unsigned long long* __re
   unsigned long long mn = KNONE;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
-    if (state[x]) { unsigned long long d = D[x]; mn = d < mn ? d : mn; }
+    const uint8_t st = state[x];
+    const unsigned long long d = D[x];  // with the state, not after it
+    if (st) mn = d < mn ? d : mn;
   }
   // one atomic per workgroup: per-wavefront atomics on the one word serialise the launch
   mn = wave_min_u64(mn);
@@ -190,11 +195,14 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
                          uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
                          SplitSel ss, Ctl c) {
   const unsigned long long k = dev_ld(c.k);
-  if (k == KNONE || dev_ld(c.len + 5)) return;
+  const uint32_t found = dev_ld(c.len + 5);
+  if (k == KNONE || found) return;
   uint32_t nsel = 0;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t x = ids ? ids[p] : (uint32_t)p;
-    const bool sel = state[x] == 1 && D[x] == k;
+    const uint8_t st = state[x];
+    const unsigned long long d = D[x];  // with the state, not after it
+    const bool sel = st == 1 && d == k;
     uint32_t w = 0;
     if (sel) {
       state[x] = 2;
