@@ -520,18 +520,23 @@ __device__ __forceinline__ void vs_count(const VSplitArgs& a, const TeamShared<2
 // m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
 __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
   __shared__ TeamShared<256> sh;
-  if (dev_ld(a.k) == KNONE) return;
+  const unsigned long long k = dev_ld(a.k);
   const uint32_t nrows = vs_nrows(a);
-  if (nrows == 0) return;
-  uint32_t total = 0;
+  if (k == KNONE || nrows == 0) return;
+  uint32_t total = 0, u0 = 0, nch0 = 0;
   for (uint32_t r = 0; r < nrows; r++) {
     const uint32_t u = dev_ld(a.list + r);
-    total += (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
+    const uint32_t nch = (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
+    if (r == 0) { u0 = u; nch0 = nch; }
+    total += nch;
   }
   for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-    uint32_t r, c;
-    vs_locate(a, nrows, g, &r, &c);
-    const uint32_t u = dev_ld(a.list + r);
+    // the chunks of row 0 (most rounds have one row) need no search
+    uint32_t r = 0, c = g, u = u0;
+    if (g >= nch0) {
+      vs_locate(a, nrows, g, &r, &c);
+      u = dev_ld(a.list + r);
+    }
     const uint32_t T = vs_chunk(a, sh, u, c, g < a.pre_cap ? a.pre + (size_t)g * VS_CHUNK : nullptr);
     const uint32_t nunits = (uint32_t)(((uint64_t)T + VS_PCH - 1) / VS_PCH);
     if (nunits > 1 && threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[1], nunits - 1);
@@ -551,8 +556,9 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
 // while the current unit runs.
 __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
   __shared__ TeamShared<256> sh;
-  if (dev_ld(a.k) == KNONE) return;
+  const unsigned long long k = dev_ld(a.k);
   const uint32_t listed = dev_ld(a.ctr + 1);
+  if (k == KNONE) return;
   const uint32_t n = listed < a.sub_cap ? listed : a.sub_cap;
   const uint32_t per = (uint32_t)(((uint64_t)n + gridDim.x - 1) / gridDim.x);
   const uint64_t q0 = (uint64_t)blockIdx.x * per;
