@@ -89,3 +89,26 @@ def test_tip_peel_split_walks(ctx):
     v = O.count_vert(h, use_v)
     tip = pb.as_u64(ctx.peel(h, use_v, v))
     assert np.array_equal(tip, O.peel_vert(h, use_v, v))
+
+
+def test_tip_peel_split_round_beyond_kept_prefixes(ctx):
+    # 32 hubs, every pair sharing t degree-2 centres: each hub is in 31 * C(t, 2) butterflies,
+    # all 32 peel in one round as split rows, and that round has 32 * ceil(31 t / 256) = 65,888
+    # chunks, past the 2^16 whose scanned prefixes the chunk phase keeps (the rest are scanned
+    # again by the unit phase)
+    t = 17000
+    hubs = 32
+    I, J = np.triu_indices(hubs, 1)
+    npairs = len(I)
+    centres = np.arange(npairs * t, dtype=np.uint32)
+    v = np.concatenate([np.repeat(I.astype(np.uint32), t), np.repeat(J.astype(np.uint32), t)])
+    u = np.concatenate([centres, centres])
+    g = pb.Graph.from_edges(hubs, npairs * t, v, u)
+    use_v, _ = ctx.side(g)
+    assert not use_v  # counts on the hubs
+    want = (hubs - 1) * (t * (t - 1) // 2)
+    counts = pb.as_u64(ctx.count(g, use_v, "VERT"))
+    assert (counts == want).all()
+    tip = pb.as_u64(ctx.peel(g, use_v, counts))
+    assert (tip == want).all()
+    assert ctx.timing()["rounds"] <= 32  # one batch of rounds: the hubs go in the first
