@@ -241,6 +241,119 @@ __global__ void k_update_retire(const uint32_t* __restrict__ touched, uint32_t* 
   }
 }
 
+// ---------------- vertex peel: minimum of D by blocks of items ----------------
+// The S items are cut into nb <= 256 blocks of 2^bs_log; vmin[b] is the exact minimum of D over
+// the live items of block b at the start of every round.  A round flags the blocks whose items
+// it peeled or touched (plain stores of 1: an atomic min per touched item on 256 words
+// serialised), and the next round's k_vblocks_prepare scans only those blocks again (retiring
+// the peeled items).  So the new k is the minimum of the nb block minima, only the blocks
+// holding it are searched for the frontier, and late rounds, which touch few blocks, scan little.
+struct VBlocks {
+  unsigned long long* vmin;  // [nb]
+  uint32_t* flag;            // [nb] 1: the block had frontier or touched items (or needs its first scan)
+  uint32_t nb;
+  uint32_t bs_log;
+};
+
+// Round start: counters reset (workgroup 0), then every flagged block retires its current items
+// and recomputes its minimum.
+__global__ __launch_bounds__(256) void k_vblocks_prepare(VBlocks vb, uint8_t* __restrict__ state,
+                                                         const unsigned long long* __restrict__ D, int64_t n, Ctl c) {
+  __shared__ unsigned long long s_min[4];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int q = 0; q < 7; q++) c.len[q] = 0;
+    for (int q = 0; q < CTL_QUEUES; q++) c.queue[q] = 0;
+  }
+  const uint32_t b = blockIdx.x;
+  if (dev_ld(vb.flag + b) == 0u) return;
+  const int64_t lo = (int64_t)b << vb.bs_log;
+  const int64_t hi = lo + (int64_t(1) << vb.bs_log) < n ? lo + (int64_t(1) << vb.bs_log) : n;
+  unsigned long long mn = KNONE;
+  for (int64_t x = lo + threadIdx.x; x < hi; x += blockDim.x) {
+    uint8_t st = state[x];
+    const unsigned long long d = D[x];
+    if (st == 2) { state[x] = 0; st = 0; }
+    if (st == 1) mn = d < mn ? d : mn;
+  }
+  mn = wave_min_u64(mn);
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = mn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) mn = s_min[w] < mn ? s_min[w] : mn;
+    __hip_atomic_store(vb.vmin + b, mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(vb.flag + b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Frontier selection: every workgroup reduces the block minima to k (workgroup 0 publishes it);
+// workgroup b then lists the items of block b with D == k when its minimum is k, as k_select
+// does, and flags the block for the next round's scan.
+__global__ __launch_bounds__(256) void k_vblocks_select(VBlocks vb, const unsigned long long* __restrict__ D,
+                                                        const unsigned long long* __restrict__ B, uint8_t* __restrict__ state,
+                                                        int64_t n, const uint32_t* __restrict__ work, uint32_t small_max,
+                                                        uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
+                                                        SplitSel ss, Ctl c) {
+  __shared__ unsigned long long s_min[4];
+  unsigned long long mn = KNONE;
+  for (uint32_t i = threadIdx.x; i < vb.nb; i += blockDim.x) {
+    const unsigned long long v = dev_ld(vb.vmin + i);
+    mn = v < mn ? v : mn;
+  }
+  mn = wave_min_u64(mn);
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = mn;
+  __syncthreads();
+  unsigned long long k = s_min[0];
+  for (int w = 1; w < 4; w++) k = s_min[w] < k ? s_min[w] : k;
+  const uint32_t b = blockIdx.x;
+  if (b == 0 && threadIdx.x == 0) __hip_atomic_store(c.k, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (k == KNONE || dev_ld(vb.vmin + b) != k) return;
+  const int64_t lo = (int64_t)b << vb.bs_log;
+  const int64_t hi = lo + (int64_t(1) << vb.bs_log) < n ? lo + (int64_t(1) << vb.bs_log) : n;
+  uint32_t nsel = 0;
+  // every thread runs the same number of iterations: the appends are per wavefront
+  for (int64_t x0 = lo; x0 < hi; x0 += blockDim.x) {
+    const int64_t xx = x0 + threadIdx.x;
+    const uint32_t x = (uint32_t)xx;
+    bool sel = false;
+    uint32_t w = 0;
+    if (xx < hi) {
+      const uint8_t st = state[x];
+      const unsigned long long d = D[x];
+      sel = st == 1 && d == k;
+      if (sel) {
+        state[x] = 2;
+        nsel++;
+        // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
+        w = B[x] ? (work[x] & WORK_MASK) : 0u;
+      }
+    }
+    bool sp = false;
+    if (ss.paths)
+      sp = wave_append_cap(sel && w != 0 && (w > ss.max_work || ss.paths[x] >= ss.min_paths), x, ss.list, &c.len[6], ss.cap);
+    wave_append(sel && !sp && w != 0 && w <= small_max, x, small_list, &c.len[0]);
+    wave_append(sel && !sp && w > small_max, x, large_list, &c.len[1]);
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(vb.flag + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  nsel = pbf_wave_sum(nsel);
+  if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
+}
+
+// Round end (vertex peel): touched items get D[x] = max(B[x], k), lose their flag and flag their
+// block.  The frontier items are retired by the next round's prepare pass.
+__global__ void k_vupdate(VBlocks vb, const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
+                          const unsigned long long* __restrict__ B, unsigned long long* __restrict__ D, Ctl c) {
+  const unsigned long long k = dev_ld(c.k);
+  const uint32_t nt = dev_ld(c.len + 3);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    const uint32_t x = touched[t];
+    tflag[x] = 0;
+    const unsigned long long b = B[x];
+    const unsigned long long d = b > k ? b : k;
+    D[x] = d;
+    vb.flag[x >> vb.bs_log] = 1u;
+  }
+}
+
 __device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* touched, uint32_t* ntouched) {
   wave_append(atomicExch(&tflag[x], 1u) == 0u, x, touched, ntouched);
 }
@@ -981,7 +1094,6 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   uint32_t* small_list = pb.lists;
   uint32_t* large_list = pb.lists + ns;
   uint32_t* ovf_list = pb.lists + 2 * ns;
-  const unsigned sel_grid = grid_for(ns, 256, 4096);
   // split walks: up to vrows rows per round, each a dense row of ns words and its block flags
   const int64_t no = use_v ? g.nv : g.nu;
   const uint32_t vrows = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(VS_MAX_ROWS, (int64_t(1) << 31) / (8 * ns)));
@@ -990,10 +1102,11 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   // kept chunk prefixes: every chunk of a round up to 2^16 of them (2 KB each)
   const uint32_t pre_cap = (uint32_t)std::min<int64_t>((int64_t)vrows * (no / VS_CHUNK + 2), int64_t(1) << 16);
   VSplitArgs vs{};
+  VBlocks vb{};
   {
     const size_t words = (size_t)vrows * ((size_t)ns + nblk) + (size_t)vrows;
     void* sp;
-    PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words + 8 * (size_t)sub_cap + 8 * (size_t)pre_cap * VS_CHUNK + 64, &sp));
+    PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words + 8 * (size_t)sub_cap + 8 * (size_t)pre_cap * VS_CHUNK + 12 * 256 + 64, &sp));
     uint32_t* w = static_cast<uint32_t*>(sp);
     vs.rows = w;
     vs.flags = w + (size_t)vrows * ns;
@@ -1005,6 +1118,13 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     vs.nblk = nblk;
     vs.pre = reinterpret_cast<uint2*>(vs.sub + sub_cap);
     vs.pre_cap = pre_cap;
+    // the block minima of D: at most 256 blocks of at least 1024 items, all scanned in round 1
+    vb.bs_log = 10;
+    while (((ns - 1) >> vb.bs_log) + 1 > 256) vb.bs_log++;
+    vb.nb = (uint32_t)(((ns - 1) >> vb.bs_log) + 1);
+    vb.vmin = reinterpret_cast<unsigned long long*>(vs.pre + (size_t)pre_cap * VS_CHUNK);
+    vb.flag = reinterpret_cast<uint32_t*>(vb.vmin + 256);
+    PBF_CHECK(pbf_fill_u32(ctx, vb.flag, 1u, vb.nb));
   }
   const unsigned credit_grid = grid_for(nblk, VS_CREDIT_WAVES, ctx->num_cus * 2);  // a wavefront per block
   vs.offS = a.offS; vs.edgS = a.edgS; vs.offO = a.offO; vs.edgO = a.edgO; vs.ns = ns;
@@ -1016,13 +1136,9 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   // beyond the large LDS table's fill limit a walk would overflow into the one-workgroup HBM path
   const SplitSel ss{paths, VS_MIN_PATHS, (V_LARGE_CAP / 4) * 3 - V_LARGE_TEAM, vrows, const_cast<uint32_t*>(vs.list)};
   auto round = [&]() -> int {
-    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
-    hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
-                       pb.work, small_max, small_list, large_list, ss, pb.c);
-    hipLaunchKernelGGL(k_min_live, dim3(grid_for(ns, 1024, 2048)), dim3(256), 0, ctx->stream, pb.D, pb.state, nullptr, ns,
-                       pb.c);
-    hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, nullptr, ns, pb.work, small_max,
-                       small_list, large_list, ss, pb.c);
+    hipLaunchKernelGGL(k_vblocks_prepare, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.state, pb.D, ns, pb.c);
+    hipLaunchKernelGGL(k_vblocks_select, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.D, pb.B, pb.state, ns, pb.work,
+                       small_max, small_list, large_list, ss, pb.c);
     PeelVArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
     s.ovf_list = ovf_list; s.ovf_len = pb.c.len + 2;
@@ -1036,8 +1152,8 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL(k_vsplit_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(64 * VS_CREDIT_WAVES), 0, ctx->stream, vs);
-    hipLaunchKernelGGL(k_update_retire, dim3(std::max(grid_for(ns, 256, 2048), sel_grid)), dim3(256), 0, ctx->stream,
-                       pb.touched, pb.tflag, pb.B, pb.D, pb.state, nullptr, ns, pb.c);
+    hipLaunchKernelGGL(k_vupdate, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, vb, pb.touched, pb.tflag, pb.B,
+                       pb.D, pb.c);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
   };
