@@ -294,19 +294,18 @@ __global__ __launch_bounds__(256) void k_vblocks_select(VBlocks vb, const unsign
                                                         uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
                                                         SplitSel ss, Ctl c) {
   __shared__ unsigned long long s_min[4];
-  unsigned long long mn = KNONE;
-  for (uint32_t i = threadIdx.x; i < vb.nb; i += blockDim.x) {
-    const unsigned long long v = dev_ld(vb.vmin + i);
-    mn = v < mn ? v : mn;
-  }
-  mn = wave_min_u64(mn);
+  __shared__ unsigned long long s_own;
+  // nb <= 256 = the workgroup size: thread i holds block i's minimum
+  const uint32_t b = blockIdx.x;
+  const unsigned long long v = threadIdx.x < vb.nb ? dev_ld(vb.vmin + threadIdx.x) : KNONE;
+  if (threadIdx.x == b) s_own = v;
+  const unsigned long long mn = wave_min_u64(v);
   if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = mn;
   __syncthreads();
   unsigned long long k = s_min[0];
   for (int w = 1; w < 4; w++) k = s_min[w] < k ? s_min[w] : k;
-  const uint32_t b = blockIdx.x;
   if (b == 0 && threadIdx.x == 0) __hip_atomic_store(c.k, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (k == KNONE || dev_ld(vb.vmin + b) != k) return;
+  if (k == KNONE || s_own != k) return;
   const int64_t lo = (int64_t)b << vb.bs_log;
   const int64_t hi = lo + (int64_t(1) << vb.bs_log) < n ? lo + (int64_t(1) << vb.bs_log) : n;
   uint32_t nsel = 0;
