@@ -244,13 +244,13 @@ __global__ void k_update_retire(const uint32_t* __restrict__ touched, uint32_t* 
 // ---------------- vertex peel: minimum of D by blocks of items ----------------
 // The S items are cut into nb <= 256 blocks of 2^bs_log; vmin[b] is the exact minimum of D over
 // the live items of block b at the start of every round.  A round flags the blocks whose items
-// it peeled or touched (plain stores of 1: an atomic min per touched item on 256 words
-// serialised), and the next round's k_vblocks_prepare scans only those blocks again (retiring
-// the peeled items).  So the new k is the minimum of the nb block minima, only the blocks
+// it peeled or decremented (plain stores of 1: an atomic min per decremented item on 256 words
+// serialised), and the next round's k_vblocks_prepare scans only those blocks again: it retires
+// the peeled items and sets D = max(B, k) (updateBuckets) on the live ones.  So the new k is the minimum of the nb block minima, only the blocks
 // holding it are searched for the frontier, and late rounds, which touch few blocks, scan little.
 struct VBlocks {
   unsigned long long* vmin;  // [nb]
-  uint32_t* flag;            // [nb] 1: the block had frontier or touched items (or needs its first scan)
+  uint32_t* flag;            // [nb] 1: the block had frontier or decremented items (or needs its first scan)
   uint32_t nb;
   uint32_t bs_log;
 };
@@ -258,8 +258,10 @@ struct VBlocks {
 // Round start: counters reset (workgroup 0), then every flagged block retires its current items
 // and recomputes its minimum.
 __global__ __launch_bounds__(256) void k_vblocks_prepare(VBlocks vb, uint8_t* __restrict__ state,
-                                                         const unsigned long long* __restrict__ D, int64_t n, Ctl c) {
+                                                         const unsigned long long* __restrict__ B,
+                                                         unsigned long long* __restrict__ D, int64_t n, Ctl c) {
   __shared__ unsigned long long s_min[4];
+  const unsigned long long k = dev_ld(c.k);  // the previous round's level (KNONE before the first)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     for (int q = 0; q < 7; q++) c.len[q] = 0;
     for (int q = 0; q < CTL_QUEUES; q++) c.queue[q] = 0;
@@ -271,9 +273,19 @@ __global__ __launch_bounds__(256) void k_vblocks_prepare(VBlocks vb, uint8_t* __
   unsigned long long mn = KNONE;
   for (int64_t x = lo + threadIdx.x; x < hi; x += blockDim.x) {
     uint8_t st = state[x];
-    const unsigned long long d = D[x];
+    unsigned long long d = D[x];
+    const unsigned long long bx = B[x];
     if (st == 2) { state[x] = 0; st = 0; }
-    if (st == 1) mn = d < mn ? d : mn;
+    if (st == 1) {
+      // D = max(B, k) holds for every live item, decremented this round or not: an untouched
+      // one with B < k has D == k already (D >= k, or it would have been peeled; D was set to
+      // max(B, k') with k' <= k)
+      if (k != KNONE) {
+        const unsigned long long nd = bx > k ? bx : k;
+        if (nd != d) { D[x] = nd; d = nd; }
+      }
+      mn = d < mn ? d : mn;
+    }
   }
   mn = wave_min_u64(mn);
   if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = mn;
@@ -337,22 +349,6 @@ __global__ __launch_bounds__(256) void k_vblocks_select(VBlocks vb, const unsign
   if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
 }
 
-// Round end (vertex peel): touched items get D[x] = max(B[x], k), lose their flag and flag their
-// block.  The frontier items are retired by the next round's prepare pass.
-__global__ void k_vupdate(VBlocks vb, const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
-                          const unsigned long long* __restrict__ B, unsigned long long* __restrict__ D, Ctl c) {
-  const unsigned long long k = dev_ld(c.k);
-  const uint32_t nt = dev_ld(c.len + 3);
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-    const uint32_t x = touched[t];
-    tflag[x] = 0;
-    const unsigned long long b = B[x];
-    const unsigned long long d = b > k ? b : k;
-    D[x] = d;
-    vb.flag[x >> vb.bs_log] = 1u;
-  }
-}
-
 __device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* touched, uint32_t* ntouched) {
   wave_append(atomicExch(&tflag[x], 1u) == 0u, x, touched, ntouched);
 }
@@ -379,6 +375,8 @@ struct PeelVArgs {
   uint32_t* ntouched;
   uint32_t* pool;
   const unsigned long long* k;
+  uint32_t* bflag;          // block flags of the block minima (see VBlocks)
+  uint32_t bs_log;
 };
 
 template <int TEAM, int CAP, bool USED, int KIND>
@@ -470,7 +468,7 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
         }
         if (c > 1u && a.D[key] > k) {
           atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
-          touch(key, a.tflag, a.touched, a.ntouched);
+          a.bflag[key >> a.bs_log] = 1u;  // the next prepare pass updates D of the block
         }
       }
     } else if (tid == 0) {
@@ -537,9 +535,8 @@ struct VSplitArgs {
   uint32_t* ctr;            // [1] units listed
   unsigned long long* B;
   const unsigned long long* D;
-  uint32_t* tflag;
-  uint32_t* touched;
-  uint32_t* ntouched;
+  uint32_t* bflag;          // block flags of the block minima (see VBlocks)
+  uint32_t bs_log;
   const unsigned long long* k;
 };
 
@@ -699,59 +696,38 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
 }
 
 // One wavefront per block of 64 keys, over all rows at once: lane r < cap reads row r's flag
-// of the block (rows past the round's count are never flagged), every lane its key's D and
-// tflag, all before any of them is used; then C(c_r, 2) summed over the flagged rows comes
-// off B of every live key, and the counters and flags are cleared.  Each key has one lane in
-// the launch, so tflag (set by this round's walks, or here) needs no exchange.  The touched
-// keys of a workgroup's 16 blocks take one counter atomic (mid-run rounds touch most of the
-// side, and per-wavefront atomics on the one counter serialise).
+// of the block (rows past the round's count are never flagged), every lane its key's D, both
+// before either is used; then C(c_r, 2) summed over the flagged rows comes off B of every live
+// key, the counters and flags are cleared, and a block with a decremented key is flagged for
+// the next round's prepare pass (which sets D = max(B, k) over the block).
 constexpr uint32_t VS_CREDIT_WAVES = 16;
 __global__ __launch_bounds__(64 * VS_CREDIT_WAVES) void k_vsplit_credit(VSplitArgs a) {
-  __shared__ uint32_t s_cnt[VS_CREDIT_WAVES];
-  __shared__ uint32_t s_base;
   const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
   const unsigned long long k = dev_ld(a.k);
   for (uint64_t b0 = (uint64_t)blockIdx.x * VS_CREDIT_WAVES; b0 < a.nblk; b0 += (uint64_t)gridDim.x * VS_CREDIT_WAVES) {
     const uint64_t b = b0 + wid;
+    if (b >= a.nblk) continue;
     const uint32_t key = (uint32_t)b * 64u + lane;
-    bool add = false;
-    if (b < a.nblk) {
-      uint32_t* fl = a.flags + (size_t)lane * a.nblk + b;
-      const uint32_t f = lane < a.cap ? dev_ld(fl) : 0u;
-      const bool in = key < (uint64_t)a.ns;
-      const unsigned long long d = in ? a.D[key] : 0ull;
-      const uint32_t tf = in ? a.tflag[key] : 1u;
-      uint64_t fm = __ballot(f != 0u);
-      if (fm != 0ull) {
-        if (f != 0u) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned long long sub = 0;
-        while (fm) {
-          const uint32_t r = (uint32_t)__ffsll((long long)fm) - 1u;
-          fm &= fm - 1ull;
-          uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
-          const uint32_t c = in ? dev_ld(row + key) : 0u;
-          if (c != 0u) __hip_atomic_store(row + key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          sub += (unsigned long long)c * (c - (c != 0u)) / 2u;
-        }
-        const bool hit = k != KNONE && sub != 0ull && d > k;
-        if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)sub));
-        add = hit && tf == 0u;
-        if (add) a.tflag[key] = 1u;
-      }
+    uint32_t* fl = a.flags + (size_t)lane * a.nblk + b;
+    const uint32_t f = lane < a.cap ? dev_ld(fl) : 0u;
+    const bool in = key < (uint64_t)a.ns;
+    const unsigned long long d = in ? a.D[key] : 0ull;
+    uint64_t fm = __ballot(f != 0u);
+    if (fm == 0ull) continue;
+    if (f != 0u) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long sub = 0;
+    while (fm) {
+      const uint32_t r = (uint32_t)__ffsll((long long)fm) - 1u;
+      fm &= fm - 1ull;
+      uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+      const uint32_t c = in ? dev_ld(row + key) : 0u;
+      if (c != 0u) __hip_atomic_store(row + key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sub += (unsigned long long)c * (c - (c != 0u)) / 2u;
     }
-    const uint64_t am = __ballot(add);
-    if (lane == 0) s_cnt[wid] = (uint32_t)__popcll(am);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t t = 0;
-      for (uint32_t w = 0; w < VS_CREDIT_WAVES; w++) t += s_cnt[w];
-      s_base = t ? atomicAdd(a.ntouched, t) : 0u;
-    }
-    __syncthreads();
-    uint32_t off = s_base;
-    for (uint32_t w = 0; w < wid; w++) off += s_cnt[w];
-    if (add) a.touched[off + __popcll(am & pbf_lanemask_lt())] = key;
-    __syncthreads();
+    const bool hit = k != KNONE && sub != 0ull && d > k;
+    if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)sub));
+    // the 64 keys lie in one block of the block minima: one flag for the wavefront
+    if (__ballot(hit) != 0ull && lane == 0) a.bflag[key >> a.bs_log] = 1u;
   }
 }
 
@@ -1131,11 +1107,13 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   vs.cap = vrows;
   vs.sub_cap = sub_cap;
   vs.ctr = pb.c.queue + 3;
-  vs.B = pb.B; vs.D = pb.D; vs.tflag = pb.tflag; vs.touched = pb.touched; vs.ntouched = pb.c.len + 3; vs.k = pb.c.k;
+  vs.B = pb.B; vs.D = pb.D; vs.k = pb.c.k;
+  vs.bflag = vb.flag; vs.bs_log = vb.bs_log;
+  a.bflag = vb.flag; a.bs_log = vb.bs_log;
   // beyond the large LDS table's fill limit a walk would overflow into the one-workgroup HBM path
   const SplitSel ss{paths, VS_MIN_PATHS, (V_LARGE_CAP / 4) * 3 - V_LARGE_TEAM, vrows, const_cast<uint32_t*>(vs.list)};
   auto round = [&]() -> int {
-    hipLaunchKernelGGL(k_vblocks_prepare, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.state, pb.D, ns, pb.c);
+    hipLaunchKernelGGL(k_vblocks_prepare, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.state, pb.B, pb.D, ns, pb.c);
     hipLaunchKernelGGL(k_vblocks_select, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.D, pb.B, pb.state, ns, pb.work,
                        small_max, small_list, large_list, ss, pb.c);
     PeelVArgs s = a;
@@ -1151,8 +1129,6 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL(k_vsplit_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(64 * VS_CREDIT_WAVES), 0, ctx->stream, vs);
-    hipLaunchKernelGGL(k_vupdate, dim3(grid_for(ns, 256, 2048)), dim3(256), 0, ctx->stream, vb, pb.touched, pb.tflag, pb.B,
-                       pb.D, pb.c);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
   };
