@@ -55,12 +55,16 @@ static inline uint32_t log_key(int64_t d) {
   return k;
 }
 
-/* cmpWedgeCounts (butterfly_utils.h:1061-1086): wV = sum_V C(d,2), wU = sum_U C(d,2);
- * use_v = (wV <= wU); the returned wedge count is the smaller one. */
+/* wedgeF (butterfly_utils.h:1061-1069): C(d,2) with the product d * (d - 1) taken in the
+ * 32-bit degree type, so it wraps modulo 2^32 for d > 65536 before the halving. */
+static inline int64_t side_term(uint32_t d) { return (int64_t)((uint32_t)(d * (d - 1u)) / 2u); }
+
+/* cmpWedgeCounts (butterfly_utils.h:1080-1086): wV = sum_V wedgeF, wU = sum_U wedgeF (64-bit
+ * sums of the wrapped terms); use_v = (wV <= wU); the returned wedge count is the smaller one. */
 void orc_side(const orc_graph* g, int* use_v, int64_t* wedges) {
   int64_t wv = 0, wu = 0;
-  for (int64_t v = 0; v < g->nv; v++) { int64_t d = g->offV[v + 1] - g->offV[v]; wv += d * (d - 1) / 2; }
-  for (int64_t u = 0; u < g->nu; u++) { int64_t d = g->offU[u + 1] - g->offU[u]; wu += d * (d - 1) / 2; }
+  for (int64_t v = 0; v < g->nv; v++) wv += side_term(g->offV[v + 1] - g->offV[v]);
+  for (int64_t u = 0; u < g->nu; u++) wu += side_term(g->offU[u + 1] - g->offU[u]);
   *use_v = (wv <= wu);
   *wedges = (wv <= wu) ? wv : wu;
 }

@@ -68,14 +68,8 @@ __global__ __launch_bounds__(256) void k_batch_side(const BGraph* __restrict__ g
   for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
     const BGraph b = gs[g];
     unsigned long long wv = 0, wu = 0;
-    for (uint32_t v = threadIdx.x; v < b.nv; v += blockDim.x) {
-      const unsigned long long d = b.offV[v + 1] - b.offV[v];
-      wv += d * (d - (d ? 1 : 0)) / 2;
-    }
-    for (uint32_t u = threadIdx.x; u < b.nu; u += blockDim.x) {
-      const unsigned long long d = b.offU[u + 1] - b.offU[u];
-      wu += d * (d - (d ? 1 : 0)) / 2;
-    }
+    for (uint32_t v = threadIdx.x; v < b.nv; v += blockDim.x) wv += (unsigned long long)pbf_side_term(b.offV[v + 1] - b.offV[v]);
+    for (uint32_t u = threadIdx.x; u < b.nu; u += blockDim.x) wu += (unsigned long long)pbf_side_term(b.offU[u + 1] - b.offU[u]);
     wv = block_sum(wv, sm);
     wu = block_sum(wu, sm);
     const bool use_v = wv <= wu;  // ties count on U

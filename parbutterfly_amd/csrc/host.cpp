@@ -179,8 +179,8 @@ extern "C" int pbf_validate(const pbf_bipartite_csr* g) {
 extern "C" int pbf_side_host(const pbf_bipartite_csr* g, int* use_v, int64_t* num_wedges) {
   if (!g || !use_v || !num_wedges) return PBF_ERR_ARG;
   int64_t wv = 0, wu = 0;
-  for (int64_t v = 0; v < g->nv; v++) { int64_t d = g->offsetsV[v + 1] - g->offsetsV[v]; wv += d * (d - 1) / 2; }
-  for (int64_t u = 0; u < g->nu; u++) { int64_t d = g->offsetsU[u + 1] - g->offsetsU[u]; wu += d * (d - 1) / 2; }
+  for (int64_t v = 0; v < g->nv; v++) wv += pbf_side_term(g->offsetsV[v + 1] - g->offsetsV[v]);
+  for (int64_t u = 0; u < g->nu; u++) wu += pbf_side_term(g->offsetsU[u + 1] - g->offsetsU[u]);
   *use_v = wv <= wu;
   *num_wedges = wv <= wu ? wv : wu;
   return PBF_OK;

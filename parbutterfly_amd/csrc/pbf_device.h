@@ -222,3 +222,8 @@ __device__ __forceinline__ T pbf_wave_sum(T x) {
   for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
   return x;
 }
+
+// wedgeF of cmpWedgeCounts (butterfly_utils.h:1061-1069): C(d,2) with the product d * (d - 1)
+// in the 32-bit degree type, i.e. modulo 2^32 (it wraps for d > 65536), then halved.  The
+// side choice sums these terms in 64 bits, as the reference does.
+__device__ __forceinline__ int64_t pbf_side_term(uint32_t d) { return (int64_t)((uint32_t)(d * (d - 1u)) / 2u); }

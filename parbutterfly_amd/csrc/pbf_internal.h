@@ -19,3 +19,9 @@ static inline double pbf_u01(uint64_t r) { return double(r >> 11) * (1.0 / 90071
 static inline uint64_t pbf_bounded(uint64_t r, uint64_t n) {
   return (uint64_t)(((unsigned __int128)r * n) >> 64);
 }
+
+// wedgeF of cmpWedgeCounts (butterfly_utils.h:1061-1069): C(d,2) of a degree d, with the
+// product d * (d - 1) taken in the 32-bit degree type as there, i.e. modulo 2^32 (it wraps
+// for d > 65536) before the halving.  The side choice compares 64-bit sums of these terms, so
+// it is the reference's choice on every graph.
+static inline int64_t pbf_side_term(uint32_t d) { return (int64_t)((uint32_t)(d * (d - 1u)) / 2u); }

@@ -24,8 +24,7 @@ __global__ void k_side_partial(const uint32_t* __restrict__ off, int64_t n, unsi
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t s = 0;
   for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t d = off[i + 1] - off[i];
-    s += d * (d - 1) / 2;
+    s += (uint64_t)pbf_side_term(off[i + 1] - off[i]);
   }
   s = pbf_wave_sum(s);
   if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, (unsigned long long)s);

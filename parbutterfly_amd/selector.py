@@ -10,6 +10,7 @@ the choice: the variants only change the order of work.
     python -m parbutterfly_amd.selector collect --first-seed 1000000 --count 20000 --out DIR
     python -m parbutterfly_amd.selector train --data DIR --out MODEL.npz [--epochs 300]
     python -m parbutterfly_amd.selector evaluate --model MODEL.npz --data DIR2 [--out JSON]
+    python -m parbutterfly_amd.selector evaluate-fixture [--fixture NPZ] [--out JSON]
 """
 import argparse
 import glob
@@ -25,6 +26,8 @@ import parbutterfly_amd as pb
 NVAR = 9
 FIXED_VARIANT = 3  # ADEG with the default knobs: the method without the selector
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_model.npz")
+HELDOUT_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden",
+                            "selector_heldout512.npz")
 
 
 def variant_settings(v):
@@ -71,6 +74,12 @@ def collect(ctx, first_seed, count, out_dir, chunk=512, workers=16):
         del dgs, hg
         ctx.trim()
         print(f"collect: seeds {seeds[0]}..{seeds[-1]} in {time.time() - t0:.1f}s", flush=True)
+
+
+def load_fixture(path=HELDOUT_PATH):
+    """(F, T, seeds) of a held-out fixture written by tests/golden/make_selector_heldout.py."""
+    z = np.load(path)
+    return z["F"], z["T"], z["seeds"]
 
 
 def load_dir(d):
@@ -184,6 +193,10 @@ def main():
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--data", required=True)
     e.add_argument("--out", default=None)
+    x = sub.add_parser("evaluate-fixture")
+    x.add_argument("--model", default=MODEL_PATH)
+    x.add_argument("--fixture", default=HELDOUT_PATH)
+    x.add_argument("--out", default=None)
     a = ap.parse_args()
     ctx = pb.Context(0)
     if a.cmd == "collect":
@@ -197,7 +210,10 @@ def main():
         print(json.dumps({"samples": int(len(F)), "epochs": a.epochs, "final_loss": float(m["loss"]),
                           "train_s": round(time.time() - t0, 2)}), flush=True)
     else:
-        F, T = load_dir(a.data)
+        if a.cmd == "evaluate-fixture":
+            F, T, _ = load_fixture(a.fixture)
+        else:
+            F, T = load_dir(a.data)
         r = evaluate(ctx, load(a.model), F, T)
         txt = json.dumps(r)
         if a.out:

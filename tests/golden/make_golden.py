@@ -2,13 +2,14 @@
 
 Run in a container that has the reference tree (oracle/Makefile builds oracle/_ref/ref_harness
 from the reference's own headers):
-    python tests/golden/make_golden.py [--sparse-only]
+    python tests/golden/make_golden.py [--sparse-only | --side-only]
 For every graph below it stores the AdjacencyHypergraph input (<name>.adj) and the reference's
 outputs (<name>.npz): cmpWedgeCounts, per-vertex counts (SERIAL and BATCHS), per-edge counts,
 totals, tip numbers (PBATCHS), wing numbers (PBATCHS), and the reference's ADEG / ACOCORE orders
 with the wedge count W each gives; sparse_cases.npz holds the graphs that the reference's
-sparsifiers make of them (make_sparse).  The reference prints only totals (BipartiteButterflies.C:
-215, 234, 262); the harness dumps the arrays that Compute discards.
+sparsifiers make of them (make_sparse); side_cases.npz holds cmpWedgeCounts (and the counts)
+of graphs on which its 32-bit C(d,2) wraps (make_side).  The reference prints only totals
+(BipartiteButterflies.C:215, 234, 262); the harness dumps the arrays that Compute discards.
 """
 import os
 import sys
@@ -59,12 +60,52 @@ def make_sparse(tmp):
     print("sparse_cases", len(rec) // 3)
 
 
+SIDE_CASES = {
+    # case -> whether the counts are stored too.  hub_wrap: the per-vertex term wraps and flips
+    # the side choice (tests/graphs.py); the headline graph: hub degrees up to 2^18, the
+    # wrapped sum is the reference's num_wedges
+    "hub_wrap": True,
+    "chunglu_2p20x2p18_m2p24_g21_s2": False,
+}
+
+
+def side_case_graph(name):
+    if name == "hub_wrap":
+        from tests.graphs import hub_wrap_graph
+        return hub_wrap_graph()
+    return pb.Graph.chung_lu(1 << 20, 1 << 18, 1 << 24, 2.1, 2.1, 2)
+
+
+def make_side(tmp):
+    """side_cases.npz: <case>__use_v, <case>__side_wedges from the reference's cmpWedgeCounts,
+    and for the small case its BATCHS / ADEG per-vertex and per-edge counts and total."""
+    out_bin = os.path.join(tmp, "s.bin")
+    rec = {}
+    for name, counts in SIDE_CASES.items():
+        g = side_case_graph(name)
+        gp = os.path.join(tmp, name + ".bcsr")
+        g.write_binary(gp)
+        use_v, sw, _, _, _ = O.run_ref("side", gp, out_bin)
+        rec[name + "__use_v"] = np.array([use_v])
+        rec[name + "__side_wedges"] = np.array([sw], np.int64)
+        if counts:
+            for op in ("vert", "edge", "total"):
+                _, _, c, _, _ = O.run_ref(op, gp, out_bin, "BATCHS", "ADEG")
+                rec[name + "__" + op] = c
+        print("side case", name, use_v, sw)
+    np.savez_compressed(os.path.join(HERE, "side_cases.npz"), **rec)
+
+
 def main():
     O.build()
     if not O.ref_available():
         raise SystemExit("oracle/_ref/ref_harness is not built (needs the reference tree)")
     tmp = tempfile.mkdtemp()
+    if "--side-only" in sys.argv:
+        make_side(tmp)
+        return
     make_sparse(tmp)
+    make_side(tmp)
     if "--sparse-only" in sys.argv:
         return
     out_bin = os.path.join(tmp, "o.bin")

@@ -24,3 +24,16 @@ def small_graphs():
     out.append(("K_1_7", complete_bipartite(1, 7)))
     out.append(("rmat-6x6", pb.Graph.rmat(6, 6, 600, 0.5, 0.1, 0.1, 0.3, 3)))
     return out
+
+
+def hub_wrap_graph():
+    """A graph on which cmpWedgeCounts' 32-bit C(d,2) wraps and flips the side choice
+    (butterfly_utils.h:1061-1086): V vertex 0 is adjacent to all 65537 U vertices (its term
+    65537 * 65536 wraps to 65536 modulo 2^32, i.e. C(d,2) = 32768 instead of 2147516416), and
+    V vertices 1..999 are adjacent to U vertices 0 and 1.  Wrapped, wV = 32768 + 999 <= wU =
+    2 * C(1000, 2), so the counts live on U; exactly, wV > wU.  Butterflies: every pair of V
+    vertices with U vertices 0 and 1, C(1000, 2) in all."""
+    nu = 65537
+    v = np.concatenate([np.zeros(nu, np.uint32), np.repeat(np.arange(1, 1000, dtype=np.uint32), 2)])
+    u = np.concatenate([np.arange(nu, dtype=np.uint32), np.tile(np.array([0, 1], np.uint32), 999)])
+    return pb.Graph.from_edges(1000, nu, v, u)

@@ -186,3 +186,27 @@ def test_hbm_fallback_reuse(ctx):
     ob, os_ = O.count_vert(big, True), O.count_vert(small, True)
     for k, v in enumerate(res):
         assert np.array_equal(v, ob if k % 2 == 0 else os_), k
+
+
+def test_side_wraps_as_reference(ctx):
+    # cmpWedgeCounts' C(d,2) wraps in 32 bits for a degree above 65536; on this graph that
+    # flips the side choice, and the reference's choice and counts are in side_cases.npz
+    import os
+    from tests.graphs import hub_wrap_graph
+    with np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "side_cases.npz")) as z:
+        rec = {k: z[k] for k in z.files}
+    g = hub_wrap_graph()
+    dg = g.to_device(ctx)
+    exp = (bool(rec["hub_wrap__use_v"][0]), int(rec["hub_wrap__side_wedges"][0]))
+    assert ctx.side(g) == ctx.side(dg) == exp
+    use_v = exp[0]
+    for rt in RANKS:
+        assert np.array_equal(pb.as_u64(ctx.count(dg, use_v, "VERT", "GPU", rt)), rec["hub_wrap__vert"].astype(np.uint64))
+        assert np.array_equal(pb.as_u64(ctx.count(dg, use_v, "EDGE", "GPU", rt)), rec["hub_wrap__edge"].astype(np.uint64))
+        assert ctx.count(dg, use_v, "TOTAL", "GPU", rt) == int(rec["hub_wrap__total"][0])
+    # a batch takes each graph's side by the same rule
+    vert, tot = ctx.count_batch([g, complete_bipartite(3, 4)], "ADEG", mode="union")
+    assert np.array_equal(vert[0], rec["hub_wrap__vert"].astype(np.uint64)) and int(tot[0]) == int(rec["hub_wrap__total"][0])
+    h = pb.Graph.chung_lu(1 << 20, 1 << 18, 1 << 24, 2.1, 2.1, 2)
+    k = "chunglu_2p20x2p18_m2p24_g21_s2"
+    assert ctx.side(h.to_device(ctx)) == (bool(rec[k + "__use_v"][0]), int(rec[k + "__side_wedges"][0]))

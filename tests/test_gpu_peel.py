@@ -112,3 +112,27 @@ def test_tip_peel_split_round_beyond_kept_prefixes(ctx):
     tip = pb.as_u64(ctx.peel(g, use_v, counts))
     assert (tip == want).all()
     assert ctx.timing()["rounds"] <= 32  # one batch of rounds: the hubs go in the first
+
+
+@pytest.mark.parametrize("nv, nu, m, seed, nb, bs_log", [
+    # n_S = 2^19: 256 blocks of 2^11 items (the whole block-minimum reduction, blocks above 1024)
+    (1 << 19, 1 << 19, 1 << 21, 5, 256, 11),
+    # 2^17 < n_S < 2^18: 245 blocks of 2^10 items, the last one partial
+    (250000, 250000, 1 << 21, 6, 245, 10),
+])
+def test_tip_peel_many_blocks(ctx, nv, nu, m, seed, nb, bs_log):
+    # sparse power-law graphs (gamma 2.5) with 1400-3000 distinct tip levels: the tip peel keeps
+    # per-block minima of D over at most 256 blocks of at least 1024 S vertices each
+    g = pb.Graph.chung_lu(nv, nu, m, 2.5, 2.5, seed)
+    use_v, _ = ctx.side(g)
+    ns = g.nu if use_v else g.nv
+    b = 10
+    while ((ns - 1) >> b) + 1 > 256:
+        b += 1
+    assert (b, ((ns - 1) >> b) + 1) == (bs_log, nb), "case no longer covers the intended block layout"
+    v = pb.as_u64(ctx.count(g, use_v, "VERT"))
+    assert np.array_equal(v, O.count_vert(g, use_v, "ADEG"))
+    tip = pb.as_u64(ctx.peel(g, use_v, v))
+    ref = O.peel_vert(g, use_v, v)
+    assert len(np.unique(ref)) > 1000
+    assert np.array_equal(tip, ref)

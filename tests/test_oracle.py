@@ -151,3 +151,26 @@ def test_planted_mistake_is_caught():
     bad_e = O.count_edge(g, use_v).copy()
     bad_e[0] += 1
     assert int(bad_e.sum()) != 4 * tot
+
+
+def _side_cases():
+    with np.load(os.path.join(GOLDEN, "side_cases.npz")) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_oracle_side_wraps_as_reference():
+    # cmpWedgeCounts takes C(d,2) in the 32-bit degree type: the reference's side choice and
+    # num_wedges on a graph where that wraps and flips the choice, and on the headline graph
+    from tests.graphs import hub_wrap_graph
+    rec = _side_cases()
+    g = hub_wrap_graph()
+    assert O.side(g) == (bool(rec["hub_wrap__use_v"][0]), int(rec["hub_wrap__side_wedges"][0])) == (True, 33767)
+    assert g.side() == O.side(g)  # pbf_side_host
+    use_v = True
+    v = O.count_vert(g, use_v, "ADEG")
+    assert np.array_equal(v, rec["hub_wrap__vert"].astype(np.uint64))
+    assert np.array_equal(O.count_edge(g, use_v, "ADEG"), rec["hub_wrap__edge"].astype(np.uint64))
+    assert O.count_total(g, use_v, "ADEG") == int(rec["hub_wrap__total"][0]) == 1000 * 999 // 2
+    h = pb.Graph.chung_lu(1 << 20, 1 << 18, 1 << 24, 2.1, 2.1, 2)
+    k = "chunglu_2p20x2p18_m2p24_g21_s2"
+    assert O.side(h) == h.side() == (bool(rec[k + "__use_v"][0]), int(rec[k + "__side_wedges"][0]))
