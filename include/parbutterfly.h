@@ -182,6 +182,21 @@ int pbf_count_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int us
 /* CountTotal (butterfly_count_total.h:1014): total count, written to host *total. */
 int pbf_count_total(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
                     int count_type, int rank_type, uint64_t* total);
+/* pbf_count_vert without allocating.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as pbf_count_vert with PBF_MEM_DEVICE
+ * (graph and out are device pointers on the context's device; there is no mem argument); it
+ * calls no hipMalloc / hipFree and no allocating library routine.  Its scratch is one fixed
+ * array of PBF_NOALLOC_SCRATCH_BYTES per device inside the library, held by every device the
+ * library runs on until the process ends, and its address is looked up by pbf_create.  A graph
+ * whose scratch does not fit returns PBF_ERR_CAPACITY and leaves out unwritten (about 110 bytes
+ * per vertex plus 32 per edge, plus up to 17 MB of fixed tables; every graph with nv + nu <=
+ * 2^20 and m <= 2^21 fits).  All calls on one device share that device's array and it keeps
+ * the last call's contents: calls that overlap on one device (other host threads, other
+ * contexts on the device) race, and the caller must serialise them. */
+#define PBF_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int pbf_count_vert_noalloc(pbf_context* ctx, const pbf_bipartite_csr* g, int use_v, int count_type,
+                           int rank_type, uint64_t* out);
 /* edgeToIdxs (butterfly_utils.h:2034-2062): eti[p] = O-CSR position of S-CSR position p. */
 int pbf_edge_to_idxs(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v,
                      uint32_t* eti);

@@ -117,6 +117,7 @@ _SIGS = {
     "pbf_side": (_INT, [_P, _PCSR, _INT, _PI, _PI64]),
     "pbf_rank": (_INT, [_P, _PCSR, _INT, _INT, _P, _P, _PI64]),
     "pbf_count_vert": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
+    "pbf_count_vert_noalloc": (_INT, [_P, _PCSR, _INT, _INT, _INT, _P]),
     "pbf_count_edge": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
     "pbf_count_total": (_INT, [_P, _PCSR, _INT, _INT, _INT, _INT, _P]),
     "pbf_edge_to_idxs": (_INT, [_P, _PCSR, _INT, _INT, _P]),
@@ -470,6 +471,20 @@ class Context:
         fn = _lib.pbf_count_vert if per == 0 else _lib.pbf_count_edge
         _check(fn(self._ctx, ctypes.byref(g._csr), mem, int(bool(use_v)), ct, rt, self._ptr(out) if n_out else 1),
                "pbf_count_vert" if per == 0 else "pbf_count_edge")
+        return out
+
+    def count_vert_noalloc(self, g, use_v, count_type="GPU", rank_type="ADEG", out=None):
+        """Count per vertex through pbf_count_vert_noalloc (device graph and output; the
+        library's fixed per-device scratch, no allocation inside the call)."""
+        if not isinstance(g, DeviceGraph):
+            raise TypeError("pbf_count_vert_noalloc takes device graphs")
+        n_out = g.nu if use_v else g.nv
+        if out is None:
+            out = DeviceArray(self, n_out, np.uint64)
+        _check(_lib.pbf_count_vert_noalloc(self._ctx, ctypes.byref(g._csr), int(bool(use_v)),
+                                           _enum(COUNT_TYPES, count_type, "count type"),
+                                           _enum(RANK_TYPES, rank_type, "rank type"), out.ptr if n_out else 1),
+               "pbf_count_vert_noalloc")
         return out
 
     def edge_to_idxs(self, g, use_v):

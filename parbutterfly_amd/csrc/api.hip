@@ -3,8 +3,15 @@
 // a pbf_status code instead of printing and exiting.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "pbf_device.h"
+
+// Scratch of pbf_count_vert_noalloc: one array of PBF_NOALLOC_SCRATCH_BYTES per device, held
+// by every device the library runs on until the process ends.  In host code its name is only a
+// stand-in, never a device address: the address is looked up on each device (noalloc_address,
+// called by pbf_create with the device current) and only that pointer is used.
+static __device__ __align__(256) unsigned char scratch[PBF_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 int alloc_dev(void** p, size_t bytes) {
@@ -197,6 +204,19 @@ int count_common(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_
 
 int pbf_scratch(pbf_context* ctx, int slot, size_t bytes, void** out) {
   if (slot < 0 || slot >= S_NUM_SLOTS) return PBF_ERR_ARG;
+  if (ctx->fixed.base) {
+    // fixed region: a slot keeps its piece while it is large enough, else gets a new piece
+    // past the last one (the old piece is not reused within the call)
+    if (ctx->cap[slot] < bytes || !ctx->buf[slot]) {
+      const size_t off = (ctx->fixed.used + 255) & ~size_t(255);
+      if (off > ctx->fixed.cap || bytes > ctx->fixed.cap - off) return PBF_ERR_CAPACITY;
+      ctx->buf[slot] = ctx->fixed.base + off;
+      ctx->cap[slot] = bytes;
+      ctx->fixed.used = off + bytes;
+    }
+    *out = ctx->buf[slot];
+    return PBF_OK;
+  }
   if (ctx->cap[slot] < bytes || !ctx->buf[slot]) {
     if (ctx->buf[slot]) {
       // the stream may still use the old buffer
@@ -212,6 +232,36 @@ int pbf_scratch(pbf_context* ctx, int slot, size_t bytes, void** out) {
   *out = ctx->buf[slot];
   return PBF_OK;
 }
+
+size_t pbf_scratch_room(const pbf_context* ctx) {
+  if (!ctx->fixed.base) return SIZE_MAX;
+  const size_t off = (ctx->fixed.used + 255) & ~size_t(255);
+  return off < ctx->fixed.cap ? ctx->fixed.cap - off : 0;
+}
+
+namespace {
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex noalloc_mu;
+bool noalloc_done[NOALLOC_DEVICES] = {};
+void* noalloc_ptr[NOALLOC_DEVICES] = {};
+
+// Address of the fixed scratch array on `device`, which must be current: looked up once per
+// device (each device holds its own copy of the array at its own address).  Null on failure.
+void* noalloc_address(int device) {
+  if (device < 0 || device >= NOALLOC_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_done[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+    }
+    noalloc_ptr[device] = p;
+    noalloc_done[device] = true;
+  }
+  return noalloc_ptr[device];
+}
+}  // namespace
 
 extern "C" int pbf_create(pbf_context** out, int device) {
   if (!out) return PBF_ERR_ARG;
@@ -233,6 +283,9 @@ extern "C" int pbf_create(pbf_context** out, int device) {
     pbf_destroy(ctx);
     return PBF_ERR_OOM;
   }
+  // the no-allocation entry point's array on this device (a failed lookup only disables that
+  // entry point, which then returns PBF_ERR_HIP)
+  ctx->noalloc = static_cast<unsigned char*>(noalloc_address(device));
   *out = ctx;
   return PBF_OK;
 }
@@ -379,6 +432,43 @@ extern "C" int pbf_rank(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, i
 extern "C" int pbf_count_vert(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int count_type,
                               int rank_type, uint64_t* out) {
   return count_common(ctx, g, mem, use_v, count_type, rank_type, MODE_VERT, out);
+}
+
+extern "C" int pbf_count_vert_noalloc(pbf_context* ctx, const pbf_bipartite_csr* g, int use_v, int count_type,
+                                      int rank_type, uint64_t* out) {
+  if (!ctx || !g || !out) return PBF_ERR_ARG;
+  if (!ctx->noalloc) return PBF_ERR_HIP;
+  // The arena hands out pieces of the device's fixed array for this call only: the context's
+  // own buffers are set aside and restored afterwards, untouched.  The array keeps what the
+  // last call left, so the buffers that must start at zero (hub accumulators, HBM-fallback
+  // counters) are cleared again.
+  struct Session {
+    pbf_context* c;
+    void* buf[S_NUM_SLOTS];
+    size_t cap[S_NUM_SLOTS];
+    bool hot_clear;
+    size_t pool_clean;
+    explicit Session(pbf_context* ctx) : c(ctx), hot_clear(ctx->hot_clear), pool_clean(ctx->pool_clean) {
+      memcpy(buf, c->buf, sizeof(buf));
+      memcpy(cap, c->cap, sizeof(cap));
+      memset(c->buf, 0, sizeof(c->buf));
+      memset(c->cap, 0, sizeof(c->cap));
+      c->hot_clear = false;
+      c->pool_clean = 0;
+      c->fixed.base = c->noalloc;
+      c->fixed.cap = PBF_NOALLOC_SCRATCH_BYTES;
+      c->fixed.used = 0;
+    }
+    ~Session() {
+      c->fixed.base = nullptr;
+      c->fixed.cap = c->fixed.used = 0;
+      memcpy(c->buf, buf, sizeof(buf));
+      memcpy(c->cap, cap, sizeof(cap));
+      c->hot_clear = hot_clear;
+      c->pool_clean = pool_clean;
+    }
+  } session(ctx);
+  return count_common(ctx, g, PBF_MEM_DEVICE, use_v, count_type, rank_type, MODE_VERT, out);
 }
 
 extern "C" int pbf_count_edge(pbf_context* ctx, const pbf_bipartite_csr* g, int mem, int use_v, int count_type,

@@ -1601,6 +1601,14 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
     ctx->timing.hub_vertices = novf;
     const int64_t most = std::max<int64_t>(HBM_BLOCKS, fit);
     blocks = (int)std::min<int64_t>((int64_t)novf, most);
+    // a fixed scratch region holds as many blocks' counters as fit (the work queue serves any
+    // number of blocks); not even one is a capacity error
+    const size_t room = pbf_scratch_room(ctx), per_block = sizeof(uint32_t) * 2 * (size_t)a.kn;
+    if (novf && room != SIZE_MAX) {
+      const size_t fit_room = room > 512 ? (room - 512) / per_block : 0;
+      if (fit_room == 0) return PBF_ERR_CAPACITY;
+      blocks = (int)std::min<size_t>((size_t)blocks, fit_room);
+    }
   } else {
     ctx->timing.hub_vertices = -1;  // not read back
   }

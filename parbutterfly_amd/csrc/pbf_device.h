@@ -88,10 +88,22 @@ struct pbf_context {
   void* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   bool stage_busy[2] = {false, false};
+  // The device's fixed scratch array (the no-allocation entry point), looked up at creation;
+  // null when the lookup failed.
+  unsigned char* noalloc = nullptr;
+  // While fixed.base is set, pbf_scratch carves its slots from [base, base + cap) instead of
+  // allocating (the no-allocation entry point), and fails with PBF_ERR_CAPACITY past the end.
+  struct {
+    unsigned char* base = nullptr;
+    size_t cap = 0, used = 0;
+  } fixed;
 };
 
 // Returns a device buffer of at least `bytes` for the slot (grow-only; contents undefined).
 int pbf_scratch(pbf_context* ctx, int slot, size_t bytes, void** out);
+// Bytes pbf_scratch can still hand out without allocating: the rest of the fixed region, or
+// SIZE_MAX when the arena allocates.
+size_t pbf_scratch_room(const pbf_context* ctx);
 
 #define PBF_HIP(call)                                  \
   do {                                                 \
