@@ -435,7 +435,11 @@ __device__ __forceinline__ Chunk load_chunk(const CountArgs& a, uint32_t slot, b
   return ch;
 }
 
-// One window: lane l holds wedge indices b0 + 4l + k, k < nval.
+// One window: lane l holds wedge indices b0 + 4l + k (blocked layout) or b0 + 64k + l
+// (interleaved layout), k < nval.  In the interleaved layout the k-th wedges of consecutive
+// lanes are consecutive entries of N(v): a per-wedge credit to the slot of each is one
+// coalesced atomic per k (64-bit atomics reach about 3.6x the rate of the blocked layout's
+// 4-word lane stride on this part).
 struct Win {
   uint32_t ent[WIN];  // ranked entry (rank << 1 | flag) of u2
   uint32_t pos[WIN];  // its slot in the ranked adjacency
@@ -461,7 +465,7 @@ __device__ __forceinline__ uint32_t next_tag(WaveCtx& wc) {
 // before it.  The segment covering b0 itself (it may have started in an earlier window) is
 // the highest lane with L > 0 and P <= b0.  Stale flags from earlier windows of the chunk
 // hold smaller segments, those of earlier chunks smaller epochs: the max ignores both.
-template <bool MULTI>
+template <bool MULTI, bool IL>
 __device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint32_t b0, const Chunk& ch, uint32_t tag,
                                         Win& wv) {
   const uint32_t lane = __lane_id();
@@ -469,6 +473,29 @@ __device__ __forceinline__ void win_map(const CountArgs& a, lds_u32* flags, uint
   const uint64_t cm = __ballot(ch.L != 0u && ch.P <= b0);
   const uint32_t carry = tag | (63u - (uint32_t)__clzll(cm));
   wave_lds_fence();
+  if constexpr (IL) {
+    // row k = indices b0 + 64k .. b0 + 64k + 63, one per lane: a max-scan per row, each row
+    // starting from the maximum of everything before it
+    uint32_t f[WIN];
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) f[k] = flags[64 * k + lane];
+    wave_lds_fence();
+    const uint32_t q0 = b0 + lane;
+    wv.nval = q0 < ch.T ? min((ch.T - q0 + 63u) / 64u, WIN) : 0u;
+    uint32_t run = carry;
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) {
+      const uint32_t m = max(wave_incl_max(f[k]), run);
+      run = lane_read(m, 63);
+      const uint32_t sg = m & 63u;
+      wv.seg[k] = sg;
+      wv.pos[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(sg << 2), (int)ch.base) + q0 + 64u * k;
+      if constexpr (MULTI) wv.td[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(sg << 2), (int)ch.td);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) wv.ent[k] = a.redg[k < wv.nval ? wv.pos[k] : 0u];
+    return;
+  }
   const u32x4 f = *(lds_u32x4*)(flags + 4 * lane);
   wave_lds_fence();
   const uint32_t m0 = f.x, m1 = max(m0, f.y), m2 = max(m1, f.z), m3 = max(m2, f.w);
@@ -585,11 +612,29 @@ __device__ __forceinline__ void win_get(const Tab& t, const Win& wv, uint32_t* c
 // (a segment boundary inside the lane) go straight to acc.  Lane sums are then reduced per
 // segment with a segmented scan; the last lane of each run adds the run's sum.  The u32 sums
 // hold at most 256 values below deg(i) < 2^24.
-template <int MODE, int TM, bool MULTI>
+template <int MODE, int TM, bool MULTI, bool IL>
 __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, const Win& wv, lds_u64* acc) {
   uint32_t val[WIN], cnt[WIN];
   uint32_t last = wv.seg[0];
   win_get<TM, MULTI>(t, wv, cnt);
+  if constexpr (IL) {
+    // row by row: the credits of a row go to consecutive slots, and the row's segment sums
+    // come from a segmented scan over its lanes (segments are non-decreasing along a row)
+#pragma unroll
+    for (uint32_t k = 0; k < WIN; k++) {
+      const uint32_t c = cnt[k];
+      const uint32_t v = (k < wv.nval && c > 1u) ? c - 1u : 0u;
+      if constexpr (MODE == MODE_EDGE) {
+        if (v) atomicAdd(&a.B[wv.pos[k]], (unsigned long long)v);
+      }
+      if (__ballot(v != 0u) == 0ull) continue;
+      const uint32_t key = k < wv.nval ? wv.seg[k] : 64u;
+      const uint32_t x = wave_seg_incl_add(v, key);
+      const uint32_t knext = wave_shl1_or(key, ~0u);
+      if (key < 64u && x && knext != key) lds_add(acc + key, (unsigned long long)x);
+    }
+    return;
+  }
 #pragma unroll
   for (uint32_t k = 0; k < WIN; k++) {
     val[k] = 0;
@@ -620,15 +665,21 @@ __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, con
 // Runs windows wstart, wstart + wstep, ... of a chunk.  The next window is mapped and its
 // loads issued before the current one is aggregated.  Pass 2 ends with the per-segment
 // credits (centre v, or edge i -> v).  Returns false when a table overflowed in pass 1.
+// Window layout of a pass: interleaved for the per-edge credit pass (a coalesced atomic per
+// wedge row), blocked otherwise (a lane's wedges mostly share its segment).
+template <int MODE, bool PASS2>
+constexpr bool win_interleaved() { return MODE == MODE_EDGE && PASS2; }
+
 template <int MODE, bool PASS2, int TM, bool MULTI>
 __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, const Chunk& ch, uint32_t wstart,
                                               uint32_t wstep, WaveCtx& wc, const Track& tr) {
+  constexpr bool IL = win_interleaved<MODE, PASS2>();
   const uint32_t lane = __lane_id();
   if (ch.T <= wstart) return true;
   const uint32_t tag = next_tag(wc);
   Win cur, nxt;
   uint32_t b0 = wstart;
-  win_map<MULTI>(a, wc.flags, b0, ch, tag, cur);
+  win_map<MULTI, IL>(a, wc.flags, b0, ch, tag, cur);
   // the first window's loads and lane exchanges complete before the loop: with nothing of
   // `cur` pending on entry, the waits inside the loop cover only what they must (otherwise
   // they also wait for the next window's loads and exchanges, issued just before, and the
@@ -637,7 +688,7 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
   while (true) {
     const uint32_t nb = b0 + wstep;
     const bool more = nb < ch.T;
-    if (more) win_map<MULTI>(a, wc.flags, nb, ch, tag, nxt);
+    if (more) win_map<MULTI, IL>(a, wc.flags, nb, ch, tag, nxt);
     if constexpr (!PASS2) {
       const uint32_t r = win_insert<TM, MULTI>(t, cur);
       const bool failed = __ballot(r >= 256u) != 0ull;
@@ -651,7 +702,7 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
         return false;
       }
     } else {
-      win_credit<MODE, TM, MULTI>(a, t, cur, wc.acc);
+      win_credit<MODE, TM, MULTI, IL>(a, t, cur, wc.acc);
     }
     if (!more) break;
     cur = nxt;
