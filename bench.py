@@ -123,20 +123,20 @@ def batched_bench(pb, ctx, count=4096, first_seed=1000):
     res["bit_exact_autotuned_vs_plain"] = same(out_t, out_p)
     res["speedup_vs_plain"] = round(t_plain / t_tuned, 2)
     ctx.set_tuning()
-    # batched + learned: features, MLP selection, one union pass per chosen variant
+    # batched + learned (pbf_count_batch_learned): features, MLP selection, one union pass per
+    # chosen ranking
     from parbutterfly_amd import selector as S
     if os.path.exists(S.MODEL_PATH):
         model = S.load()
-        S.run_learned(ctx, model, dgs)  # warms every variant's scratch
-        best_t, parts, pick = None, None, None
+        out_l, pick = S.run_learned(ctx, model, dgs)  # warms every part's scratch
+        best_t = None
         for _ in range(2):
             ctx.synchronize()
             t = time.perf_counter()
-            parts, pick = S.run_learned(ctx, model, dgs)
-            ctx.synchronize()
+            out_l, pick = S.run_learned(ctx, model, dgs, outputs=out_l)
             dt = time.perf_counter() - t
             best_t = dt if best_t is None else min(best_t, dt)
-        verts, tots = S.gather(parts, count)
+        verts, tots = S.gather(out_l, count)
         vp, op, tp = (x.numpy() for x in out_p)
         exact = bool(np.array_equal(tots, tp[:count]) and
                      all(np.array_equal(verts[k], vp[int(op[k]):int(op[k + 1])]) for k in range(count)))

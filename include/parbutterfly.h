@@ -134,8 +134,15 @@ typedef struct pbf_context pbf_context;
  * scratch is cached in the context and grows on demand; one context per host thread. */
 int pbf_create(pbf_context** out, int device);
 int pbf_destroy(pbf_context* ctx);
-/* The context's HIP stream (hipStream_t) as an opaque pointer. */
+/* The HIP stream (hipStream_t, as an opaque pointer) the context queues its work on. */
 void* pbf_stream(pbf_context* ctx);
+/* Queues the context's work on the caller's HIP stream (a hipStream_t of the context's device,
+ * as an opaque pointer) instead of the context's own; NULL returns to the context's own
+ * stream.  Work queued on the previous stream is finished first.  Afterwards every entry point
+ * queues its kernels and copies on that stream, and the synchronous ones wait for that stream
+ * only, so a caller can order the library's work with its own.  The stream must outlive its
+ * use by the context (set NULL before destroying it). */
+int pbf_set_stream(pbf_context* ctx, void* stream);
 int pbf_synchronize(pbf_context* ctx);
 int pbf_last_timing(pbf_context* ctx, pbf_timing* out);
 /* Releases cached device scratch. */
@@ -287,6 +294,31 @@ int pbf_mlp_grad(pbf_context* ctx, const float* params, const float* X, const fl
                  float* grads, float* loss);
 /* Predictions Y[n][9]. */
 int pbf_mlp_predict(pbf_context* ctx, const float* params, const float* X, int64_t n, float* Y);
+
+/* A trained selector: network parameters (pbf_mlp_* layout) and the standardisation of its
+ * inputs (feature f enters as (x - fmu[f]) / fsd[f]) and outputs (variant v's predicted log2
+ * runtime is y[v] * tsd[v] + tmu[v]). */
+typedef struct {
+  float params[PBF_MLP_PARAMS];
+  float fmu[PBF_MLP_IN], fsd[PBF_MLP_IN];
+  float tmu[PBF_MLP_OUT], tsd[PBF_MLP_OUT];
+} pbf_selector_model;
+/* The method without the selector: ADEG ranking with the default counting knobs. */
+#define PBF_VARIANT_FIXED 3
+/* Chooses a variant for every graph of a batch (host int variants[count]): features
+ * (pbf_batch_features), one batched inference, and the variant of least predicted runtime.
+ * model == NULL, or a prediction that is not finite, gives the fixed variant.  The choice only
+ * orders the work: every variant counts exactly.  Arguments as pbf_batch_features. */
+int pbf_select(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
+               const pbf_selector_model* model, int* variants);
+/* Batched + learned counting: pbf_select, then one union pass (PBF_BATCH_UNION) per chosen
+ * ranking with the knob setting most chosen among its graphs; outputs in pbf_count_batch's
+ * layout and order (graph g's counts at [vert_off[g], vert_off[g+1]), totals[g]), and the
+ * chosen variants in variants[count] (host, may be NULL).  With model == NULL it is one fixed
+ * ADEG union pass.  Synchronous. */
+int pbf_count_batch_learned(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
+                            const pbf_selector_model* model, uint64_t* vert_out, int64_t vert_cap,
+                            uint64_t* vert_off, uint64_t* totals, int out_mem, int* variants);
 
 #ifdef __cplusplus
 }

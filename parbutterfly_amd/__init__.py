@@ -81,6 +81,22 @@ _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _U64 = ctypes.c_uint64
 _INT = ctypes.c_int
+class SelectorModel(ctypes.Structure):
+    """pbf_selector_model: network parameters and the standardisation of inputs and outputs."""
+    _fields_ = [("params", ctypes.c_float * 6288), ("fmu", ctypes.c_float * 16), ("fsd", ctypes.c_float * 16),
+                ("tmu", ctypes.c_float * 9), ("tsd", ctypes.c_float * 9)]
+
+    @classmethod
+    def from_dict(cls, m):
+        c = cls()
+        for k in ("params", "fmu", "fsd", "tmu", "tsd"):
+            a = np.ascontiguousarray(m[k], np.float32)
+            if a.size != len(getattr(c, k)):
+                raise ValueError(f"selector model field {k}: {a.size} values")
+            ctypes.memmove(getattr(c, k), a.ctypes.data, a.nbytes)
+        return c
+
+
 _PI = ctypes.POINTER(ctypes.c_int)
 _PI64 = ctypes.POINTER(ctypes.c_int64)
 _PCSR = ctypes.POINTER(_CSR)
@@ -106,6 +122,7 @@ _SIGS = {
     "pbf_create": (_INT, [ctypes.POINTER(_P), _INT]),
     "pbf_destroy": (_INT, [_P]),
     "pbf_stream": (_P, [_P]),
+    "pbf_set_stream": (_INT, [_P, _P]),
     "pbf_synchronize": (_INT, [_P]),
     "pbf_last_timing": (_INT, [_P, ctypes.POINTER(Timing)]),
     "pbf_trim": (_INT, [_P]),
@@ -138,6 +155,9 @@ _SIGS = {
     "pbf_mlp_train": (_INT, [_P, _P, _P, _P, _I64, _INT, ctypes.c_float, _U64, ctypes.POINTER(ctypes.c_float)]),
     "pbf_mlp_grad": (_INT, [_P, _P, _P, _P, _I64, _P, ctypes.POINTER(ctypes.c_float)]),
     "pbf_mlp_predict": (_INT, [_P, _P, _P, _I64, _P]),
+    "pbf_select": (_INT, [_P, _I64, _PCSR, _INT, ctypes.POINTER(SelectorModel), _P]),
+    "pbf_count_batch_learned": (_INT, [_P, _I64, _PCSR, _INT, ctypes.POINTER(SelectorModel), _P, _I64, _P, _P, _INT,
+                                       _P]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -407,6 +427,11 @@ class Context:
     def stream(self):
         return _lib.pbf_stream(self._ctx)
 
+    def set_stream(self, stream):
+        """Queues the context's work on a caller's hipStream_t (an integer handle); None
+        returns to the context's own stream."""
+        _check(_lib.pbf_set_stream(self._ctx, stream), "pbf_set_stream")
+
     def synchronize(self):
         _check(_lib.pbf_synchronize(self._ctx), "pbf_synchronize")
 
@@ -630,6 +655,42 @@ class Context:
                                     len(X), Y.ctypes.data), "pbf_mlp_predict")
         return Y[:len(X)]
 
+    @staticmethod
+    def _model(model):
+        if model is None:
+            return None
+        if not isinstance(model, SelectorModel):
+            model = SelectorModel.from_dict(model)
+        return ctypes.byref(model)
+
+    def select(self, graphs, model=None):
+        """pbf_select: the chosen variant (0..8) of every graph; model None = the fixed variant."""
+        mem, arr, _ = self._batch_args(graphs)
+        v = np.zeros(max(len(graphs), 1), np.int32)
+        _check(_lib.pbf_select(self._ctx, len(graphs), arr, mem, self._model(model), v.ctypes.data), "pbf_select")
+        return v[:len(graphs)]
+
+    def count_batch_learned(self, graphs, model=None, outputs=None):
+        """pbf_count_batch_learned: (outputs as count_batch gives them, chosen variants)."""
+        mem, arr, cap = self._batch_args(graphs)
+        cnt = len(graphs)
+        var = np.zeros(max(cnt, 1), np.int32)
+        if mem == MEM_DEVICE:
+            if outputs is None:
+                outputs = (DeviceArray(self, max(cap, 1), np.uint64), DeviceArray(self, cnt + 1, np.uint64),
+                           DeviceArray(self, max(cnt, 1), np.uint64))
+            vout, voff, tot = outputs
+            _check(_lib.pbf_count_batch_learned(self._ctx, cnt, arr, mem, self._model(model), vout.ptr, vout.n, voff.ptr,
+                                                tot.ptr, MEM_DEVICE, var.ctypes.data), "pbf_count_batch_learned")
+            return outputs, var[:cnt]
+        vout = np.zeros(max(cap, 1), np.uint64)
+        voff = np.zeros(cnt + 1, np.uint64)
+        tot = np.zeros(max(cnt, 1), np.uint64)
+        _check(_lib.pbf_count_batch_learned(self._ctx, cnt, arr, mem, self._model(model), vout.ctypes.data, len(vout),
+                                            voff.ctypes.data, tot.ctypes.data, MEM_HOST, var.ctypes.data),
+               "pbf_count_batch_learned")
+        return ([vout[int(voff[k]):int(voff[k + 1])] for k in range(cnt)], tot[:cnt]), var[:cnt]
+
     def _as_u64(self, g, counts):
         if isinstance(g, DeviceGraph):
             if isinstance(counts, DeviceArray):
@@ -639,6 +700,7 @@ class Context:
 
 
 MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 16, 64, 9, 6288
+VARIANT_FIXED = 3  # PBF_VARIANT_FIXED: ADEG with the default knobs
 RANK_NAMES = {v: k for k, v in RANK_TYPES.items()}
 
 

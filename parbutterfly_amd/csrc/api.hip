@@ -276,7 +276,8 @@ extern "C" int pbf_create(pbf_context** out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->num_cus = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return PBF_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return PBF_ERR_HIP; }
+  ctx->stream = ctx->own_stream;
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) { pbf_destroy(ctx); return PBF_ERR_HIP; }
   if (hipHostMalloc(reinterpret_cast<void**>(&ctx->pinned), 4096, hipHostMallocDefault) != hipSuccess) {
@@ -315,12 +316,25 @@ extern "C" int pbf_destroy(pbf_context* ctx) {
     if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
   }
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream && ctx->stream != ctx->own_stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return PBF_OK;
 }
 
 extern "C" void* pbf_stream(pbf_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+extern "C" int pbf_set_stream(pbf_context* ctx, void* stream) {
+  if (!ctx) return PBF_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  hipStream_t next = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+  if (next == ctx->stream) return PBF_OK;
+  // the previous stream's work (and the staging copies recorded on it) completes first
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 2; k++) ctx->stage_busy[k] = false;
+  ctx->stream = next;
+  return PBF_OK;
+}
 
 extern "C" int pbf_synchronize(pbf_context* ctx) {
   if (!ctx) return PBF_ERR_ARG;

@@ -13,6 +13,7 @@
 // replayed by identical calls.
 // PBF_BATCH_PLAIN: the same kernels, launched graph by graph from a host loop (the baseline).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -639,5 +640,193 @@ extern "C" int pbf_set_tuning(pbf_context* ctx, int light_max, int light_work, i
   ctx->tune.light_max = (uint32_t)light_max;
   ctx->tune.light_work = (uint32_t)light_work;
   ctx->tune.blocks_per_cu = (uint32_t)blocks_per_cu;
+  return PBF_OK;
+}
+
+// ---------------- learned selection (no counterpart in the reference) ----------------
+namespace {
+constexpr int FIXED_VARIANT = 3;  // ADEG with the default knobs: the method without the selector
+
+// Where one graph's outputs go: its part's packed segment (src) to the caller's layout (dst).
+struct Place {
+  uint64_t src, dst;      // first count in the parts' packed array / in the caller's array
+  uint32_t len, src_tot;  // counts of the graph (n_S) / its total in the parts' totals
+  uint32_t dst_tot, pad;
+};
+
+// One workgroup per graph: copies its counts and total into the caller's layout.
+__global__ __launch_bounds__(256) void k_place(const Place* __restrict__ pl, int64_t count,
+                                               const unsigned long long* __restrict__ pv,
+                                               const unsigned long long* __restrict__ pt,
+                                               unsigned long long* __restrict__ vout, unsigned long long* __restrict__ tot) {
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const Place p = pl[g];
+    for (uint32_t x = threadIdx.x; x < p.len; x += blockDim.x) vout[p.dst + x] = pv[p.src + x];
+    if (threadIdx.x == 0) tot[p.dst_tot] = pt[p.src_tot];
+  }
+}
+}  // namespace
+
+extern "C" int pbf_select(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
+                          const pbf_selector_model* model, int* variants) {
+  if (!ctx || count < 0 || (count > 0 && (!graphs || !variants))) return PBF_ERR_ARG;
+  if (count == 0) return PBF_OK;
+  if (!model) {
+    for (int64_t g = 0; g < count; g++) variants[g] = FIXED_VARIANT;
+    return PBF_OK;
+  }
+  std::vector<float> F((size_t)count * PBF_MLP_IN), Y((size_t)count * PBF_MLP_OUT);
+  PBF_CHECK(pbf_batch_features(ctx, count, graphs, mem, F.data()));
+  for (int64_t g = 0; g < count; g++)
+    for (int f = 0; f < PBF_MLP_IN; f++) {
+      float& x = F[(size_t)g * PBF_MLP_IN + f];
+      x = (x - model->fmu[f]) / model->fsd[f];
+    }
+  PBF_CHECK(pbf_mlp_predict(ctx, model->params, F.data(), count, Y.data()));
+  for (int64_t g = 0; g < count; g++) {
+    // predicted log2 runtimes (plain float multiply and add, no fused multiply-add, so that
+    // a caller's float32 y * tsd + tmu gives the same values); a prediction that is not finite
+    // is rejected and the graph keeps the fixed variant
+#pragma clang fp contract(off)
+    int best = FIXED_VARIANT;
+    float bt = 0;
+    bool ok = true;
+    for (int v = 0; v < PBF_MLP_OUT; v++) {
+      const float t = Y[(size_t)g * PBF_MLP_OUT + v] * model->tsd[v] + model->tmu[v];
+      if (!std::isfinite(t)) { ok = false; break; }
+      if (v == 0 || t < bt) { bt = t; best = v; }
+    }
+    variants[g] = ok ? best : FIXED_VARIANT;
+  }
+  return PBF_OK;
+}
+
+extern "C" int pbf_count_batch_learned(pbf_context* ctx, int64_t count, const pbf_bipartite_csr* graphs, int mem,
+                                       const pbf_selector_model* model, uint64_t* vert_out, int64_t vert_cap,
+                                       uint64_t* vert_off, uint64_t* totals, int out_mem, int* variants) {
+  if (!ctx || count < 0 || (count > 0 && (!graphs || !vert_out || !vert_off || !totals))) return PBF_ERR_ARG;
+  if (mem != PBF_MEM_DEVICE && mem != PBF_MEM_HOST) return PBF_ERR_ARG;
+  if (out_mem != PBF_MEM_DEVICE && out_mem != PBF_MEM_HOST) return PBF_ERR_ARG;
+  int64_t cap_need = 0;
+  for (int64_t g = 0; g < count; g++) {
+    const pbf_bipartite_csr& d = graphs[g];
+    if (d.nv < 0 || d.nu < 0 || d.m < 0) return PBF_ERR_ARG;
+    cap_need += std::max(d.nv, d.nu);
+  }
+  if (vert_cap < cap_need) return PBF_ERR_ARG;
+  if (count == 0) return PBF_OK;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+  if (prev != ctx->device) (void)hipSetDevice(ctx->device);
+  struct Restore {
+    int p, d;
+    ~Restore() { if (p >= 0 && p != d) (void)hipSetDevice(p); }
+  } restore{prev, ctx->device};
+  // events 6 and 7 bracket the whole call (the union passes inside use events 0 and 4)
+  hipEvent_t t0 = ctx->ev[6], t1 = ctx->ev[7];
+  PBF_HIP(hipEventRecord(t0, ctx->stream));
+  std::vector<int> var((size_t)count);
+  PBF_CHECK(pbf_select(ctx, count, graphs, mem, model, var.data()));
+  if (variants) std::copy(var.begin(), var.end(), variants);
+  // parts: the graphs of each chosen ranking, each run as one union pass with the part's most
+  // chosen knob setting
+  std::vector<std::vector<int64_t>> parts(3);
+  for (int64_t g = 0; g < count; g++) parts[(size_t)var[(size_t)g] / 3].push_back(g);
+  int nparts = 0;
+  for (auto& p : parts) nparts += p.empty() ? 0 : 1;
+  const CountTune saved = ctx->tune;
+  auto use_knobs = [&](const std::vector<int64_t>& idx) {
+    int votes[3] = {0, 0, 0};
+    for (int64_t g : idx) votes[var[(size_t)g] % 3]++;
+    const int k = (int)(std::max_element(votes, votes + 3) - votes);
+    int rt, lm, lw, bpc;
+    (void)pbf_variant_tuning(k, &rt, &lm, &lw, &bpc);
+    ctx->tune.light_max = (uint32_t)lm;
+    ctx->tune.light_work = (uint32_t)lw;
+    ctx->tune.blocks_per_cu = (uint32_t)bpc;
+  };
+  struct Untune {
+    pbf_context* c;
+    CountTune t;
+    ~Untune() { c->tune = t; }
+  } untune{ctx, saved};
+  static const int rank_of_part[3] = {PBF_RANK_SIDE, PBF_RANK_ADEG, PBF_RANK_ACOCORE};
+  if (nparts == 1) {
+    for (int r = 0; r < 3; r++)
+      if (!parts[r].empty()) {
+        use_knobs(parts[r]);
+        PBF_CHECK(pbf_count_batch(ctx, count, graphs, mem, rank_of_part[r], PBF_BATCH_UNION, vert_out, vert_cap, vert_off,
+                                  totals, out_mem));
+      }
+    PBF_HIP(hipEventRecord(t1, ctx->stream));
+    PBF_HIP(hipEventSynchronize(t1));
+    ctx->timing.total_ms = elapsed_ms(t0, t1);
+    ctx->timing.count_ms = ctx->timing.total_ms;
+    return PBF_OK;
+  }
+  // several parts: each part's outputs packed into S_LEARN0, then placed in the caller's layout
+  void *lp, *pp;
+  const size_t part_words = (size_t)cap_need + 2 * (size_t)count + 3;
+  PBF_CHECK(pbf_scratch(ctx, S_LEARN0, 8 * part_words, &lp));
+  unsigned long long* pv = static_cast<unsigned long long*>(lp);
+  unsigned long long* pt = pv + cap_need;
+  unsigned long long* po = pt + count;  // per part: its count + 1 offsets, one part after another
+  std::vector<Place> place((size_t)count);
+  std::vector<uint64_t> ns((size_t)count);
+  std::vector<pbf_bipartite_csr> sub;
+  int64_t vbase = 0, tbase = 0;
+  for (int r = 0; r < 3; r++) {
+    const auto& idx = parts[r];
+    if (idx.empty()) continue;
+    sub.clear();
+    int64_t cap = 0;
+    for (int64_t g : idx) { sub.push_back(graphs[g]); cap += std::max(graphs[g].nv, graphs[g].nu); }
+    const int64_t c = (int64_t)idx.size();
+    use_knobs(idx);
+    PBF_CHECK(pbf_count_batch(ctx, c, sub.data(), mem, rank_of_part[r], PBF_BATCH_UNION,
+                              reinterpret_cast<uint64_t*>(pv + vbase), cap, reinterpret_cast<uint64_t*>(po + tbase + r),
+                              reinterpret_cast<uint64_t*>(pt + tbase), PBF_MEM_DEVICE));
+    std::vector<uint64_t> off((size_t)c + 1);
+    PBF_CHECK(pbf_d2h(ctx, off.data(), po + tbase + r, 8 * (size_t)(c + 1)));
+    for (int64_t j = 0; j < c; j++) {
+      const int64_t g = idx[(size_t)j];
+      ns[(size_t)g] = off[(size_t)j + 1] - off[(size_t)j];
+      place[(size_t)g].src = (uint64_t)vbase + off[(size_t)j];
+      place[(size_t)g].len = (uint32_t)ns[(size_t)g];
+      place[(size_t)g].src_tot = (uint32_t)(tbase + j);
+      place[(size_t)g].dst_tot = (uint32_t)g;
+    }
+    vbase += cap;
+    tbase += c;
+  }
+  std::vector<uint64_t> goff((size_t)count + 1, 0);
+  for (int64_t g = 0; g < count; g++) {
+    place[(size_t)g].dst = goff[(size_t)g];
+    goff[(size_t)g + 1] = goff[(size_t)g] + ns[(size_t)g];
+  }
+  PBF_CHECK(pbf_scratch(ctx, S_LEARN1, sizeof(Place) * (size_t)count, &pp));
+  PBF_CHECK(pbf_h2d(ctx, pp, place.data(), sizeof(Place) * (size_t)count));
+  unsigned long long *vdst = reinterpret_cast<unsigned long long*>(vert_out), *tdst = reinterpret_cast<unsigned long long*>(totals);
+  if (out_mem == PBF_MEM_HOST) {
+    void* fp;
+    PBF_CHECK(pbf_scratch(ctx, S_LEARN2, 8 * ((size_t)goff[(size_t)count] + (size_t)count), &fp));
+    vdst = static_cast<unsigned long long*>(fp);
+    tdst = vdst + goff[(size_t)count];
+  }
+  const unsigned gb = (unsigned)std::min<int64_t>(count, (int64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_place, dim3(gb), dim3(256), 0, ctx->stream, static_cast<const Place*>(pp), count, pv, pt, vdst, tdst);
+  PBF_HIP(hipGetLastError());
+  PBF_HIP(hipEventRecord(t1, ctx->stream));
+  if (out_mem == PBF_MEM_HOST) {
+    PBF_CHECK(pbf_d2h(ctx, vert_out, vdst, 8 * (size_t)goff[(size_t)count]));
+    PBF_CHECK(pbf_d2h(ctx, totals, tdst, 8 * (size_t)count));
+    memcpy(vert_off, goff.data(), 8 * (size_t)(count + 1));
+  } else {
+    PBF_CHECK(pbf_h2d(ctx, vert_off, goff.data(), 8 * (size_t)(count + 1)));
+  }
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->timing.total_ms = elapsed_ms(t0, t1);
+  ctx->timing.count_ms = ctx->timing.total_ms;
+  ctx->timing.wedges = -1;
   return PBF_OK;
 }

@@ -60,6 +60,9 @@ enum ScratchSlot {
   S_TEXT0,       // device reader: the file's bytes, tile token counts
   S_TEXT1,       // device reader: list-start flags, tile owners
   S_TEXT2,       // device reader: first-error key
+  S_LEARN0,      // learned batch: every part's packed outputs (counts, offsets, totals)
+  S_LEARN1,      // learned batch: placement table of the parts' graphs
+  S_LEARN2,      // learned batch: final outputs for host callers
   S_NUM_SLOTS
 };
 
@@ -72,7 +75,8 @@ struct CountTune {
 
 struct pbf_context {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where all work is queued: own_stream or the caller's
+  hipStream_t own_stream = nullptr;  // created with the context
   void* buf[S_NUM_SLOTS] = {};
   size_t cap[S_NUM_SLOTS] = {};
   hipEvent_t ev[8] = {};

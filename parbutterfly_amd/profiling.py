@@ -1,13 +1,24 @@
-"""HBM traffic of the counting phase from rocprofv3 counter runs.
+"""Profiling helpers: workloads to run under rocprofv3, and HBM traffic from its counter runs.
 
-Two ``rocprofv3 --pmc`` runs of the same command (one with FETCH_SIZE, one with WRITE_SIZE,
+    python -m parbutterfly_amd.profiling workload NAME [--reps N]
+    python -m parbutterfly_amd.profiling stats DIR [--top N]   # rocprofv3 --stats table
+
+runs one workload on device 0 (graph generated and uploaded first, one untimed warm-up call,
+then N calls), so that a kernel trace or a counter run covers that workload alone.  NAME:
+  vert-chunglu   per-vertex counts, Chung-Lu nv = 2^20, nu = 2^18, m = 2^24, gamma 2.1, seed 2, ADEG
+  edge-chunglu   per-edge counts of the same graph (ADEG)
+  edge-rmat      per-edge counts, R-MAT 2^22 x 2^22, m = 1.2e8, (.5, .1, .1, .3), seed 3, ACOCORE
+  tip-chunglu    the tip peel of the Chung-Lu graph from its per-vertex counts (counts untimed)
+  wing-rmat      the wing peel of the R-MAT graph from its per-edge counts (counts untimed)
+
+HBM traffic: two ``rocprofv3 --pmc`` runs of the same command (one with FETCH_SIZE, one with WRITE_SIZE,
 ``--output-format csv``) give per-dispatch KiB figures for every kernel.  Each counter is
 scaled by what it reads for ``pbf_copy_device`` of a known size (kernel ``k_copy``, which
 reads and writes exactly that many bytes), run under the same two counters.  The traffic of
 one counting phase is the sum over its kernels, averaged over the phases in the run (one per
 ``k_count2<.., 2>`` dispatch).
 
-    python -m parbutterfly_amd.profiling --fetch DIR --write DIR --cal-fetch DIR --cal-write DIR \\
+    python -m parbutterfly_amd.profiling traffic --fetch DIR --write DIR --cal-fetch DIR --cal-write DIR \\
         --cal-bytes N [--out JSON]
 """
 import argparse
@@ -71,15 +82,85 @@ def count_phase_traffic(fetch_dir, write_dir, cal_fetch, cal_write):
     return out
 
 
+def kernel_stats(run_dir, top=30):
+    """Markdown table of rocprofv3 --stats (kernel_stats.csv): calls, average and total time."""
+    files = glob.glob(os.path.join(run_dir, "**", "*kernel_stats.csv"), recursive=True)
+    if not files:
+        raise FileNotFoundError(f"no kernel_stats.csv under {run_dir}")
+    rows = defaultdict(lambda: [0, 0.0])
+    for fn in files:
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                name = r["Name"].replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0].strip()
+                rows[name][0] += int(r["Calls"])
+                rows[name][1] += float(r["TotalDurationNs"])
+    total = sum(v[1] for v in rows.values())
+    out = ["| kernel | calls | avg us | total ms | % |", "|---|---:|---:|---:|---:|"]
+    for name, (calls, ns) in sorted(rows.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| `{name}` | {calls} | {ns / calls / 1e3:.1f} | {ns / 1e6:.2f} | {100 * ns / total:.1f} |")
+    out.append(f"\nAll kernels: {total / 1e6:.2f} ms.")
+    return "\n".join(out)
+
+
+WORKLOADS = ("vert-chunglu", "edge-chunglu", "edge-rmat", "tip-chunglu", "wing-rmat")
+
+
+def run_workload(name, reps):
+    import sys
+    import time
+
+    import parbutterfly_amd as pb
+    if name not in WORKLOADS:
+        raise SystemExit(f"unknown workload {name}; one of {WORKLOADS}")
+    if name.endswith("chunglu"):
+        g = pb.Graph.chung_lu(1 << 20, 1 << 18, 1 << 24, 2.1, 2.1, 2)
+        rank = "ADEG"
+    else:
+        g = pb.Graph.rmat(22, 22, 120_000_000, 0.5, 0.1, 0.1, 0.3, 3)
+        rank = "ACOCORE"
+    ctx = pb.Context(0)
+    dg = g.to_device(ctx)
+    use_v, _ = ctx.side(dg)
+    per = "VERT" if name.startswith(("vert", "tip")) else "EDGE"
+    counts = ctx.count(dg, use_v, per, "GPU", rank)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        if name.startswith("tip"):
+            ctx.peel(dg, use_v, counts)
+        elif name.startswith("wing"):
+            ctx.peel_edge(dg, use_v, counts)
+        else:
+            ctx.count(dg, use_v, per, "GPU", rank, out=counts)
+    ctx.synchronize()
+    print(json.dumps({"workload": name, "reps": reps, "ms_per_call": (time.perf_counter() - t0) * 1e3 / reps,
+                      "timing": ctx.timing()}), flush=True)
+    sys.stdout.flush()
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--cal-fetch", required=True)
-    ap.add_argument("--cal-write", required=True)
-    ap.add_argument("--cal-bytes", type=int, required=True)
-    ap.add_argument("--out", default=None)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    w = sub.add_parser("workload")
+    w.add_argument("name")
+    w.add_argument("--reps", type=int, default=2)
+    st = sub.add_parser("stats")
+    st.add_argument("dir")
+    st.add_argument("--top", type=int, default=30)
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--cal-fetch", required=True)
+    t.add_argument("--cal-write", required=True)
+    t.add_argument("--cal-bytes", type=int, required=True)
+    t.add_argument("--out", default=None)
     a = ap.parse_args()
+    if a.cmd == "workload":
+        run_workload(a.name, a.reps)
+        return
+    if a.cmd == "stats":
+        print(kernel_stats(a.dir, a.top))
+        return
     cf = calibration(a.cal_fetch, "FETCH_SIZE", a.cal_bytes)
     cw = calibration(a.cal_write, "WRITE_SIZE", a.cal_bytes)
     res = count_phase_traffic(a.fetch, a.write, cf, cw)

@@ -136,43 +136,26 @@ def evaluate(ctx, model, F, T):
             "variant_histogram": np.bincount(pick, minlength=NVAR).tolist()}
 
 
-def run_learned(ctx, model, graphs):
-    """Batched + learned: features, selection, then one union pass per chosen ranking.
-    The knob half of a variant (table bounds, workgroups per CU) is tuned for one graph run
-    alone, where launch shape dominates; a union pass keeps the context's own knobs (default,
-    or the batch autotuner's).  Returns [(indices, outputs of that pass)] (device outputs for
-    device graphs) and the chosen variant of every graph."""
-    F = ctx.batch_features(graphs)
-    pick = predict(ctx, model, F).argmin(1)
-    parts = []
-    ranking = pick // 3
-    for r in np.unique(ranking):
-        idx = np.nonzero(ranking == r)[0]
-        rank, _ = variant_settings(int(r) * 3)
-        parts.append((idx, ctx.count_batch([graphs[k] for k in idx], rank, mode="union")))
-    return parts, pick
+def run_learned(ctx, model, graphs, outputs=None):
+    """Batched + learned through pbf_count_batch_learned: features, one batched inference, then
+    one union pass per chosen ranking with the knob setting most chosen among its graphs; the
+    outputs land in pbf_count_batch's layout.  Returns (outputs as Context.count_batch gives
+    them, the chosen variant of every graph)."""
+    return ctx.count_batch_learned(graphs, model, outputs)
 
 
-def gather(parts, n):
-    """Per-graph VERT arrays and totals (host) from run_learned's parts."""
-    verts = [None] * n
-    tots = np.zeros(n, np.uint64)
-    for idx, res in parts:
-        if len(res) == 3:  # device outputs
-            vout, voff, tot = (x.numpy() for x in res)
-            segs = [vout[int(voff[j]):int(voff[j + 1])] for j in range(len(idx))]
-        else:
-            segs, tot = res
-        for j, k in enumerate(idx):
-            verts[k] = segs[j]
-            tots[k] = tot[j]
-    return verts, tots
+def gather(res, n):
+    """Per-graph VERT arrays and totals (host) from run_learned's outputs."""
+    if len(res) == 3:  # device outputs
+        vout, voff, tot = (x.numpy() for x in res)
+        return [vout[int(voff[k]):int(voff[k + 1])] for k in range(n)], tot[:n]
+    return res
 
 
 def count_learned(ctx, model, graphs):
     """run_learned + gather: (per-graph VERT arrays, totals, chosen variants)."""
-    parts, pick = run_learned(ctx, model, graphs)
-    verts, tots = gather(parts, len(graphs))
+    res, pick = run_learned(ctx, model, graphs)
+    verts, tots = gather(res, len(graphs))
     return verts, tots, pick
 
 

@@ -83,3 +83,26 @@ def test_learned_batch_exact(ctx, heldout):
         ov = O.count_vert(g, use_v, "ADEG") if g.m else np.zeros(g.nu if use_v else g.nv, np.uint64)
         assert np.array_equal(np.asarray(verts[k], np.uint64), ov), f"graph {k} (seed {seeds[k]}) VERT"
         assert int(tots[k]) == int(ov.sum()) // 2, f"graph {k} total"
+
+
+def test_select_abi_and_fallback(ctx, heldout):
+    # pbf_select picks the argmin of the model's predictions; without a model, or with a model
+    # whose predictions are not finite, every graph gets the fixed variant
+    _, _, seeds = heldout
+    gs = [pb.Graph.batch_graph(int(s)) for s in seeds[:24]]
+    model = S.load()
+    v = ctx.select(gs, model)
+    assert np.array_equal(v, S.predict(ctx, model, ctx.batch_features(gs)).argmin(1))
+    assert np.all(ctx.select(gs, None) == pb.VARIANT_FIXED)
+    bad = dict(model)
+    bad["tsd"] = np.full(9, np.nan, np.float32)
+    assert np.all(ctx.select(gs, bad) == pb.VARIANT_FIXED)
+    # fixed fallback and learned runs give identical counts (host and device graphs)
+    (vf, tf), pf = ctx.count_batch_learned(gs, None)
+    assert np.all(pf == pb.VARIANT_FIXED)
+    outs, pl = ctx.count_batch_learned([g.to_device(ctx) for g in gs], model)
+    vl, tl = S.gather(outs, len(gs))
+    assert np.array_equal(pl, v)
+    assert np.array_equal(tf, tl)
+    for a, b in zip(vf, vl):
+        assert np.array_equal(a, b)
