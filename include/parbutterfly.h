@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 1
+#define PBF_ABI_VERSION 2
 
 typedef enum {
   PBF_OK = 0,
@@ -86,6 +86,7 @@ typedef struct {
   int64_t rounds;     /* peeling rounds */
   int64_t hub_vertices;   /* endpoints that took the HBM-histogram path */
   double sparsify_ms;     /* sparsification (pbf_sparsify / pbf_count_total_sparse) */
+  int64_t batches;        /* rank ranges the last count ran in (pbf_set_limits) */
 } pbf_timing;
 
 const char* pbf_status_str(int status);
@@ -147,6 +148,15 @@ int pbf_synchronize(pbf_context* ctx);
 int pbf_last_timing(pbf_context* ctx, pbf_timing* out);
 /* Releases cached device scratch. */
 int pbf_trim(pbf_context* ctx);
+/* Memory bounds of the counting passes, the reference's -m and -a (BipartiteButterflies.C:
+ * 174-175; 0 = no bound, the default).  max_wedges > 0: the ranked endpoints are counted in
+ * consecutive rank ranges of at most max_wedges wedges each, cut by the wedge prefix as
+ * getNextWedgeIdx does (butterfly_utils.h:1700-1714, 1948-1959; butterfly_count.h:1578-1587);
+ * an endpoint with more wedges on its own makes the count fail with PBF_ERR_CAPACITY (the
+ * reference exits there).  max_array_size > 0: the dense HBM histograms of the endpoints no
+ * LDS table holds use at most that many 32-bit counters, 2 (nv + nu) per concurrently counted
+ * endpoint, and at least one such endpoint's worth.  Results do not depend on either. */
+int pbf_set_limits(pbf_context* ctx, int64_t max_wedges, int64_t max_array_size);
 
 /* Device buffers on the context's device (for PBF_MEM_DEVICE callers without their own HIP
  * allocator).  Copies are synchronous with respect to the host and ordered on the context's

@@ -71,7 +71,7 @@ class Timing(ctypes.Structure):
     _fields_ = [("rank_ms", ctypes.c_double), ("relabel_ms", ctypes.c_double), ("count_ms", ctypes.c_double),
                 ("convert_ms", ctypes.c_double), ("peel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("wedges", ctypes.c_int64), ("rounds", ctypes.c_int64), ("hub_vertices", ctypes.c_int64),
-                ("sparsify_ms", ctypes.c_double)]
+                ("sparsify_ms", ctypes.c_double), ("batches", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -126,6 +126,7 @@ _SIGS = {
     "pbf_synchronize": (_INT, [_P]),
     "pbf_last_timing": (_INT, [_P, ctypes.POINTER(Timing)]),
     "pbf_trim": (_INT, [_P]),
+    "pbf_set_limits": (_INT, [_P, _I64, _I64]),
     "pbf_device_alloc": (_INT, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "pbf_device_free": (_INT, [_P, _P]),
     "pbf_copy_to_device": (_INT, [_P, _P, _P, ctypes.c_size_t]),
@@ -437,6 +438,10 @@ class Context:
 
     def trim(self):
         _check(_lib.pbf_trim(self._ctx), "pbf_trim")
+
+    def set_limits(self, max_wedges=0, max_array_size=0):
+        """pbf_set_limits: the reference's -m / -a bounds of the counting passes (0 = none)."""
+        _check(_lib.pbf_set_limits(self._ctx, int(max_wedges), int(max_array_size)), "pbf_set_limits")
 
     def timing(self):
         t = Timing()

@@ -306,6 +306,13 @@ extern "C" int pbf_trim(pbf_context* ctx) {
   return PBF_OK;
 }
 
+extern "C" int pbf_set_limits(pbf_context* ctx, int64_t max_wedges, int64_t max_array_size) {
+  if (!ctx || max_wedges < 0 || max_array_size < 0) return PBF_ERR_ARG;
+  ctx->max_wedges = max_wedges;
+  ctx->max_array = max_array_size;
+  return PBF_OK;
+}
+
 extern "C" int pbf_destroy(pbf_context* ctx) {
   if (!ctx) return PBF_ERR_ARG;
   DeviceGuard guard(ctx->device);

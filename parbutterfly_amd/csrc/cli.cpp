@@ -256,6 +256,12 @@ int main(int argc, char** argv) {
   pbf_context* ctx = nullptr;
   int st = pbf_create(&ctx, (int)P.num("-device", 0));
   if (st) return fail("pbf_create", st);
+  // -m / -a (:174-175) bound the counting passes only when given: the reference's defaults
+  // exist to bound its CPU scratch, which the GPU path does not have
+  if (P.has("-m") || P.has("-a")) {
+    st = pbf_set_limits(ctx, P.num("-m", 0), P.num("-a", 0));
+    if (st) { pbf_destroy(ctx); return fail("pbf_set_limits", st); }
+  }
   int rc = 0;
   const bool host_read = P.has("-hostRead");
   const int mem = host_read ? PBF_MEM_HOST : PBF_MEM_DEVICE;

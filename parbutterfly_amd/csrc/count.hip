@@ -19,6 +19,9 @@
 // aggregated in LDS: dense counters indexed by u2 when u2 < i is small, otherwise an
 // open-addressing hash.  Counts are scattered with 64-bit atomics; integer adds commute, so
 // the results are exact and deterministic.
+#include <algorithm>
+#include <vector>
+
 #include "pbf_device.h"
 #include "team.h"
 
@@ -1091,6 +1094,7 @@ struct ClassOut {
   uint32_t light_max;   // <= LIGHT_MAX
   uint32_t light_work;
   const uint32_t* lo;   // graph-local ranks (CountArgs::lo), or null
+  uint32_t rlo, rhi;    // only endpoints with ranks in [rlo, rhi) are listed (a batch)
 };
 
 __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, const ClassOut& co) {
@@ -1102,6 +1106,7 @@ __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t
   const uint32_t cb = co.cb;
   const uint32_t lane = threadIdx.x & 63;
   uint32_t cls = 0;  // 0 none, 1 wavefront, 2 workgroup, 3 HBM
+  valid = valid && i >= co.rlo && i < co.rhi;
   if (valid && T) {
     const uint32_t il = co.lo ? i - co.lo[i] : i;  // distinct keys are local ranks below il
     const uint32_t b = (uint32_t)(T < (uint64_t)il ? T : (uint64_t)il);
@@ -1167,11 +1172,12 @@ __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ r
     wave_lds_fence();
     const uint64_t T = s_T[wid][lane];
     classify_append((uint32_t)i, i < n && small, T, deg, co);
-    const uint64_t mb = __ballot(!small);
+    const bool big_item = !small && i >= (int64_t)co.rlo && i < (int64_t)co.rhi;
+    const uint64_t mb = __ballot(big_item);
     uint32_t bb = 0;
     if (lane == 0 && mb) bb = atomicAdd(&co.lens[2], (uint32_t)__popcll(mb));
     bb = __shfl(bb, 0, 64);
-    if (!small) {
+    if (big_item) {
       const uint32_t slot = bb + __popcll(mb & pbf_lanemask_lt());
       big[slot] = (uint32_t)i;
       big_chunks[slot] = (deg + CL_CHUNK - 1) / CL_CHUNK;
@@ -1536,10 +1542,55 @@ inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
 
 constexpr int HBM_TEAM = 256, HBM_BLOCKS = 32, HBM_BLOCKS_NOSYNC = 4;
 
+// Wedges of every endpoint (the sum of its slots' segment lengths), a wavefront per endpoint.
+__global__ __launch_bounds__(256) void k_endpoint_wedges(const uint32_t* __restrict__ roff, const uint2* __restrict__ sinfo,
+                                                         int64_t n, unsigned long long* __restrict__ T) {
+  const uint32_t lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += nw) {
+    const uint32_t b = roff[i], e = roff[i + 1];
+    unsigned long long t = 0;
+    for (uint32_t p = b + lane; p < e; p += 64) t += sinfo[p].y;
+    t = pbf_wave_sum(t);
+    if (lane == 0) T[i] = t;
+  }
+}
+
+// Rank ranges of at most max_wedges wedges each, consecutive and greedy (getNextWedgeIdx,
+// butterfly_utils.h:1700-1714, 1948-1959: the binary search over the wedge prefix for the
+// first endpoint that no longer fits).  An endpoint with more wedges than max_wedges on its own
+// is a capacity error (the reference exits: "Space must accomodate seagulls originating from
+// one vertex").
+int wedge_batches(pbf_context* ctx, const RankedGraph& R, const uint2* sinfo, int64_t max_wedges,
+                  std::vector<uint32_t>* bounds) {
+  const int64_t n = R.n;
+  void* tp;
+  PBF_CHECK(pbf_scratch(ctx, S_WBATCH, 16 * (size_t)(n + 1), &tp));
+  unsigned long long* T = static_cast<unsigned long long*>(tp);
+  unsigned long long* P = T + n + 1;
+  hipLaunchKernelGGL(k_endpoint_wedges, dim3(grid_for(n, 4, 8192)), dim3(256), 0, ctx->stream, R.roff, sinfo, n, T);
+  PBF_CHECK(pbf_scan_excl_u64(ctx, reinterpret_cast<const uint64_t*>(T), reinterpret_cast<uint64_t*>(P), n,
+                              reinterpret_cast<uint64_t*>(P + n)));
+  std::vector<uint64_t> pre((size_t)n + 1);
+  PBF_CHECK(pbf_d2h(ctx, pre.data(), P, 8 * (size_t)(n + 1)));
+  bounds->assign(1, 0u);
+  int64_t lo = 0;
+  while (lo < n) {
+    // largest hi with pre[hi] - pre[lo] <= max_wedges
+    const uint64_t lim = pre[(size_t)lo] + (uint64_t)max_wedges;
+    const int64_t hi = (int64_t)(std::upper_bound(pre.begin() + lo, pre.end(), lim) - pre.begin()) - 1;
+    if (hi <= lo) return PBF_ERR_CAPACITY;
+    bounds->push_back((uint32_t)hi);
+    lo = hi;
+  }
+  return PBF_OK;
+}
+
 // Runs the counting phase: classification, the fused kernel, the hub fold and (only if some
-// endpoint needs it) the HBM fallback.  lens/queues are cleared with a fill kernel.
+// endpoint needs it) the HBM fallback.  lens/queues are cleared with a fill kernel.  With a
+// wedge bound (ctx->max_wedges) the endpoints are counted rank range by rank range.
 template <int MODE>
-int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lists, uint32_t* ctl, bool sync) {
+int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* lists, uint32_t* ctl, bool sync) {
   const int64_t n = R.n;
   // ctl: [0] light len, [1] heavy len, [2] big len, [3] heavy queue, [4] light-group queue,
   //      [5] light groups, [6] re-run len, [7] total big chunks, [8] split len,
@@ -1560,7 +1611,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
     uint2* rinfo = static_cast<uint2*>(rip);
     hipLaunchKernelGGL(k_rinfo, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.inpref, n, rinfo);
     hipLaunchKernelGGL(k_slotinfo, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, rinfo, R.redg, R.rown, R.lo,
-                       R.m2, const_cast<uint2*>(a.sinfo));
+                       R.m2, const_cast<uint2*>(a0.sinfo));
   }
   PBF_CHECK(pbf_scratch(ctx, S_LPLAN, 4 * (64 + 2 * LG_BLOCKS * LBUCKETS) + 8 * LBUCKETS, &pl));
   uint32_t* plan = static_cast<uint32_t*>(pl);
@@ -1570,21 +1621,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   uint4* lrec = static_cast<uint4*>(lrp);
   uint32_t* bo = bh + LG_BLOCKS * LBUCKETS;
   unsigned long long* wbase = reinterpret_cast<unsigned long long*>(bo + LG_BLOCKS * LBUCKETS);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctl, 0u, (int64_t)64);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, plan, 0u, (int64_t)64);
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
   uint32_t* hbm_list = lists + 5 * n + 1;
-  // light-table bound: the context's, or the mode's default (per-edge counting has a second
-  // pass with a 64-bit credit per wedge for every endpoint, and keeps larger tables light)
-  const uint32_t want = ctx->tune.light_max ? ctx->tune.light_max : (MODE == MODE_EDGE ? LIGHT_MAX : 767u);
-  const uint32_t lmax = want < LIGHT_MAX ? want : LIGHT_MAX;
-  const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb, lmax, ctx->tune.light_work, R.lo};
-  const unsigned grid = (unsigned)(ctx->num_cus * (ctx->tune.blocks_per_cu ? ctx->tune.blocks_per_cu : 4));
-  hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
-                     n, co, big, big_chunks, w64);
-  PBF_CHECK(pbf_scan_excl_u32(ctx, big_chunks, chunk_base, n, ctl + 7));
-  hipLaunchKernelGGL(k_wedge_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
-                     big, chunk_base, ctl, ctl + 7, w64);
   // split endpoints: list, chunk counts, chunk bases, dense rows
   void *sp, *gp;
   // a single graph has at most SPLIT_IMAX split endpoints (distinct ranks below it); a union
@@ -1596,89 +1633,113 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a, uint32_t* lis
   uint32_t* split_chunks = split + max_split;
   uint32_t* split_base = split + 2 * max_split;
   uint32_t* G = static_cast<uint32_t*>(gp);
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split, 256, 64)), dim3(256), 0, ctx->stream, split_chunks, 0u, max_split);
-  hipLaunchKernelGGL(k_classify_big, dim3(64), dim3(256), 0, ctx->stream, R.roff, big, ctl, w64, co, split, split_chunks,
-                     (uint32_t)max_split);
-  PBF_CHECK(pbf_scan_excl_u32(ctx, split_chunks, split_base, max_split, ctl + 9));
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split * SPLIT_IMAX, 256, 4096)), dim3(256), 0, ctx->stream, G, 0u,
-                     max_split * (int64_t)SPLIT_IMAX);
-  hipLaunchKernelGGL(k_split_pass1<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G,
-                     (uint32_t)max_split);
-  hipLaunchKernelGGL(k_split_credit<MODE>, dim3(256), dim3(TPB), 0, ctx->stream, a, split, ctl, G, (uint32_t)max_split);
-  if (MODE != MODE_TOTAL)
-    hipLaunchKernelGGL(k_split_pass2<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G,
+  // light-table bound: the context's, or the mode's default (per-edge counting has a second
+  // pass with a 64-bit credit per wedge for every endpoint, and keeps larger tables light)
+  const uint32_t want = ctx->tune.light_max ? ctx->tune.light_max : (MODE == MODE_EDGE ? LIGHT_MAX : 767u);
+  const uint32_t lmax = want < LIGHT_MAX ? want : LIGHT_MAX;
+  const unsigned grid = (unsigned)(ctx->num_cus * (ctx->tune.blocks_per_cu ? ctx->tune.blocks_per_cu : 4));
+  std::vector<uint32_t> bounds{0u, (uint32_t)n};
+  if (ctx->max_wedges > 0 && n > 0) PBF_CHECK(wedge_batches(ctx, R, a0.sinfo, ctx->max_wedges, &bounds));
+  ctx->timing.batches = (int64_t)bounds.size() - 1;
+  ctx->timing.hub_vertices = sync ? 0 : -1;
+  for (size_t bi = 0; bi + 1 < bounds.size(); bi++) {
+    CountArgs a = a0;
+    const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb, lmax, ctx->tune.light_work,
+                      R.lo, bounds[bi], bounds[bi + 1]};
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctl, 0u, (int64_t)64);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, plan, 0u, (int64_t)64);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, big_chunks, 0u, n);
+    hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
+                       n, co, big, big_chunks, w64);
+    PBF_CHECK(pbf_scan_excl_u32(ctx, big_chunks, chunk_base, n, ctl + 7));
+    hipLaunchKernelGGL(k_wedge_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, R.roff, R.redg, R.inpref, a.sinfo,
+                       big, chunk_base, ctl, ctl + 7, w64);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split, 256, 64)), dim3(256), 0, ctx->stream, split_chunks, 0u, max_split);
+    hipLaunchKernelGGL(k_classify_big, dim3(64), dim3(256), 0, ctx->stream, R.roff, big, ctl, w64, co, split, split_chunks,
                        (uint32_t)max_split);
-  // light endpoints: sort into groups
-  hipLaunchKernelGGL(k_light_hist<MODE>, dim3(LG_BLOCKS), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound, lists, ctl,
-                     bh);
-  hipLaunchKernelGGL(k_light_plan, dim3(1), dim3(LG_BLOCKS), 0, ctx->stream, bh, bo, plan, wbase, ctl, lists + 4 * n);
-  hipLaunchKernelGGL(k_light_scatter<MODE>, dim3(LG_BLOCKS), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound, lists, ctl,
-                     bo, plan, wbase, lists + 3 * n, lrec, lists + 4 * n);
-  a.wlist = lists + 3 * n;
-  a.gstart = lists + 4 * n;
-  a.lrec = lrec;
-  a.blist = lists + n;
-  a.nblock = ctl + 1;
-  a.ngroups = ctl + 5;
-  a.ngroups0 = ctl + 10;
-  a.queues = ctl + 3;
-  a.ovf_list = hbm_list;
-  a.ovf_len = ctl + 11;
-  a.rerun_list = lists + 2 * n;
-  a.rerun_len = ctl + 6;
-  a.rerun = 0;
-  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
-  hipLaunchKernelGGL((k_count2<MODE, 2>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
-  // light endpoints whose optimistic table overflowed: again, one workgroup each, full tables
-  a.blist = lists + 2 * n;
-  a.nblock = ctl + 6;
-  a.ngroups = ctl + 14;
-  a.ngroups0 = ctl + 14;
-  a.queues = ctl + 16;
-  a.rerun = 1;
-  hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
-  PBF_HIP(hipGetLastError());
-  if (MODE != MODE_EDGE && MODE != MODE_TOTAL)
-    hipLaunchKernelGGL(k_hot_reduce, dim3(grid_for(HOT_N, 256, 64)), dim3(256), 0, ctx->stream, a.hot, a.B, n);
-  uint32_t novf = 0;
-  // each block needs 2 kn words of dense counters: with small key ranges (a union of small
-  // graphs) a 256 MB pool holds many more blocks
-  const int64_t fit = std::min<int64_t>((int64_t)ctx->num_cus * 4, ((int64_t)256 << 20) / (8 * std::max<int64_t>(a.kn, 1)));
-  int blocks = (int)std::max<int64_t>(HBM_BLOCKS_NOSYNC, fit);
-  if (sync) {
-    PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 11, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    PBF_HIP(hipStreamSynchronize(ctx->stream));
-    novf = (uint32_t)(ctx->pinned[16] & 0xFFFFFFFFu);
-    ctx->timing.hub_vertices = novf;
-    const int64_t most = std::max<int64_t>(HBM_BLOCKS, fit);
-    blocks = (int)std::min<int64_t>((int64_t)novf, most);
-    // a fixed scratch region holds as many blocks' counters as fit (the work queue serves any
-    // number of blocks); not even one is a capacity error
-    const size_t room = pbf_scratch_room(ctx), per_block = sizeof(uint32_t) * 2 * (size_t)a.kn;
-    if (novf && room != SIZE_MAX) {
-      const size_t fit_room = room > 512 ? (room - 512) / per_block : 0;
-      if (fit_room == 0) return PBF_ERR_CAPACITY;
-      blocks = (int)std::min<size_t>((size_t)blocks, fit_room);
-    }
-  } else {
-    ctx->timing.hub_vertices = -1;  // not read back
-  }
-  if (novf || !sync) {
-    // endpoints no LDS table could hold: dense HBM histograms (every counter and used-list
-    // entry is restored to zero after its endpoint, so the pool is cleared only when first
-    // allocated)
-    void* pp;
-    const size_t pool_bytes = sizeof(uint32_t) * 2 * (size_t)a.kn * blocks;
-    PBF_CHECK(pbf_scratch(ctx, S_POOL, pool_bytes, &pp));
-    if (ctx->pool_clean < pool_bytes) {
-      PBF_CHECK(pbf_fill_u32(ctx, static_cast<uint32_t*>(pp), 0u, (int64_t)(pool_bytes / 4)));
-      ctx->pool_clean = pool_bytes;
-    }
-    a.pool = static_cast<uint32_t*>(pp);
-    a.list = hbm_list; a.list_len = ctl + 11; a.queue = ctl + 13;
-    hipLaunchKernelGGL((k_count_hbm<MODE, HBM_TEAM>), dim3(blocks), dim3(HBM_TEAM), 0, ctx->stream, a);
+    PBF_CHECK(pbf_scan_excl_u32(ctx, split_chunks, split_base, max_split, ctl + 9));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(max_split * SPLIT_IMAX, 256, 4096)), dim3(256), 0, ctx->stream, G, 0u,
+                       max_split * (int64_t)SPLIT_IMAX);
+    hipLaunchKernelGGL(k_split_pass1<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G,
+                       (uint32_t)max_split);
+    hipLaunchKernelGGL(k_split_credit<MODE>, dim3(256), dim3(TPB), 0, ctx->stream, a, split, ctl, G, (uint32_t)max_split);
+    if (MODE != MODE_TOTAL)
+      hipLaunchKernelGGL(k_split_pass2<MODE>, dim3(ctx->num_cus * 2), dim3(TPB), 0, ctx->stream, a, split, split_base, ctl, G,
+                         (uint32_t)max_split);
+    // light endpoints: sort into groups
+    hipLaunchKernelGGL(k_light_hist<MODE>, dim3(LG_BLOCKS), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound, lists, ctl,
+                       bh);
+    hipLaunchKernelGGL(k_light_plan, dim3(1), dim3(LG_BLOCKS), 0, ctx->stream, bh, bo, plan, wbase, ctl, lists + 4 * n);
+    hipLaunchKernelGGL(k_light_scatter<MODE>, dim3(LG_BLOCKS), dim3(256), 0, ctx->stream, R.roff, R.redg, a.bound, lists, ctl,
+                       bo, plan, wbase, lists + 3 * n, lrec, lists + 4 * n);
+    a.wlist = lists + 3 * n;
+    a.gstart = lists + 4 * n;
+    a.lrec = lrec;
+    a.blist = lists + n;
+    a.nblock = ctl + 1;
+    a.ngroups = ctl + 5;
+    a.ngroups0 = ctl + 10;
+    a.queues = ctl + 3;
+    a.ovf_list = hbm_list;
+    a.ovf_len = ctl + 11;
+    a.rerun_list = lists + 2 * n;
+    a.rerun_len = ctl + 6;
+    a.rerun = 0;
+    hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_count2<MODE, 2>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
+    // light endpoints whose optimistic table overflowed: again, one workgroup each, full tables
+    a.blist = lists + 2 * n;
+    a.nblock = ctl + 6;
+    a.ngroups = ctl + 14;
+    a.ngroups0 = ctl + 14;
+    a.queues = ctl + 16;
+    a.rerun = 1;
+    hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
     PBF_HIP(hipGetLastError());
+    uint32_t novf = 0;
+    // each block needs 2 kn words of dense counters: with small key ranges (a union of small
+    // graphs) a 256 MB pool holds many more blocks
+    const size_t per_block = sizeof(uint32_t) * 2 * (size_t)a.kn;
+    const int64_t fit = std::min<int64_t>((int64_t)ctx->num_cus * 4, ((int64_t)256 << 20) / (int64_t)per_block);
+    int blocks = (int)std::max<int64_t>(HBM_BLOCKS_NOSYNC, fit);
+    if (sync) {
+      PBF_HIP(hipMemcpyAsync(ctx->pinned + 16, ctl + 11, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+      PBF_HIP(hipStreamSynchronize(ctx->stream));
+      novf = (uint32_t)(ctx->pinned[16] & 0xFFFFFFFFu);
+      ctx->timing.hub_vertices += novf;
+      const int64_t most = std::max<int64_t>(HBM_BLOCKS, fit);
+      blocks = (int)std::min<int64_t>((int64_t)novf, most);
+      // a fixed scratch region holds as many blocks' counters as fit (the work queue serves any
+      // number of blocks); not even one is a capacity error
+      const size_t room = pbf_scratch_room(ctx);
+      if (novf && room != SIZE_MAX) {
+        const size_t fit_room = room > 512 ? (room - 512) / per_block : 0;
+        if (fit_room == 0) return PBF_ERR_CAPACITY;
+        blocks = (int)std::min<size_t>((size_t)blocks, fit_room);
+      }
+    }
+    // an array bound (-a) caps the histograms: as many blocks as it holds, at least one
+    if (ctx->max_array > 0)
+      blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, ctx->max_array / (2 * std::max<int64_t>(a.kn, 1))));
+    if (novf || !sync) {
+      // endpoints no LDS table could hold: dense HBM histograms (every counter and used-list
+      // entry is restored to zero after its endpoint, so the pool is cleared only when first
+      // allocated)
+      void* pp;
+      const size_t pool_bytes = per_block * blocks;
+      PBF_CHECK(pbf_scratch(ctx, S_POOL, pool_bytes, &pp));
+      if (ctx->pool_clean < pool_bytes) {
+        PBF_CHECK(pbf_fill_u32(ctx, static_cast<uint32_t*>(pp), 0u, (int64_t)(pool_bytes / 4)));
+        ctx->pool_clean = pool_bytes;
+      }
+      a.pool = static_cast<uint32_t*>(pp);
+      a.list = hbm_list; a.list_len = ctl + 11; a.queue = ctl + 13;
+      hipLaunchKernelGGL((k_count_hbm<MODE, HBM_TEAM>), dim3(blocks), dim3(HBM_TEAM), 0, ctx->stream, a);
+      PBF_HIP(hipGetLastError());
+    }
   }
+  if (MODE != MODE_EDGE && MODE != MODE_TOTAL)
+    hipLaunchKernelGGL(k_hot_reduce, dim3(grid_for(HOT_N, 256, 64)), dim3(256), 0, ctx->stream, a0.hot, a0.B, n);
   return PBF_OK;
 }
 }  // namespace

@@ -63,6 +63,7 @@ enum ScratchSlot {
   S_LEARN0,      // learned batch: every part's packed outputs (counts, offsets, totals)
   S_LEARN1,      // learned batch: placement table of the parts' graphs
   S_LEARN2,      // learned batch: final outputs for host callers
+  S_WBATCH,      // wedge-bounded counting: wedges per endpoint and their prefix
   S_NUM_SLOTS
 };
 
@@ -86,6 +87,11 @@ struct pbf_context {
   bool hot_clear = false;  // S_HOT has been zeroed
   size_t pool_clean = 0;   // bytes of S_POOL known to be zero (the HBM kernel leaves them zero)
   CountTune tune;
+  // Memory bounds of the counting pass (pbf_set_limits; 0 = none): endpoints are counted in
+  // rank ranges of at most max_wedges wedges, and the HBM-fallback histograms hold at most
+  // max_array 32-bit counters (one endpoint's worth at least).
+  int64_t max_wedges = 0;
+  int64_t max_array = 0;
   void* batch = nullptr;   // batch state (batch.hip)
   // pinned staging for host transfers (pbf_h2d / pbf_d2h): two buffers in turn, so the host
   // copy of one chunk overlaps the DMA of the other

@@ -37,3 +37,17 @@ def hub_wrap_graph():
     v = np.concatenate([np.zeros(nu, np.uint32), np.repeat(np.arange(1, 1000, dtype=np.uint32), 2)])
     u = np.concatenate([np.arange(nu, dtype=np.uint32), np.tile(np.array([0, 1], np.uint32), 999)])
     return pb.Graph.from_edges(1000, nu, v, u)
+
+
+def hbm_graph(keys, pad):
+    """S = U (wV <= wU) under SIDE ranking: U vertex `keys + pad` shares the V hub v0 with U
+    vertices 0..keys-1, so as an endpoint it has `keys` distinct keys at a rank above 16384
+    (beyond every LDS table: the HBM-histogram path).  A U hub with keys + 1 leaves in V makes
+    wU > wV."""
+    i = keys + pad
+    v = [0] * (keys + 1)
+    u = list(range(keys)) + [i]
+    leaves = keys + 1
+    v += list(range(1, 1 + leaves))
+    u += [i + 1] * leaves
+    return pb.Graph.from_edges(1 + leaves, i + 2, np.array(v, np.uint32), np.array(u, np.uint32))

@@ -8,7 +8,7 @@ import pytest
 
 import parbutterfly_amd as pb
 from oracle import oracle as O
-from tests.graphs import complete_bipartite, empty_graph, small_graphs
+from tests.graphs import complete_bipartite, empty_graph, hbm_graph, small_graphs
 
 pytestmark = pytest.mark.gpu
 
@@ -157,24 +157,10 @@ def test_side_ranking(ctx):
         assert w == int((d * (d - 1) // 2).sum()), tag
 
 
-def _hbm_graph(keys, pad):
-    """S = U (wV <= wU) under SIDE ranking: U vertex `keys + pad` shares the V hub v0 with U
-    vertices 0..keys-1, so as an endpoint it has `keys` distinct keys at a rank above 16384
-    (beyond every LDS table: the HBM-histogram path).  A U hub with keys + 1 leaves in V makes
-    wU > wV."""
-    i = keys + pad
-    v = [0] * (keys + 1)
-    u = list(range(keys)) + [i]
-    leaves = keys + 1
-    v += list(range(1, 1 + leaves))
-    u += [i + 1] * leaves
-    return pb.Graph.from_edges(1 + leaves, i + 2, np.array(v, np.uint32), np.array(u, np.uint32))
-
-
 def test_hbm_fallback_reuse(ctx):
     # the HBM path must leave its pool slice zero, used lists included: a later call with
     # another vertex count lays its counters over that storage
-    big, small = _hbm_graph(45000, 5000), _hbm_graph(10000, 10000)
+    big, small = hbm_graph(45000, 5000), hbm_graph(10000, 10000)
     for g in (big, small):
         use_v, _ = ctx.side(g)
         assert use_v

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_status_strings():
-    assert pb._lib.pbf_abi_version() == 1
+    assert pb._lib.pbf_abi_version() == 2
     for s in range(11):
         assert pb._lib.pbf_status_str(s)
 
