@@ -30,6 +30,11 @@ using namespace pbf;
 namespace {
 enum TableKind { T_LDS = 0, T_HBM = 1 };
 
+// The lists, their lengths and the group counts a kernel reads (list_len, nblock, ngroups,
+// rerun and overflow lists) are written by earlier launches on the stream (classification,
+// grouping, the previous counting launch); the launch boundary orders those writes before any
+// load, so plain loads read them.  Within a launch, workgroups share only the work queues,
+// through device-scope atomics whose returned values carry the claims.
 struct CountArgs {
   const uint32_t* roff;
   const uint32_t* redg;

@@ -26,10 +26,13 @@ constexpr unsigned long long KNONE = ~0ull;
 // work[x] of a peeled item: its walk cost; bit 31 is the side an edge walk starts from
 constexpr uint32_t WORK_MASK = 0x7FFFFFFFu, WORK_SSIDE = 0x80000000u;
 
-// Control words and list entries written by earlier launches (or other workgroups) are read
-// with device-scope atomic loads, which are vector loads served by the device-coherent level:
-// a plain load of a uniform address compiles to a scalar load, and such loads were seen to
-// return values older than the previous launch's writes.
+// Control words, list entries and walk targets are read with relaxed agent-scope atomic loads
+// (vector loads; a plain load of a uniform address would be a scalar-cache load).  Every such
+// value is written by an earlier launch on the stream: no kernel here reads what another
+// workgroup of the same launch stored, since within a launch workgroups share data only through
+// device-scope atomics whose returned values carry it (queue claims, list reservations).  The
+// end of the writing launch and the start of the reading one already order those writes before
+// any load, plain or atomic; the atomic form is the one these kernels have been tested with.
 __device__ __forceinline__ uint32_t dev_ld(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
