@@ -10,7 +10,13 @@ value = W / (seconds per step) / 1e9, W = rank-filtered wedges (rankWedgeF) unde
 roofline: the counting phase, algorithmic bytes B_alg(VERT) = 2 * (4 W + 32 m) + 4 (n + 1) + 8 n_S
 over its duration measured with HIP events on the library's stream, against 8 TB/s.
 cpu_baseline: the reference's own BATCHS path (oracle/_ref, compiled from the reference's
-headers) on the same graph, all host cores, median of its timed rounds.
+headers) on the same graph, all host cores (at most 16: a one-GPU job's share on the GPU hosts),
+median of 3 timed rounds after a warm-up, compared bit-exactly.  The other sections carry their
+own baselines beside the GPU figures: per_edge (CountE, same graph), rmat_count_peel (CountE on
+the same graph with its peak memory; PeelE on a bounded R-MAT sample), tip_peel (Peel on a
+bounded Chung-Lu sample), batched (the per-graph loop over the first 512 graphs) and config1
+(the CPU-runnable case: SERIAL on one core and BATCHS).  Peels report their algorithmic bytes
+(tip: 4 sum_O deg^2 + 32 m; wing: 8 sum_e deg(u) deg(v)) against the HBM peak.
 
 traffic: HBM bytes of one counting phase from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, each
 scaled by the same counter's reading for pbf_copy_device of a known size (--calibrate-copy),
@@ -40,28 +46,53 @@ def b_alg_vert(W, m, n, ns):
     return 2 * (4 * W + 32 * m) + 4 * (n + 1) + 8 * ns
 
 
-def cpu_baseline(graph, use_v, rounds=2):
-    """Times the compiled reference (BATCHS, ADEG, per vertex) on the same graph."""
+def host_cores():
+    """Host threads for the CPU baselines: the cores this process may use, at most 16, which is
+    the CPU share a one-GPU job gets on the GPU hosts (their nproc can show many more)."""
+    cores = os.cpu_count() or 1
+    try:
+        cores = min(cores, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    return max(1, min(cores, 16))
+
+
+def ref_run(op, graph, count_type, rank_type, rounds=1, threads=None, peel_type="BATCHS", timeout=900):
+    """Runs the compiled reference (oracle/_ref) on a graph (or, for op "vertlist", a list of
+    graphs); returns (use_v, counts, peel, info, seconds), seconds = the median of the timed
+    rounds after the first (the warm-up) when there are several.  None when not built."""
     from oracle import oracle as O
     if not O.ref_available():
         return None
-    cores = os.cpu_count() or 1
-    try:
-        aff = len(os.sched_getaffinity(0))
-        cores = min(cores, aff)
-    except AttributeError:
-        pass
-    cores = min(cores, 16)
+    import resource
     with tempfile.TemporaryDirectory() as td:
-        gp = os.path.join(td, "g.bcsr")
-        graph.write_binary(gp)
-        t0 = time.time()
-        uv, _, counts, _, info = O.run_ref("vert", gp, os.path.join(td, "o.bin"), "BATCHS", "ADEG",
-                                           rounds=rounds, threads=cores, timeout=900)
-        wall = time.time() - t0
-    secs = sorted(info["seconds"])
-    return dict(seconds=secs[len(secs) // 2], all_seconds=info["seconds"], cores=cores, wall=wall,
-                use_v=uv, counts=counts)
+        if op == "vertlist":
+            paths = []
+            for k, g in enumerate(graph):
+                paths.append(os.path.join(td, f"g{k}.bcsr"))
+                g.write_binary(paths[-1])
+            gp = os.path.join(td, "list.txt")
+            with open(gp, "w") as f:
+                f.write("\n".join(paths) + "\n")
+        else:
+            gp = os.path.join(td, "g.bcsr")
+            graph.write_binary(gp)
+        uv, _, counts, peel, info = O.run_ref(op, gp, os.path.join(td, "o.bin"), count_type, rank_type,
+                                              peel_type=peel_type, rounds=rounds, threads=threads or host_cores(),
+                                              timeout=timeout)
+    info["max_rss_gb"] = round(resource.getrusage(resource.RUSAGE_CHILDREN).ru_maxrss / 2 ** 20, 2)
+    secs = info["peel_seconds"] if op in ("peelv", "peele") else info["seconds"]
+    timed = secs[1:] if len(secs) > 1 else secs
+    return uv, counts, peel, info, sorted(timed)[len(timed) // 2]
+
+
+def cpu_baseline(graph, use_v, rounds=4):
+    """The reference's BATCHS / ADEG Count on the headline graph, all host cores."""
+    r = ref_run("vert", graph, "BATCHS", "ADEG", rounds=rounds, timeout=900)
+    if r is None:
+        return None
+    uv, counts, _, info, sec = r
+    return dict(seconds=sec, all_seconds=info["seconds"], cores=host_cores(), use_v=uv, counts=counts)
 
 
 def gen_batch(seeds, workers=16):
@@ -73,7 +104,7 @@ def gen_batch(seeds, workers=16):
         return list(ex.map(pb.Graph.batch_graph, seeds))
 
 
-def batched_bench(pb, ctx, count=4096, first_seed=1000):
+def batched_bench(pb, ctx, count=4096, first_seed=1000, cpu_sample=512):
     """Batched workload: `count` graphs (seed first_seed + g; nv, nu log-uniform in [64, 65536],
     average degree log-uniform in [2, 32], exponent uniform in [2, 3]), all CSRs resident in HBM,
     per-graph VERT and TOTAL counts into HBM.  graphs/s of the plain per-graph loop, the
@@ -85,6 +116,7 @@ def batched_bench(pb, ctx, count=4096, first_seed=1000):
     dgs = [g.to_device(ctx) for g in hg]
     edges = int(sum(g.m for g in hg))
     print(f"# batch: {count} graphs, {edges} edges, generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    sample = hg[:cpu_sample] if cpu_sample else []
     del hg
     res = {"graphs": count, "edges": edges}
 
@@ -144,13 +176,27 @@ def batched_bench(pb, ctx, count=4096, first_seed=1000):
         res["learned_speedup_vs_plain"] = round(t_plain / best_t, 2)
         res["bit_exact_learned_vs_plain"] = exact
         res["learned_variant_histogram"] = np.bincount(pick, minlength=S.NVAR).tolist()
+    if sample:
+        # the reference's per-graph loop (Compute's side choice + BATCHS / ADEG Count, graph after
+        # graph) over the first graphs of the set, all host cores
+        r = ref_run("vertlist", sample, "BATCHS", "ADEG", rounds=3)
+        if r is not None:
+            _, tots, _, info, sec = r
+            tp = out_p[2].numpy()
+            res["cpu_baseline"] = {"value": round(len(sample) / sec, 1), "unit": "graphs/s", "cores": host_cores(),
+                                   "kind": "reference",
+                                   "sample": f"the first {len(sample)} graphs of the set, reference BATCHS/ADEG Count() "
+                                             "per graph, median of 2 rounds after a warm-up",
+                                   "seconds": round(sec, 3),
+                                   "bit_exact_vs_gpu": bool(np.array_equal(tots.astype(np.uint64), tp[:len(sample)]))}
     ctx.trim()
     return res
 
 
-def edge_bench(pb, ctx, dg, use_v, W, m, n, vert_sum, steps=5):
+def edge_bench(pb, ctx, dg, use_v, W, m, n, vert_sum, steps=5, host_graph=None):
     """Per-edge counts of the headline graph: throughput, roofline fraction of the counting phase,
-    and sum E = 4 TOTAL = 2 sum_S B (the per-vertex run's sum)."""
+    and sum E = 4 TOTAL = 2 sum_S B (the per-vertex run's sum); with host_graph, the reference's
+    BATCHS / ADEG CountE on the same graph as the CPU baseline, compared bit-exactly."""
     eout = pb.DeviceArray(ctx, m, np.uint64)
     ctx.count(dg, use_v, "EDGE", "GPU", "ADEG", out=eout)
     ctx.synchronize()
@@ -163,17 +209,53 @@ def edge_bench(pb, ctx, dg, use_v, W, m, n, vert_sum, steps=5):
     edge_ms = (time.perf_counter() - t0) * 1e3 / steps
     edge_count_ms = float(np.mean([p["count_ms"] for p in eph]))
     balg = 2 * (4 * W + 32 * m) + 4 * (n + 1) + 8 * m
-    esum = int(pb.as_u64(eout).sum(dtype=np.uint64))
+    E = pb.as_u64(eout)
+    esum = int(E.sum(dtype=np.uint64))
     del eout
-    return {"gwedges_s": round(W / (edge_ms * 1e-3) / 1e9, 3), "ms_per_step": round(edge_ms, 4),
-            "count_ms": round(edge_count_ms, 4), "b_alg_bytes": balg,
-            "roofline_frac": round(balg / (edge_count_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "sum_is_4_total": esum == 2 * vert_sum}
+    res = {"gwedges_s": round(W / (edge_ms * 1e-3) / 1e9, 3), "ms_per_step": round(edge_ms, 4),
+           "count_ms": round(edge_count_ms, 4), "b_alg_bytes": balg,
+           "roofline_frac": round(balg / (edge_count_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "sum_is_4_total": esum == 2 * vert_sum}
+    if host_graph is not None:
+        r = ref_run("edge", host_graph, "BATCHS", "ADEG", rounds=3)
+        if r is not None:
+            uv, counts, _, info, sec = r
+            res["cpu_baseline"] = {"value": round(W / sec / 1e9, 4), "unit": "Gwedges/s", "cores": host_cores(),
+                                   "kind": "reference",
+                                   "sample": "full workload (same graph), reference BATCHS/ADEG CountE(), median of 2 "
+                                             "rounds after a warm-up",
+                                   "seconds": round(sec, 3), "max_rss_gb": info["max_rss_gb"],
+                                   "bit_exact_vs_gpu": bool(uv == use_v and np.array_equal(counts.astype(np.uint64), E))}
+    return res
 
 
-def tip_bench(pb, ctx, dg, use_v, counts):
+def tip_b_alg(g, use_v):
+    """Algorithmic bytes of a tip peel: every S vertex u is a frontier item once, and its walk
+    reads the lists of its centres, 4 * sum_{v in N(u)} deg(v) + 32 * deg(u) bytes (the
+    two-hop walk of PeelOrigParallel, butterfly_peel.h:670-702), i.e. 4 sum_{v in O} deg(v)^2
+    + 32 m in all."""
+    dO = np.diff((g.offsetsV if use_v else g.offsetsU).astype(np.int64))
+    return int(4 * (dO * dO).sum() + 32 * g.m)
+
+
+def wing_b_alg(g, use_v):
+    """Algorithmic bytes of a wing peel: every edge e = (u, v) (u on S) is a frontier item once,
+    and for each u2 in N(v) its walk merges N(u) with N(u2), 4 (deg u + deg u2) bytes
+    (PeelEOrigParallel, butterfly_epeel.h:364-429): 8 sum_e deg(u) deg(v) in all."""
+    offS = (g.offsetsU if use_v else g.offsetsV).astype(np.int64)
+    edgS = (g.edgesU if use_v else g.edgesV).astype(np.int64)
+    dS = np.diff(offS)
+    dO = np.diff((g.offsetsV if use_v else g.offsetsU).astype(np.int64))
+    return int(8 * (np.repeat(dS, dS) * dO[edgS]).sum())
+
+
+def tip_bench(pb, ctx, dg, use_v, counts, host_graph, cpu=True):
     """Tip numbers of the headline graph from its per-vertex counts (Peel, the CLI's default
-    after VERT counts): one call, timed with the library's events and the host clock."""
+    after VERT counts): one call, timed with the library's events and the host clock, its
+    algorithmic bytes against the HBM peak, and the frontiers (items peeled per round and per
+    tip level).  The reference's Peel takes hours at this size, so the CPU baseline is a bounded
+    sample: Chung-Lu nv = 2^16, nu = 2^14, m = 2^20, gamma 2.1, seed 2, peeled by the reference's
+    PBATCHS Peel and by the GPU, compared bit-exactly."""
     t0 = time.perf_counter()
     tip = ctx.peel(dg, use_v, counts)
     ctx.synchronize()
@@ -181,8 +263,37 @@ def tip_bench(pb, ctx, dg, use_v, counts):
     tm = ctx.timing()
     T = tip.numpy()
     B = pb.as_u64(counts)
-    return {"peel_s": round(wall, 3), "peel_ms_events": round(tm["peel_ms"], 1), "rounds": int(tm["rounds"]),
-            "levels": int(len(np.unique(T))), "max_tip": int(T.max()), "tip_le_count": bool(np.all(T <= B))}
+    _, per_level = np.unique(T, return_counts=True)
+    balg = tip_b_alg(host_graph, use_v)
+    res = {"peel_s": round(wall, 3), "peel_ms_events": round(tm["peel_ms"], 1), "rounds": int(tm["rounds"]),
+           "levels": int(len(per_level)), "max_tip": int(T.max()), "tip_le_count": bool(np.all(T <= B)),
+           "frontier": {"items": int(len(T)), "mean_per_round": round(len(T) / max(tm["rounds"], 1), 3),
+                        "max_per_level": int(per_level.max())},
+           "b_alg_bytes": balg, "roofline_frac": round(balg / (tm["peel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+    if cpu:
+        sg = pb.Graph.chung_lu(1 << 16, 1 << 14, 1 << 20, 2.1, 2.1, 2)
+        sdg = sg.to_device(ctx)
+        su, _ = ctx.side(sdg)
+        sc = ctx.count(sdg, su, "VERT")
+        ctx.peel(sdg, su, sc)  # warm
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        stip = ctx.peel(sdg, su, sc)
+        ctx.synchronize()
+        gsec = time.perf_counter() - t0
+        rounds = int(ctx.timing()["rounds"])
+        r = ref_run("peelv", sg, "BATCHS", "ADEG", rounds=1, peel_type="BATCHS")
+        if r is not None:
+            uv, _, rtip, info, sec = r
+            res["cpu_baseline"] = {"value": round(sec, 3), "unit": "s", "higher_is_better": False,
+                                   "cores": host_cores(), "kind": "reference",
+                                   "sample": "Chung-Lu nv=2^16 nu=2^14 m=2^20 gamma=2.1 seed=2, reference PBATCHS Peel() "
+                                             "of the reference's own counts, one run",
+                                   "gpu_s": round(gsec, 4), "rounds": rounds,
+                                   "bit_exact_vs_gpu": bool(uv == su and np.array_equal(rtip.astype(np.uint64),
+                                                                                        stip.numpy()))}
+        del sdg, sc, stip
+    return res
 
 
 def sparse_bench(pb, ctx, dg, exact_total, denom=25, seed=0, reps=5):
@@ -246,7 +357,7 @@ def read_bench(pb, ctx, g, reps=3):
     return out
 
 
-def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000):
+def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000, cpu=True):
     """Count + peel workload: R-MAT bipartite graph (a, b, c, d) = (0.5, 0.1, 0.1, 0.3), 2^22 x 2^22
     vertices, 1.2e8 distinct edges, seed 3, ACOCORE ranking, per-edge counts, then the full wing
     peel.  Properties checked at full size: identical per-edge counts under ADEG and ACOCORE,
@@ -293,6 +404,8 @@ def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000):
     ctx.synchronize()
     res["peel_s"] = round(time.perf_counter() - t, 3)
     res["peel_rounds"] = int(ctx.timing()["rounds"])
+    res["peel_b_alg_bytes"] = wing_b_alg(g, use_v)
+    res["peel_roofline_frac"] = round(res["peel_b_alg_bytes"] / res["peel_s"] / 1e9 / HBM_PEAK_GBS, 5)
     wn = wing.numpy()
     res["max_wing"] = int(wn.max())
     res["wing_le_count"] = bool(np.all(wn <= E))
@@ -318,8 +431,79 @@ def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000):
         ok = mn >= k and (k != int(levels[0]) or mn == k)
         checks.append({"k": k, "edges": int(sel.sum()), "min_butterflies": mn, "ok": bool(ok)})
     res["wing_subgraph_checks"] = checks
-    del e, wing, dg
+    del wing, dg
+    if cpu:
+        # the reference's BATCHS / ACOCORE CountE on the same graph (dense arrays: its peak
+        # resident memory is reported), then a bounded wing-peel sample on both sides
+        r = ref_run("edge", g, "BATCHS", "ACOCORE", rounds=1, timeout=1200)
+        if r is not None:
+            uv, counts, _, info, sec = r
+            res["cpu_baseline"] = {"value": round(W / sec / 1e9, 4), "unit": "Gwedges/s", "cores": host_cores(),
+                                   "kind": "reference",
+                                   "sample": "full workload (same graph), reference BATCHS/ACOCORE CountE(), one run",
+                                   "seconds": round(sec, 3), "max_rss_gb": info["max_rss_gb"],
+                                   "bit_exact_vs_gpu": bool(uv == use_v and np.array_equal(counts.astype(np.uint64), E))}
+            del counts
+        sg = pb.Graph.rmat(14, 14, 1_000_000, 0.5, 0.1, 0.1, 0.3, 3)
+        sdg = sg.to_device(ctx)
+        su, _ = ctx.side(sdg)
+        se = ctx.count(sdg, su, "EDGE", "GPU", "ACOCORE")
+        ctx.peel_edge(sdg, su, se)  # warm
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sw = ctx.peel_edge(sdg, su, se)
+        ctx.synchronize()
+        gsec = time.perf_counter() - t0
+        srounds = int(ctx.timing()["rounds"])
+        r = ref_run("peele", sg, "BATCHS", "ACOCORE", rounds=1, peel_type="BATCHS")
+        if r is not None:
+            uv, _, rw, info, sec = r
+            res["peel_cpu_baseline"] = {"value": round(sec, 3), "unit": "s", "higher_is_better": False,
+                                        "cores": host_cores(), "kind": "reference",
+                                        "sample": "R-MAT 2^14 x 2^14 m=1e6 (.5,.1,.1,.3) seed=3, reference PBATCHS PeelE() "
+                                                  "of the reference's own ACOCORE counts, one run",
+                                        "gpu_s": round(gsec, 4), "rounds": srounds,
+                                        "b_alg_bytes": wing_b_alg(sg, su),
+                                        "bit_exact_vs_gpu": bool(uv == su and np.array_equal(rw.astype(np.uint64),
+                                                                                             sw.numpy()))}
+        del sdg, se, sw
+    del e, E
     ctx.trim()
+    return res
+
+
+def config1_bench(pb, ctx, steps=50, cpu=True):
+    """The reference's CPU-runnable case: uniform random bipartite graph, nv = nu = 1000, 10,000
+    distinct edges, seed 1, ADEG, per-vertex counts.  GPU: the graph resident in HBM, one step
+    = ranking + relabel + counting + conversion; CPU: the reference's SERIAL Count on one core
+    and its BATCHS Count on all host cores (medians of 3 rounds after a warm-up), compared
+    bit-exactly."""
+    g = pb.Graph.uniform(1000, 1000, 10000, 1)
+    dg = g.to_device(ctx)
+    use_v, _ = ctx.side(dg)
+    out = pb.DeviceArray(ctx, g.nu if use_v else g.nv, np.uint64)
+    for _ in range(5):
+        ctx.count(dg, use_v, "VERT", "GPU", "ADEG", out=out)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.count(dg, use_v, "VERT", "GPU", "ADEG", out=out)
+    ctx.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    W = int(ctx.timing()["wedges"])
+    gc = pb.as_u64(out)
+    res = {"workload": "uniform bipartite nv=nu=1000 m=10000 seed=1, ADEG, per-vertex counts", "wedges": W,
+           "gpu_ms_per_step": round(sec * 1e3, 4), "gpu_gwedges_s": round(W / sec / 1e9, 4)}
+    if cpu:
+        for ct, thr in (("SERIAL", 1), ("BATCHS", host_cores())):
+            r = ref_run("vert", g, ct, "ADEG", rounds=4, threads=thr)
+            if r is None:
+                continue
+            uv, counts, _, info, csec = r
+            res["cpu_" + ct.lower()] = {"ms": round(csec * 1e3, 4), "gwedges_s": round(W / csec / 1e9, 4),
+                                        "cores": thr, "kind": "reference",
+                                        "bit_exact_vs_gpu": bool(uv == use_v and
+                                                                 np.array_equal(counts.astype(np.uint64), gc))}
     return res
 
 
@@ -400,11 +584,12 @@ def main():
     # per-edge counts on the same graph (same ranking, same W): B_alg(EDGE) = 2 B_pass + 4 (n + 1) + 8 m
     edge = None
     if not args.no_edge:
-        edge = edge_bench(pb, ctx, dg, use_v, W, g.m, n, int(gpu_counts.sum(dtype=np.uint64)))
+        edge = edge_bench(pb, ctx, dg, use_v, W, g.m, n, int(gpu_counts.sum(dtype=np.uint64)),
+                          host_graph=None if args.no_cpu_baseline else g)
 
     tipres = None
     if not args.no_tip:
-        tipres = tip_bench(pb, ctx, dg, use_v, out)
+        tipres = tip_bench(pb, ctx, dg, use_v, out, g, cpu=not args.no_cpu_baseline)
         print(f"# tip peel {tipres['peel_s']} s, {tipres['rounds']} rounds", file=sys.stderr, flush=True)
 
     sparse = None
@@ -449,9 +634,10 @@ def main():
     if not args.no_batch:
         del dg, out
         ctx.trim()
-        rec["batched"] = batched_bench(pb, ctx, count=args.batch_count)
+        rec["batched"] = batched_bench(pb, ctx, count=args.batch_count, cpu_sample=0 if args.no_cpu_baseline else 512)
     if not args.no_rmat:
-        rec["rmat_count_peel"] = rmat_bench(pb, ctx)
+        rec["rmat_count_peel"] = rmat_bench(pb, ctx, cpu=not args.no_cpu_baseline)
+    rec["config1"] = config1_bench(pb, ctx, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(g, use_v)
         if cb is not None:
@@ -459,7 +645,7 @@ def main():
             rec["cpu_baseline"] = {"value": round(W / cb["seconds"] / 1e9, 4), "unit": "Gwedges/s",
                                    "cores": cb["cores"], "kind": "reference",
                                    "sample": "full workload (same graph), reference BATCHS/ADEG Count(), "
-                                             f"median of {len(cb['all_seconds'])} rounds",
+                                             f"median of {len(cb['all_seconds']) - 1} rounds after a warm-up",
                                    "seconds": round(cb["seconds"], 3), "bit_exact_vs_gpu": bool(ok)}
     print(json.dumps(rec), flush=True)
 
