@@ -72,6 +72,12 @@ struct CountArgs {
   // entry of r's list holds (rank - lo[r]) << 1 | flag, and the tables are keyed by the local
   // rank, so each graph behaves as if counted alone.  Null for a single graph (lo = 0).
   const uint32_t* lo;
+  // Log of credits to ranks below HOT_N (per-vertex counts): entries (rank << 52) | value,
+  // appended by wavefronts in reserved chunks and summed by k_hlog_reduce.  hlog_cap == 0:
+  // no log, those credits are replicated atomics.
+  unsigned long long* hlog;
+  uint32_t* hlog_len;      // entries reserved (may pass hlog_cap; the rest are not stored)
+  uint32_t hlog_cap;
 };
 
 __device__ __forceinline__ uint32_t lo_of(const CountArgs& a, uint32_t i) { return a.lo ? a.lo[i] : 0u; }
@@ -399,12 +405,77 @@ __device__ __forceinline__ void credit(const CountArgs& a, uint32_t x, unsigned 
   else atomicAdd(&a.B[x], val);
 }
 
+// ---------------- the hub-credit log ----------------
+// Most endpoint credits go to the hubs (ranks below HOT_N): on a power-law graph over 90 % of
+// them.  Scattered 64-bit device atomics run at a small fraction of the rate of coalesced
+// ones, so a wavefront appends its hub credits to an HBM log instead: consecutive entries of
+// a chunk of HLOG_CHUNK it reserved with one counter atomic (coalesced stores), and
+// k_hlog_reduce sums the log per rank in LDS.  Entries are (rank << 52) | value; a value of
+// 2^52 or more, an entry past the log's capacity, and every credit to another rank take the
+// direct atomic.  An abandoned chunk's unused entries are zeroed (credits of 0).
+constexpr uint32_t HLOG_CHUNK = 1024;
+constexpr unsigned long long HLOG_VMAX = 1ull << 52;
+
+struct HotLog {
+  uint32_t base = 0;           // the wavefront's chunk (wave-uniform)
+  uint32_t fill = HLOG_CHUNK;  // entries of it used; HLOG_CHUNK: no chunk yet
+};
+
+// Zeroes the unused entries of the wavefront's chunk.
+__device__ __forceinline__ void hlog_pad(const CountArgs& a, const HotLog& hl) {
+  for (uint32_t q = hl.fill + __lane_id(); q < HLOG_CHUNK; q += 64) {
+    const uint64_t p = (uint64_t)hl.base + q;
+    if (p < a.hlog_cap) a.hlog[p] = 0ull;
+  }
+}
+
+// Credits val to rank x on the lanes with act; every lane of the wavefront calls it together.
+__device__ __forceinline__ void credit_wave(const CountArgs& a, HotLog& hl, bool act, uint32_t x,
+                                            unsigned long long val) {
+  const bool logged = act && x < HOT_N && val < HLOG_VMAX && a.hlog_cap != 0u;
+  if (act && !logged) credit(a, x, val);
+  const uint64_t lm = __ballot(logged);
+  if (lm == 0ull) return;
+  const uint32_t cnt = (uint32_t)__popcll(lm);
+  if (hl.fill + cnt > HLOG_CHUNK) {
+    hlog_pad(a, hl);
+    uint32_t b = 0;
+    if (__lane_id() == (uint32_t)__ffsll((long long)__ballot(1)) - 1u) b = atomicAdd(a.hlog_len, HLOG_CHUNK);
+    hl.base = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+    hl.fill = 0;
+  }
+  if (logged) {
+    const uint64_t p = (uint64_t)hl.base + hl.fill + (uint32_t)__popcll(lm & pbf_lanemask_lt());
+    if (p < a.hlog_cap) a.hlog[p] = ((unsigned long long)x << 52) | val;
+    else credit(a, x, val);
+  }
+  hl.fill += cnt;
+}
+
+// Sums the log per rank in LDS and adds each rank's sum to its count (coalesced atomics).
+__global__ __launch_bounds__(256) void k_hlog_reduce(const unsigned long long* __restrict__ hlog,
+                                                     const uint32_t* __restrict__ len, uint32_t cap,
+                                                     unsigned long long* __restrict__ B, int64_t n) {
+  __shared__ unsigned long long acc[HOT_N];
+  for (uint32_t x = threadIdx.x; x < HOT_N; x += blockDim.x) acc[x] = 0;
+  __syncthreads();
+  const uint32_t L = min(*len, cap);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < L; p += gridDim.x * blockDim.x) {
+    const unsigned long long e = hlog[p];
+    if (e) atomicAdd(&acc[e >> 52], e & (HLOG_VMAX - 1ull));
+  }
+  __syncthreads();
+  for (uint32_t x = threadIdx.x; x < HOT_N && x < n; x += blockDim.x)
+    if (acc[x]) atomicAdd(&B[x], acc[x]);
+}
+
 // ---------------- windowed wedge enumeration ----------------
 // Per wavefront LDS: the window's segment-start flags and the per-segment partial sums.
 struct WaveCtx {
   lds_u32* flags;   // WIN_SPAN words, 16-byte aligned
   lds_u64* acc;     // 64 partial sums (pass 2), kept zero between chunks
   uint32_t epoch;   // tag generation of the flags
+  HotLog hl;        // the wavefront's chunk of the hub-credit log
 };
 
 // Overflow tracking of a shared hash table (one workgroup).
@@ -719,10 +790,11 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
   if constexpr (PASS2) {
     wave_lds_fence();
     const unsigned long long mine = wc.acc[lane];
-    if (mine) {
-      wc.acc[lane] = 0;
-      if constexpr (MODE == MODE_EDGE) atomicAdd(&a.B[ch.slot], mine);
-      else credit(a, ch.v, mine);
+    if (mine) wc.acc[lane] = 0;
+    if constexpr (MODE == MODE_EDGE) {
+      if (mine) atomicAdd(&a.B[ch.slot], mine);
+    } else {
+      credit_wave(a, wc.hl, mine != 0ull, ch.v, mine);
     }
     wave_lds_fence();
   }
@@ -748,29 +820,31 @@ __device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint
 }
 
 // Endpoint credits from a finished table: C(c,2) to every key (VERT) and the sum (returned).
-// Cells q = gt, gt + gn, ... of `cells`.
+// Cells q = gt, gt + gn, ... of `cells` (gt - lane is the wavefront's first).
 template <int MODE, int TM>
 __device__ __forceinline__ unsigned long long sweep(const CountArgs& a, const Tab& t, uint32_t cells, uint32_t gt,
-                                                    uint32_t gn, uint32_t lo) {
+                                                    uint32_t gn, uint32_t lo, HotLog& hl) {
   unsigned long long sum = 0;
-  for (uint32_t q = gt; q < cells; q += gn) {
-    uint32_t key, c;
-    if constexpr (TM == TM_D32) {
-      key = q; c = t.w[q];
-    } else if constexpr (TM == TM_D16) {
-      key = q; c = (t.w[q >> 1] >> ((q & 1u) << 4)) & 0xFFFFu;
-    } else if constexpr (TM == TM_HASH32) {
-      const uint32_t cur = t.w[q];
-      key = cur >> t.cb; c = cur == EMPTY ? 0u : (cur & ((1u << t.cb) - 1u));
-    } else {
-      const unsigned long long cur = ((const lds_u64*)t.w)[q];
-      key = (uint32_t)(cur >> 32); c = cur == ~0ull ? 0u : (uint32_t)cur;
+  // every lane of a wavefront runs the same iterations (the credits are wavefront-wide)
+  for (uint32_t qb = gt - __lane_id(); qb < cells; qb += gn) {
+    const uint32_t q = qb + __lane_id();
+    uint32_t key = 0, c = 0;
+    if (q < cells) {
+      if constexpr (TM == TM_D32) {
+        key = q; c = t.w[q];
+      } else if constexpr (TM == TM_D16) {
+        key = q; c = (t.w[q >> 1] >> ((q & 1u) << 4)) & 0xFFFFu;
+      } else if constexpr (TM == TM_HASH32) {
+        const uint32_t cur = t.w[q];
+        key = cur >> t.cb; c = cur == EMPTY ? 0u : (cur & ((1u << t.cb) - 1u));
+      } else {
+        const unsigned long long cur = ((const lds_u64*)t.w)[q];
+        key = (uint32_t)(cur >> 32); c = cur == ~0ull ? 0u : (uint32_t)cur;
+      }
     }
-    if (c > 1u) {
-      const unsigned long long b = (unsigned long long)c * (c - 1u) / 2u;
-      sum += b;
-      if constexpr (MODE == MODE_VERT) credit(a, key + lo, b);
-    }
+    const unsigned long long b = c > 1u ? (unsigned long long)c * (c - 1u) / 2u : 0ull;
+    sum += b;
+    if constexpr (MODE == MODE_VERT) credit_wave(a, hl, c > 1u, key + lo, b);
   }
   return sum;
 }
@@ -786,7 +860,7 @@ __device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint
   __syncthreads();
   if (lds_load(tr.ovf)) return false;
   if (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S)) {
-    unsigned long long sum = pbf_wave_sum(sweep<MODE, TM>(a, t, cells, tid, TPB, lo));
+    unsigned long long sum = pbf_wave_sum(sweep<MODE, TM>(a, t, cells, tid, TPB, lo, wc.hl));
     if (lane == 0) red[w] = sum;
     __syncthreads();
     if (tid == 0) {
@@ -952,11 +1026,9 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const uint32_t c = cur[k] == EMPTY ? 0u : (cur[k] & cmask);
-        if (c > 1u) {
-          const uint32_t b = c * (c - 1u) / 2u;
-          sum += b;
-          if constexpr (MODE == MODE_VERT) credit(a, (cur[k] >> a.cb) + lo, b);
-        }
+        const uint32_t b = c > 1u ? c * (c - 1u) / 2u : 0u;
+        sum += b;
+        if constexpr (MODE == MODE_VERT) credit_wave(a, wc.hl, c > 1u, (cur[k] >> a.cb) + lo, b);
       }
       const uint32_t k = in ? kq : 64u;
       const uint32_t x = wave_seg_incl_add(sum, k);
@@ -965,14 +1037,12 @@ __device__ bool light_group(const CountArgs& a, uint32_t nk, const uint4& rc, bo
     }
     if (words >= wm) wm = 0;  // swept clean
     wave_lds_fence();
-    if (lane < nk) {
-      const unsigned long long s = iacc[lane];
-      if (s) {
-        iacc[lane] = 0;
-        if constexpr (MODE == MODE_VERT) credit(a, i_k, s);
-        else *total_acc += s;
-      }
+    const unsigned long long s = lane < nk ? iacc[lane] : 0ull;
+    if (s) {
+      iacc[lane] = 0;
+      if constexpr (MODE != MODE_VERT) *total_acc += s;
     }
+    if constexpr (MODE == MODE_VERT) credit_wave(a, wc.hl, s != 0ull, i_k, s);
     wave_lds_fence();
   }
   at_c();
@@ -1058,6 +1128,7 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
     gn = gnn; ns = nns; ne = nne;
   }
   }
+  if constexpr (MODE == MODE_VERT) hlog_pad(a, wc.hl);
   if constexpr (MODE == MODE_TOTAL) {
     total_acc = pbf_wave_sum(total_acc);
     if (lane == 0 && total_acc) atomicAdd(a.total, total_acc);
@@ -1416,6 +1487,7 @@ __global__ __launch_bounds__(TPB) void k_split_credit(CountArgs a, const uint32_
                                                       uint32_t split_cap) {
   __shared__ unsigned long long red[NWAVE];
   const uint32_t nsplit = min(ctl[8], split_cap);
+  HotLog hl;
   for (uint32_t b = blockIdx.x; b < nsplit; b += gridDim.x) {
     const uint32_t i = split[b];
     const bool i_in_S = (a.redg[a.roff[i]] & 1u) == 0u;
@@ -1423,13 +1495,13 @@ __global__ __launch_bounds__(TPB) void k_split_credit(CountArgs a, const uint32_
     const uint32_t* row = G + (size_t)b * SPLIT_IMAX;
     const uint32_t glo = lo_of(a, i), il = i - glo;
     unsigned long long sum = 0;
-    for (uint32_t q = threadIdx.x; q < il; q += TPB) {
-      const uint32_t c = row[q];
-      if (c > 1u) {
-        const unsigned long long v = (unsigned long long)c * (c - 1u) / 2u;
-        sum += v;
-        if constexpr (MODE == MODE_VERT) credit(a, q + glo, v);
-      }
+    // every lane of a wavefront runs the same iterations (the credits are wavefront-wide)
+    for (uint32_t qb = threadIdx.x & ~63u; qb < il; qb += TPB) {
+      const uint32_t q = qb + (threadIdx.x & 63u);
+      const uint32_t c = q < il ? row[q] : 0u;
+      const unsigned long long v = c > 1u ? (unsigned long long)c * (c - 1u) / 2u : 0ull;
+      sum += v;
+      if constexpr (MODE == MODE_VERT) credit_wave(a, hl, c > 1u, q + glo, v);
     }
     sum = pbf_wave_sum(sum);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
@@ -1444,6 +1516,7 @@ __global__ __launch_bounds__(TPB) void k_split_credit(CountArgs a, const uint32_
     }
     __syncthreads();
   }
+  if constexpr (MODE == MODE_VERT) hlog_pad(a, hl);
 }
 
 // Split endpoints, pass 2 (VERT with the endpoint on O, or EDGE): counts looked up in G.
@@ -1473,6 +1546,7 @@ __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t
     const uint32_t j1 = min(deg, (c + 1) * SPLIT_CHUNK);
     enum_pass<MODE, true, TM_G32>(a, t, i, io, lo_of(a, i), c * SPLIT_CHUNK, j1, wid, NWAVE, wc, tr);
   }
+  if constexpr (MODE == MODE_VERT) hlog_pad(a, wc.hl);
 }
 
 // O-CSR index of every ranked slot: the payload is an O-CSR position when the slot's owner
@@ -1649,6 +1723,7 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* li
   ctx->timing.hub_vertices = sync ? 0 : -1;
   for (size_t bi = 0; bi + 1 < bounds.size(); bi++) {
     CountArgs a = a0;
+    a.hlog_len = ctl + 20;  // cleared with ctl below
     const ClassOut co{const_cast<uint32_t*>(a.bound), lists, lists + n, hbm_list, ctl, a.cb, lmax, ctx->tune.light_work,
                       R.lo, bounds[bi], bounds[bi + 1]};
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctl, 0u, (int64_t)64);
@@ -1700,6 +1775,9 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* li
     a.queues = ctl + 16;
     a.rerun = 1;
     hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
+    if (MODE == MODE_VERT && a.hlog_cap)
+      hipLaunchKernelGGL(k_hlog_reduce, dim3(ctx->num_cus * 2), dim3(256), 0, ctx->stream, a.hlog, a.hlog_len, a.hlog_cap,
+                         a.B, n);
     PBF_HIP(hipGetLastError());
     uint32_t novf = 0;
     // each block needs 2 kn words of dense counters: with small key ranges (a union of small
@@ -1785,6 +1863,15 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
   if (mode == MODE_VERT) {
     PBF_CHECK(pbf_scratch(ctx, S_COUNTS, 8 * (size_t)(n ? n : 1), &cp));
     a.B = static_cast<unsigned long long*>(cp);
+    // the hub-credit log: up to 16 entries per slot (at most 2^27), none in a fixed scratch
+    // region (there the hub credits stay replicated atomics)
+    if (pbf_scratch_room(ctx) == SIZE_MAX && n > 0) {
+      const int64_t cap = std::min<int64_t>(int64_t(1) << 27, 16 * m2 + (1 << 16));
+      void* hl;
+      PBF_CHECK(pbf_scratch(ctx, S_HLOG, 8 * (size_t)cap, &hl));
+      a.hlog = static_cast<unsigned long long*>(hl);
+      a.hlog_cap = (uint32_t)cap;
+    }
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, a.B, 0ull, n);
     PBF_CHECK(run_count<MODE_VERT>(ctx, R, a, lists, ctl, opts.sync));
     if (opts.B_out) *opts.B_out = a.B;

@@ -64,6 +64,7 @@ enum ScratchSlot {
   S_LEARN1,      // learned batch: placement table of the parts' graphs
   S_LEARN2,      // learned batch: final outputs for host callers
   S_WBATCH,      // wedge-bounded counting: wedges per endpoint and their prefix
+  S_HLOG,        // per-vertex counting: the log of credits to hub ranks
   S_NUM_SLOTS
 };
 
