@@ -825,6 +825,32 @@ template <int MODE, int TM>
 __device__ __forceinline__ unsigned long long sweep(const CountArgs& a, const Tab& t, uint32_t cells, uint32_t gt,
                                                     uint32_t gn, uint32_t lo, HotLog& hl) {
   unsigned long long sum = 0;
+  if constexpr (TM == TM_HASH32) {
+    // packed hash tables (a power of two of at least 64 cells): 4 cells per lane per step,
+    // and a step whose cells hold no count above 1 costs one ballot
+    const uint32_t cmask = (1u << t.cb) - 1u;
+    for (uint32_t qb = 4 * (gt - __lane_id()); qb < cells; qb += 4 * gn) {
+      const uint32_t q = qb + 4 * __lane_id();
+      u32x4 f = u32x4{EMPTY, EMPTY, EMPTY, EMPTY};
+      if (q < cells) f = *(lds_u32x4*)(t.w + q);
+      const uint32_t cur[4] = {f.x, f.y, f.z, f.w};
+      uint32_t c[4];
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        c[k] = cur[k] == EMPTY ? 0u : (cur[k] & cmask);
+        any |= c[k] > 1u;
+      }
+      if (__ballot(any) == 0ull) continue;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const unsigned long long b = c[k] > 1u ? (unsigned long long)c[k] * (c[k] - 1u) / 2u : 0ull;
+        sum += b;
+        if constexpr (MODE == MODE_VERT) credit_wave(a, hl, c[k] > 1u, (cur[k] >> t.cb) + lo, b);
+      }
+    }
+    return sum;
+  }
   // every lane of a wavefront runs the same iterations (the credits are wavefront-wide)
   for (uint32_t qb = gt - __lane_id(); qb < cells; qb += gn) {
     const uint32_t q = qb + __lane_id();
@@ -914,7 +940,8 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
     fill = EMPTY;
     cells = cap;
   }
-  for (uint32_t q = tid; q < words; q += TPB) pool[q] = fill;
+  // cleared 4 words per thread per step (the pool has room for words rounded up to 4)
+  for (uint32_t q = 4 * tid; q < words; q += 4 * TPB) *(lds_u32x4*)(pool + q) = u32x4{fill, fill, fill, fill};
   if (tid == 0) { *nused = 0; *ovf = 0; }
   __syncthreads();
   switch (tm) {
