@@ -97,6 +97,12 @@ int pbf_abi_version(void);
 /* readBipartite (apps/hyper/butterfly_utils.h:1008-1058): AdjacencyHypergraph text, or the
  * library's binary "PBFCSR1" layout (detected by magic).  Validates like pbf_validate. */
 int pbf_read_bipartite(const char* path, pbf_graph* out);
+/* A KONECT bipartite edge list (readKONECT, utils/graphIO.h:1032-1098, as utils/
+ * KONECTtoHyperAdj.C converts it): leading '%' lines skipped, the first two columns of every
+ * row a V id and a U id (further columns ignored), each side's ids renumbered densely in
+ * increasing order, lists sorted and duplicates dropped.  Ids are non-negative and below 2^32
+ * (else PBF_ERR_RANGE); fewer than two columns is PBF_ERR_FORMAT. */
+int pbf_read_konect(const char* path, pbf_graph* out);
 /* Writes AdjacencyHypergraph text (utils/graphIO.h:779-800 layout) or PBFCSR1 binary. */
 int pbf_write_bipartite(const char* path, const pbf_bipartite_csr* g);
 int pbf_write_binary(const char* path, const pbf_bipartite_csr* g);

@@ -106,6 +106,7 @@ _SIGS = {
     "pbf_status_str": (ctypes.c_char_p, [_INT]),
     "pbf_abi_version": (_INT, []),
     "pbf_read_bipartite": (_INT, [ctypes.c_char_p, _PG]),
+    "pbf_read_konect": (_INT, [ctypes.c_char_p, _PG]),
     "pbf_write_bipartite": (_INT, [ctypes.c_char_p, _PCSR]),
     "pbf_write_binary": (_INT, [ctypes.c_char_p, _PCSR]),
     "pbf_validate": (_INT, [_PCSR]),
@@ -223,6 +224,13 @@ class Graph:
         """readBipartite: AdjacencyHypergraph text or PBFCSR1 binary."""
         g = _Graph()
         _check(_lib.pbf_read_bipartite(os.fsencode(path), ctypes.byref(g)), "pbf_read_bipartite")
+        return Graph._adopt(g)
+
+    @staticmethod
+    def read_konect(path):
+        """pbf_read_konect: a KONECT bipartite edge list (readKONECT + KONECTtoHyperAdj)."""
+        g = _Graph()
+        _check(_lib.pbf_read_konect(os.fsencode(path), ctypes.byref(g)), "pbf_read_konect")
         return Graph._adopt(g)
 
     @staticmethod

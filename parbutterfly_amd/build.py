@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libparbutterfly_amd.so")
 CLI = os.path.join(HERE, "bin", "bipartite_butterflies")
+KONECT = os.path.join(HERE, "bin", "konect_to_hyperadj")
 
 HIP_SOURCES = ["scan.hip", "rank.hip", "count.hip", "peel.hip", "batch.hip", "mlp.hip", "sparse.hip", "textio.hip", "api.hip"]
 HOST_SOURCES = ["host.cpp"]
@@ -64,11 +65,12 @@ def build(verbose=False, jobs=None):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp"], verbose)
     os.replace(tmp, LIB)
-    cli_src = os.path.join(CSRC, "cli.cpp")
-    if os.path.exists(cli_src):
-        _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), cli_src, "-o", CLI + ".tmp",
-              "-L" + HERE, "-lparbutterfly_amd", "-Wl,-rpath,$ORIGIN/.."], verbose)
-        os.replace(CLI + ".tmp", CLI)
+    for src, exe in (("cli.cpp", CLI), ("konect.cpp", KONECT)):
+        path = os.path.join(CSRC, src)
+        if os.path.exists(path):
+            _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), path, "-o", exe + ".tmp",
+                  "-L" + HERE, "-lparbutterfly_amd", "-Wl,-rpath,$ORIGIN/.."], verbose)
+            os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -300,6 +300,84 @@ extern "C" int pbf_read_bipartite(const char* path, pbf_graph* out) {
   return st;
 }
 
+// KONECT bipartite edge lists (readKONECT, utils/graphIO.h:1032-1098, turned into a graph as
+// utils/KONECTtoHyperAdj.C does through hypergraphFromHyperedges, :378-440).  Leading lines
+// that start with '%' are skipped; the number of tokens on the first remaining line is the
+// number of columns (a token starts at a non-space after a space of isSpace, and C isspace
+// ends one, as there); the rest is tokenised on isSpace and cut into rows of that many tokens
+// (a partial last row is dropped); each row's first column is a V id and its second a U id,
+// read as atol reads them (further columns, such as weights or times, are ignored).  Each
+// side's ids are renumbered in increasing order over the ids that occur; the lists are
+// sorted and duplicate edges dropped.  Ids must be below 2^32 and not negative.
+extern "C" int pbf_read_konect(const char* path, pbf_graph* out) {
+  if (!path || !out) return PBF_ERR_ARG;
+  memset(out, 0, sizeof(*out));
+  Mapped f;
+  if (!f.open(path)) return PBF_ERR_IO;
+  const char* p = f.p;
+  const size_t n = f.n;
+  size_t k = 0;
+  while (k < n && p[k] == '%') {
+    while (k < n && p[k] != '\n') k++;
+    if (k < n) k++;
+  }
+  int64_t cols = 0;
+  bool prev_space = true;
+  for (size_t t = k; t < n && p[t] != '\n'; t++) {
+    const char c = p[t];
+    if (!is_space(c) && prev_space) { prev_space = false; cols++; }
+    else if (c == ' ' || c == '\t' || c == '\v' || c == '\f' || c == '\r') prev_space = true;
+  }
+  if (cols < 2) return PBF_ERR_FORMAT;
+  std::vector<uint32_t> vv, uu;
+  Tok t{p + k, p + n};
+  const char *ws, *we;
+  int64_t col = 0;
+  uint32_t row[2] = {0, 0};
+  // atol of a token: an optional '+' and the leading digits (0 when there are none)
+  auto value = [](const char* a, const char* b, uint32_t* x) -> int {
+    if (a < b && *a == '-') {
+      for (const char* q = a + 1; q < b && *q >= '0' && *q <= '9'; q++)
+        if (*q != '0') return PBF_ERR_RANGE;
+      *x = 0;
+      return PBF_OK;
+    }
+    if (a < b && *a == '+') a++;
+    uint64_t v = 0;
+    for (; a < b && *a >= '0' && *a <= '9'; a++) {
+      v = v * 10 + uint64_t(*a - '0');
+      if (v >> 32) return PBF_ERR_RANGE;
+    }
+    *x = (uint32_t)v;
+    return PBF_OK;
+  };
+  while (t.next_word(&ws, &we)) {
+    if (col < 2) {
+      const int st = value(ws, we, &row[col]);
+      if (st) return st;
+    }
+    if (++col == cols) {
+      vv.push_back(row[0]);
+      uu.push_back(row[1]);
+      col = 0;
+    }
+  }
+  const int64_t m = (int64_t)vv.size();
+  if (2 * m >= (int64_t(1) << 31)) return PBF_ERR_CAPACITY;
+  // dense ids per side, in increasing order of the original ids
+  auto compact = [](std::vector<uint32_t>& ids) -> int64_t {
+    std::vector<uint32_t> seen(ids);
+    __gnu_parallel::sort(seen.begin(), seen.end());
+    seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)ids.size(); i++)
+      ids[i] = (uint32_t)(std::lower_bound(seen.begin(), seen.end(), ids[i]) - seen.begin());
+    return (int64_t)seen.size();
+  };
+  const int64_t nv = compact(vv), nu = compact(uu);
+  return pbf_graph_from_edges(nv, nu, m, vv.data(), uu.data(), out);
+}
+
 extern "C" int pbf_write_bipartite(const char* path, const pbf_bipartite_csr* g) {
   if (!path || !g) return PBF_ERR_ARG;
   FILE* f = fopen(path, "w");

@@ -2,16 +2,18 @@
 
 Run in a container that has the reference tree (oracle/Makefile builds oracle/_ref/ref_harness
 from the reference's own headers):
-    python tests/golden/make_golden.py [--sparse-only | --side-only]
+    python tests/golden/make_golden.py [--sparse-only | --side-only | --konect-only]
 For every graph below it stores the AdjacencyHypergraph input (<name>.adj) and the reference's
 outputs (<name>.npz): cmpWedgeCounts, per-vertex counts (SERIAL and BATCHS), per-edge counts,
 totals, tip numbers (PBATCHS), wing numbers (PBATCHS), and the reference's ADEG / ACOCORE orders
 with the wedge count W each gives; sparse_cases.npz holds the graphs that the reference's
 sparsifiers make of them (make_sparse); side_cases.npz holds cmpWedgeCounts (and the counts)
-of graphs on which its 32-bit C(d,2) wraps (make_side).  The reference prints only totals
+of graphs on which its 32-bit C(d,2) wraps (make_side); konect_*.tsv / .adj are KONECT edge
+lists and what the reference's utils/KONECTtoHyperAdj.C writes for them (make_konect).  The reference prints only totals
 (BipartiteButterflies.C:215, 234, 262); the harness dumps the arrays that Compute discards.
 """
 import os
+import subprocess
 import sys
 import tempfile
 
@@ -96,6 +98,40 @@ def make_side(tmp):
     np.savez_compressed(os.path.join(HERE, "side_cases.npz"), **rec)
 
 
+def konect_inputs():
+    """KONECT edge lists (name -> text): comment lines, 2 to 4 columns, duplicate edges, sparse
+    and unsorted ids."""
+    rng = np.random.default_rng(11)
+    out = {}
+    rows = ["% bip unweighted", "% 14 6 7"]
+    for v, h in [(3, 10), (3, 11), (50, 10), (7, 2), (3, 10), (50, 2), (7, 11), (9, 40), (9, 2), (7, 10), (50, 11),
+                 (3, 2), (9, 10), (7, 40)]:
+        rows.append(f"{v}\t{h}")
+    out["konect_small"] = "\n".join(rows) + "\n"
+    rows = ["% bip weighted positive", "%  note: extra header line"]
+    vs, hs = rng.integers(1, 400, 3000) * 3 + 1, rng.integers(1, 150, 3000) * 7
+    for v, h, w, t in zip(vs, hs, rng.integers(1, 9, 3000), rng.integers(10 ** 9, 2 * 10 ** 9, 3000)):
+        rows.append(f"{v} {h} {w} {t}")
+    out["konect_weighted"] = "\n".join(rows) + "\n"
+    g = pb.Graph.chung_lu(3000, 1000, 20000, 2.1, 2.3, 13)
+    owner = np.repeat(np.arange(g.nv), np.diff(g.offsetsV.astype(np.int64)))
+    perm = rng.permutation(g.m)
+    rows = ["% sym unweighted"] + [f"{owner[i] + 1}   {g.edgesV[i] + 1}  1" for i in perm]
+    out["konect_chunglu"] = "\n".join(rows) + "\n"
+    return out
+
+
+def make_konect(tmp):
+    """<name>.tsv (the input) and <name>.adj (the reference's KONECTtoHyperAdj output)."""
+    conv = os.path.join(ROOT, "oracle", "_ref", "konect_to_hyperadj")
+    for name, text in konect_inputs().items():
+        src = os.path.join(HERE, name + ".tsv")
+        with open(src, "w") as f:
+            f.write(text)
+        subprocess.run([conv, src, os.path.join(HERE, name + ".adj")], check=True, stdout=subprocess.DEVNULL)
+        print("konect", name)
+
+
 def main():
     O.build()
     if not O.ref_available():
@@ -104,6 +140,10 @@ def main():
     if "--side-only" in sys.argv:
         make_side(tmp)
         return
+    if "--konect-only" in sys.argv:
+        make_konect(tmp)
+        return
+    make_konect(tmp)
     make_sparse(tmp)
     make_side(tmp)
     if "--sparse-only" in sys.argv:

@@ -166,3 +166,42 @@ def test_validate_error_codes():
     oV, eV, oU, eU = parts()
     oU[-1] -= 1
     assert pb.Graph(g.nv, g.nu, oV, eV, oU, eU).validate() == 2
+
+
+KONECT_CASES = ["konect_small", "konect_weighted", "konect_chunglu"]
+
+
+@pytest.mark.parametrize("name", KONECT_CASES)
+def test_konect_matches_reference(name, tmp_path):
+    # the graph and the AdjacencyHypergraph text the reference's KONECTtoHyperAdj wrote for it
+    import subprocess
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    g = pb.Graph.read_konect(os.path.join(gold, name + ".tsv"))
+    ref = pb.Graph.read(os.path.join(gold, name + ".adj"))
+    assert (g.nv, g.nu, g.m) == (ref.nv, ref.nu, ref.m)
+    for k in ("offsetsV", "edgesV", "offsetsU", "edgesU"):
+        assert np.array_equal(getattr(g, k), getattr(ref, k)), k
+    conv = os.path.join(os.path.dirname(pb.library_path()), "bin", "konect_to_hyperadj")
+    out = tmp_path / "out.adj"
+    subprocess.run([conv, os.path.join(gold, name + ".tsv"), str(out)], check=True)
+    with open(os.path.join(gold, name + ".adj"), "rb") as f:
+        assert out.read_bytes() == f.read()
+
+
+def test_konect_errors(tmp_path):
+    cases = {"one column": ("% x\n1\n2\n3\n", "PBF_ERR_FORMAT"), "empty": ("% only comments\n", "PBF_ERR_FORMAT"),
+             "negative id": ("1 2\n-3 4\n", "PBF_ERR_RANGE"), "id beyond 32 bits": ("1 4294967296\n", "PBF_ERR_RANGE")}
+    for tag, (text, err) in cases.items():
+        p = tmp_path / (tag.replace(" ", "_") + ".tsv")
+        p.write_text(text)
+        with pytest.raises(pb.PbfError) as e:
+            pb.Graph.read_konect(str(p))
+        assert e.value.name == err, tag
+    with pytest.raises(pb.PbfError) as e:
+        pb.Graph.read_konect(str(tmp_path / "missing.tsv"))
+    assert e.value.name == "PBF_ERR_IO"
+    # a partial last row is dropped, "+" signs and leading zeros are read as atol reads them
+    p = tmp_path / "partial.tsv"
+    p.write_text("%c\n+7 005 1\n3 5 2\n7\n")
+    g = pb.Graph.read_konect(str(p))
+    assert (g.nv, g.nu, g.m) == (2, 1, 2)
