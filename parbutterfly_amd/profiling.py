@@ -2,6 +2,7 @@
 
     python -m parbutterfly_amd.profiling workload NAME [--reps N]
     python -m parbutterfly_amd.profiling stats DIR [--top N]   # rocprofv3 --stats table
+    python -m parbutterfly_amd.profiling counters DIR [DIR ...] --kernels "K1;K2"   # --pmc table
 
 runs one workload on device 0 (graph generated and uploaded first, one untimed warm-up call,
 then N calls), so that a kernel trace or a counter run covers that workload alone.  NAME:
@@ -32,7 +33,7 @@ from collections import defaultdict
 COUNT_PHASE = ("k_rinfo", "k_slotinfo", "k_fill_u32", "k_fill_u64", "k_classify", "k_classify_big", "k_wedge_chunks",
                "k_scan_tiles", "k_tile_sums", "k_scan_sums", "k_light_hist", "k_light_plan", "k_light_scatter",
                "k_split_pass1", "k_split_credit", "k_split_pass2", "k_count2", "k_count_hbm", "k_hot_reduce",
-               "k_convert_vert", "k_oidx", "k_edge_out_s", "k_edge_out_o")
+               "k_convert_vert", "k_oidx", "k_edge_out_s", "k_edge_out_o", "k_hlog_reduce", "k_endpoint_wedges")
 
 
 def _short(name):
@@ -102,6 +103,36 @@ def kernel_stats(run_dir, top=30):
     return "\n".join(out)
 
 
+def counter_table(run_dirs, kernels):
+    """Markdown table of rocprofv3 --pmc counters (one or more runs): each counter's value per
+    dispatch (summed over the device), averaged over the dispatches of each kernel (matched on
+    the full name's prefix, e.g. "k_count2<0, 1>")."""
+    vals = defaultdict(lambda: defaultdict(list))
+    for d in run_dirs:
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            per = defaultdict(float)
+            name = {}
+            with open(fn) as f:
+                for r in csv.DictReader(f):
+                    key = (int(r["Dispatch_Id"]), r["Counter_Name"])
+                    per[key] += float(r["Counter_Value"])
+                    name[int(r["Dispatch_Id"])] = r["Kernel_Name"].replace("(anonymous namespace)::", "").replace("void ", "")
+            for (disp, cn), v in per.items():
+                full = name[disp]
+                for k in kernels:
+                    if full.startswith(k):
+                        vals[cn][k].append(v)
+    out = ["| counter (per dispatch) | " + " | ".join(f"`{k}`" for k in kernels) + " |",
+           "|---|" + "---:|" * len(kernels)]
+    for cn in sorted(vals):
+        row = []
+        for k in kernels:
+            xs = vals[cn].get(k, [])
+            row.append(f"{sum(xs) / len(xs):.3g}" if xs else "")
+        out.append(f"| {cn} | " + " | ".join(row) + " |")
+    return "\n".join(out)
+
+
 WORKLOADS = ("vert-chunglu", "edge-chunglu", "edge-rmat", "tip-chunglu", "wing-rmat")
 
 
@@ -147,6 +178,9 @@ def main():
     st = sub.add_parser("stats")
     st.add_argument("dir")
     st.add_argument("--top", type=int, default=30)
+    cnt = sub.add_parser("counters")
+    cnt.add_argument("dirs", nargs="+")
+    cnt.add_argument("--kernels", required=True, help="kernel name prefixes separated by semicolons")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -160,6 +194,9 @@ def main():
         return
     if a.cmd == "stats":
         print(kernel_stats(a.dir, a.top))
+        return
+    if a.cmd == "counters":
+        print(counter_table(a.dirs, [k.strip() for k in a.kernels.split(";")]))
         return
     cf = calibration(a.cal_fetch, "FETCH_SIZE", a.cal_bytes)
     cw = calibration(a.cal_write, "WRITE_SIZE", a.cal_bytes)
