@@ -78,6 +78,13 @@ struct CountArgs {
   unsigned long long* hlog;
   uint32_t* hlog_len;      // entries reserved (may pass hlog_cap; the rest are not stored)
   uint32_t hlog_cap;
+  // Per-edge counts: the per-wedge credits of pass 2 go to 32-bit per-slot counters where no
+  // sum can reach 2^32 (the output adds them to B).  A slot (v -> u2) receives part of the
+  // butterflies of edge (v, u2), at most (deg(v) - 1) (deg(u2) - 1) <= (deg(v) - 1) (maxdeg - 1),
+  // so the slots of a centre that keeps that product below 2^32 take 32-bit credits.  Null:
+  // every credit is 64-bit.
+  uint32_t* B32;
+  const uint32_t* maxdeg;  // the largest degree of the ranked graph (written by k_rinfo)
 };
 
 __device__ __forceinline__ uint32_t lo_of(const CountArgs& a, uint32_t i) { return a.lo ? a.lo[i] : 0u; }
@@ -692,7 +699,8 @@ __device__ __forceinline__ void win_get(const Tab& t, const Win& wv, uint32_t* c
 // segment with a segmented scan; the last lane of each run adds the run's sum.  The u32 sums
 // hold at most 256 values below deg(i) < 2^24.
 template <int MODE, int TM, bool MULTI, bool IL>
-__device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, const Win& wv, lds_u64* acc) {
+__device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, const Win& wv, lds_u64* acc,
+                                           uint64_t wide) {
   uint32_t val[WIN], cnt[WIN];
   uint32_t last = wv.seg[0];
   win_get<TM, MULTI>(t, wv, cnt);
@@ -704,7 +712,11 @@ __device__ __forceinline__ void win_credit(const CountArgs& a, const Tab& t, con
       const uint32_t c = cnt[k];
       const uint32_t v = (k < wv.nval && c > 1u) ? c - 1u : 0u;
       if constexpr (MODE == MODE_EDGE) {
-        if (v) atomicAdd(&a.B[wv.pos[k]], (unsigned long long)v);
+        if (v) {
+          // wide: the segments whose centre's slots need 64-bit credits
+          if (wide != 0ull && ((wide >> wv.seg[k]) & 1ull)) atomicAdd(&a.B[wv.pos[k]], (unsigned long long)v);
+          else atomicAdd(a.B32 + wv.pos[k], v);
+        }
       }
       if (__ballot(v != 0u) == 0ull) continue;
       const uint32_t key = k < wv.nval ? wv.seg[k] : 64u;
@@ -756,6 +768,18 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
   const uint32_t lane = __lane_id();
   if (ch.T <= wstart) return true;
   const uint32_t tag = next_tag(wc);
+  uint64_t wide = ~0ull;  // per segment: its centre's slots take 64-bit credits
+  if constexpr (PASS2 && MODE == MODE_EDGE) {
+    if (a.B32 != nullptr) {
+      bool w = false;
+      if (ch.L != 0u) {
+        const uint64_t d = a.roff[ch.v + 1] - a.roff[ch.v];
+        const uint32_t md = *a.maxdeg;
+        w = (d - 1u) * (uint64_t)(md > 0u ? md - 1u : 0u) >= (1ull << 32);
+      }
+      wide = __ballot(w);
+    }
+  }
   Win cur, nxt;
   uint32_t b0 = wstart;
   win_map<MULTI, IL>(a, wc.flags, b0, ch, tag, cur);
@@ -781,7 +805,7 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
         return false;
       }
     } else {
-      win_credit<MODE, TM, MULTI, IL>(a, t, cur, wc.acc);
+      win_credit<MODE, TM, MULTI, IL>(a, t, cur, wc.acc, wide);
     }
     if (!more) break;
     cur = nxt;
@@ -1582,15 +1606,18 @@ __global__ __launch_bounds__(TPB) void k_split_pass2(CountArgs a, const uint32_t
 // Per-edge output from the per-slot counts: every edge has one slot owned by its S end (flag 0)
 // and one owned by its O end (flag 1); the first pass stores, the second adds, each writing
 // every edge exactly once (no atomics).
+// A slot's count is its 64-bit counter plus its 32-bit one.
 __global__ void k_edge_out_s(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ oidx,
-                             const unsigned long long* __restrict__ Bs, int64_t m2, unsigned long long* __restrict__ out) {
+                             const unsigned long long* __restrict__ Bs, const uint32_t* __restrict__ B32, int64_t m2,
+                             unsigned long long* __restrict__ out) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x)
-    if ((redg[p] & 1u) == 0u) out[oidx[p]] = Bs[p];
+    if ((redg[p] & 1u) == 0u) out[oidx[p]] = Bs[p] + B32[p];
 }
 __global__ void k_edge_out_o(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ oidx,
-                             const unsigned long long* __restrict__ Bs, int64_t m2, unsigned long long* __restrict__ out) {
+                             const unsigned long long* __restrict__ Bs, const uint32_t* __restrict__ B32, int64_t m2,
+                             unsigned long long* __restrict__ out) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m2; p += (int64_t)gridDim.x * blockDim.x)
-    if (redg[p] & 1u) out[oidx[p]] += Bs[p];
+    if (redg[p] & 1u) out[oidx[p]] += Bs[p] + B32[p];
 }
 
 __global__ void k_oidx(const uint32_t* __restrict__ redg, const uint32_t* __restrict__ rpos, const uint32_t* __restrict__ eti,
@@ -1604,10 +1631,19 @@ __global__ void k_oidx(const uint32_t* __restrict__ redg, const uint32_t* __rest
 // inside N(y) (a binary search over the in-prefix of N(y)).  The counting kernels read both
 // beside the slot, which keeps two dependent loads and the search off their critical path.
 // rinfo[r] = (roff[r], in(r)): one 8-byte gather per slot instead of two 4-byte ones.
+// With maxdeg (cleared beforehand), also the largest degree.
 __global__ void k_rinfo(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ inpref, int64_t n,
-                        uint2* __restrict__ rinfo) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
-    rinfo[r] = make_uint2(roff[r], inpref[r]);
+                        uint2* __restrict__ rinfo, uint32_t* __restrict__ maxdeg) {
+  uint32_t md = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = roff[r];
+    rinfo[r] = make_uint2(o, inpref[r]);
+    md = max(md, roff[r + 1] - o);
+  }
+  if (maxdeg) {
+    md = wave_incl_max(md);
+    if (__lane_id() == 63u && md) atomicMax(maxdeg, md);
+  }
 }
 
 __global__ void k_slotinfo(const uint2* __restrict__ rinfo, const uint32_t* __restrict__ redg,
@@ -1715,7 +1751,8 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* li
     void* rip;
     PBF_CHECK(pbf_scratch(ctx, S_RINFO, 8 * (size_t)(n ? n : 1), &rip));
     uint2* rinfo = static_cast<uint2*>(rip);
-    hipLaunchKernelGGL(k_rinfo, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.inpref, n, rinfo);
+    hipLaunchKernelGGL(k_rinfo, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, R.roff, R.inpref, n, rinfo,
+                       const_cast<uint32_t*>(a0.maxdeg));
     hipLaunchKernelGGL(k_slotinfo, dim3(grid_for(R.m2, 256, 16384)), dim3(256), 0, ctx->stream, rinfo, R.redg, R.rown, R.lo,
                        R.m2, const_cast<uint2*>(a0.sinfo));
   }
@@ -1912,12 +1949,21 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
     if (m2) hipLaunchKernelGGL(k_oidx, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, R.rpos, eti, m2, oidx);
     PBF_CHECK(pbf_scratch(ctx, S_COUNTS, 8 * (size_t)(m2 ? m2 : 1), &cp));
     a.B = static_cast<unsigned long long*>(cp);
+    void* b32;
+    PBF_CHECK(pbf_scratch(ctx, S_B32, 4 * (size_t)(m2 ? m2 : 1), &b32));
+    a.B32 = static_cast<uint32_t*>(b32);
+    uint32_t* md = static_cast<uint32_t*>(mp) + 28;
+    a.maxdeg = md;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, md, 0u, (int64_t)1);
     if (m2) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(m2, 256, 8192)), dim3(256), 0, ctx->stream, a.B, 0ull, m2);
+    if (m2) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(m2, 256, 8192)), dim3(256), 0, ctx->stream, a.B32, 0u, m2);
     PBF_CHECK(run_count<MODE_EDGE>(ctx, R, a, lists, ctl, opts.sync));
     if (m2) {
       unsigned long long* out = reinterpret_cast<unsigned long long*>(out_dev);
-      hipLaunchKernelGGL(k_edge_out_s, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, oidx, a.B, m2, out);
-      hipLaunchKernelGGL(k_edge_out_o, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, oidx, a.B, m2, out);
+      hipLaunchKernelGGL(k_edge_out_s, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, oidx, a.B, a.B32, m2,
+                         out);
+      hipLaunchKernelGGL(k_edge_out_o, dim3(grid_for(m2, 256, 16384)), dim3(256), 0, ctx->stream, R.redg, oidx, a.B, a.B32, m2,
+                         out);
     }
   } else {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, a.total, 0ull, (int64_t)1);

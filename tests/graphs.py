@@ -39,6 +39,20 @@ def hub_wrap_graph():
     return pb.Graph.from_edges(1000, nu, v, u)
 
 
+def wide_centre_graph(k=69999):
+    """Two adjacent hubs of degree k + 1: V vertex 0 on U vertices 0..k, U vertex 0 on V
+    vertices 0..k, plus the edges (j, j) for j = 1..k.  The butterflies are {0, j} x {0, j}, one
+    per j: edge (0, 0) lies in k of them, every other edge in one.  Under ADEG the two hubs rank
+    first and the U hub is the centre of a wedge from every V vertex j; with k + 1 > 65536,
+    deg(hub) - 1 times the largest degree - 1 reaches 2^32, so the per-edge pass credits those
+    chunks in 64 bits."""
+    j = np.arange(1, k + 1, dtype=np.uint32)
+    z = np.zeros(k, np.uint32)
+    v = np.concatenate([np.zeros(1, np.uint32), z, j, j])
+    u = np.concatenate([np.zeros(1, np.uint32), j, z, j])
+    return pb.Graph.from_edges(k + 1, k + 1, v, u)
+
+
 def hbm_graph(keys, pad):
     """S = U (wV <= wU) under SIDE ranking: U vertex `keys + pad` shares the V hub v0 with U
     vertices 0..keys-1, so as an endpoint it has `keys` distinct keys at a rank above 16384

@@ -8,7 +8,7 @@ import pytest
 
 import parbutterfly_amd as pb
 from oracle import oracle as O
-from tests.graphs import complete_bipartite, empty_graph, hbm_graph, small_graphs
+from tests.graphs import complete_bipartite, empty_graph, hbm_graph, small_graphs, wide_centre_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -196,3 +196,18 @@ def test_side_wraps_as_reference(ctx):
     h = pb.Graph.chung_lu(1 << 20, 1 << 18, 1 << 24, 2.1, 2.1, 2)
     k = "chunglu_2p20x2p18_m2p24_g21_s2"
     assert ctx.side(h.to_device(ctx)) == (bool(rec[k + "__use_v"][0]), int(rec[k + "__side_wedges"][0]))
+
+
+def test_edge_wide_centres(ctx):
+    # per-edge pass 2 credits a chunk in 64 bits when one of its centres could pass 2^32 in a
+    # 32-bit slot counter; here every chunk through the U hub does (closed form: edge (0, 0) in
+    # k butterflies, every other edge in one)
+    k = 69999
+    g = wide_centre_graph(k)
+    for rt in RANKS:
+        use_v, _ = ctx.side(g)
+        e = pb.as_u64(ctx.count(g, use_v, "EDGE", "GPU", rt))
+        assert e.shape == (3 * k + 1,)
+        assert int(e.max()) == k and int((e == k).sum()) == 1 and int((e == 1).sum()) == 3 * k, rt
+        assert np.array_equal(e, O.count_edge(g, use_v, rt)), rt
+        assert ctx.count(g, use_v, "TOTAL", "GPU", rt) == k
