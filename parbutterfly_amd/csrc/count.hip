@@ -899,16 +899,35 @@ __device__ __forceinline__ unsigned long long sweep(const CountArgs& a, const Ta
   return sum;
 }
 
+// A heavy endpoint's record: the endpoint, roff[i], deg(i), its distinct-key bound, its first
+// entry (whose flag says whether i lies on S) and its graph's first rank.
+struct HRec {
+  uint32_t i, io, deg, bnd, e0, lo;
+};
+
+__device__ __forceinline__ void hrec_lists(const CountArgs& a, HRec& r) {
+  r.io = a.roff[r.i];
+  r.deg = a.roff[r.i + 1] - r.io;
+  r.bnd = a.bound[r.i];
+  r.lo = lo_of(a, r.i);
+}
+
 // Heavy endpoint, whole workgroup.  Returns false when its hash table overflowed (nothing was
-// credited).
-template <int MODE, int TM>
+// credited).  after_pass1(0) runs after pass 1, after_pass1(1) after the barrier that follows,
+// at_end before the last barrier (all three on every path): the item pipeline's loads.
+template <int MODE, int TM, typename FA, typename FB>
 __device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t lo, uint32_t deg,
                               bool i_in_S, uint32_t cells, WaveCtx& wc, const Track& tr, unsigned long long* red,
-                              unsigned long long* total_acc) {
+                              unsigned long long* total_acc, FA&& after_pass1, FB&& at_end) {
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   enum_pass<MODE, false, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr);
+  after_pass1(0);
   __syncthreads();
-  if (lds_load(tr.ovf)) return false;
+  after_pass1(1);
+  if (lds_load(tr.ovf)) {
+    at_end();
+    return false;
+  }
   if (MODE == MODE_TOTAL || (MODE == MODE_VERT && i_in_S)) {
     unsigned long long sum = pbf_wave_sum(sweep<MODE, TM>(a, t, cells, tid, TPB, lo, wc.hl));
     if (lane == 0) red[w] = sum;
@@ -924,17 +943,18 @@ __device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint
   }
   if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S))
     enum_pass<MODE, true, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr);
+  at_end();
   __syncthreads();
   return true;
 }
 
-template <int MODE>
-__device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCtx& wc, lds_u32* nused, lds_u32* ovf,
-                           unsigned long long* red, unsigned long long* total_acc) {
+template <int MODE, typename FA, typename FB>
+__device__ bool block_item(const CountArgs& a, const HRec& r, lds_u32* pool, WaveCtx& wc, lds_u32* nused, lds_u32* ovf,
+                           unsigned long long* red, unsigned long long* total_acc, FA&& after_pass1, FB&& at_end) {
   const uint32_t tid = threadIdx.x;
-  const uint32_t io = a.roff[i], deg = a.roff[i + 1] - io;
-  const bool i_in_S = (a.redg[io] & 1u) == 0u;
-  const uint32_t lo = lo_of(a, i), il = i - lo;  // keys are local ranks below il
+  const uint32_t i = r.i, io = r.io, deg = r.deg;
+  const bool i_in_S = (r.e0 & 1u) == 0u;
+  const uint32_t lo = r.lo, il = i - lo;  // keys are local ranks below il
   Tab t{pool, 0, 0, a.cb, nullptr, 0};
   Track tr{nused, ovf, 0xFFFFFFFFu, false};
   int tm;
@@ -944,7 +964,7 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
   } else if (il < D16_MAX && deg < 65536u) {
     tm = TM_D16; words = (il + 1) >> 1;
   } else {
-    const uint32_t bnd = a.bound[i];
+    const uint32_t bnd = r.bnd;
     const bool packed = a.cb < 32u && deg < (1u << a.cb);
     const uint32_t capmax = packed ? POOL_WORDS : POOL_WORDS / 2;
     const uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
@@ -969,10 +989,15 @@ __device__ bool block_item(const CountArgs& a, uint32_t i, lds_u32* pool, WaveCt
   if (tid == 0) { *nused = 0; *ovf = 0; }
   __syncthreads();
   switch (tm) {
-    case TM_D32: return block_item_tm<MODE, TM_D32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
-    case TM_D16: return block_item_tm<MODE, TM_D16>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
-    case TM_HASH32: return block_item_tm<MODE, TM_HASH32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
-    default: return block_item_tm<MODE, TM_HASH>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc);
+    case TM_D32:
+      return block_item_tm<MODE, TM_D32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1, at_end);
+    case TM_D16:
+      return block_item_tm<MODE, TM_D16>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1, at_end);
+    case TM_HASH32:
+      return block_item_tm<MODE, TM_HASH32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1,
+                                            at_end);
+    default:
+      return block_item_tm<MODE, TM_HASH>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1, at_end);
   }
 }
 
@@ -1109,6 +1134,7 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   __shared__ unsigned long long s_iacc[NWAVE][32];
   __shared__ unsigned long long s_red[NWAVE];
   __shared__ uint32_t s_item, s_nused, s_ovf, s_wovf[NWAVE];
+  __shared__ HRec s_hrec;
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned long long total_acc = 0;
   *(lds_u32x4*)((lds_u32*)s_flags[wid] + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
@@ -1116,23 +1142,53 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   if (lane < 32) s_iacc[wid][lane] = 0;
   __syncthreads();
   WaveCtx wc{(lds_u32*)s_flags[wid], (lds_u64*)s_acc[wid], 0};
-  // phase 1: heavy endpoints, whole workgroup; the next one is claimed while this one runs
+  // phase 1: heavy endpoints, whole workgroup.  Software pipeline over items k, k+1, k+2 (thread
+  // 0): while item k runs, item k+2 is claimed and item k+1's record is loaded -- its endpoint
+  // at the start, its list bounds and key bound after pass 1, its first entry at the end -- so
+  // an item starts from its record in LDS, not from three dependent loads.
   if constexpr ((PH & 1) != 0) {
   const uint32_t nb = *a.nblock;
-  uint32_t claim = 0;
-  if (threadIdx.x == 0) s_item = atomicAdd(&a.queues[0], 1u);
-  __syncthreads();
-  uint32_t idx = s_item;
-  __syncthreads();
-  while (idx < nb) {
-    if (threadIdx.x == 0) claim = atomicAdd(&a.queues[0], 1u);
-    const uint32_t i = a.blist[idx];
-    if (!block_item<MODE>(a, i, (lds_u32*)pool, wc, (lds_u32*)&s_nused, (lds_u32*)&s_ovf, s_red, &total_acc)) {
-      if (threadIdx.x == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = i;
+  uint32_t n1 = 0, n2 = 0;  // thread 0: the next item and the one after it
+  HRec r1{};                // thread 0: the next item's record
+  if (threadIdx.x == 0) {
+    const uint32_t q0 = atomicAdd(&a.queues[0], 1u);
+    n1 = atomicAdd(&a.queues[0], 1u);
+    HRec r{};
+    if (q0 < nb) {
+      r.i = a.blist[q0];
+      hrec_lists(a, r);
+      r.e0 = a.redg[r.io];  // a listed endpoint has wedges, so a non-empty list
     }
-    if (threadIdx.x == 0) s_item = claim;
-    __syncthreads();
-    idx = s_item;
+    s_hrec = r;
+    s_item = q0;
+  }
+  __syncthreads();
+  while (true) {
+    const uint32_t idx = s_item;
+    const HRec r = s_hrec;
+    __syncthreads();  // every thread holds the record before thread 0 stores the next one
+    if (idx >= nb) break;
+    if (threadIdx.x == 0) {
+      n2 = atomicAdd(&a.queues[0], 1u);
+      if (n1 < nb) r1.i = a.blist[n1];
+    }
+    auto after_pass1 = [&](int stage) {
+      if (threadIdx.x == 0 && n1 < nb) {
+        if (stage == 0) hrec_lists(a, r1);
+        else r1.e0 = a.redg[r1.io];
+      }
+    };
+    auto at_end = [&]() {
+      if (threadIdx.x == 0) {
+        s_hrec = r1;
+        s_item = n1;
+        n1 = n2;
+      }
+    };
+    if (!block_item<MODE>(a, r, (lds_u32*)pool, wc, (lds_u32*)&s_nused, (lds_u32*)&s_ovf, s_red, &total_acc, after_pass1,
+                          at_end)) {
+      if (threadIdx.x == 0) a.ovf_list[atomicAdd(a.ovf_len, 1u)] = r.i;
+    }
     __syncthreads();
   }
   }
@@ -1776,9 +1832,10 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* li
   uint32_t* split_chunks = split + max_split;
   uint32_t* split_base = split + 2 * max_split;
   uint32_t* G = static_cast<uint32_t*>(gp);
-  // light-table bound: the context's, or the mode's default (per-edge counting has a second
-  // pass with a 64-bit credit per wedge for every endpoint, and keeps larger tables light)
-  const uint32_t want = ctx->tune.light_max ? ctx->tune.light_max : (MODE == MODE_EDGE ? LIGHT_MAX : 767u);
+  // light-table bound: the context's, or the default 767 distinct keys (at most 1024 slots per
+  // sub-table); measured on the Chung-Lu 2^20 x 2^18 graph, larger light tables were slower for
+  // both per-vertex and per-edge counts (per-edge: 18.1 ms at 767, 19.5 ms at 1535)
+  const uint32_t want = ctx->tune.light_max ? ctx->tune.light_max : 767u;
   const uint32_t lmax = want < LIGHT_MAX ? want : LIGHT_MAX;
   const unsigned grid = (unsigned)(ctx->num_cus * (ctx->tune.blocks_per_cu ? ctx->tune.blocks_per_cu : 4));
   std::vector<uint32_t> bounds{0u, (uint32_t)n};
