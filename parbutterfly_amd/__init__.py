@@ -478,7 +478,7 @@ class Context:
         return bool(uv.value), nw.value
 
     def rank(self, g, rank_type="ADEG"):
-        """Returns (order, rank_of, W) as uint32 arrays (torch tensors for device graphs)."""
+        """Returns (order, rank_of, W) as uint32 arrays (DeviceArrays for device graphs)."""
         order = self._alloc_out(g, g.n, np.uint32)
         rank_of = self._alloc_out(g, g.n, np.uint32)
         w = ctypes.c_int64()

@@ -57,6 +57,19 @@ def test_cli_host_read_under_asan(cli, name):
             assert len(re.findall(r"number of butterflies: (\d+)", r.stdout)) == rounds, (args, r.stdout)
 
 
+def test_cli_timing_labels(cli):
+    # every count type runs the GPU kernels; the timing line says so and names the requested type
+    adj = os.path.join(GOLD, "k_4_5.adj")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1", OMP_NUM_THREADS="4")
+    for args, want in ((["-countType", "SERIAL", "-per", "TOTAL"], "GPU (requested Serial) : "),
+                       (["-countType", "BATCHS", "-per", "VERT", "-peelType", "BATCHS"], "GPU Peel (requested Par Peel) : "),
+                       (["-countType", "GPU", "-per", "EDGE"], "GPU : ")):
+        r = subprocess.run([cli, "-hostRead", "-r", "0", *args, adj], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, env=env, timeout=120)
+        assert r.returncode == 0 and want in r.stdout, (args, r.stdout)
+        assert not re.search(r"^(Par|Serial|Par Peel) : ", r.stdout, re.M), r.stdout
+
+
 def test_cli_malformed_input_under_asan(cli, tmp_path):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1", OMP_NUM_THREADS="4")
     bad = {"empty": "", "header only": "AdjacencyHypergraph\n", "truncated": "AdjacencyHypergraph\n2\n3\n2\n3\n0\n1\n0\n",
