@@ -358,6 +358,9 @@ __device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* tou
 
 // ---------------- vertex peel walk ----------------
 enum TableKind { T_LDS = 0, T_HBM = 1 };
+// Vertex walk teams: wavefronts with 1024-slot tables for small items, workgroups with
+// 8192-slot tables for large ones, 32 workgroups with HBM rows for the rest.
+constexpr int V_SMALL_TEAM = 64, V_SMALL_CAP = 1024, V_LARGE_TEAM = 256, V_LARGE_CAP = 8192, V_HBM_BLOCKS = 32;
 
 struct PeelVArgs {
   const uint32_t* offS;
@@ -382,22 +385,45 @@ struct PeelVArgs {
   uint32_t bs_log;
 };
 
+// A walk team's LDS: its shared scratch and, for LDS tables, keys, counts and used slots.
 template <int TEAM, int CAP, bool USED, int KIND>
-__global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
-  __shared__ TeamShared<TEAM> sh;
-  __shared__ uint32_t lkeys[KIND == T_LDS ? CAP : 1];
-  __shared__ uint32_t lcnts[KIND == T_LDS ? CAP : 1];
-  __shared__ uint32_t lused[(KIND == T_LDS && USED) ? CAP : 1];
-  const uint32_t tid = threadIdx.x;
+struct WalkLds {
+  TeamShared<TEAM> sh;
+  uint32_t keys[KIND == T_LDS ? CAP : 1];
+  uint32_t cnts[KIND == T_LDS ? CAP : 1];
+  uint32_t used[(KIND == T_LDS && USED) ? CAP : 1];
+};
+
+// The barrier of a walk team: the workgroup, or for a one-wavefront team inside a larger
+// workgroup the wavefront (an LDS fence and a wave barrier).
+template <int TEAM>
+__device__ __forceinline__ void walk_barrier() {
+  if constexpr (TEAM == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
+
+// Frontier walks of one team (team index `team` of its role, thread `tid` of TEAM): items are
+// drained from the list's queue.  A TEAM of 256 is a whole workgroup; a TEAM of 64 is one
+// wavefront and synchronises only itself.
+template <int TEAM, int CAP, bool USED, int KIND>
+__device__ void peelv_walk(const PeelVArgs& a, WalkLds<TEAM, CAP, USED, KIND>& L, uint32_t team, uint32_t tid) {
+  TeamShared<TEAM>& sh = L.sh;
+  uint32_t* lkeys = L.keys;
+  uint32_t* lcnts = L.cnts;
+  uint32_t* lused = L.used;
   const unsigned long long k = dev_ld(a.k);
   if (k == KNONE) return;
   const uint32_t nlist = dev_ld(a.list_len);
-  // the first nlist workgroups drain the queue: the others leave without touching its counter
-  if (blockIdx.x >= nlist) return;
+  // the first nlist teams drain the queue: the others leave without touching its counter
+  if (team >= nlist) return;
   uint32_t* dense = nullptr;
   uint32_t* dused = nullptr;
   if constexpr (KIND == T_HBM) {
-    dense = a.pool + (size_t)blockIdx.x * (size_t)a.ns * 2;
+    dense = a.pool + (size_t)team * (size_t)a.ns * 2;
     dused = dense + a.ns;
   }
   if constexpr (KIND == T_LDS && USED) {
@@ -405,9 +431,9 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
   }
   while (true) {
     if (tid == 0) sh.item = atomicAdd(a.queue, 1u);
-    __syncthreads();
+    walk_barrier<TEAM>();
     const uint32_t idx = sh.item;
-    __syncthreads();
+    walk_barrier<TEAM>();
     if (idx >= nlist) break;
     const uint32_t u = dev_ld(a.list + idx);
     const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
@@ -429,7 +455,7 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
       }
     }
     if (tid == 0) { sh.nused = 0; sh.ovf = 0; }
-    __syncthreads();
+    walk_barrier<TEAM>();
     for (uint32_t c0 = 0; c0 < deg; c0 += TEAM) {
       const uint32_t j = c0 + tid;
       uint32_t L = 0, vo = 0;
@@ -442,7 +468,7 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
       const uint32_t incl = team_incl_scan<TEAM>(L, sh, &T);
       sh.P[tid] = incl;
       sh.vs[tid] = vo;
-      __syncthreads();
+      walk_barrier<TEAM>();
       for (uint32_t w = tid; w < T; w += TEAM) {
         const uint32_t s = seg_of<TEAM>(sh.P, w);
         const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
@@ -454,7 +480,7 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
           if (atomicAdd(&dense[u2], 1u) == 0u) dused[atomicAdd(&sh.nused, 1u)] = u2;
         }
       }
-      __syncthreads();
+      walk_barrier<TEAM>();
     }
     const bool overflowed = sh.ovf != 0;
     if (!overflowed) {
@@ -477,7 +503,7 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
     } else if (tid == 0) {
       a.ovf_list[atomicAdd(a.ovf_len, 1u)] = u;
     }
-    __syncthreads();
+    walk_barrier<TEAM>();
     if constexpr (KIND == T_LDS && USED) {
       const uint32_t nu_ = sh.nused < CAP ? sh.nused : CAP;
       for (uint32_t q = tid; q < nu_; q += TEAM) { const uint32_t slot = lused[q]; lkeys[slot] = EMPTY; lcnts[slot] = 0; }
@@ -485,7 +511,22 @@ __global__ __launch_bounds__(TEAM) void k_peelv_walk(PeelVArgs a) {
     if constexpr (KIND == T_HBM) {
       for (uint32_t q = tid; q < sh.nused; q += TEAM) dense[dused[q]] = 0;
     }
-    __syncthreads();
+    walk_barrier<TEAM>();
+  }
+}
+
+// Launch 1 of a round's walks: wavefront teams (4 per workgroup) on the small list in
+// workgroups [0, small_blocks), workgroup teams on the large list in the others.
+__global__ __launch_bounds__(256) void k_peelv_walks(PeelVArgs small, PeelVArgs large, uint32_t small_blocks) {
+  __shared__ union {
+    WalkLds<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS> s[4];
+    WalkLds<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS> l;
+  } lds;
+  if (blockIdx.x < small_blocks) {
+    const uint32_t w = threadIdx.x >> 6;
+    peelv_walk<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS>(small, lds.s[w], blockIdx.x * 4 + w, threadIdx.x & 63);
+  } else {
+    peelv_walk<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS>(large, lds.l, blockIdx.x - small_blocks, threadIdx.x);
   }
 }
 
@@ -626,12 +667,11 @@ __device__ __forceinline__ void vs_count(const VSplitArgs& a, const TeamShared<2
   }
 }
 
-// Phase 1: workgroups take whole chunks (chunk g, g + grid, ...) and count their first unit;
-// a chunk with more than VS_PCH paths reserves consecutive room for its other units and lists
-// them for phase 2.  The list has room for every unit of a round: a row has at most
-// m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
-__global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
-  __shared__ TeamShared<256> sh;
+// Phase 1: workgroups take whole chunks (chunk g, g + nb, ... for workgroup bid of nb) and
+// count their first unit; a chunk with more than VS_PCH paths reserves consecutive room for its
+// other units and lists them for phase 2.  The list has room for every unit of a round: a row
+// has at most m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
+__device__ void vsplit_chunks(const VSplitArgs& a, TeamShared<256>& sh, uint32_t bid, uint32_t nb) {
   const unsigned long long k = dev_ld(a.k);
   const uint32_t nrows = vs_nrows(a);
   if (k == KNONE || nrows == 0) return;
@@ -642,7 +682,7 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
     if (r == 0) { u0 = u; nch0 = nch; }
     total += nch;
   }
-  for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+  for (uint32_t g = bid; g < total; g += nb) {
     // the chunks of row 0 (most rounds have one row) need no search
     uint32_t r = 0, c = g, u = u0;
     if (g >= nch0) {
@@ -660,6 +700,20 @@ __global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a) {
       if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 59) | ((unsigned long long)g << 27) | s;
     }
   }
+}
+
+// Launch 2 of a round's walks: the HBM-row walks of the items no LDS table held (their list is
+// complete, launch 1 having ended) in workgroups [0, V_HBM_BLOCKS), phase 1 of the split walks
+// in the others.
+__global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a, PeelVArgs hbm) {
+  __shared__ union {
+    TeamShared<256> sh;
+    WalkLds<V_LARGE_TEAM, 1, true, T_HBM> h;
+  } lds;
+  if (blockIdx.x < (uint32_t)V_HBM_BLOCKS)
+    peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>(hbm, lds.h, blockIdx.x, threadIdx.x);
+  else
+    vsplit_chunks(a, lds.sh, blockIdx.x - V_HBM_BLOCKS, gridDim.x - V_HBM_BLOCKS);
 }
 
 // Phase 2: workgroup b takes the listed units [b * per, (b + 1) * per); consecutive units of
@@ -960,7 +1014,6 @@ __global__ void k_init_peel(const unsigned long long* __restrict__ counts, unsig
   }
 }
 
-constexpr int V_SMALL_TEAM = 64, V_SMALL_CAP = 1024, V_LARGE_TEAM = 256, V_LARGE_CAP = 8192, V_HBM_BLOCKS = 32;
 // Edge walks are latency bound: the wavefront teams keep a small LDS copy of N(b) (longer lists
 // are searched in place) so that 8 teams fit per SIMD.
 constexpr int E_SMALL_TEAM = 64, E_SMALL_UCAP = 256, E_SMALL_PER_CU = 32;
@@ -1119,17 +1172,19 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL(k_vblocks_prepare, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.state, pb.B, pb.D, ns, pb.c);
     hipLaunchKernelGGL(k_vblocks_select, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.D, pb.B, pb.state, ns, pb.work,
                        small_max, small_list, large_list, ss, pb.c);
+    // walks: wavefront and workgroup teams in one launch (3 * num_cus workgroups of 4 wavefront
+    // teams, 2 * num_cus workgroup teams), then the HBM-row walks of what overflowed together
+    // with phase 1 of the split walks
     PeelVArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
     s.ovf_list = ovf_list; s.ovf_len = pb.c.len + 2;
-    hipLaunchKernelGGL((k_peelv_walk<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS>), dim3(ctx->num_cus * 12), dim3(V_SMALL_TEAM), 0,
-                       ctx->stream, s);
-    s.list = large_list; s.list_len = pb.c.len + 1; s.queue = pb.c.queue + 1;
-    hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS>), dim3(ctx->num_cus * 2), dim3(V_LARGE_TEAM), 0,
-                       ctx->stream, s);
-    s.list = ovf_list; s.list_len = pb.c.len + 2; s.queue = pb.c.queue + 2;
-    hipLaunchKernelGGL((k_peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>), dim3(V_HBM_BLOCKS), dim3(V_LARGE_TEAM), 0, ctx->stream, s);
-    hipLaunchKernelGGL(k_vsplit_chunks, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
+    PeelVArgs l = s;
+    l.list = large_list; l.list_len = pb.c.len + 1; l.queue = pb.c.queue + 1;
+    PeelVArgs h = s;
+    h.list = ovf_list; h.list_len = pb.c.len + 2; h.queue = pb.c.queue + 2;
+    const uint32_t small_blocks = (uint32_t)ctx->num_cus * 3;
+    hipLaunchKernelGGL(k_peelv_walks, dim3(small_blocks + ctx->num_cus * 2), dim3(256), 0, ctx->stream, s, l, small_blocks);
+    hipLaunchKernelGGL(k_vsplit_chunks, dim3(V_HBM_BLOCKS + ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs, h);
     hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
     hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(64 * VS_CREDIT_WAVES), 0, ctx->stream, vs);
     PBF_HIP(hipGetLastError());
