@@ -1832,10 +1832,12 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* li
   uint32_t* split_chunks = split + max_split;
   uint32_t* split_base = split + 2 * max_split;
   uint32_t* G = static_cast<uint32_t*>(gp);
-  // light-table bound: the context's, or the default 767 distinct keys (at most 1024 slots per
-  // sub-table); measured on the Chung-Lu 2^20 x 2^18 graph, larger light tables were slower for
-  // both per-vertex and per-edge counts (per-edge: 18.1 ms at 767, 19.5 ms at 1535)
-  const uint32_t want = ctx->tune.light_max ? ctx->tune.light_max : 767u;
+  // light-table bound: the context's, or the mode's default.  Per-vertex counting: 767 keys
+  // (1024-slot sub-tables; 13.2 ms against 14.6 ms at 1535 on the Chung-Lu 2^20 x 2^18 graph).
+  // Per-edge counting: 1535 (2048-slot sub-tables), which its second pass over every endpoint
+  // favours on the R-MAT 2^22 x 2^22 graph (82 ms against 94 ms at 767) more than the Chung-Lu
+  // graph favours 767 (18.0 against 19.2 ms).
+  const uint32_t want = ctx->tune.light_max ? ctx->tune.light_max : (MODE == MODE_EDGE ? LIGHT_MAX : 767u);
   const uint32_t lmax = want < LIGHT_MAX ? want : LIGHT_MAX;
   const unsigned grid = (unsigned)(ctx->num_cus * (ctx->tune.blocks_per_cu ? ctx->tune.blocks_per_cu : 4));
   std::vector<uint32_t> bounds{0u, (uint32_t)n};
@@ -1896,8 +1898,9 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* li
     a.queues = ctl + 16;
     a.rerun = 1;
     hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
+    // the log's reducer: four workgroups per CU (32 KB of accumulators each)
     if (MODE == MODE_VERT && a.hlog_cap)
-      hipLaunchKernelGGL(k_hlog_reduce, dim3(ctx->num_cus * 2), dim3(256), 0, ctx->stream, a.hlog, a.hlog_len, a.hlog_cap,
+      hipLaunchKernelGGL(k_hlog_reduce, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, a.hlog, a.hlog_len, a.hlog_cap,
                          a.B, n);
     PBF_HIP(hipGetLastError());
     uint32_t novf = 0;
