@@ -515,21 +515,6 @@ __device__ void peelv_walk(const PeelVArgs& a, WalkLds<TEAM, CAP, USED, KIND>& L
   }
 }
 
-// Launch 1 of a round's walks: wavefront teams (4 per workgroup) on the small list in
-// workgroups [0, small_blocks), workgroup teams on the large list in the others.
-__global__ __launch_bounds__(256) void k_peelv_walks(PeelVArgs small, PeelVArgs large, uint32_t small_blocks) {
-  __shared__ union {
-    WalkLds<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS> s[4];
-    WalkLds<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS> l;
-  } lds;
-  if (blockIdx.x < small_blocks) {
-    const uint32_t w = threadIdx.x >> 6;
-    peelv_walk<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS>(small, lds.s[w], blockIdx.x * 4 + w, threadIdx.x & 63);
-  } else {
-    peelv_walk<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS>(large, lds.l, blockIdx.x - small_blocks, threadIdx.x);
-  }
-}
-
 // bound[u] = min(sum_{v in N(u)} deg(v), ns): distinct two-hop neighbours of u; paths[u] = the
 // sum itself (two-hop paths, saturating at 2^31 - 1).
 __global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ edgS,
@@ -624,7 +609,7 @@ __device__ uint32_t vs_chunk(const VSplitArgs& a, TeamShared<256>& sh, uint32_t 
   return T;
 }
 
-// The stored prefix of global chunk g (written by k_vsplit_chunks) into sh; returns its total.
+// The stored prefix of global chunk g (written by phase 1, vsplit_chunks) into sh; returns its total.
 __device__ uint32_t vs_chunk_stored(const VSplitArgs& a, TeamShared<256>& sh, uint32_t g) {
   const uint32_t tid = threadIdx.x;
   const uint32_t* q = reinterpret_cast<const uint32_t*>(a.pre + (size_t)g * VS_CHUNK + tid);
@@ -702,32 +687,17 @@ __device__ void vsplit_chunks(const VSplitArgs& a, TeamShared<256>& sh, uint32_t
   }
 }
 
-// Launch 2 of a round's walks: the HBM-row walks of the items no LDS table held (their list is
-// complete, launch 1 having ended) in workgroups [0, V_HBM_BLOCKS), phase 1 of the split walks
-// in the others.
-__global__ __launch_bounds__(256) void k_vsplit_chunks(VSplitArgs a, PeelVArgs hbm) {
-  __shared__ union {
-    TeamShared<256> sh;
-    WalkLds<V_LARGE_TEAM, 1, true, T_HBM> h;
-  } lds;
-  if (blockIdx.x < (uint32_t)V_HBM_BLOCKS)
-    peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>(hbm, lds.h, blockIdx.x, threadIdx.x);
-  else
-    vsplit_chunks(a, lds.sh, blockIdx.x - V_HBM_BLOCKS, gridDim.x - V_HBM_BLOCKS);
-}
-
 // Phase 2: workgroup b takes the listed units [b * per, (b + 1) * per); consecutive units of
 // one chunk reuse the chunk's prefix already in LDS, which is read from the copy phase 1 kept
 // (the first pre_cap chunks of a round) or else scanned again; the next list entry is loaded
 // while the current unit runs.
-__global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
-  __shared__ TeamShared<256> sh;
+__device__ void vsplit_units(const VSplitArgs& a, TeamShared<256>& sh, uint32_t bid, uint32_t nb) {
   const unsigned long long k = dev_ld(a.k);
   const uint32_t listed = dev_ld(a.ctr + 1);
   if (k == KNONE) return;
   const uint32_t n = listed < a.sub_cap ? listed : a.sub_cap;
-  const uint32_t per = (uint32_t)(((uint64_t)n + gridDim.x - 1) / gridDim.x);
-  const uint64_t q0 = (uint64_t)blockIdx.x * per;
+  const uint32_t per = (uint32_t)(((uint64_t)n + nb - 1) / nb);
+  const uint64_t q0 = (uint64_t)bid * per;
   const uint64_t q1 = q0 + per < n ? q0 + per : n;
   if (q0 >= q1) return;
   uint32_t cr = ~0u, cg = ~0u, u = 0, T = 0;
@@ -750,6 +720,41 @@ __global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a) {
     vs_count(a, sh, r, u, (uint32_t)e & 0x7FFFFFFu, T);
     e = en;
   }
+}
+
+// A round's walks run in two launches.  Launch 1: phase 1 of the split walks in workgroups
+// [0, chunk_blocks), wavefront teams (4 per workgroup) on the small list in the next
+// small_blocks, workgroup teams on the large list in the rest.
+__global__ __launch_bounds__(256) void k_peelv_walks(PeelVArgs small, PeelVArgs large, VSplitArgs vs, uint32_t chunk_blocks,
+                                                     uint32_t small_blocks) {
+  __shared__ union {
+    TeamShared<256> sh;
+    WalkLds<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS> s[4];
+    WalkLds<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS> l;
+  } lds;
+  if (blockIdx.x < chunk_blocks) {
+    vsplit_chunks(vs, lds.sh, blockIdx.x, chunk_blocks);
+  } else if (blockIdx.x < chunk_blocks + small_blocks) {
+    const uint32_t w = threadIdx.x >> 6;
+    peelv_walk<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS>(small, lds.s[w], (blockIdx.x - chunk_blocks) * 4 + w,
+                                                        threadIdx.x & 63);
+  } else {
+    peelv_walk<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS>(large, lds.l, blockIdx.x - chunk_blocks - small_blocks,
+                                                         threadIdx.x);
+  }
+}
+
+// Launch 2: the HBM-row walks of the items no LDS table held (their list is complete, launch 1
+// having ended) in workgroups [0, V_HBM_BLOCKS), phase 2 of the split walks in the others.
+__global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a, PeelVArgs hbm) {
+  __shared__ union {
+    TeamShared<256> sh;
+    WalkLds<V_LARGE_TEAM, 1, true, T_HBM> h;
+  } lds;
+  if (blockIdx.x < (uint32_t)V_HBM_BLOCKS)
+    peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>(hbm, lds.h, blockIdx.x, threadIdx.x);
+  else
+    vsplit_units(a, lds.sh, blockIdx.x - V_HBM_BLOCKS, gridDim.x - V_HBM_BLOCKS);
 }
 
 // One wavefront per block of 64 keys, over all rows at once: lane r < cap reads row r's flag
@@ -1172,9 +1177,9 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     hipLaunchKernelGGL(k_vblocks_prepare, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.state, pb.B, pb.D, ns, pb.c);
     hipLaunchKernelGGL(k_vblocks_select, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.D, pb.B, pb.state, ns, pb.work,
                        small_max, small_list, large_list, ss, pb.c);
-    // walks: wavefront and workgroup teams in one launch (3 * num_cus workgroups of 4 wavefront
-    // teams, 2 * num_cus workgroup teams), then the HBM-row walks of what overflowed together
-    // with phase 1 of the split walks
+    // walks: phase 1 of the split walks (2 * num_cus workgroups), wavefront teams (3 * num_cus
+    // workgroups of 4) and workgroup teams (2 * num_cus) in one launch; then the HBM-row walks of
+    // what overflowed together with phase 2 of the split walks (8 * num_cus workgroups)
     PeelVArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
     s.ovf_list = ovf_list; s.ovf_len = pb.c.len + 2;
@@ -1182,10 +1187,10 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     l.list = large_list; l.list_len = pb.c.len + 1; l.queue = pb.c.queue + 1;
     PeelVArgs h = s;
     h.list = ovf_list; h.list_len = pb.c.len + 2; h.queue = pb.c.queue + 2;
-    const uint32_t small_blocks = (uint32_t)ctx->num_cus * 3;
-    hipLaunchKernelGGL(k_peelv_walks, dim3(small_blocks + ctx->num_cus * 2), dim3(256), 0, ctx->stream, s, l, small_blocks);
-    hipLaunchKernelGGL(k_vsplit_chunks, dim3(V_HBM_BLOCKS + ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs, h);
-    hipLaunchKernelGGL(k_vsplit_units, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs);
+    const uint32_t chunk_blocks = (uint32_t)ctx->num_cus * 2, small_blocks = (uint32_t)ctx->num_cus * 3;
+    hipLaunchKernelGGL(k_peelv_walks, dim3(chunk_blocks + small_blocks + ctx->num_cus * 2), dim3(256), 0, ctx->stream, s, l,
+                       vs, chunk_blocks, small_blocks);
+    hipLaunchKernelGGL(k_vsplit_units, dim3(V_HBM_BLOCKS + ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs, h);
     hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(64 * VS_CREDIT_WAVES), 0, ctx->stream, vs);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
