@@ -20,7 +20,8 @@ bounded Chung-Lu sample), batched (the per-graph loop over the first 512 graphs)
 
 traffic: HBM bytes of one counting phase from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, each
 scaled by the same counter's reading for pbf_copy_device of a known size (--calibrate-copy),
-computed by parbutterfly_amd/profiling.py into profiles/traffic_count_chunglu_2p24.json.
+computed by parbutterfly_amd/profiling.py into profiles/traffic_count_chunglu_2p24.json (per-edge:
+profiles/traffic_count_edge_chunglu_2p24.json, reported as per_edge.traffic).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--traffic JSON]
     python bench.py --calibrate-copy BYTES     # one device copy of BYTES (counter calibration)
@@ -604,6 +605,11 @@ def main():
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    # the per-edge phase's HBM traffic from its own counter runs
+    etraffic = os.path.join(ROOT, "profiles", "traffic_count_edge_chunglu_2p24.json")
+    if edge is not None and os.path.exists(etraffic):
+        with open(etraffic) as f:
+            edge["traffic"] = json.load(f).get("bytes_per_launch")
 
     rec = {
         "metric": "Gwedges/s per-vertex count",
