@@ -504,6 +504,23 @@ struct Chunk {
   uint32_t td;     // grouped tables: the endpoint's sub-table (word offset | log2 size << 16)
 };
 
+// A chunk from its lanes' slot records (si, the slot's entry e), loaded beforehand.
+__device__ __forceinline__ Chunk make_chunk(uint2 si, uint32_t e, uint32_t slot, bool valid, uint32_t td, uint32_t lo) {
+  Chunk ch;
+  ch.v = 0; ch.L = 0; ch.slot = slot; ch.td = td;
+  uint32_t vo = 0;
+  if (valid) {
+    ch.v = (e >> 1) + lo;  // global rank of the centre
+    vo = si.x;
+    ch.L = si.y;
+  }
+  const uint32_t incl = wave_incl_add(ch.L);
+  ch.T = lane_read(incl, 63);
+  ch.P = incl - ch.L;
+  ch.base = vo - ch.P;
+  return ch;
+}
+
 __device__ __forceinline__ Chunk load_chunk(const CountArgs& a, uint32_t slot, bool valid, uint32_t td, uint32_t lo) {
   Chunk ch;
   ch.v = 0; ch.L = 0; ch.slot = slot; ch.td = td;
@@ -831,13 +848,14 @@ __device__ __forceinline__ bool chunk_windows(const CountArgs& a, const Tab& t, 
 template <int MODE, bool PASS2, int TM>
 __device__ __forceinline__ bool enum_pass(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t lo,
                                           uint32_t jbeg, uint32_t jend, uint32_t w, uint32_t nw, WaveCtx& wc,
-                                          const Track& tr) {
+                                          const Track& tr, const Chunk* first = nullptr) {
   const bool split = nw > 1 && jend - jbeg >= 8 * 64;
   const uint32_t cstart = split ? w * 64 : 0u, cstep = split ? nw * 64 : 64u;
   const uint32_t wstart = split ? 0u : w * WIN_SPAN, wstep = split ? WIN_SPAN : nw * WIN_SPAN;
   for (uint32_t c0 = jbeg + cstart; c0 < jend; c0 += cstep) {
     const uint32_t j = c0 + __lane_id();
-    const Chunk ch = load_chunk(a, io + j, j < jend, 0u, lo);
+    // `first`: the chunk at jbeg, already loaded
+    const Chunk ch = (first != nullptr && c0 == jbeg) ? *first : load_chunk(a, io + j, j < jend, 0u, lo);
     if (!chunk_windows<MODE, PASS2, TM, false>(a, t, ch, wstart, wstep, wc, tr)) return false;
   }
   return true;
@@ -918,9 +936,10 @@ __device__ __forceinline__ void hrec_lists(const CountArgs& a, HRec& r) {
 template <int MODE, int TM, typename FA, typename FB>
 __device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint32_t io, uint32_t lo, uint32_t deg,
                               bool i_in_S, uint32_t cells, WaveCtx& wc, const Track& tr, unsigned long long* red,
-                              unsigned long long* total_acc, FA&& after_pass1, FB&& at_end) {
+                              unsigned long long* total_acc, const Chunk& one, FA&& after_pass1, FB&& at_end) {
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  enum_pass<MODE, false, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr);
+  // the list's first chunk was loaded before the table was cleared, and serves both passes
+  enum_pass<MODE, false, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr, &one);
   after_pass1(0);
   __syncthreads();
   after_pass1(1);
@@ -942,7 +961,7 @@ __device__ bool block_item_tm(const CountArgs& a, const Tab& t, uint32_t i, uint
     }
   }
   if (MODE == MODE_EDGE || (MODE == MODE_VERT && !i_in_S))
-    enum_pass<MODE, true, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr);
+    enum_pass<MODE, true, TM>(a, t, i, io, lo, 0, deg, w, NWAVE, wc, tr, &one);
   at_end();
   __syncthreads();
   return true;
@@ -955,6 +974,15 @@ __device__ bool block_item(const CountArgs& a, const HRec& r, lds_u32* pool, Wav
   const uint32_t i = r.i, io = r.io, deg = r.deg;
   const bool i_in_S = (r.e0 & 1u) == 0u;
   const uint32_t lo = r.lo, il = i - lo;  // keys are local ranks below il
+  // the list's first chunk (its first 64 slots): loaded now, in flight while the table is cleared
+  const uint32_t lane = tid & 63u;
+  const bool in1 = lane < deg;
+  uint2 si1 = make_uint2(0u, 0u);
+  uint32_t e1 = 0;
+  if (in1) {
+    si1 = a.sinfo[io + lane];
+    e1 = a.redg[io + lane];
+  }
   Tab t{pool, 0, 0, a.cb, nullptr, 0};
   Track tr{nused, ovf, 0xFFFFFFFFu, false};
   int tm;
@@ -988,16 +1016,20 @@ __device__ bool block_item(const CountArgs& a, const HRec& r, lds_u32* pool, Wav
   for (uint32_t q = 4 * tid; q < words; q += 4 * TPB) *(lds_u32x4*)(pool + q) = u32x4{fill, fill, fill, fill};
   if (tid == 0) { *nused = 0; *ovf = 0; }
   __syncthreads();
+  const Chunk one = make_chunk(si1, e1, io + lane, in1, 0u, lo);
   switch (tm) {
     case TM_D32:
-      return block_item_tm<MODE, TM_D32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1, at_end);
+      return block_item_tm<MODE, TM_D32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, one, after_pass1,
+                                         at_end);
     case TM_D16:
-      return block_item_tm<MODE, TM_D16>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1, at_end);
+      return block_item_tm<MODE, TM_D16>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, one, after_pass1,
+                                         at_end);
     case TM_HASH32:
-      return block_item_tm<MODE, TM_HASH32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1,
+      return block_item_tm<MODE, TM_HASH32>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, one, after_pass1,
                                             at_end);
     default:
-      return block_item_tm<MODE, TM_HASH>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, after_pass1, at_end);
+      return block_item_tm<MODE, TM_HASH>(a, t, i, io, lo, deg, i_in_S, cells, wc, tr, red, total_acc, one, after_pass1,
+                                          at_end);
   }
 }
 
