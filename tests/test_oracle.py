@@ -174,3 +174,59 @@ def test_oracle_side_wraps_as_reference():
     h = pb.Graph.chung_lu(1 << 20, 1 << 18, 1 << 24, 2.1, 2.1, 2)
     k = "chunglu_2p20x2p18_m2p24_g21_s2"
     assert O.side(h) == h.side() == (bool(rec[k + "__use_v"][0]), int(rec[k + "__side_wedges"][0]))
+
+
+def _level_minimum(g, use_v, tip, k):
+    """Least butterfly count, in the subgraph induced by the S vertices with tip >= k and all of
+    O, over those S vertices (the oracle's count of that subgraph)."""
+    offS, edgS = (g.offsetsU, g.edgesU) if use_v else (g.offsetsV, g.edgesV)
+    owner = np.repeat(np.arange(len(offS) - 1, dtype=np.uint32), np.diff(offS.astype(np.int64)))
+    keep = tip[owner] >= k
+    s_ids, o_ids = owner[keep], edgS[keep].astype(np.uint32)
+    vv, uu = (o_ids, s_ids) if use_v else (s_ids, o_ids)
+    h = pb.Graph.from_edges(g.nv, g.nu, np.ascontiguousarray(vv), np.ascontiguousarray(uu))
+    return int(O.count_vert(h, use_v)[tip >= k].min())
+
+
+def test_tip_levels_are_exact_minima():
+    # when the peel reaches level k, the live S vertices are those with tip >= k and each one's D
+    # is its butterfly count among them, so k is the least such count at every distinct level
+    # (the property the full-size GPU test checks).  A tip raised by one breaks it: the vertex
+    # would join a subgraph in which, by the maximality of tips, some vertex has fewer
+    g = pb.Graph.chung_lu(1500, 800, 10000, 2.1, 2.1, 4)
+    use_v, _ = O.side(g)
+    tip = O.peel_vert(g, use_v, O.count_vert(g, use_v))
+    levels = np.unique(tip)
+    assert len(levels) > 100
+    for k in levels:
+        assert _level_minimum(g, use_v, tip, int(k)) == int(k), int(k)
+    bad = tip.copy()
+    j = int(np.argmin(tip))
+    bad[j] += 1
+    assert _level_minimum(g, use_v, bad, int(bad[j])) != int(bad[j])
+
+
+def test_wing_levels_are_exact_minima():
+    # the edge-peel counterpart: at every distinct level k, the edges with wing >= k hold at least
+    # k butterflies each among themselves and some edge exactly k; a wing raised by one breaks it
+    # (by the maximality of wings)
+    g = pb.Graph.chung_lu(800, 600, 8000, 2.1, 2.1, 8)
+    use_v, _ = O.side(g)
+    wing = O.peel_edge(g, use_v, O.count_edge(g, use_v))
+    offO, edgO = (g.offsetsV, g.edgesV) if use_v else (g.offsetsU, g.edgesU)
+    owner = np.repeat(np.arange(len(offO) - 1, dtype=np.uint32), np.diff(offO.astype(np.int64)))
+    vv, uu = (owner, edgO) if use_v else (edgO, owner)
+
+    def least(w, k):
+        sel = w >= k
+        h = pb.Graph.from_edges(g.nv, g.nu, np.ascontiguousarray(vv[sel]), np.ascontiguousarray(uu[sel]))
+        return int(O.count_edge(h, O.side(h)[0]).min())
+
+    levels = np.unique(wing)
+    assert len(levels) > 50
+    for k in levels:
+        assert least(wing, int(k)) == int(k), int(k)
+    bad = wing.copy()
+    j = int(np.argmin(wing))
+    bad[j] += 1
+    assert least(bad, int(bad[j])) != int(bad[j])
