@@ -476,7 +476,8 @@ def rmat_bench(pb, ctx, log_n=22, m=120_000_000, sub_edges=4_000_000, cpu=True):
 def config1_bench(pb, ctx, steps=50, cpu=True):
     """The reference's CPU-runnable case: uniform random bipartite graph, nv = nu = 1000, 10,000
     distinct edges, seed 1, ADEG, per-vertex counts.  GPU: the graph resident in HBM, one step
-    = ranking + relabel + counting + conversion; CPU: the reference's SERIAL Count on one core
+    = ranking + relabel + counting + conversion, as one call and as a captured one-graph batch
+    (replayed as a HIP graph); CPU: the reference's SERIAL Count on one core
     and its BATCHS Count on all host cores (medians of 3 rounds after a warm-up), compared
     bit-exactly."""
     g = pb.Graph.uniform(1000, 1000, 10000, 1)
@@ -495,6 +496,20 @@ def config1_bench(pb, ctx, steps=50, cpu=True):
     gc = pb.as_u64(out)
     res = {"workload": "uniform bipartite nv=nu=1000 m=10000 seed=1, ADEG, per-vertex counts", "wedges": W,
            "gpu_ms_per_step": round(sec * 1e3, 4), "gpu_gwedges_s": round(W / sec / 1e9, 4)}
+    # the same graph as a one-graph captured batch: the whole step replayed as one HIP graph, with
+    # no host synchronisation inside it (a graph this small is bound by launches and syncs)
+    outs = ctx.count_batch([dg], "ADEG", mode="union", capture=True)
+    outs = ctx.count_batch([dg], "ADEG", mode="union", capture=True, outputs=outs)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs = ctx.count_batch([dg], "ADEG", mode="union", capture=True, outputs=outs)
+    ctx.synchronize()
+    csec = (time.perf_counter() - t0) / steps
+    vb = outs[0].numpy()[:len(gc)]
+    res["gpu_captured_ms_per_step"] = round(csec * 1e3, 4)
+    res["gpu_captured_gwedges_s"] = round(W / csec / 1e9, 4)
+    res["captured_bit_exact"] = bool(np.array_equal(vb, gc))
     if cpu:
         for ct, thr in (("SERIAL", 1), ("BATCHS", host_cores())):
             r = ref_run("vert", g, ct, "ADEG", rounds=4, threads=thr)
