@@ -66,6 +66,7 @@ enum ScratchSlot {
   S_WBATCH,      // wedge-bounded counting: wedges per endpoint and their prefix
   S_HLOG,        // per-vertex counting: the log of credits to hub ranks
   S_B32,         // per-edge counting: 32-bit per-slot credits of the wedge pass [2m]
+  S_RKEY,        // relabel: (rank, key) per vertex [n]
   S_NUM_SLOTS
 };
 

@@ -88,6 +88,12 @@ __global__ __launch_bounds__(1024) void k_key_scatter(const uint8_t* __restrict_
   (void)lane;
 }
 
+__global__ void k_rank_key(const uint32_t* __restrict__ rank_of, const uint8_t* __restrict__ key, int64_t n,
+                           uint2* __restrict__ rk) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+    rk[x] = make_uint2(rank_of[x], key[x]);
+}
+
 __global__ void k_ranked_degree(DevGraph g, const uint32_t* __restrict__ order, uint32_t* __restrict__ rdeg) {
   int64_t n = g.nv + g.nu;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
@@ -103,8 +109,7 @@ __device__ __forceinline__ uint32_t entry_flag(int use_v, const uint8_t* __restr
 }
 
 __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
-                                                 const uint8_t* __restrict__ key,
-                                                 const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
+                                                 const uint2* __restrict__ rk, const uint32_t* __restrict__ roff,
                                                  const uint32_t* __restrict__ lo, uint32_t* __restrict__ redg,
                                                  uint32_t* __restrict__ rpos, uint32_t* __restrict__ inpref,
                                                  uint32_t* __restrict__ rown) {
@@ -121,13 +126,14 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
     const int64_t nbr_shift = xv ? g.nv : 0;  // combined id of neighbour = adj[j] + shift
     const uint32_t flag = entry_flag(use_v, sflag, x, xv);
-    const uint32_t r = rank_of[x];
+    const uint2 rkx = rk[x];
+    const uint32_t r = rkx.x;
     const uint32_t lox = lo ? lo[r] : 0u;  // graph-local entries (a union ranked graph by graph)
     if (lane < KEYS + 1) { cnt[wid][lane] = 0; run[wid][lane] = 0; }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     // pass A: key histogram of the neighbours
-    for (uint32_t j = lane; j < deg; j += 64) atomicAdd(&cnt[wid][key[adj[j] + nbr_shift]], 1u);
+    for (uint32_t j = lane; j < deg; j += 64) atomicAdd(&cnt[wid][rk[adj[j] + nbr_shift].y], 1u);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     // exclusive prefix in descending key order: start[k] = sum_{k' > k} cnt[k']
@@ -137,7 +143,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
-      const uint32_t kself = key[x];
+      const uint32_t kself = rkx.y;
       // in(r): neighbours of strictly higher key, plus same-key neighbours when x is in U
       inpref[r] = run[wid][kself] + (xv ? 0u : cnt[wid][kself]);
     }
@@ -146,21 +152,22 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     for (uint32_t j0 = 0; j0 < deg; j0 += 64) {
       const uint32_t j = j0 + lane;
       const bool act = j < deg;
-      uint32_t y = 0, ky = 0;
-      if (act) { y = (uint32_t)(adj[j] + nbr_shift); ky = key[y]; }
+      uint2 ry = make_uint2(0u, 0u);
+      if (act) ry = rk[adj[j] + nbr_shift];  // the neighbour's rank and key, one 8-byte gather
+      const uint32_t ky = ry.y;
       const uint64_t active = __ballot(act);
       const uint64_t mm = pbf_match_key6(ky, active);
-      const uint32_t rk = __popcll(mm & pbf_lanemask_lt());
+      const uint32_t rkn = __popcll(mm & pbf_lanemask_lt());
       uint32_t slot = 0;
       if (act) {
-        slot = rbase + run[wid][ky] + rk;
-        redg[slot] = ((rank_of[y] - lox) << 1) | flag;
+        slot = rbase + run[wid][ky] + rkn;
+        redg[slot] = ((ry.x - lox) << 1) | flag;
         rown[slot] = r;
         if (rpos) rpos[slot] = obase + j;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      if (act && rk == 0) run[wid][ky] += __popcll(mm);
+      if (act && rkn == 0) run[wid][ky] += __popcll(mm);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
@@ -171,7 +178,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
 // the position of entry j counted directly (entries of a higher key, then earlier entries of
 // the same key), so a wavefront serves 64 vertices instead of one.
 __global__ __launch_bounds__(256) void k_relabel_small(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
-                                                       const uint8_t* __restrict__ key, const uint32_t* __restrict__ rank_of,
+                                                       const uint2* __restrict__ rk,
                                                        const uint32_t* __restrict__ roff, const uint32_t* __restrict__ lo,
                                                        uint32_t* __restrict__ redg, uint32_t* __restrict__ rpos,
                                                        uint32_t* __restrict__ inpref, uint32_t* __restrict__ rown) {
@@ -184,14 +191,17 @@ __global__ __launch_bounds__(256) void k_relabel_small(DevGraph g, int use_v, co
     const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
     const int64_t sh = xv ? g.nv : 0;
     const uint32_t flag = entry_flag(use_v, sflag, x, xv);
-    const uint32_t r = rank_of[x];
+    const uint2 rkx = rk[x];
+    const uint32_t r = rkx.x;
     const uint32_t lox = lo ? lo[r] : 0u;
-    const uint32_t kself = key[x];
-    uint32_t y[RL_SMALL], ky[RL_SMALL];
+    const uint32_t kself = rkx.y;
+    // each neighbour's rank and key: one 8-byte gather
+    uint32_t yr[RL_SMALL], ky[RL_SMALL];
 #pragma unroll
     for (uint32_t j = 0; j < RL_SMALL; j++) {
-      y[j] = j < deg ? (uint32_t)(adj[j] + sh) : 0u;
-      ky[j] = j < deg ? (uint32_t)key[y[j]] : 0u;
+      const uint2 ry = j < deg ? rk[adj[j] + sh] : make_uint2(0u, 0u);
+      yr[j] = ry.x;
+      ky[j] = ry.y;
     }
     // in(r): neighbours of strictly higher key, plus same-key neighbours when x is in U
     uint32_t in = 0;
@@ -208,7 +218,7 @@ __global__ __launch_bounds__(256) void k_relabel_small(DevGraph g, int use_v, co
         for (uint32_t k = 0; k < RL_SMALL; k++)
           pos += (k < deg && (ky[k] > ky[j] || (k < j && ky[k] == ky[j]))) ? 1u : 0u;
         const uint32_t slot = rbase + pos;
-        redg[slot] = ((rank_of[y[j]] - lox) << 1) | flag;
+        redg[slot] = ((yr[j] - lox) << 1) | flag;
         rown[slot] = r;
         if (rpos) rpos[slot] = obase + j;
       }
@@ -247,7 +257,7 @@ __device__ __forceinline__ void vertex_list(const DevGraph& g, uint32_t x, uint3
   *shift = xv ? g.nv : 0;
 }
 
-__global__ __launch_bounds__(256) void k_rl_hist(DevGraph g, const uint8_t* __restrict__ key, LongLists L) {
+__global__ __launch_bounds__(256) void k_rl_hist(DevGraph g, const uint2* __restrict__ rk, LongLists L) {
   __shared__ uint32_t h[4][KEYS + 1];
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint32_t nl = *L.nlong, tot = *L.nitems;
@@ -262,7 +272,7 @@ __global__ __launch_bounds__(256) void k_rl_hist(DevGraph g, const uint8_t* __re
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const uint32_t j1 = min(deg, (c + 1) * RL_CHUNK);
-    for (uint32_t j = c * RL_CHUNK + lane; j < j1; j += 64) atomicAdd(&h[wid][key[adj[j] + sh]], 1u);
+    for (uint32_t j = c * RL_CHUNK + lane; j < j1; j += 64) atomicAdd(&h[wid][rk[adj[j] + sh].y], 1u);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane < KEYS) L.hist[(size_t)item * KEYS + lane] = h[wid][lane];
@@ -304,8 +314,7 @@ __global__ __launch_bounds__(256) void k_rl_offsets(DevGraph g, const uint8_t* _
 }
 
 __global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const uint8_t* __restrict__ sflag,
-                                                    const uint8_t* __restrict__ key,
-                                                    const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ roff,
+                                                    const uint2* __restrict__ rk, const uint32_t* __restrict__ roff,
                                                     const uint32_t* __restrict__ lo, LongLists L, uint32_t* __restrict__ redg,
                                                     uint32_t* __restrict__ rpos, uint32_t* __restrict__ rown) {
   __shared__ uint32_t run[4][KEYS + 1];
@@ -320,7 +329,7 @@ __global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const
     vertex_list(g, x, &obase, &deg, &adj, &sh);
     const bool xv = x < g.nv;
     const uint32_t flag = entry_flag(use_v, sflag, x, xv);
-    const uint32_t rx = rank_of[x];
+    const uint32_t rx = rk[x].x;
     const uint32_t lox = lo ? lo[rx] : 0u;
     const uint32_t rbase = roff[rx];
     if (lane < KEYS) run[wid][lane] = L.hist[(size_t)item * KEYS + lane];
@@ -330,19 +339,20 @@ __global__ __launch_bounds__(256) void k_rl_scatter(DevGraph g, int use_v, const
     for (uint32_t j0 = c * RL_CHUNK; j0 < j1; j0 += 64) {
       const uint32_t j = j0 + lane;
       const bool act = j < j1;
-      uint32_t y = 0, ky = 0;
-      if (act) { y = (uint32_t)(adj[j] + sh); ky = key[y]; }
+      uint2 ry = make_uint2(0u, 0u);
+      if (act) ry = rk[adj[j] + sh];
+      const uint32_t ky = ry.y;
       const uint64_t mm = pbf_match_key6(ky, __ballot(act));
-      const uint32_t rk = __popcll(mm & pbf_lanemask_lt());
+      const uint32_t rkn = __popcll(mm & pbf_lanemask_lt());
       if (act) {
-        const uint32_t slot = rbase + run[wid][ky] + rk;
-        redg[slot] = ((rank_of[y] - lox) << 1) | flag;
+        const uint32_t slot = rbase + run[wid][ky] + rkn;
+        redg[slot] = ((ry.x - lox) << 1) | flag;
         rown[slot] = rx;
         if (rpos) rpos[slot] = obase + j;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      if (act && rk == 0) run[wid][ky] += __popcll(mm);
+      if (act && rkn == 0) run[wid][ky] += __popcll(mm);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
@@ -609,12 +619,18 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
   }
   if (n == 0) { R->wedges = 0; return PBF_OK; }
   const uint8_t* key = static_cast<const uint8_t*>(ctx->buf[S_KEY]);
+  // (rank, key) per vertex in one 8-byte record: a neighbour's two are one gather (one line)
+  void* rkp;
+  PBF_CHECK(pbf_scratch(ctx, S_RKEY, 8 * (size_t)n, &rkp));
+  const uint2* rk = static_cast<const uint2*>(rkp);
+  hipLaunchKernelGGL(k_rank_key, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, R->rank_of, key, n,
+                     const_cast<uint2*>(rk));
   hipLaunchKernelGGL(k_ranked_degree, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, g, R->order, R->roff);
   PBF_CHECK(pbf_scan_excl_u32(ctx, R->roff, R->roff, n + 1, nullptr));
-  hipLaunchKernelGGL(k_relabel_small, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, key,
-                     R->rank_of, R->roff, R->lo, R->redg, R->rpos, R->inpref, R->rown);
-  hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 4, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of,
+  hipLaunchKernelGGL(k_relabel_small, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, rk,
                      R->roff, R->lo, R->redg, R->rpos, R->inpref, R->rown);
+  hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 4, 16384)), dim3(256), 0, ctx->stream, g, use_v, sflag, rk, R->roff,
+                     R->lo, R->redg, R->rpos, R->inpref, R->rown);
   // long lists, chunk-parallel
   {
     const int64_t max_items = 2 * (2 * g.m / RL_CHUNK) + 64;
@@ -630,10 +646,10 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
     hipLaunchKernelGGL(k_rl_find_long, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, g, llist, lchunks, lctl);
     PBF_CHECK(pbf_scan_excl_u32(ctx, lchunks, lbase, n, lctl + 1));
     LongLists L{llist, lbase, lctl, lctl + 1, static_cast<uint32_t*>(hp2)};
-    hipLaunchKernelGGL(k_rl_hist, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, key, L);
+    hipLaunchKernelGGL(k_rl_hist, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, rk, L);
     hipLaunchKernelGGL(k_rl_offsets, dim3(grid_for(n, 4, 2048)), dim3(256), 0, ctx->stream, g, key, R->rank_of, L, R->inpref);
-    hipLaunchKernelGGL(k_rl_scatter, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, use_v, sflag, key, R->rank_of, R->roff,
-                       R->lo, L, R->redg, R->rpos, R->rown);
+    hipLaunchKernelGGL(k_rl_scatter, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->stream, g, use_v, sflag, rk, R->roff, R->lo,
+                       L, R->redg, R->rpos, R->rown);
   }
   unsigned long long* acc = static_cast<unsigned long long*>(mp) + 8;
   PBF_CHECK(pbf_fill_u64(ctx, reinterpret_cast<uint64_t*>(acc), 0, 1));
