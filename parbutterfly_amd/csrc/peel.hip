@@ -49,26 +49,15 @@ inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
 
 // ---------------- shared round kernels ----------------
 // ctl layout (u64): [0] k, [1] live count, [2] k of the previous round; u32 view from word 8:
-// list lengths, then queues.
+// list lengths, then queues.  The vertex peel uses len[0..2] its own way (k_vpeel_*).
 struct Ctl {
   unsigned long long* k;
   unsigned long long* live;
   unsigned long long* kprev;
-  uint32_t* len;    // [0] small, [1] large, [2] ovf, [3] ntouched, [4] previous ntouched, [5] found,
-                    // [6] split (reservations; the list holds at most its capacity)
-  uint32_t* queue;  // [0] small, [1] large, [2] ovf, [3..5] split walk counters; all reset per round
+  uint32_t* len;    // edge peel: [0] small, [1] large, [3] ntouched, [4] previous ntouched, [5] found
+  uint32_t* queue;  // edge peel: [0] small, [1] large; reset per round
 };
 constexpr int VS_MAX_ROWS = 32, CTL_QUEUES = 8;
-
-// The split class of the vertex peel: frontier items with at least min_paths two-hop paths, or
-// more possible distinct two-hop neighbours (work) than max_work, are walked by many
-// workgroups at once (k_vsplit_*), up to cap of them per round.  paths == null: no split
-// class (edge peel).
-struct SplitSel {
-  const uint32_t* paths;
-  uint32_t min_paths, max_work, cap;
-  uint32_t* list;
-};
 
 __global__ void k_ctl_init(Ctl c) {
   if (threadIdx.x == 0) { *c.k = KNONE; *c.kprev = KNONE; }
@@ -83,7 +72,6 @@ __global__ void k_round_begin(Ctl c) {
     c.len[5] = 0;
     *c.k = KNONE;
     for (int q = 0; q < 4; q++) c.len[q] = 0;
-    c.len[6] = 0;
     for (int q = 0; q < CTL_QUEUES; q++) c.queue[q] = 0;
   }
 }
@@ -112,22 +100,6 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t x, uint32_t* lis
   if (flag) list[base + __popcll(fm & pbf_lanemask_lt())] = x;
 }
 
-// wave_append into a list of capacity cap: true for the lanes whose x found room (the counter
-// still counts every request).
-__device__ __forceinline__ bool wave_append_cap(bool flag, uint32_t x, uint32_t* list, uint32_t* len, uint32_t cap) {
-  const uint64_t fm = __ballot(flag);
-  if (fm == 0ull) return false;
-  const uint32_t leader = (uint32_t)__ffsll((long long)__ballot(1)) - 1u;
-  uint32_t base = 0;
-  if (__lane_id() == leader) base = atomicAdd(len, (uint32_t)__popcll(fm));
-  base = (uint32_t)__shfl((int)base, (int)leader, 64);
-  if (!flag) return false;
-  const uint32_t pos = base + __popcll(fm & pbf_lanemask_lt());
-  if (pos >= cap) return false;
-  list[pos] = x;
-  return true;
-}
-
 // Frontier from the previous round's touched items.  After a round at level k every live item
 // has D >= k, and the live items with D == k are all touched ones (an untouched item keeps its
 // D, which was not k or it would have been peeled).  So when a touched item has D == k the new
@@ -136,7 +108,7 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
                                  uint8_t* __restrict__ state,
                                  const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
                                  uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
-                                 SplitSel ss, Ctl c) {
+                                 Ctl c) {
   // control words, state and D are loaded before any of them is tested
   const unsigned long long k = dev_ld(c.kprev);
   const uint32_t nt = dev_ld(c.len + 4);
@@ -154,11 +126,8 @@ __global__ void k_select_touched(const unsigned long long* __restrict__ D, const
       // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
       w = B[x] ? (work[x] & WORK_MASK) : 0u;
     }
-    bool sp = false;
-    if (ss.paths)
-      sp = wave_append_cap(sel && w != 0 && (w > ss.max_work || ss.paths[x] >= ss.min_paths), x, ss.list, &c.len[6], ss.cap);
-    wave_append(sel && !sp && w != 0 && w <= small_max, x, small_list, &c.len[0]);
-    wave_append(sel && !sp && w > small_max, x, large_list, &c.len[1]);
+    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
+    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) {
@@ -196,7 +165,7 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
                          uint8_t* __restrict__ state,
                          const uint32_t* __restrict__ ids, int64_t n, const uint32_t* __restrict__ work,
                          uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
-                         SplitSel ss, Ctl c) {
+                         Ctl c) {
   const unsigned long long k = dev_ld(c.k);
   const uint32_t found = dev_ld(c.len + 5);
   if (k == KNONE || found) return;
@@ -213,11 +182,8 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
       // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
       w = B[x] ? (work[x] & WORK_MASK) : 0u;
     }
-    bool sp = false;
-    if (ss.paths)
-      sp = wave_append_cap(sel && w != 0 && (w > ss.max_work || ss.paths[x] >= ss.min_paths), x, ss.list, &c.len[6], ss.cap);
-    wave_append(sel && !sp && w != 0 && w <= small_max, x, small_list, &c.len[0]);
-    wave_append(sel && !sp && w > small_max, x, large_list, &c.len[1]);
+    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
+    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
@@ -244,154 +210,198 @@ __global__ void k_update_retire(const uint32_t* __restrict__ touched, uint32_t* 
   }
 }
 
-// ---------------- vertex peel: minimum of D by blocks of items ----------------
-// The S items are cut into nb <= 256 blocks of 2^bs_log; vmin[b] is the exact minimum of D over
-// the live items of block b at the start of every round.  A round flags the blocks whose items
-// it peeled or decremented (plain stores of 1: an atomic min per decremented item on 256 words
-// serialised), and the next round's k_vblocks_prepare scans only those blocks again: it retires
-// the peeled items and sets D = max(B, k) (updateBuckets) on the live ones.  So the new k is the minimum of the nb block minima, only the blocks
-// holding it are searched for the frontier, and late rounds, which touch few blocks, scan little.
-struct VBlocks {
-  unsigned long long* vmin;  // [nb]
-  uint32_t* flag;            // [nb] 1: the block had frontier or decremented items (or needs its first scan)
-  uint32_t nb;
-  uint32_t bs_log;
-};
-
-// Round start: counters reset (workgroup 0), then every flagged block retires its current items
-// and recomputes its minimum.
-__global__ __launch_bounds__(256) void k_vblocks_prepare(VBlocks vb, uint8_t* __restrict__ state,
-                                                         const unsigned long long* __restrict__ B,
-                                                         unsigned long long* __restrict__ D, int64_t n, Ctl c) {
-  __shared__ unsigned long long s_min[4];
-  const unsigned long long k = dev_ld(c.k);  // the previous round's level (KNONE before the first)
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    for (int q = 0; q < 7; q++) c.len[q] = 0;
-    for (int q = 0; q < CTL_QUEUES; q++) c.queue[q] = 0;
-  }
-  const uint32_t b = blockIdx.x;
-  if (dev_ld(vb.flag + b) == 0u) return;
-  const int64_t lo = (int64_t)b << vb.bs_log;
-  const int64_t hi = lo + (int64_t(1) << vb.bs_log) < n ? lo + (int64_t(1) << vb.bs_log) : n;
-  unsigned long long mn = KNONE;
-  for (int64_t x = lo + threadIdx.x; x < hi; x += blockDim.x) {
-    uint8_t st = state[x];
-    unsigned long long d = D[x];
-    const unsigned long long bx = B[x];
-    if (st == 2) { state[x] = 0; st = 0; }
-    if (st == 1) {
-      // D = max(B, k) holds for every live item, decremented this round or not: an untouched
-      // one with B < k has D == k already (D >= k, or it would have been peeled; D was set to
-      // max(B, k') with k' <= k)
-      if (k != KNONE) {
-        const unsigned long long nd = bx > k ? bx : k;
-        if (nd != d) { D[x] = nd; d = nd; }
-      }
-      mn = d < mn ? d : mn;
-    }
-  }
-  mn = wave_min_u64(mn);
-  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = mn;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < 4; w++) mn = s_min[w] < mn ? s_min[w] : mn;
-    __hip_atomic_store(vb.vmin + b, mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(vb.flag + b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// Frontier selection: every workgroup reduces the block minima to k (workgroup 0 publishes it);
-// workgroup b then lists the items of block b with D == k when its minimum is k, as k_select
-// does, and flags the block for the next round's scan.
-__global__ __launch_bounds__(256) void k_vblocks_select(VBlocks vb, const unsigned long long* __restrict__ D,
-                                                        const unsigned long long* __restrict__ B, uint8_t* __restrict__ state,
-                                                        int64_t n, const uint32_t* __restrict__ work, uint32_t small_max,
-                                                        uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
-                                                        SplitSel ss, Ctl c) {
-  __shared__ unsigned long long s_min[4];
-  __shared__ unsigned long long s_own;
-  // nb <= 256 = the workgroup size: thread i holds block i's minimum
-  const uint32_t b = blockIdx.x;
-  const unsigned long long v = threadIdx.x < vb.nb ? dev_ld(vb.vmin + threadIdx.x) : KNONE;
-  if (threadIdx.x == b) s_own = v;
-  const unsigned long long mn = wave_min_u64(v);
-  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = mn;
-  __syncthreads();
-  unsigned long long k = s_min[0];
-  for (int w = 1; w < 4; w++) k = s_min[w] < k ? s_min[w] : k;
-  if (b == 0 && threadIdx.x == 0) __hip_atomic_store(c.k, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (k == KNONE || s_own != k) return;
-  const int64_t lo = (int64_t)b << vb.bs_log;
-  const int64_t hi = lo + (int64_t(1) << vb.bs_log) < n ? lo + (int64_t(1) << vb.bs_log) : n;
-  uint32_t nsel = 0;
-  // every thread runs the same number of iterations: the appends are per wavefront
-  for (int64_t x0 = lo; x0 < hi; x0 += blockDim.x) {
-    const int64_t xx = x0 + threadIdx.x;
-    const uint32_t x = (uint32_t)xx;
-    bool sel = false;
-    uint32_t w = 0;
-    if (xx < hi) {
-      const uint8_t st = state[x];
-      const unsigned long long d = D[x];
-      sel = st == 1 && d == k;
-      if (sel) {
-        state[x] = 2;
-        nsel++;
-        // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
-        w = B[x] ? (work[x] & WORK_MASK) : 0u;
-      }
-    }
-    bool sp = false;
-    if (ss.paths)
-      sp = wave_append_cap(sel && w != 0 && (w > ss.max_work || ss.paths[x] >= ss.min_paths), x, ss.list, &c.len[6], ss.cap);
-    wave_append(sel && !sp && w != 0 && w <= small_max, x, small_list, &c.len[0]);
-    wave_append(sel && !sp && w > small_max, x, large_list, &c.len[1]);
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(vb.flag + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  nsel = pbf_wave_sum(nsel);
-  if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
-}
-
 __device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* touched, uint32_t* ntouched) {
   wave_append(atomicExch(&tflag[x], 1u) == 0u, x, touched, ntouched);
 }
 
-// ---------------- vertex peel walk ----------------
-enum TableKind { T_LDS = 0, T_HBM = 1 };
-// Vertex walk teams: wavefronts with 1024-slot tables for small items, workgroups with
-// 8192-slot tables for large ones, 32 workgroups with HBM rows for the rest.
-constexpr int V_SMALL_TEAM = 64, V_SMALL_CAP = 1024, V_LARGE_TEAM = 256, V_LARGE_CAP = 8192, V_HBM_BLOCKS = 32;
+// ---------------- vertex (tip) peel ----------------
+// A round is two launches.
+//   k_vpeel_prepare, one workgroup per block of S items (at most 256 blocks of at least 1024):
+//     retires the previous round's frontier, subtracts C(c, 2) for the counts the previous
+//     round's split walks left in their rows, sets D = max(B, k) on the live items of every
+//     block that changed (updateBuckets, butterfly_putils.h:1036-1043), and keeps per block the
+//     exact minimum of D over its live items, how many items reach it and the first NARG of them.
+//   k_vpeel_round, every workgroup: reduces the block minima to the level k, builds the
+//     frontier (live items with D == k: at most FR_CAP of them, at most vrows of the split
+//     class) from the blocks holding k, the same list in every workgroup, and walks it.
+// An item of the split class (many two-hop paths, or more possible two-hop neighbours than an
+// LDS table holds) is walked by many workgroups: its centres are cut into chunks of 256 and the
+// paths of a chunk into units of at most 256, numbered by static tables (sv, cu); the counts
+// c(u, x) go to a dense row of the item in HBM and reach B in the next prepare pass.  Small and
+// large items are counted in LDS tables by one wavefront or one workgroup, which subtract
+// C(c, 2) from B directly.
+// Items at level k that a round leaves out (the caps) keep D == k and are taken by the next
+// rounds, still at level k.  A butterfly has exactly two S vertices, so removing the level's
+// items in parts subtracts from every other item x the same C(c(u, x), 2) per removed u as
+// removing them at once, and no item with D <= k is ever decremented: the tip numbers are
+// those of PeelOrigParallel.
+// Both launches are short chains of dependent loads (a round is one level, and most levels of a
+// power-law graph peel one vertex), so the code keeps those chains short: static per-item
+// records, block scans with every load of a step issued together, no flags on the split rows.
+constexpr int V_SMALL_TEAM = 64, V_SMALL_CAP = 1024, V_LARGE_TEAM = 256, V_LARGE_CAP = 8192;
+constexpr uint32_t FR_CAP = 1024;     // frontier items per round (4 per thread of a workgroup)
+constexpr uint32_t NARG = 8;          // items at a block's minimum that the prepare pass lists
+constexpr uint32_t VS_PCH = 256;      // paths per split unit, one per thread
+constexpr uint32_t VS_MIN_PATHS = 1u << 15;
+constexpr uint32_t V_ROUND_PER_CU = 2;
+constexpr uint32_t PREP_PER = 8;      // items per thread in one step of the prepare pass
+constexpr uint32_t V_SMALL_MAX = (V_SMALL_CAP / 4) * 3 - V_SMALL_TEAM;  // the tables' fill limits
+constexpr uint32_t V_LARGE_MAX = (V_LARGE_CAP / 4) * 3 - V_LARGE_TEAM;
+enum FrontierClass { FC_NONE = 0, FC_SMALL = 1, FC_LARGE = 2, FC_SPLIT = 3 };
 
-struct PeelVArgs {
+struct VPeel {
   const uint32_t* offS;
   const uint32_t* edgS;
   const uint32_t* offO;
   const uint32_t* edgO;
+  const uint2* sv;           // per S slot (u -> v_j): (two-hop paths of u through v_0 .. v_{j-1}, offO[v_j])
+  const uint32_t* cu;        // per chunk of 256 centres of an item: the item's units before it
+  const uint4* rec;          // per S item u: offS[u], deg(u), two-hop paths T(u), class by work
+  const uint2* rec2;         // per S item u: its first chunk in cu, its units
   int64_t ns;
   unsigned long long* B;
-  const unsigned long long* D;
-  const uint32_t* bound;
-  const uint32_t* list;
-  const uint32_t* list_len;
-  uint32_t* queue;
-  uint32_t* ovf_list;
-  uint32_t* ovf_len;
-  uint32_t* tflag;
-  uint32_t* touched;
-  uint32_t* ntouched;
-  uint32_t* pool;
-  const unsigned long long* k;
-  uint32_t* bflag;          // block flags of the block minima (see VBlocks)
-  uint32_t bs_log;
+  unsigned long long* D;
+  uint8_t* state;            // 1 live, 0 retired
+  unsigned long long* vmin;  // [nb] minimum of D over the block's live items (KNONE: none)
+  uint32_t* vcnt;            // [nb] items at that minimum, at most NARG + 1
+  uint32_t* varg;            // [nb * NARG] the first of them, ascending
+  uint32_t* vflag;           // [nb] 1: the block has decremented items (or needs its first scan)
+  uint32_t nb, bs_log;
+  uint32_t* fr;              // [FR_CAP] the round's frontier (~0u: left for a later round)
+  uint32_t* ctl;             // [0] frontier entries, [1] split rows of the round, [2] error flag
+  unsigned long long* k;     // level of the last round (KNONE before the first)
+  unsigned long long* live;
+  uint32_t* rows;            // [vrows][ns] split-walk counts, zero outside a round
+  uint32_t vrows;
 };
 
-// A walk team's LDS: its shared scratch and, for LDS tables, keys, counts and used slots.
-template <int TEAM, int CAP, bool USED, int KIND>
+// Retire, credit, update, block minimum.  A step holds PREP_PER items per thread in registers,
+// all their loads issued before any is used; a block of at most 2048 items is one step, and its
+// items at the minimum are listed from the registers.
+__global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
+  __shared__ unsigned long long s_min[4];
+  __shared__ uint32_t s_wc[PREP_PER][4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+  const uint32_t b = blockIdx.x;
+  const unsigned long long k = dev_ld(a.k);
+  const uint32_t nfr = dev_ld(a.ctl + 0), nrows = dev_ld(a.ctl + 1), flag = dev_ld(a.vflag + b);
+  const int64_t lo = (int64_t)b << a.bs_log;
+  const int64_t hi = lo + (int64_t(1) << a.bs_log) < a.ns ? lo + (int64_t(1) << a.bs_log) : a.ns;
+  // the previous round's frontier items of this block retire (their D is their tip number)
+  uint32_t ret = 0;
+  for (uint32_t j = tid; j < nfr; j += 256) {
+    const uint32_t x = dev_ld(a.fr + j);
+    if ((int64_t)x >= lo && (int64_t)x < hi) { a.state[x] = 0; ret++; }
+  }
+  ret = pbf_wave_sum(ret);
+  if (lane == 0 && ret) atomicAdd(a.live, (unsigned long long)(-(long long)ret));
+  // a block changes when it held frontier items, decremented items or (any) split-row counts
+  if (!__syncthreads_or(ret != 0 || flag != 0 || nrows != 0)) return;
+  unsigned long long mn = KNONE;
+  uint8_t st[PREP_PER];
+  unsigned long long d[PREP_PER];
+  for (int64_t x0 = lo; x0 < hi; x0 += 256 * PREP_PER) {
+    unsigned long long bx[PREP_PER], sub[PREP_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < PREP_PER; i++) {
+      const int64_t xx = x0 + 256 * i + tid;
+      const bool in = xx < hi;
+      st[i] = in ? a.state[xx] : 0;
+      d[i] = in ? a.D[xx] : 0ull;
+      bx[i] = in ? a.B[xx] : 0ull;
+      sub[i] = 0;
+    }
+    for (uint32_t r = 0; r < nrows; r++) {
+      uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+      uint32_t c[PREP_PER];
+#pragma unroll
+      for (uint32_t i = 0; i < PREP_PER; i++) {
+        const int64_t xx = x0 + 256 * i + tid;
+        c[i] = xx < hi ? dev_ld(row + xx) : 0u;
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < PREP_PER; i++) {
+        if (c[i]) {
+          row[x0 + 256 * i + tid] = 0u;
+          sub[i] += (unsigned long long)c[i] * (c[i] - 1u) / 2u;
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < PREP_PER; i++) {
+      if (st[i] != 1) continue;
+      const int64_t xx = x0 + 256 * i + tid;
+      // as the LDS walks: items at or below the level (the rest of the level) keep B
+      if (sub[i] && d[i] > k) { bx[i] -= sub[i]; a.B[xx] = bx[i]; }
+      if (k != KNONE) {
+        const unsigned long long nd = bx[i] > k ? bx[i] : k;
+        if (nd != d[i]) { a.D[xx] = nd; d[i] = nd; }
+      }
+      mn = d[i] < mn ? d[i] : mn;
+    }
+  }
+  mn = wave_min_u64(mn);
+  if (lane == 0) s_min[wid] = mn;
+  __syncthreads();
+  mn = s_min[0];
+  for (int w = 1; w < 4; w++) mn = s_min[w] < mn ? s_min[w] : mn;
+  // the first NARG live items at the minimum, ascending; counting stops past NARG
+  uint32_t found = 0;
+  if (mn != KNONE && hi - lo <= 256 * (int64_t)PREP_PER) {
+    uint64_t bm[PREP_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < PREP_PER; i++) {
+      bm[i] = __ballot(st[i] == 1 && d[i] == mn);
+      if (lane == 0) s_wc[i][wid] = (uint32_t)__popcll(bm[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < PREP_PER; i++) {
+      uint32_t before = found;
+      for (uint32_t w = 0; w < 4; w++) {
+        if (w < wid) before += s_wc[i][w];
+        found += s_wc[i][w];
+      }
+      if ((bm[i] >> lane) & 1ull) {
+        const uint32_t pos = before + (uint32_t)__popcll(bm[i] & pbf_lanemask_lt());
+        if (pos < NARG) a.varg[(size_t)b * NARG + pos] = (uint32_t)(lo + 256 * i + tid);
+      }
+    }
+  } else if (mn != KNONE) {
+    for (int64_t x0 = lo; x0 < hi && found <= NARG; x0 += 256) {
+      const int64_t xx = x0 + tid;
+      bool sel = false;
+      if (xx < hi) sel = a.state[xx] == 1 && a.D[xx] == mn;
+      const uint64_t bm = __ballot(sel);
+      __syncthreads();  // s_wc of the previous step has been read
+      if (lane == 0) s_wc[0][wid] = (uint32_t)__popcll(bm);
+      __syncthreads();
+      uint32_t before = found, tot = 0;
+      for (uint32_t w = 0; w < 4; w++) {
+        if (w < wid) before += s_wc[0][w];
+        tot += s_wc[0][w];
+      }
+      if (sel) {
+        const uint32_t pos = before + (uint32_t)__popcll(bm & pbf_lanemask_lt());
+        if (pos < NARG) a.varg[(size_t)b * NARG + pos] = (uint32_t)xx;
+      }
+      found += tot;
+    }
+  }
+  if (tid == 0) {
+    __hip_atomic_store(a.vmin + b, mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a.vcnt + b, found < NARG + 1 ? found : NARG + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a.vflag + b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// A walk team's LDS: its shared scratch and the table's keys, counts and used slots.
+template <int TEAM, int CAP, bool USED>
 struct WalkLds {
   TeamShared<TEAM> sh;
-  uint32_t keys[KIND == T_LDS ? CAP : 1];
-  uint32_t cnts[KIND == T_LDS ? CAP : 1];
-  uint32_t used[(KIND == T_LDS && USED) ? CAP : 1];
+  uint32_t keys[CAP];
+  uint32_t cnts[CAP];
+  uint32_t used[USED ? CAP : 1];
 };
 
 // The barrier of a walk team: the workgroup, or for a one-wavefront team inside a larger
@@ -406,391 +416,355 @@ __device__ __forceinline__ void walk_barrier() {
   }
 }
 
-// Frontier walks of one team (team index `team` of its role, thread `tid` of TEAM): items are
-// drained from the list's queue.  A TEAM of 256 is a whole workgroup; a TEAM of 64 is one
-// wavefront and synchronises only itself.
-template <int TEAM, int CAP, bool USED, int KIND>
-__device__ void peelv_walk(const PeelVArgs& a, WalkLds<TEAM, CAP, USED, KIND>& L, uint32_t team, uint32_t tid) {
+// The walk of one frontier item u (its record r) by one team (thread tid of TEAM): every
+// two-hop path u -> v -> u2 (u2 != u) counted in an LDS hash (PeelOrigParallel,
+// butterfly_peel.h:670-702), then C(c, 2) comes off B[u2] for every u2 with D[u2] > k, and u2's
+// block is flagged for the next prepare pass.  USED tables keep a list of their used slots; the
+// others are sized from the item's bound on distinct keys, min(T(u), ns), and swept whole.  The
+// classes keep every item's distinct keys below the table's fill limit, so a table cannot fill
+// up; if one did, the round's error flag is set and the peel fails.  A step takes TEAM centres,
+// whose paths are numbered by the slot records: the centre j of the step starts at sv[j].x.
+template <int TEAM, int CAP, bool USED>
+__device__ void vwalk_item(const VPeel& a, WalkLds<TEAM, CAP, USED>& L, uint32_t u, uint4 r, unsigned long long k,
+                           uint32_t tid) {
   TeamShared<TEAM>& sh = L.sh;
-  uint32_t* lkeys = L.keys;
-  uint32_t* lcnts = L.cnts;
-  uint32_t* lused = L.used;
-  const unsigned long long k = dev_ld(a.k);
-  if (k == KNONE) return;
-  const uint32_t nlist = dev_ld(a.list_len);
-  // the first nlist teams drain the queue: the others leave without touching its counter
-  if (team >= nlist) return;
-  uint32_t* dense = nullptr;
-  uint32_t* dused = nullptr;
-  if constexpr (KIND == T_HBM) {
-    dense = a.pool + (size_t)team * (size_t)a.ns * 2;
-    dused = dense + a.ns;
+  const uint32_t uo = r.x, deg = r.y, T = r.z;
+  uint32_t cap = CAP;
+  if constexpr (!USED) {
+    const uint32_t bnd = (int64_t)T < a.ns ? T : (uint32_t)a.ns;
+    const uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
+    cap = 256;
+    while (cap < want && cap < CAP) cap <<= 1;
   }
-  if constexpr (KIND == T_LDS && USED) {
-    for (uint32_t s = tid; s < CAP; s += TEAM) { lkeys[s] = EMPTY; lcnts[s] = 0; }
-  }
-  while (true) {
-    if (tid == 0) sh.item = atomicAdd(a.queue, 1u);
+  const uint32_t lg = 31 - __clz(cap);
+  LdsTable t;
+  t.keys = L.keys; t.cnts = L.cnts; t.used = USED ? L.used : nullptr;
+  t.shift = 32 - lg; t.mask = cap - 1;
+  t.maxfill = (cap == CAP) ? (CAP / 4) * 3 - TEAM : 0xFFFFFFFFu;
+  for (uint32_t s = tid; s < cap; s += TEAM) { L.keys[s] = EMPTY; L.cnts[s] = 0; }
+  if (tid == 0) { sh.nused = 0; sh.ovf = 0; }
+  walk_barrier<TEAM>();
+  for (uint32_t c0 = 0; c0 < deg; c0 += TEAM) {
+    // the step's paths [cs, ce) and, per centre, its first path and list start
+    const uint32_t cs = a.sv[uo + c0].x;
+    const uint32_t ce = c0 + TEAM < deg ? a.sv[uo + c0 + TEAM].x : T;
+    const uint32_t j = c0 + tid;
+    uint2 e = make_uint2(0xFFFFFFFFu, 0u);
+    if (j < deg) e = a.sv[uo + j];
+    // path p of centre j is N(v_j)[p - e.x]; with w = p - cs that is (e.y - (e.x - cs)) + w
+    sh.P[tid] = e.x == 0xFFFFFFFFu ? e.x : e.x - cs;
+    sh.vs[tid] = e.y - (e.x - cs);
     walk_barrier<TEAM>();
-    const uint32_t idx = sh.item;
-    walk_barrier<TEAM>();
-    if (idx >= nlist) break;
-    const uint32_t u = dev_ld(a.list + idx);
-    const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
-    LdsTable t;
-    if constexpr (KIND == T_LDS) {
-      const uint32_t bnd = dev_ld(a.bound + u);
-      uint32_t cap = CAP;
-      if constexpr (!USED) {
-        uint64_t want = (uint64_t)bnd + bnd / 3 + 1;
-        cap = 256;
-        while (cap < want && cap < CAP) cap <<= 1;
+    for (uint32_t w = tid; w < ce - cs; w += TEAM) {
+      // the last centre of the step starting at or before w
+      uint32_t lo = 0, hi = TEAM - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (sh.P[mid] <= w) lo = mid; else hi = mid - 1;
       }
-      const uint32_t lg = 31 - __clz(cap);
-      t.keys = lkeys; t.cnts = lcnts; t.used = USED ? lused : nullptr;
-      t.shift = 32 - lg; t.mask = cap - 1;
-      t.maxfill = (cap == CAP) ? (CAP / 4) * 3 - TEAM : 0xFFFFFFFFu;
-      if constexpr (!USED) {
-        for (uint32_t s = tid; s < cap; s += TEAM) { lkeys[s] = EMPTY; lcnts[s] = 0; }
-      }
-    }
-    if (tid == 0) { sh.nused = 0; sh.ovf = 0; }
-    walk_barrier<TEAM>();
-    for (uint32_t c0 = 0; c0 < deg; c0 += TEAM) {
-      const uint32_t j = c0 + tid;
-      uint32_t L = 0, vo = 0;
-      if (j < deg) {
-        const uint32_t v = a.edgS[uo + j];
-        vo = a.offO[v];
-        L = a.offO[v + 1] - vo;
-      }
-      uint32_t T;
-      const uint32_t incl = team_incl_scan<TEAM>(L, sh, &T);
-      sh.P[tid] = incl;
-      sh.vs[tid] = vo;
-      walk_barrier<TEAM>();
-      for (uint32_t w = tid; w < T; w += TEAM) {
-        const uint32_t s = seg_of<TEAM>(sh.P, w);
-        const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
-        const uint32_t u2 = a.edgO[sh.vs[s] + kk];
-        if (u2 == u) continue;
-        if constexpr (KIND == T_LDS) {
-          if (!sh.ovf) lds_insert<TEAM>(t, sh, u2);
-        } else {
-          if (atomicAdd(&dense[u2], 1u) == 0u) dused[atomicAdd(&sh.nused, 1u)] = u2;
-        }
-      }
-      walk_barrier<TEAM>();
-    }
-    const bool overflowed = sh.ovf != 0;
-    if (!overflowed) {
-      const uint32_t extent = (KIND == T_HBM || USED) ? sh.nused : (t.mask + 1);
-      for (uint32_t q = tid; q < extent; q += TEAM) {
-        uint32_t key, c;
-        if constexpr (KIND == T_LDS) {
-          const uint32_t slot = USED ? lused[q] : q;
-          key = lkeys[slot];
-          c = (key == EMPTY) ? 0u : lcnts[slot];
-        } else {
-          key = dused[q];
-          c = dense[key];
-        }
-        if (c > 1u && a.D[key] > k) {
-          atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
-          a.bflag[key >> a.bs_log] = 1u;  // the next prepare pass updates D of the block
-        }
-      }
-    } else if (tid == 0) {
-      a.ovf_list[atomicAdd(a.ovf_len, 1u)] = u;
-    }
-    walk_barrier<TEAM>();
-    if constexpr (KIND == T_LDS && USED) {
-      const uint32_t nu_ = sh.nused < CAP ? sh.nused : CAP;
-      for (uint32_t q = tid; q < nu_; q += TEAM) { const uint32_t slot = lused[q]; lkeys[slot] = EMPTY; lcnts[slot] = 0; }
-    }
-    if constexpr (KIND == T_HBM) {
-      for (uint32_t q = tid; q < sh.nused; q += TEAM) dense[dused[q]] = 0;
+      const uint32_t u2 = a.edgO[sh.vs[lo] + w];
+      if (u2 != u && !sh.ovf) lds_insert<TEAM>(t, sh, u2);
     }
     walk_barrier<TEAM>();
   }
+  if (sh.ovf) {
+    if (tid == 0) a.ctl[2] = 1u;
+    walk_barrier<TEAM>();
+    return;
+  }
+  const uint32_t extent = USED ? sh.nused : cap;
+  for (uint32_t q = tid; q < extent; q += TEAM) {
+    const uint32_t slot = USED ? L.used[q] : q;
+    const uint32_t key = L.keys[slot];
+    const uint32_t c = key == EMPTY ? 0u : L.cnts[slot];
+    if (c > 1u && a.D[key] > k) {
+      atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
+      a.vflag[key >> a.bs_log] = 1u;  // the next prepare pass updates D over the block
+    }
+  }
+  walk_barrier<TEAM>();
 }
 
-// bound[u] = min(sum_{v in N(u)} deg(v), ns): distinct two-hop neighbours of u; paths[u] = the
-// sum itself (two-hop paths, saturating at 2^31 - 1).
-__global__ __launch_bounds__(256) void k_peelv_bound(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ edgS,
-                                                     const uint32_t* __restrict__ offO, int64_t ns, uint32_t* __restrict__ bound,
-                                                     uint32_t* __restrict__ paths) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int64_t u = (int64_t)blockIdx.x * 4 + wid; u < ns; u += (int64_t)gridDim.x * 4) {
-    uint64_t s = 0;
-    for (uint32_t p = offS[u] + lane; p < offS[u + 1]; p += 64) {
-      const uint32_t v = edgS[p];
-      s += offO[v + 1] - offO[v];
-    }
-    s = pbf_wave_sum(s);
-    if (lane == 0) {
-      bound[u] = (uint32_t)(s < (uint64_t)ns ? s : (uint64_t)ns);
-      paths[u] = (uint32_t)(s < (uint64_t)WORK_MASK ? s : (uint64_t)WORK_MASK);
-    }
-  }
-}
-
-// ---------------- vertex peel: walks split over many workgroups ----------------
-// A frontier vertex u with many two-hop paths (the late rounds of a power-law graph peel one
-// such vertex at a time) is walked by the whole device: its centres v in N(u) are cut into
-// chunks of VS_CHUNK, a chunk's paths into units of VS_PCH, and workgroups take units; the
-// counts c(u, u2) go to a dense row of u in HBM (global atomics whose results are not waited
-// for), and every counted key sets the flag of its block of 64 keys; k_vsplit_credit then
-// sweeps the flagged blocks, subtracts C(c, 2) from every live u2 and clears rows and flags.
-constexpr uint32_t VS_CHUNK = 256, VS_PCH = 256, VS_MIN_PATHS = 1u << 15;
-
-struct VSplitArgs {
-  const uint32_t* offS;
-  const uint32_t* edgS;
-  const uint32_t* offO;
-  const uint32_t* edgO;
-  int64_t ns;
-  const uint32_t* list;     // split items, row r = list[r]
-  const uint32_t* len;      // reservations of the list (rows = min(*len, cap))
-  uint32_t cap;
-  uint32_t* rows;           // per row: dense counters [ns], zero between rounds
-  uint32_t* flags;          // per row: one word per block of 64 keys [nblk], zero between rounds
-  uint32_t nblk;
-  unsigned long long* sub;  // units after each chunk's first: (row << 59) | (global chunk << 27) | unit,
-                            // the units of one chunk consecutive
-  uint32_t sub_cap;
-  uint2* pre;               // per global chunk g < pre_cap: the chunk's (prefix, O-list start) pairs
-  uint32_t pre_cap;
-  uint32_t* ctr;            // [1] units listed
-  unsigned long long* B;
-  const unsigned long long* D;
-  uint32_t* bflag;          // block flags of the block minima (see VBlocks)
-  uint32_t bs_log;
-  const unsigned long long* k;
-};
-
-__device__ __forceinline__ uint32_t vs_nrows(const VSplitArgs& a) {
-  const uint32_t n = dev_ld(a.len);
-  return n < a.cap ? n : a.cap;
-}
-
-// Row r's chunk starts: every thread computes the prefix of chunk counts over the rows (at
-// most cap rows, read from L2).  Returns the row holding global chunk g and the chunk in it.
-__device__ __forceinline__ void vs_locate(const VSplitArgs& a, uint32_t nrows, uint32_t g, uint32_t* row, uint32_t* chunk) {
-  uint32_t before = 0;
-  for (uint32_t r = 0; r < nrows; r++) {
-    const uint32_t u = dev_ld(a.list + r);
-    const uint32_t nch = (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
-    if (g < before + nch) { *row = r; *chunk = g - before; return; }
-    before += nch;
-  }
-  *row = nrows; *chunk = 0;
-}
-
-// Chunk c of u: the inclusive prefix of its centres' path counts into sh.P and their O-list
-// starts into sh.vs (one centre per thread).  Returns the chunk's path total, which is below
-// 2^32: the centres are distinct, so their paths are distinct edges.
-__device__ uint32_t vs_chunk(const VSplitArgs& a, TeamShared<256>& sh, uint32_t u, uint32_t c, uint2* keep) {
+// Unit q of row r's item u (record r4, chunks from r2.x): its chunk c is the last one with
+// cu[c] <= q (one 256-wide probe of cu up to 256 chunks, a 256-ary search beyond), and it
+// covers the chunk's paths [256 (q - cu[c]), + 256), one per thread, counted into the row with
+// counter atomics that return nothing.  The chunk's centres are laid out in LDS (one per thread)
+// and every thread finds its path's centre there.
+__device__ void vsplit_unit(const VPeel& a, TeamShared<256>& sh, uint32_t r, uint32_t u, uint4 r4, uint2 r2, uint32_t q) {
   const uint32_t tid = threadIdx.x;
-  const uint32_t uo = a.offS[u], deg = a.offS[u + 1] - uo;
-  const uint32_t j = c * VS_CHUNK + tid;
-  uint32_t L = 0, vo = 0;
-  if (j < deg) {
-    const uint32_t v = a.edgS[uo + j];
-    vo = a.offO[v];
-    L = a.offO[v + 1] - vo;
+  const uint32_t uo = r4.x, deg = r4.y, T = r4.z;
+  const uint32_t nch = (deg + 255u) / 256u;
+  // the chunk: in [lo, lo + len); cu[lo] <= q always (cu[0] = 0)
+  uint32_t lo = 0, len = nch;
+  if (tid == 0) sh.item = 0;  // cu[lo]
+  while (len > 1) {
+    const uint32_t step = (len + 255u) / 256u;
+    const uint32_t j = lo + tid * step;
+    uint32_t cv = 0;
+    const bool le = j < lo + len && (cv = a.cu[r2.x + j]) <= q;
+    const uint32_t c = (uint32_t)__syncthreads_count(le);  // probes 0 .. c-1 lie at or before q
+    if (le && tid == c - 1u) sh.item = cv;
+    const uint32_t nlo = lo + (c - 1u) * step;
+    len = (c * step < len ? c * step : len) - (c - 1u) * step;
+    lo = nlo;
   }
-  uint32_t T;
-  const uint32_t incl = team_incl_scan<256>(L, sh, &T);
-  sh.P[tid] = incl;
-  sh.vs[tid] = vo;
-  if (keep) keep[tid] = make_uint2(incl, vo);
   __syncthreads();
-  return T;
-}
-
-// The stored prefix of global chunk g (written by phase 1, vsplit_chunks) into sh; returns its total.
-__device__ uint32_t vs_chunk_stored(const VSplitArgs& a, TeamShared<256>& sh, uint32_t g) {
-  const uint32_t tid = threadIdx.x;
-  const uint32_t* q = reinterpret_cast<const uint32_t*>(a.pre + (size_t)g * VS_CHUNK + tid);
-  const uint32_t incl = dev_ld(q), vo = dev_ld(q + 1);
-  __syncthreads();  // every thread is done with the previous chunk
-  sh.P[tid] = incl;
-  sh.vs[tid] = vo;
+  const uint32_t first = sh.item;
+  const uint32_t c0 = lo * 256u;
+  const uint32_t cs = a.sv[uo + c0].x;
+  const uint32_t ce = c0 + 256u < deg ? a.sv[uo + c0 + 256u].x : T;
+  const uint32_t j = c0 + tid;
+  uint2 e = make_uint2(0xFFFFFFFFu, 0u);
+  if (j < deg) e = a.sv[uo + j];
+  sh.P[tid] = e.x == 0xFFFFFFFFu ? e.x : e.x - cs;
+  sh.vs[tid] = e.y - (e.x - cs);
   __syncthreads();
-  return sh.P[VS_CHUNK - 1];
-}
-
-// Paths [s * VS_PCH, min((s + 1) * VS_PCH, T)) of the chunk in sh, counted into row r of u:
-// VS_PCH / 256 paths per thread, all key loads issued before the counter updates, which
-// return nothing (a key's block flag is set by a plain store of 1 instead).
-__device__ __forceinline__ void vs_count(const VSplitArgs& a, const TeamShared<256>& sh, uint32_t r, uint32_t u,
-                                         uint32_t s, uint32_t T) {
-  const uint32_t tid = threadIdx.x;
-  uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
-  uint32_t* flag = a.flags + (size_t)r * a.nblk;
-  const uint64_t w0 = (uint64_t)s * VS_PCH;
-  const uint32_t wend = (uint32_t)(w0 + VS_PCH < T ? w0 + VS_PCH : T);
-  constexpr uint32_t PER = VS_PCH / 256;
-  uint32_t key[PER];
-#pragma unroll
-  for (uint32_t q = 0; q < PER; q++) {
-    const uint32_t w = (uint32_t)w0 + q * 256 + tid;
-    key[q] = u;
-    if (w < wend) {
-      const uint32_t sg = seg_of<256>(sh.P, w);
-      const uint32_t kk = w - (sg ? sh.P[sg - 1] : 0u);
-      key[q] = a.edgO[sh.vs[sg] + kk];
+  const uint32_t w = (q - first) * 256u + tid;  // path of the chunk, from its first
+  if (w < ce - cs) {
+    uint32_t l = 0, h = 255;
+    while (l < h) {
+      const uint32_t mid = (l + h + 1) >> 1;
+      if (sh.P[mid] <= w) l = mid; else h = mid - 1;
     }
-  }
-#pragma unroll
-  for (uint32_t q = 0; q < PER; q++) {
-    if (key[q] != u) {
-      atomicAdd(&row[key[q]], 1u);
-      flag[key[q] >> 6] = 1u;
-    }
+    const uint32_t u2 = a.edgO[sh.vs[l] + w];
+    if (u2 != u) atomicAdd(a.rows + (size_t)r * (size_t)a.ns + u2, 1u);
   }
 }
 
-// Phase 1: workgroups take whole chunks (chunk g, g + nb, ... for workgroup bid of nb) and
-// count their first unit; a chunk with more than VS_PCH paths reserves consecutive room for its
-// other units and lists them for phase 2.  The list has room for every unit of a round: a row
-// has at most m / VS_PCH + (its chunks) units, as its paths u -> v -> u2 are distinct edges.
-__device__ void vsplit_chunks(const VSplitArgs& a, TeamShared<256>& sh, uint32_t bid, uint32_t nb) {
-  const unsigned long long k = dev_ld(a.k);
-  const uint32_t nrows = vs_nrows(a);
-  if (k == KNONE || nrows == 0) return;
-  uint32_t total = 0, u0 = 0, nch0 = 0;
-  for (uint32_t r = 0; r < nrows; r++) {
-    const uint32_t u = dev_ld(a.list + r);
-    const uint32_t nch = (a.offS[u + 1] - a.offS[u] + VS_CHUNK - 1) / VS_CHUNK;
-    if (r == 0) { u0 = u; nch0 = nch; }
-    total += nch;
-  }
-  for (uint32_t g = bid; g < total; g += nb) {
-    // the chunks of row 0 (most rounds have one row) need no search
-    uint32_t r = 0, c = g, u = u0;
-    if (g >= nch0) {
-      vs_locate(a, nrows, g, &r, &c);
-      u = dev_ld(a.list + r);
-    }
-    const uint32_t T = vs_chunk(a, sh, u, c, g < a.pre_cap ? a.pre + (size_t)g * VS_CHUNK : nullptr);
-    const uint32_t nunits = (uint32_t)(((uint64_t)T + VS_PCH - 1) / VS_PCH);
-    if (nunits > 1 && threadIdx.x == 0) sh.item = atomicAdd(&a.ctr[1], nunits - 1);
-    vs_count(a, sh, r, u, 0, T);
-    __syncthreads();
-    const uint32_t base = sh.item;
-    for (uint32_t s = 1 + threadIdx.x; s < nunits; s += 256) {
-      const uint32_t pos = base + s - 1;
-      if (pos < a.sub_cap) a.sub[pos] = ((unsigned long long)r << 59) | ((unsigned long long)g << 27) | s;
-    }
-  }
-}
-
-// Phase 2: workgroup b takes the listed units [b * per, (b + 1) * per); consecutive units of
-// one chunk reuse the chunk's prefix already in LDS, which is read from the copy phase 1 kept
-// (the first pre_cap chunks of a round) or else scanned again; the next list entry is loaded
-// while the current unit runs.
-__device__ void vsplit_units(const VSplitArgs& a, TeamShared<256>& sh, uint32_t bid, uint32_t nb) {
-  const unsigned long long k = dev_ld(a.k);
-  const uint32_t listed = dev_ld(a.ctr + 1);
-  if (k == KNONE) return;
-  const uint32_t n = listed < a.sub_cap ? listed : a.sub_cap;
-  const uint32_t per = (uint32_t)(((uint64_t)n + nb - 1) / nb);
-  const uint64_t q0 = (uint64_t)bid * per;
-  const uint64_t q1 = q0 + per < n ? q0 + per : n;
-  if (q0 >= q1) return;
-  uint32_t cr = ~0u, cg = ~0u, u = 0, T = 0;
-  unsigned long long e = dev_ld(a.sub + q0);
-  for (uint64_t q = q0; q < q1; q++) {
-    const unsigned long long en = q + 1 < q1 ? dev_ld(a.sub + q + 1) : 0ull;
-    const uint32_t r = (uint32_t)(e >> 59), g = (uint32_t)(e >> 27) & 0xFFFFFFFFu;
-    if (r != cr || g != cg) {
-      u = dev_ld(a.list + r);
-      if (g < a.pre_cap) {
-        T = vs_chunk_stored(a, sh, g);
-      } else {
-        uint32_t r2, c;
-        vs_locate(a, vs_nrows(a), g, &r2, &c);
-        T = vs_chunk(a, sh, u, c, nullptr);
-      }
-      cr = r;
-      cg = g;
-    }
-    vs_count(a, sh, r, u, (uint32_t)e & 0x7FFFFFFu, T);
-    e = en;
-  }
-}
-
-// A round's walks run in two launches.  Launch 1: phase 1 of the split walks in workgroups
-// [0, chunk_blocks), wavefront teams (4 per workgroup) on the small list in the next
-// small_blocks, workgroup teams on the large list in the rest.
-__global__ __launch_bounds__(256) void k_peelv_walks(PeelVArgs small, PeelVArgs large, VSplitArgs vs, uint32_t chunk_blocks,
-                                                     uint32_t small_blocks) {
-  __shared__ union {
-    TeamShared<256> sh;
-    WalkLds<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS> s[4];
-    WalkLds<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS> l;
-  } lds;
-  if (blockIdx.x < chunk_blocks) {
-    vsplit_chunks(vs, lds.sh, blockIdx.x, chunk_blocks);
-  } else if (blockIdx.x < chunk_blocks + small_blocks) {
-    const uint32_t w = threadIdx.x >> 6;
-    peelv_walk<V_SMALL_TEAM, V_SMALL_CAP, true, T_LDS>(small, lds.s[w], (blockIdx.x - chunk_blocks) * 4 + w,
-                                                        threadIdx.x & 63);
-  } else {
-    peelv_walk<V_LARGE_TEAM, V_LARGE_CAP, false, T_LDS>(large, lds.l, blockIdx.x - chunk_blocks - small_blocks,
-                                                         threadIdx.x);
-  }
-}
-
-// Launch 2: the HBM-row walks of the items no LDS table held (their list is complete, launch 1
-// having ended) in workgroups [0, V_HBM_BLOCKS), phase 2 of the split walks in the others.
-__global__ __launch_bounds__(256) void k_vsplit_units(VSplitArgs a, PeelVArgs hbm) {
-  __shared__ union {
-    TeamShared<256> sh;
-    WalkLds<V_LARGE_TEAM, 1, true, T_HBM> h;
-  } lds;
-  if (blockIdx.x < (uint32_t)V_HBM_BLOCKS)
-    peelv_walk<V_LARGE_TEAM, 1, true, T_HBM>(hbm, lds.h, blockIdx.x, threadIdx.x);
-  else
-    vsplit_units(a, lds.sh, blockIdx.x - V_HBM_BLOCKS, gridDim.x - V_HBM_BLOCKS);
-}
-
-// One wavefront per block of 64 keys, over all rows at once: lane r < cap reads row r's flag
-// of the block (rows past the round's count are never flagged), every lane its key's D, both
-// before either is used; then C(c_r, 2) summed over the flagged rows comes off B of every live
-// key, the counters and flags are cleared, and a block with a decremented key is flagged for
-// the next round's prepare pass (which sets D = max(B, k) over the block).
-constexpr uint32_t VS_CREDIT_WAVES = 16;
-__global__ __launch_bounds__(64 * VS_CREDIT_WAVES) void k_vsplit_credit(VSplitArgs a) {
+// Exclusive workgroup scan of x (256 threads); *total gets the sum.
+__device__ __forceinline__ uint64_t wg_excl_scan(uint64_t x, uint64_t* s_w, uint64_t* total) {
   const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  const unsigned long long k = dev_ld(a.k);
-  for (uint64_t b0 = (uint64_t)blockIdx.x * VS_CREDIT_WAVES; b0 < a.nblk; b0 += (uint64_t)gridDim.x * VS_CREDIT_WAVES) {
-    const uint64_t b = b0 + wid;
-    if (b >= a.nblk) continue;
-    const uint32_t key = (uint32_t)b * 64u + lane;
-    uint32_t* fl = a.flags + (size_t)lane * a.nblk + b;
-    const uint32_t f = lane < a.cap ? dev_ld(fl) : 0u;
-    const bool in = key < (uint64_t)a.ns;
-    const unsigned long long d = in ? a.D[key] : 0ull;
-    uint64_t fm = __ballot(f != 0u);
-    if (fm == 0ull) continue;
-    if (f != 0u) __hip_atomic_store(fl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long sub = 0;
-    while (fm) {
-      const uint32_t r = (uint32_t)__ffsll((long long)fm) - 1u;
-      fm &= fm - 1ull;
-      uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
-      const uint32_t c = in ? dev_ld(row + key) : 0u;
-      if (c != 0u) __hip_atomic_store(row + key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      sub += (unsigned long long)c * (c - (c != 0u)) / 2u;
-    }
-    const bool hit = k != KNONE && sub != 0ull && d > k;
-    if (hit) atomicAdd(&a.B[key], (unsigned long long)(-(long long)sub));
-    // the 64 keys lie in one block of the block minima: one flag for the wavefront
-    if (__ballot(hit) != 0ull && lane == 0) a.bflag[key >> a.bs_log] = 1u;
+  const uint64_t incl = pbf_wave_incl_scan(x);
+  __syncthreads();  // s_w of an earlier scan has been read
+  if (lane == 63) s_w[wid] = incl;
+  __syncthreads();
+  uint64_t before = 0, tot = 0;
+  for (uint32_t w = 0; w < 4; w++) {
+    if (w < wid) before += s_w[w];
+    tot += s_w[w];
   }
+  *total = tot;
+  return before + incl - x;
+}
+
+// The round: level, frontier, walks.  Tasks of the round, in order: one per large item (a
+// workgroup team), one per split unit, one per 4 small items (a wavefront team each); workgroup
+// b runs tasks b, b + grid, ...
+__global__ __launch_bounds__(256) void k_vpeel_round(VPeel a) {
+  __shared__ union {
+    TeamShared<256> sh;
+    WalkLds<V_LARGE_TEAM, V_LARGE_CAP, false> l;
+    WalkLds<V_SMALL_TEAM, V_SMALL_CAP, true> s[4];
+  } lds;
+  __shared__ uint32_t f_id[FR_CAP];
+  __shared__ uint32_t f_list[FR_CAP];   // small | large | split items
+  __shared__ uint4 s_row[VS_MAX_ROWS];  // split rows: the item's record
+  __shared__ uint2 s_row2[VS_MAX_ROWS];
+  __shared__ uint32_t s_rowu[VS_MAX_ROWS];  // the item
+  __shared__ uint32_t s_upfx[VS_MAX_ROWS + 1];  // split units before row r
+  __shared__ unsigned long long s_min[4];
+  __shared__ uint64_t s_w[4];
+  __shared__ uint64_t s_kb[4];
+  __shared__ uint32_t s_wc[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+  const bool lead = blockIdx.x == 0 && tid == 0;
+  // the level: the least block minimum (nb <= 256 = the workgroup size)
+  // with every block's first item at its minimum, loaded before the level is known
+  const unsigned long long v = tid < a.nb ? dev_ld(a.vmin + tid) : KNONE;
+  const uint32_t vc = tid < a.nb ? dev_ld(a.vcnt + tid) : 0u;
+  const uint32_t v0 = tid < a.nb ? dev_ld(a.varg + (size_t)tid * NARG) : 0u;
+  unsigned long long k = wave_min_u64(v);
+  if (lane == 0) s_min[wid] = k;
+  __syncthreads();
+  k = s_min[0];
+  for (int w = 1; w < 4; w++) k = s_min[w] < k ? s_min[w] : k;
+  if (lead) __hip_atomic_store(a.k, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (k == KNONE) {
+    if (lead) { a.ctl[0] = 0u; a.ctl[1] = 0u; }
+    return;
+  }
+  // candidates: the live items with D == k of the blocks holding k, in block order
+  const bool isk = v == k;
+  const uint32_t cnt = isk ? vc : 0u;
+  uint32_t n;
+  if (!__syncthreads_or(cnt > NARG)) {
+    uint64_t tot;
+    const uint32_t pos = (uint32_t)wg_excl_scan(cnt, s_w, &tot);
+    for (uint32_t j = 0; j < cnt && pos + j < FR_CAP; j++) f_id[pos + j] = j ? dev_ld(a.varg + (size_t)tid * NARG + j) : v0;
+    n = tot < FR_CAP ? (uint32_t)tot : FR_CAP;
+  } else {
+    const uint64_t km = __ballot(isk);
+    if (lane == 0) s_kb[wid] = km;
+    __syncthreads();
+    n = 0;
+    for (uint32_t w4 = 0; w4 < 4 && n < FR_CAP; w4++) {
+      uint64_t m = s_kb[w4];
+      while (m && n < FR_CAP) {
+        const uint32_t bb = w4 * 64u + (uint32_t)__ffsll((long long)m) - 1u;
+        m &= m - 1ull;
+        const uint32_t c = dev_ld(a.vcnt + bb);
+        if (c <= NARG) {
+          if (tid < c && n + tid < FR_CAP) f_id[n + tid] = dev_ld(a.varg + (size_t)bb * NARG + tid);
+          n += c;
+          continue;
+        }
+        const int64_t lo = (int64_t)bb << a.bs_log;
+        const int64_t hi = lo + (int64_t(1) << a.bs_log) < a.ns ? lo + (int64_t(1) << a.bs_log) : a.ns;
+        for (int64_t x0 = lo; x0 < hi && n < FR_CAP; x0 += 256) {
+          const int64_t xx = x0 + tid;
+          bool sel = false;
+          if (xx < hi) sel = a.state[xx] == 1 && a.D[xx] == k;
+          const uint64_t bm = __ballot(sel);
+          __syncthreads();
+          if (lane == 0) s_wc[wid] = (uint32_t)__popcll(bm);
+          __syncthreads();
+          uint32_t before = n, tot = 0;
+          for (uint32_t w = 0; w < 4; w++) {
+            if (w < wid) before += s_wc[w];
+            tot += s_wc[w];
+          }
+          const uint32_t pos = before + (uint32_t)__popcll(bm & pbf_lanemask_lt());
+          if (sel && pos < FR_CAP) f_id[pos] = (uint32_t)xx;
+          n += tot;
+        }
+      }
+    }
+    n = n < FR_CAP ? n : FR_CAP;
+  }
+  __syncthreads();
+  // classes; split items beyond the round's rows stay for a later round (fr entry ~0u)
+  uint32_t cls[4], xs[4];
+  uint4 rc[4];
+  uint2 rc2[4];
+  uint64_t pos[4];
+  uint64_t run = 0;  // small | large << 16 | split << 32, before this step
+  for (uint32_t q = 0; q < 4; q++) {
+    const uint32_t j = q * 256u + tid;
+    cls[q] = FC_NONE;
+    xs[q] = 0;
+    rc[q] = make_uint4(0u, 0u, 0u, 0u);
+    rc2[q] = make_uint2(0u, 0u);
+    if (j < n) {
+      const uint32_t x = f_id[j];
+      xs[q] = x;
+      rc[q] = a.rec[x];
+      rc2[q] = a.rec2[x];
+      // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
+      cls[q] = a.B[x] ? rc[q].w : (uint32_t)FC_NONE;
+    }
+    const uint64_t one = cls[q] == FC_SMALL ? 1ull : cls[q] == FC_LARGE ? (1ull << 16) : cls[q] == FC_SPLIT ? (1ull << 32) : 0ull;
+    uint64_t tot = 0;
+    if (q * 256u < n) pos[q] = run + wg_excl_scan(one, s_w, &tot);
+    else pos[q] = run;
+    run += tot;
+  }
+  const uint32_t nsmall = (uint32_t)(run & 0xFFFFu), nlarge = (uint32_t)((run >> 16) & 0xFFFFu);
+  const uint32_t nsplit_all = (uint32_t)((run >> 32) & 0xFFFFu);
+  const uint32_t nsplit = nsplit_all < a.vrows ? nsplit_all : a.vrows;
+  for (uint32_t q = 0; q < 4; q++) {
+    const uint32_t j = q * 256u + tid;
+    if (j >= n) break;
+    bool keep = true;
+    if (cls[q] == FC_SMALL) f_list[(uint32_t)(pos[q] & 0xFFFFu)] = xs[q];
+    else if (cls[q] == FC_LARGE) f_list[nsmall + (uint32_t)((pos[q] >> 16) & 0xFFFFu)] = xs[q];
+    else if (cls[q] == FC_SPLIT) {
+      const uint32_t r = (uint32_t)((pos[q] >> 32) & 0xFFFFu);
+      if (r < nsplit) { s_row[r] = rc[q]; s_row2[r] = rc2[q]; s_rowu[r] = xs[q]; }
+      else keep = false;
+    }
+    if (blockIdx.x == 0) a.fr[j] = keep ? xs[q] : ~0u;
+  }
+  if (lead) { a.ctl[0] = n; a.ctl[1] = nsplit; }
+  __syncthreads();
+  // split rows and their units
+  if (tid < 64) {
+    const uint32_t un = tid < nsplit ? s_row2[tid].y : 0u;
+    const uint32_t incl = pbf_wave_incl_scan(un);
+    if (tid < nsplit) s_upfx[tid + 1] = incl;
+    if (tid == 0) s_upfx[0] = 0;
+  }
+  __syncthreads();
+  const uint32_t nunits = s_upfx[nsplit];
+  const uint32_t ntasks = nlarge + nunits + (nsmall + 3u) / 4u;
+  for (uint32_t task = blockIdx.x; task < ntasks; task += gridDim.x) {
+    __syncthreads();  // LDS of the previous task
+    if (task < nlarge) {
+      const uint32_t u = f_list[nsmall + task];
+      vwalk_item<V_LARGE_TEAM, V_LARGE_CAP, false>(a, lds.l, u, a.rec[u], k, tid);
+    } else if (task < nlarge + nunits) {
+      const uint32_t q = task - nlarge;
+      uint32_t r = 0;
+      while (s_upfx[r + 1] <= q) r++;
+      vsplit_unit(a, lds.sh, r, s_rowu[r], s_row[r], s_row2[r], q - s_upfx[r]);
+    } else {
+      const uint32_t i = (task - nlarge - nunits) * 4u + wid;
+      if (i < nsmall) {
+        const uint32_t u = f_list[i];
+        vwalk_item<V_SMALL_TEAM, V_SMALL_CAP, true>(a, lds.s[wid], u, a.rec[u], k, lane);
+      }
+    }
+  }
+}
+
+// The static tables of the tip peel, one wavefront per S vertex u, a chunk of 256 slots per
+// step (4 consecutive slots per lane): sv[e] = (paths of u through its centres before the slot's,
+// offO[v]) for every slot e = (u -> v); cu[cbase[u] + c] = the units of u's chunks before chunk c
+// (a chunk's paths in units of 256); rec[u] = (offS[u], deg(u), T(u) = all two-hop paths of u,
+// class) and rec2[u] = (cbase[u], units of u).  T(u) is below m (the paths of u are distinct
+// edges), so 32 bits hold every prefix.  The class: split when T(u) reaches VS_MIN_PATHS or
+// the bound on distinct two-hop neighbours, min(T(u), ns), passes what the large LDS table
+// holds, else small or large by that bound, none without paths.
+__global__ __launch_bounds__(256) void k_peelv_tables(const uint32_t* __restrict__ offS, const uint32_t* __restrict__ edgS,
+                                                      const uint32_t* __restrict__ offO, int64_t ns,
+                                                      const uint32_t* __restrict__ cbase, uint2* __restrict__ sv,
+                                                      uint32_t* __restrict__ cu, uint4* __restrict__ rec, uint2* __restrict__ rec2) {
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  for (int64_t u = (int64_t)blockIdx.x * 4 + wid; u < ns; u += (int64_t)gridDim.x * 4) {
+    const uint32_t e0 = offS[u], e1 = offS[u + 1], cb = cbase[u];
+    uint32_t carry = 0, units = 0, c = 0;
+    for (uint32_t p0 = e0; p0 < e1; p0 += 256, c++) {
+      uint32_t d[4], vo[4];
+#pragma unroll
+      for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t p = p0 + 4 * lane + i;
+        d[i] = 0;
+        vo[i] = 0;
+        if (p < e1) {
+          const uint32_t v = edgS[p];
+          vo[i] = offO[v];
+          d[i] = offO[v + 1] - vo[i];
+        }
+      }
+      const uint32_t s = d[0] + d[1] + d[2] + d[3];
+      const uint32_t incl = pbf_wave_incl_scan(s);
+      uint32_t x = carry + incl - s;
+#pragma unroll
+      for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t p = p0 + 4 * lane + i;
+        if (p < e1) sv[p] = make_uint2(x, vo[i]);
+        x += d[i];
+      }
+      const uint32_t chunk = __shfl(incl, 63, 64);
+      if (lane == 0) cu[cb + c] = units;
+      units += (chunk + VS_PCH - 1u) / VS_PCH;
+      carry += chunk;
+    }
+    if (lane == 0) {
+      const uint32_t T = carry;
+      const uint32_t bnd = (int64_t)T < ns ? T : (uint32_t)ns;
+      const uint32_t cls = T == 0 ? FC_NONE
+                         : (bnd > V_LARGE_MAX || T >= VS_MIN_PATHS) ? FC_SPLIT
+                         : bnd <= V_SMALL_MAX ? FC_SMALL : FC_LARGE;
+      rec[u] = make_uint4(e0, e1 - e0, T, cls);
+      rec2[u] = make_uint2(cb, units);
+    }
+  }
+}
+
+// Chunks of 256 centres per S vertex.
+__global__ void k_peelv_nchunks(const uint32_t* __restrict__ offS, int64_t ns, uint32_t* __restrict__ nch) {
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < ns; u += (int64_t)gridDim.x * blockDim.x)
+    nch[u] = (offS[u + 1] - offS[u] + 255u) / 256u;
 }
 
 // ---------------- edge peel walk ----------------
@@ -1100,7 +1074,7 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   if (ns == 0) return PBF_OK;
   PeelBufs pb;
   PBF_CHECK(peel_bufs(ctx, ns, &pb));
-  PeelVArgs a{};
+  VPeel a{};
   a.offS = use_v ? g.offU : g.offV;
   a.edgS = use_v ? g.edgU : g.edgV;
   a.offO = use_v ? g.offV : g.offU;
@@ -1108,96 +1082,73 @@ int pbf_dev_peel_vert(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   a.ns = ns;
   a.B = pb.B;
   a.D = pb.D;
-  a.bound = pb.work;
-  a.tflag = pb.tflag;
-  a.touched = pb.touched;
-  a.ntouched = pb.c.len + 3;
+  a.state = pb.state;
   a.k = pb.c.k;
+  a.live = pb.c.live;
+  a.ctl = pb.c.len;  // [0] frontier entries, [1] split rows, [2] error flag (zeroed by k_ctl_init)
   hipLaunchKernelGGL(k_init_peel, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const unsigned long long*>(counts), pb.B, pb.D, pb.state, pb.tflag, ns);
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, ctx->stream, pb.c);
-  uint32_t* paths = pb.owner_o;  // the vertex peel has no edge owners
-  hipLaunchKernelGGL(k_peelv_bound, dim3(grid_for(ns, 4, 16384)), dim3(256), 0, ctx->stream, a.offS, a.edgS, a.offO, ns,
-                     pb.work, paths);
-  PBF_HIP(hipGetLastError());
-  // HBM fallback pool, only when some vertex could overflow the largest LDS table
-  const uint32_t small_max = (V_SMALL_CAP / 4) * 3 - V_SMALL_TEAM;
-  void* pp;
-  PBF_CHECK(pbf_scratch(ctx, S_POOL, sizeof(uint32_t) * 2 * (size_t)ns * V_HBM_BLOCKS, &pp));
-  PBF_HIP(hipMemsetAsync(pp, 0, sizeof(uint32_t) * 2 * (size_t)ns * V_HBM_BLOCKS, ctx->stream));
-  ctx->pool_clean = 0;  // the counting kernels re-clear the pool before their next use
-  a.pool = static_cast<uint32_t*>(pp);
-  uint32_t* small_list = pb.lists;
-  uint32_t* large_list = pb.lists + ns;
-  uint32_t* ovf_list = pb.lists + 2 * ns;
-  // split walks: up to vrows rows per round, each a dense row of ns words and its block flags
-  const int64_t no = use_v ? g.nv : g.nu;
-  const uint32_t vrows = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(VS_MAX_ROWS, (int64_t(1) << 31) / (8 * ns)));
-  const uint32_t sub_cap = vrows * (uint32_t)(g.m / VS_PCH + no / VS_CHUNK + 2);
-  const uint32_t nblk = (uint32_t)((ns + 63) / 64);
-  // kept chunk prefixes: every chunk of a round up to 2^16 of them (2 KB each)
-  const uint32_t pre_cap = (uint32_t)std::min<int64_t>((int64_t)vrows * (no / VS_CHUNK + 2), int64_t(1) << 16);
-  VSplitArgs vs{};
-  VBlocks vb{};
+  // split walks: up to vrows rows per round, each a dense row of ns counters (at most 2 GB of
+  // rows); the static path prefix of every S slot and the record of every S item
+  const uint32_t vrows = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(VS_MAX_ROWS, (int64_t(1) << 31) / (4 * ns)));
+  uint32_t nb, bs_log = 10;
+  while (((ns - 1) >> bs_log) + 1 > 256) bs_log++;
+  nb = (uint32_t)(((ns - 1) >> bs_log) + 1);
   {
-    const size_t words = (size_t)vrows * ((size_t)ns + nblk) + (size_t)vrows;
+    const size_t row_words = (size_t)vrows * (size_t)ns;
+    // chunks of 256 centres: at most m / 256 + ns of them
+    const size_t nchunk_max = (size_t)g.m / 256 + (size_t)ns + 1;
+    const size_t words = row_words + 4 * (size_t)ns + 2 * (size_t)ns + 2 * (size_t)g.m + 2 * (size_t)ns + nchunk_max +
+                         FR_CAP + 256 * (2 + 1 + 1 + NARG) + 64;
     void* sp;
-    PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words + 8 * (size_t)sub_cap + 8 * (size_t)pre_cap * VS_CHUNK + 12 * 256 + 64, &sp));
+    PBF_CHECK(pbf_scratch(ctx, S_PEEL7, 4 * words, &sp));
     uint32_t* w = static_cast<uint32_t*>(sp);
-    vs.rows = w;
-    vs.flags = w + (size_t)vrows * ns;
-    uint32_t* vlist = vs.flags + (size_t)vrows * nblk;
-    vs.sub = reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(vlist + vrows + 2) & ~uintptr_t(7));
-    // rows and flags start at zero; every round's credit pass leaves them so
-    PBF_CHECK(pbf_fill_u32(ctx, vs.rows, 0u, (int64_t)vrows * ((int64_t)ns + nblk)));
-    vs.list = vlist;
-    vs.nblk = nblk;
-    vs.pre = reinterpret_cast<uint2*>(vs.sub + sub_cap);
-    vs.pre_cap = pre_cap;
-    // the block minima of D: at most 256 blocks of at least 1024 items, all scanned in round 1
-    vb.bs_log = 10;
-    while (((ns - 1) >> vb.bs_log) + 1 > 256) vb.bs_log++;
-    vb.nb = (uint32_t)(((ns - 1) >> vb.bs_log) + 1);
-    vb.vmin = reinterpret_cast<unsigned long long*>(vs.pre + (size_t)pre_cap * VS_CHUNK);
-    vb.flag = reinterpret_cast<uint32_t*>(vb.vmin + 256);
-    PBF_CHECK(pbf_fill_u32(ctx, vb.flag, 1u, vb.nb));
+    a.rows = w;
+    w += row_words;
+    w = reinterpret_cast<uint32_t*>((reinterpret_cast<uintptr_t>(w) + 15) & ~uintptr_t(15));
+    a.rec = reinterpret_cast<const uint4*>(w);
+    w += 4 * (size_t)ns;
+    a.rec2 = reinterpret_cast<const uint2*>(w);
+    w += 2 * (size_t)ns;
+    a.sv = reinterpret_cast<const uint2*>(w);
+    w += 2 * (size_t)g.m;
+    a.vmin = reinterpret_cast<unsigned long long*>(w);
+    w = reinterpret_cast<uint32_t*>(a.vmin + 256);
+    a.vcnt = w; w += 256;
+    a.vflag = w; w += 256;
+    a.varg = w; w += 256 * NARG;
+    a.fr = w; w += FR_CAP;
+    uint32_t* nch = w; w += ns;   // chunks per item, then their exclusive prefix (the item's first chunk)
+    uint32_t* cbase = w; w += ns;
+    a.cu = w;
+    // rows start at zero; every prepare pass leaves the ones it credited so
+    PBF_CHECK(pbf_fill_u32(ctx, a.rows, 0u, (int64_t)row_words));
+    PBF_CHECK(pbf_fill_u32(ctx, a.vflag, 1u, nb));
+    hipLaunchKernelGGL(k_peelv_nchunks, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, ctx->stream, a.offS, ns, nch);
+    PBF_CHECK(pbf_scan_excl_u32(ctx, nch, cbase, ns, nullptr));
+    hipLaunchKernelGGL(k_peelv_tables, dim3(grid_for(ns, 4, 16384)), dim3(256), 0, ctx->stream, a.offS, a.edgS, a.offO, ns,
+                       cbase, const_cast<uint2*>(a.sv), const_cast<uint32_t*>(a.cu), const_cast<uint4*>(a.rec),
+                       const_cast<uint2*>(a.rec2));
   }
-  const unsigned credit_grid = grid_for(nblk, VS_CREDIT_WAVES, ctx->num_cus * 2);  // a wavefront per block
-  vs.offS = a.offS; vs.edgS = a.edgS; vs.offO = a.offO; vs.edgO = a.edgO; vs.ns = ns;
-  vs.len = pb.c.len + 6;
-  vs.cap = vrows;
-  vs.sub_cap = sub_cap;
-  vs.ctr = pb.c.queue + 3;
-  vs.B = pb.B; vs.D = pb.D; vs.k = pb.c.k;
-  vs.bflag = vb.flag; vs.bs_log = vb.bs_log;
-  a.bflag = vb.flag; a.bs_log = vb.bs_log;
-  // beyond the large LDS table's fill limit a walk would overflow into the one-workgroup HBM path
-  const SplitSel ss{paths, VS_MIN_PATHS, (V_LARGE_CAP / 4) * 3 - V_LARGE_TEAM, vrows, const_cast<uint32_t*>(vs.list)};
+  a.nb = nb;
+  a.bs_log = bs_log;
+  a.vrows = vrows;
+  PBF_HIP(hipGetLastError());
+  const unsigned round_grid = (unsigned)ctx->num_cus * V_ROUND_PER_CU;
   auto round = [&]() -> int {
-    hipLaunchKernelGGL(k_vblocks_prepare, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.state, pb.B, pb.D, ns, pb.c);
-    hipLaunchKernelGGL(k_vblocks_select, dim3(vb.nb), dim3(256), 0, ctx->stream, vb, pb.D, pb.B, pb.state, ns, pb.work,
-                       small_max, small_list, large_list, ss, pb.c);
-    // walks: phase 1 of the split walks (2 * num_cus workgroups), wavefront teams (3 * num_cus
-    // workgroups of 4) and workgroup teams (2 * num_cus) in one launch; then the HBM-row walks of
-    // what overflowed together with phase 2 of the split walks (8 * num_cus workgroups)
-    PeelVArgs s = a;
-    s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
-    s.ovf_list = ovf_list; s.ovf_len = pb.c.len + 2;
-    PeelVArgs l = s;
-    l.list = large_list; l.list_len = pb.c.len + 1; l.queue = pb.c.queue + 1;
-    PeelVArgs h = s;
-    h.list = ovf_list; h.list_len = pb.c.len + 2; h.queue = pb.c.queue + 2;
-    const uint32_t chunk_blocks = (uint32_t)ctx->num_cus * 2, small_blocks = (uint32_t)ctx->num_cus * 3;
-    hipLaunchKernelGGL(k_peelv_walks, dim3(chunk_blocks + small_blocks + ctx->num_cus * 2), dim3(256), 0, ctx->stream, s, l,
-                       vs, chunk_blocks, small_blocks);
-    hipLaunchKernelGGL(k_vsplit_units, dim3(V_HBM_BLOCKS + ctx->num_cus * 8), dim3(256), 0, ctx->stream, vs, h);
-    hipLaunchKernelGGL(k_vsplit_credit, dim3(credit_grid), dim3(64 * VS_CREDIT_WAVES), 0, ctx->stream, vs);
+    hipLaunchKernelGGL(k_vpeel_prepare, dim3(nb), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_vpeel_round, dim3(round_grid), dim3(256), 0, ctx->stream, a);
     PBF_HIP(hipGetLastError());
     return PBF_OK;
   };
   int64_t rounds = 0;
   PBF_CHECK(run_rounds(ctx, pb, ns, round, &rounds));
   ctx->timing.rounds = rounds;
+  // a walk table that filled up (the classes rule it out) fails the peel rather than its result
+  PBF_HIP(hipMemcpyAsync(ctx->pinned + 34, pb.c.len + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PBF_HIP(hipStreamSynchronize(ctx->stream));
+  if (reinterpret_cast<const uint32_t*>(ctx->pinned + 34)[0] != 0u) return PBF_ERR_CAPACITY;
   PBF_HIP(hipMemcpyAsync(tip, pb.D, 8 * (size_t)ns, hipMemcpyDeviceToDevice, ctx->stream));
   return PBF_OK;
 }
@@ -1274,11 +1225,11 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
     const unsigned sel_grid = grid_for(entries, 256, 8192);
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
     hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
-                       pb.work, small_max, small_list, large_list, SplitSel{}, pb.c);
+                       pb.work, small_max, small_list, large_list, pb.c);
     hipLaunchKernelGGL(k_min_live, dim3(grid_for(entries, 1024, 2048)), dim3(256), 0, ctx->stream, pb.D, pb.state, cur.idO,
                        entries, pb.c);
     hipLaunchKernelGGL(k_select, dim3(sel_grid), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, cur.idO, entries, pb.work,
-                       small_max, small_list, large_list, SplitSel{}, pb.c);
+                       small_max, small_list, large_list, pb.c);
     PeelEArgs s = a;
     s.list = small_list; s.list_len = pb.c.len + 0; s.queue = pb.c.queue + 0;
     hipLaunchKernelGGL((k_peele_walk<E_SMALL_TEAM, E_SMALL_UCAP>), dim3(ctx->num_cus * E_SMALL_PER_CU), dim3(E_SMALL_TEAM), 0,
