@@ -40,6 +40,17 @@ __device__ __forceinline__ unsigned long long dev_ld(const unsigned long long* p
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The tip peel's kernels read what earlier launches wrote with ordinary vector loads, which may
+// hit in L2 (an agent-scope atomic load bypasses it); a control word at a uniform address is
+// read as a workgroup-scope atomic, so that it stays a vector load too rather than a
+// scalar-cache load.
+__device__ __forceinline__ uint32_t ctl_ld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ unsigned long long ctl_ld(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 inline unsigned grid_for(int64_t n, int threads, int max_blocks) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -244,7 +255,9 @@ constexpr uint32_t NARG = 8;          // items at a block's minimum that the pre
 constexpr uint32_t VS_PCH = 256;      // paths per split unit, one per thread
 constexpr uint32_t VS_MIN_PATHS = 1u << 15;
 constexpr uint32_t V_ROUND_PER_CU = 2;
-constexpr uint32_t PREP_PER = 8;      // items per thread in one step of the prepare pass
+constexpr uint32_t PREP_PER = 4;      // items per thread in one step of the prepare pass
+constexpr uint32_t WALK_PER = 8;      // paths per thread in flight in an LDS walk
+constexpr uint32_t VS_BATCH = 8;      // paths per thread in flight in a split walk
 constexpr uint32_t V_SMALL_MAX = (V_SMALL_CAP / 4) * 3 - V_SMALL_TEAM;  // the tables' fill limits
 constexpr uint32_t V_LARGE_MAX = (V_LARGE_CAP / 4) * 3 - V_LARGE_TEAM;
 enum FrontierClass { FC_NONE = 0, FC_SMALL = 1, FC_LARGE = 2, FC_SPLIT = 3 };
@@ -275,33 +288,138 @@ struct VPeel {
   uint32_t vrows;
 };
 
-// Retire, credit, update, block minimum.  A step holds PREP_PER items per thread in registers,
-// all their loads issued before any is used; a block of at most 2048 items is one step, and its
-// items at the minimum are listed from the registers.
+// Retire, credit, update, block minimum.  A block of at most 256 * PREP_PER items is one step:
+// everything the pass needs (control words, the previous frontier, the items' state, D and B,
+// the first split row's counts) is loaded at once, before anything is known, so the pass waits
+// for one round trip to memory; its items at the minimum are then listed from registers.
+// Larger blocks take the general path below, a step at a time.
 __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
   __shared__ unsigned long long s_min[4];
   __shared__ uint32_t s_wc[PREP_PER][4];
+  __shared__ uint32_t s_ret[PREP_PER * 8];  // retired items of a one-step block, a bit each
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
   const uint32_t b = blockIdx.x;
-  const unsigned long long k = dev_ld(a.k);
-  const uint32_t nfr = dev_ld(a.ctl + 0), nrows = dev_ld(a.ctl + 1), flag = dev_ld(a.vflag + b);
   const int64_t lo = (int64_t)b << a.bs_log;
   const int64_t hi = lo + (int64_t(1) << a.bs_log) < a.ns ? lo + (int64_t(1) << a.bs_log) : a.ns;
-  // the previous round's frontier items of this block retire (their D is their tip number)
+  const bool one = hi - lo <= 256 * (int64_t)PREP_PER;
+  if (tid < PREP_PER * 8) s_ret[tid] = 0u;
+  __syncthreads();
+  const unsigned long long k = ctl_ld(a.k);
+  const uint32_t nfr = ctl_ld(a.ctl + 0), nrows = ctl_ld(a.ctl + 1), flag = ctl_ld(a.vflag + b);
+  const uint32_t f0 = a.fr[tid];  // FR_CAP >= 256 entries
+  uint8_t st[PREP_PER];
+  unsigned long long d[PREP_PER], bx[PREP_PER], sub[PREP_PER];
+  uint32_t c0[PREP_PER];
+  if (one) {
+#pragma unroll
+    for (uint32_t i = 0; i < PREP_PER; i++) {
+      const int64_t xx = lo + 256 * i + tid;
+      const bool in = xx < hi;
+      st[i] = in ? a.state[xx] : 0;
+      d[i] = in ? a.D[xx] : 0ull;
+      bx[i] = in ? a.B[xx] : 0ull;
+      c0[i] = in ? a.rows[xx] : 0u;  // row 0, whether the last round had split rows or not
+    }
+    // the previous round's frontier items of this block retire (their D is their tip number)
+    uint32_t ret = 0;
+    for (uint32_t j = tid; j < nfr; j += 256) {
+      const uint32_t x = j < 256 ? f0 : a.fr[j];
+      if ((int64_t)x >= lo && (int64_t)x < hi) {
+        a.state[x] = 0;
+        atomicOr(&s_ret[(x - (uint32_t)lo) >> 5], 1u << (x & 31u));
+        ret++;
+      }
+    }
+    ret = pbf_wave_sum(ret);
+    if (lane == 0 && ret) atomicAdd(a.live, (unsigned long long)(-(long long)ret));
+    const bool need = __syncthreads_or(ret != 0 || flag != 0 || nrows != 0);
+    unsigned long long mn = KNONE;
+    if (need) {
+#pragma unroll
+      for (uint32_t i = 0; i < PREP_PER; i++) {
+        const uint32_t r = 256 * i + tid;
+        if ((s_ret[r >> 5] >> (r & 31u)) & 1u) st[i] = 0;
+        sub[i] = 0;
+        if (nrows && c0[i]) {
+          a.rows[lo + r] = 0u;
+          sub[i] = (unsigned long long)c0[i] * (c0[i] - 1u) / 2u;
+        }
+      }
+      for (uint32_t rr = 1; rr < nrows; rr++) {
+        uint32_t* row = a.rows + (size_t)rr * (size_t)a.ns;
+        uint32_t c[PREP_PER];
+#pragma unroll
+        for (uint32_t i = 0; i < PREP_PER; i++) {
+          const int64_t xx = lo + 256 * i + tid;
+          c[i] = xx < hi ? row[xx] : 0u;
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < PREP_PER; i++) {
+          if (c[i]) {
+            row[lo + 256 * i + tid] = 0u;
+            sub[i] += (unsigned long long)c[i] * (c[i] - 1u) / 2u;
+          }
+        }
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < PREP_PER; i++) {
+        if (st[i] != 1) continue;
+        const int64_t xx = lo + 256 * i + tid;
+        // as the LDS walks: items at or below the level (the rest of the level) keep B
+        if (sub[i] && d[i] > k) { bx[i] -= sub[i]; a.B[xx] = bx[i]; }
+        if (k != KNONE) {
+          const unsigned long long nd = bx[i] > k ? bx[i] : k;
+          if (nd != d[i]) { a.D[xx] = nd; d[i] = nd; }
+        }
+        mn = d[i] < mn ? d[i] : mn;
+      }
+    }
+    if (!need) return;
+    mn = wave_min_u64(mn);
+    if (lane == 0) s_min[wid] = mn;
+    __syncthreads();
+    mn = s_min[0];
+    for (int w = 1; w < 4; w++) mn = s_min[w] < mn ? s_min[w] : mn;
+    uint32_t found = 0;
+    if (mn != KNONE) {
+      uint64_t bm[PREP_PER];
+#pragma unroll
+      for (uint32_t i = 0; i < PREP_PER; i++) {
+        bm[i] = __ballot(st[i] == 1 && d[i] == mn);
+        if (lane == 0) s_wc[i][wid] = (uint32_t)__popcll(bm[i]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (uint32_t i = 0; i < PREP_PER; i++) {
+        uint32_t before = found;
+        for (uint32_t w = 0; w < 4; w++) {
+          if (w < wid) before += s_wc[i][w];
+          found += s_wc[i][w];
+        }
+        if ((bm[i] >> lane) & 1ull) {
+          const uint32_t pos = before + (uint32_t)__popcll(bm[i] & pbf_lanemask_lt());
+          if (pos < NARG) a.varg[(size_t)b * NARG + pos] = (uint32_t)(lo + 256 * i + tid);
+        }
+      }
+    }
+    if (tid == 0) {
+      __hip_atomic_store(a.vmin + b, mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.vcnt + b, found < NARG + 1 ? found : NARG + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.vflag + b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  // general path (blocks of more than 256 * PREP_PER items): retire, then a step at a time
   uint32_t ret = 0;
   for (uint32_t j = tid; j < nfr; j += 256) {
-    const uint32_t x = dev_ld(a.fr + j);
+    const uint32_t x = j < 256 ? f0 : a.fr[j];
     if ((int64_t)x >= lo && (int64_t)x < hi) { a.state[x] = 0; ret++; }
   }
   ret = pbf_wave_sum(ret);
   if (lane == 0 && ret) atomicAdd(a.live, (unsigned long long)(-(long long)ret));
-  // a block changes when it held frontier items, decremented items or (any) split-row counts
   if (!__syncthreads_or(ret != 0 || flag != 0 || nrows != 0)) return;
   unsigned long long mn = KNONE;
-  uint8_t st[PREP_PER];
-  unsigned long long d[PREP_PER];
   for (int64_t x0 = lo; x0 < hi; x0 += 256 * PREP_PER) {
-    unsigned long long bx[PREP_PER], sub[PREP_PER];
 #pragma unroll
     for (uint32_t i = 0; i < PREP_PER; i++) {
       const int64_t xx = x0 + 256 * i + tid;
@@ -317,7 +435,7 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
 #pragma unroll
       for (uint32_t i = 0; i < PREP_PER; i++) {
         const int64_t xx = x0 + 256 * i + tid;
-        c[i] = xx < hi ? dev_ld(row + xx) : 0u;
+        c[i] = xx < hi ? row[xx] : 0u;
       }
 #pragma unroll
       for (uint32_t i = 0; i < PREP_PER; i++) {
@@ -331,7 +449,6 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
     for (uint32_t i = 0; i < PREP_PER; i++) {
       if (st[i] != 1) continue;
       const int64_t xx = x0 + 256 * i + tid;
-      // as the LDS walks: items at or below the level (the rest of the level) keep B
       if (sub[i] && d[i] > k) { bx[i] -= sub[i]; a.B[xx] = bx[i]; }
       if (k != KNONE) {
         const unsigned long long nd = bx[i] > k ? bx[i] : k;
@@ -347,27 +464,7 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
   for (int w = 1; w < 4; w++) mn = s_min[w] < mn ? s_min[w] : mn;
   // the first NARG live items at the minimum, ascending; counting stops past NARG
   uint32_t found = 0;
-  if (mn != KNONE && hi - lo <= 256 * (int64_t)PREP_PER) {
-    uint64_t bm[PREP_PER];
-#pragma unroll
-    for (uint32_t i = 0; i < PREP_PER; i++) {
-      bm[i] = __ballot(st[i] == 1 && d[i] == mn);
-      if (lane == 0) s_wc[i][wid] = (uint32_t)__popcll(bm[i]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t i = 0; i < PREP_PER; i++) {
-      uint32_t before = found;
-      for (uint32_t w = 0; w < 4; w++) {
-        if (w < wid) before += s_wc[i][w];
-        found += s_wc[i][w];
-      }
-      if ((bm[i] >> lane) & 1ull) {
-        const uint32_t pos = before + (uint32_t)__popcll(bm[i] & pbf_lanemask_lt());
-        if (pos < NARG) a.varg[(size_t)b * NARG + pos] = (uint32_t)(lo + 256 * i + tid);
-      }
-    }
-  } else if (mn != KNONE) {
+  if (mn != KNONE) {
     for (int64_t x0 = lo; x0 < hi && found <= NARG; x0 += 256) {
       const int64_t xx = x0 + tid;
       bool sel = false;
@@ -455,15 +552,26 @@ __device__ void vwalk_item(const VPeel& a, WalkLds<TEAM, CAP, USED>& L, uint32_t
     sh.P[tid] = e.x == 0xFFFFFFFFu ? e.x : e.x - cs;
     sh.vs[tid] = e.y - (e.x - cs);
     walk_barrier<TEAM>();
-    for (uint32_t w = tid; w < ce - cs; w += TEAM) {
-      // the last centre of the step starting at or before w
-      uint32_t lo = 0, hi = TEAM - 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (sh.P[mid] <= w) lo = mid; else hi = mid - 1;
+    // WALK_PER paths per thread at a time, every key loaded before any is inserted
+    for (uint32_t w0 = 0; w0 < ce - cs; w0 += TEAM * WALK_PER) {
+      uint32_t key[WALK_PER];
+#pragma unroll
+      for (uint32_t i = 0; i < WALK_PER; i++) {
+        const uint32_t w = w0 + i * TEAM + tid;
+        key[i] = u;
+        if (w < ce - cs) {
+          // the last centre of the step starting at or before w
+          uint32_t lo = 0, hi = TEAM - 1;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (sh.P[mid] <= w) lo = mid; else hi = mid - 1;
+          }
+          key[i] = a.edgO[sh.vs[lo] + w];
+        }
       }
-      const uint32_t u2 = a.edgO[sh.vs[lo] + w];
-      if (u2 != u && !sh.ovf) lds_insert<TEAM>(t, sh, u2);
+#pragma unroll
+      for (uint32_t i = 0; i < WALK_PER; i++)
+        if (key[i] != u && !sh.ovf) lds_insert<TEAM>(t, sh, key[i]);
     }
     walk_barrier<TEAM>();
   }
@@ -485,94 +593,151 @@ __device__ void vwalk_item(const VPeel& a, WalkLds<TEAM, CAP, USED>& L, uint32_t
   walk_barrier<TEAM>();
 }
 
-// Unit q of row r's item u (record r4, chunks from r2.x): its chunk c is the last one with
-// cu[c] <= q (one 256-wide probe of cu up to 256 chunks, a 256-ary search beyond), and it
-// covers the chunk's paths [256 (q - cu[c]), + 256), one per thread, counted into the row with
-// counter atomics that return nothing.  The chunk's centres are laid out in LDS (one per thread)
-// and every thread finds its path's centre there.
-__device__ void vsplit_unit(const VPeel& a, TeamShared<256>& sh, uint32_t r, uint32_t u, uint4 r4, uint2 r2, uint32_t q) {
+// A split task's LDS: the chunk layout and a table that sums the task's counts per key.
+constexpr uint32_t AGG_CAP = 4096, AGG_PROBES = 16;
+struct SplitLds {
+  TeamShared<256> sh;
+  uint32_t keys[AGG_CAP];
+  uint32_t cnts[AGG_CAP];
+  uint32_t used[AGG_CAP];
+};
+
+// Counts key in the task's table (bounded probing); false when no slot was found.
+__device__ __forceinline__ bool agg_insert(SplitLds& L, uint32_t key) {
+  uint32_t h = hash_slot(key, 32u - 12u);  // AGG_CAP = 2^12 slots
+  for (uint32_t probe = 0; probe < AGG_PROBES; probe++) {
+    const uint32_t old = atomicCAS(&L.keys[h], EMPTY, key);
+    if (old == EMPTY) {
+      L.used[atomicAdd(&L.sh.nused, 1u)] = h;
+      atomicAdd(&L.cnts[h], 1u);
+      return true;
+    }
+    if (old == key) {
+      atomicAdd(&L.cnts[h], 1u);
+      return true;
+    }
+    h = (h + 1u) & (AGG_CAP - 1u);
+  }
+  return false;
+}
+
+// Paths [wa, wb) of the chunk starting at centre c0 of the item (record r4): the chunk's
+// centres are laid out in LDS (one per thread) and every thread then takes VS_BATCH paths at a
+// time, all their keys loaded before any is counted.  Keys of retired items are dropped (they
+// take no credit); the others are summed in the task's table, or counted into row r directly
+// when the table has no room for them.
+__device__ void vsplit_paths(const VPeel& a, SplitLds& L, uint32_t r, uint32_t u, uint4 r4, uint32_t c0,
+                             uint32_t wa, uint32_t wb) {
+  TeamShared<256>& sh = L.sh;
   const uint32_t tid = threadIdx.x;
-  const uint32_t uo = r4.x, deg = r4.y, T = r4.z;
+  const uint32_t uo = r4.x, deg = r4.y;
+  const uint32_t cs = a.sv[uo + c0].x;
+  const uint32_t j = c0 + tid;
+  uint2 e = make_uint2(0xFFFFFFFFu, 0u);
+  if (j < deg) e = a.sv[uo + j];
+  __syncthreads();  // the previous chunk's readers are done with sh
+  // path p of centre j is N(v_j)[p - e.x]; with w = p - cs that is (e.y - (e.x - cs)) + w
+  sh.P[tid] = e.x == 0xFFFFFFFFu ? e.x : e.x - cs;
+  sh.vs[tid] = e.y - (e.x - cs);
+  __syncthreads();
+  uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+  for (uint32_t w0 = wa; w0 < wb; w0 += 256u * VS_BATCH) {
+    uint32_t key[VS_BATCH];
+#pragma unroll
+    for (uint32_t i = 0; i < VS_BATCH; i++) {
+      const uint32_t w = w0 + i * 256u + tid;
+      key[i] = u;
+      if (w < wb) {
+        // the last centre of the chunk starting at or before w
+        uint32_t l = 0, h = 255;
+        while (l < h) {
+          const uint32_t mid = (l + h + 1) >> 1;
+          if (sh.P[mid] <= w) l = mid; else h = mid - 1;
+        }
+        key[i] = a.edgO[sh.vs[l] + w];
+      }
+    }
+    uint8_t live[VS_BATCH];
+#pragma unroll
+    for (uint32_t i = 0; i < VS_BATCH; i++) live[i] = key[i] != u ? a.state[key[i]] : 0;
+#pragma unroll
+    for (uint32_t i = 0; i < VS_BATCH; i++)
+      if (live[i] == 1 && !agg_insert(L, key[i])) atomicAdd(row + key[i], 1u);
+  }
+}
+
+// Units [q0, q1) of row r's item u (record r4, chunks from r2.x, r2.y units in all): unit q of
+// chunk c (the last chunk with cu[c] <= q) covers the chunk's paths [256 (q - cu[c]), + 256).
+// The first chunk is found by a 256-wide probe of cu (a 256-ary search beyond 256 chunks);
+// the units of one chunk are walked together.  The task's table starts empty and is added to
+// the row at the end, one atomic per key.
+__device__ void vsplit_units(const VPeel& a, SplitLds& L, uint32_t r, uint32_t u, uint4 r4, uint2 r2,
+                             uint32_t q0, uint32_t q1) {
+  TeamShared<256>& sh = L.sh;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t deg = r4.y, T = r4.z;
   const uint32_t nch = (deg + 255u) / 256u;
-  // the chunk: in [lo, lo + len); cu[lo] <= q always (cu[0] = 0)
+  for (uint32_t s = tid; s < AGG_CAP; s += 256) { L.keys[s] = EMPTY; L.cnts[s] = 0; }
   uint32_t lo = 0, len = nch;
-  if (tid == 0) sh.item = 0;  // cu[lo]
+  if (tid == 0) { sh.item = 0; sh.nused = 0; }  // cu[lo]
   while (len > 1) {
     const uint32_t step = (len + 255u) / 256u;
     const uint32_t j = lo + tid * step;
     uint32_t cv = 0;
-    const bool le = j < lo + len && (cv = a.cu[r2.x + j]) <= q;
-    const uint32_t c = (uint32_t)__syncthreads_count(le);  // probes 0 .. c-1 lie at or before q
+    const bool le = j < lo + len && (cv = a.cu[r2.x + j]) <= q0;
+    const uint32_t c = (uint32_t)__syncthreads_count(le);  // probes 0 .. c-1 lie at or before q0
     if (le && tid == c - 1u) sh.item = cv;
     const uint32_t nlo = lo + (c - 1u) * step;
     len = (c * step < len ? c * step : len) - (c - 1u) * step;
     lo = nlo;
   }
   __syncthreads();
-  const uint32_t first = sh.item;
-  const uint32_t c0 = lo * 256u;
-  const uint32_t cs = a.sv[uo + c0].x;
-  const uint32_t ce = c0 + 256u < deg ? a.sv[uo + c0 + 256u].x : T;
-  const uint32_t j = c0 + tid;
-  uint2 e = make_uint2(0xFFFFFFFFu, 0u);
-  if (j < deg) e = a.sv[uo + j];
-  sh.P[tid] = e.x == 0xFFFFFFFFu ? e.x : e.x - cs;
-  sh.vs[tid] = e.y - (e.x - cs);
-  __syncthreads();
-  const uint32_t w = (q - first) * 256u + tid;  // path of the chunk, from its first
-  if (w < ce - cs) {
-    uint32_t l = 0, h = 255;
-    while (l < h) {
-      const uint32_t mid = (l + h + 1) >> 1;
-      if (sh.P[mid] <= w) l = mid; else h = mid - 1;
-    }
-    const uint32_t u2 = a.edgO[sh.vs[l] + w];
-    if (u2 != u) atomicAdd(a.rows + (size_t)r * (size_t)a.ns + u2, 1u);
+  uint32_t cfirst = sh.item;  // units before chunk lo
+  for (uint32_t c = lo; c < nch && cfirst < q1; c++) {
+    // the chunk's units [cfirst, cnext) and paths [0, ce - cs) from its first
+    const uint32_t c0 = c * 256u;
+    const uint32_t cnext = c + 1 < nch ? a.cu[r2.x + c + 1] : r2.y;
+    const uint32_t cs = a.sv[r4.x + c0].x;
+    const uint32_t ce = c0 + 256u < deg ? a.sv[r4.x + c0 + 256u].x : T;
+    const uint32_t qa = q0 > cfirst ? q0 : cfirst, qb = q1 < cnext ? q1 : cnext;
+    const uint32_t wa = (qa - cfirst) * 256u;
+    const uint32_t wb = (qb - cfirst) * 256u < ce - cs ? (qb - cfirst) * 256u : ce - cs;
+    vsplit_paths(a, L, r, u, r4, c0, wa, wb);
+    cfirst = cnext;
   }
-}
-
-// Exclusive workgroup scan of x (256 threads); *total gets the sum.
-__device__ __forceinline__ uint64_t wg_excl_scan(uint64_t x, uint64_t* s_w, uint64_t* total) {
-  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  const uint64_t incl = pbf_wave_incl_scan(x);
-  __syncthreads();  // s_w of an earlier scan has been read
-  if (lane == 63) s_w[wid] = incl;
   __syncthreads();
-  uint64_t before = 0, tot = 0;
-  for (uint32_t w = 0; w < 4; w++) {
-    if (w < wid) before += s_w[w];
-    tot += s_w[w];
+  uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
+  for (uint32_t q = tid; q < sh.nused; q += 256) {
+    const uint32_t slot = L.used[q];
+    atomicAdd(row + L.keys[slot], L.cnts[slot]);
   }
-  *total = tot;
-  return before + incl - x;
 }
 
 // The round: level, frontier, walks.  Tasks of the round, in order: one per large item (a
-// workgroup team), one per split unit, one per 4 small items (a wavefront team each); workgroup
-// b runs tasks b, b + grid, ...
+// workgroup team), split-unit ranges (at most one per workgroup of the grid), one per 4 small
+// items (a wavefront team each); workgroup b runs tasks b, b + grid, ...
 __global__ __launch_bounds__(256) void k_vpeel_round(VPeel a) {
   __shared__ union {
-    TeamShared<256> sh;
+    SplitLds sp;
     WalkLds<V_LARGE_TEAM, V_LARGE_CAP, false> l;
     WalkLds<V_SMALL_TEAM, V_SMALL_CAP, true> s[4];
   } lds;
   __shared__ uint32_t f_id[FR_CAP];
-  __shared__ uint32_t f_list[FR_CAP];   // small | large | split items
+  __shared__ uint32_t f_list[FR_CAP];   // small items from the front, large ones from the end
   __shared__ uint4 s_row[VS_MAX_ROWS];  // split rows: the item's record
   __shared__ uint2 s_row2[VS_MAX_ROWS];
   __shared__ uint32_t s_rowu[VS_MAX_ROWS];  // the item
   __shared__ uint32_t s_upfx[VS_MAX_ROWS + 1];  // split units before row r
   __shared__ unsigned long long s_min[4];
-  __shared__ uint64_t s_w[4];
-  __shared__ uint64_t s_kb[4];
-  __shared__ uint32_t s_wc[4];
+  __shared__ uint64_t s_kb[4], s_big[4];
+  __shared__ uint32_t s_wc[4], s_cc[4][3];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
   const bool lead = blockIdx.x == 0 && tid == 0;
-  // the level: the least block minimum (nb <= 256 = the workgroup size)
-  // with every block's first item at its minimum, loaded before the level is known
-  const unsigned long long v = tid < a.nb ? dev_ld(a.vmin + tid) : KNONE;
-  const uint32_t vc = tid < a.nb ? dev_ld(a.vcnt + tid) : 0u;
-  const uint32_t v0 = tid < a.nb ? dev_ld(a.varg + (size_t)tid * NARG) : 0u;
+  // the level: the least block minimum (nb <= 256 = the workgroup size), with every block's
+  // count and first item at its minimum loaded before the level is known
+  const unsigned long long v = tid < a.nb ? a.vmin[tid] : KNONE;
+  const uint32_t vc = tid < a.nb ? a.vcnt[tid] : 0u;
+  const uint32_t v0 = tid < a.nb ? a.varg[(size_t)tid * NARG] : 0u;
   unsigned long long k = wave_min_u64(v);
   if (lane == 0) s_min[wid] = k;
   __syncthreads();
@@ -586,25 +751,32 @@ __global__ __launch_bounds__(256) void k_vpeel_round(VPeel a) {
   // candidates: the live items with D == k of the blocks holding k, in block order
   const bool isk = v == k;
   const uint32_t cnt = isk ? vc : 0u;
-  uint32_t n;
-  if (!__syncthreads_or(cnt > NARG)) {
-    uint64_t tot;
-    const uint32_t pos = (uint32_t)wg_excl_scan(cnt, s_w, &tot);
-    for (uint32_t j = 0; j < cnt && pos + j < FR_CAP; j++) f_id[pos + j] = j ? dev_ld(a.varg + (size_t)tid * NARG + j) : v0;
-    n = tot < FR_CAP ? (uint32_t)tot : FR_CAP;
+  const uint32_t incl = pbf_wave_incl_scan(cnt);
+  {
+    const uint64_t km = __ballot(isk), bg = __ballot(cnt > NARG);
+    if (lane == 63) s_wc[wid] = incl;
+    if (lane == 0) { s_kb[wid] = km; s_big[wid] = bg; }
+  }
+  __syncthreads();
+  uint32_t n = 0, before = 0;
+  for (uint32_t w = 0; w < 4; w++) {
+    if (w < wid) before += s_wc[w];
+    n += s_wc[w];
+  }
+  if ((s_big[0] | s_big[1] | s_big[2] | s_big[3]) == 0ull) {
+    // every block holding k listed all its items at k
+    const uint32_t pos = before + incl - cnt;
+    for (uint32_t j = 0; j < cnt && pos + j < FR_CAP; j++) f_id[pos + j] = j ? a.varg[(size_t)tid * NARG + j] : v0;
   } else {
-    const uint64_t km = __ballot(isk);
-    if (lane == 0) s_kb[wid] = km;
-    __syncthreads();
     n = 0;
     for (uint32_t w4 = 0; w4 < 4 && n < FR_CAP; w4++) {
       uint64_t m = s_kb[w4];
       while (m && n < FR_CAP) {
         const uint32_t bb = w4 * 64u + (uint32_t)__ffsll((long long)m) - 1u;
         m &= m - 1ull;
-        const uint32_t c = dev_ld(a.vcnt + bb);
+        const uint32_t c = ctl_ld(a.vcnt + bb);
         if (c <= NARG) {
-          if (tid < c && n + tid < FR_CAP) f_id[n + tid] = dev_ld(a.varg + (size_t)bb * NARG + tid);
+          if (tid < c && n + tid < FR_CAP) f_id[n + tid] = a.varg[(size_t)bb * NARG + tid];
           n += c;
           continue;
         }
@@ -618,86 +790,95 @@ __global__ __launch_bounds__(256) void k_vpeel_round(VPeel a) {
           __syncthreads();
           if (lane == 0) s_wc[wid] = (uint32_t)__popcll(bm);
           __syncthreads();
-          uint32_t before = n, tot = 0;
+          uint32_t bf = n, tot = 0;
           for (uint32_t w = 0; w < 4; w++) {
-            if (w < wid) before += s_wc[w];
+            if (w < wid) bf += s_wc[w];
             tot += s_wc[w];
           }
-          const uint32_t pos = before + (uint32_t)__popcll(bm & pbf_lanemask_lt());
+          const uint32_t pos = bf + (uint32_t)__popcll(bm & pbf_lanemask_lt());
           if (sel && pos < FR_CAP) f_id[pos] = (uint32_t)xx;
           n += tot;
         }
       }
     }
-    n = n < FR_CAP ? n : FR_CAP;
   }
+  n = n < FR_CAP ? n : FR_CAP;
   __syncthreads();
-  // classes; split items beyond the round's rows stay for a later round (fr entry ~0u)
-  uint32_t cls[4], xs[4];
-  uint4 rc[4];
-  uint2 rc2[4];
-  uint64_t pos[4];
-  uint64_t run = 0;  // small | large << 16 | split << 32, before this step
-  for (uint32_t q = 0; q < 4; q++) {
-    const uint32_t j = q * 256u + tid;
-    cls[q] = FC_NONE;
-    xs[q] = 0;
-    rc[q] = make_uint4(0u, 0u, 0u, 0u);
-    rc2[q] = make_uint2(0u, 0u);
+  // classes, placed by ballots; split items beyond the round's rows stay for a later round
+  // (frontier entry ~0u)
+  uint32_t nsmall = 0, nlarge = 0, nsplit_all = 0;
+  const uint64_t lt = pbf_lanemask_lt();
+  for (uint32_t j0 = 0; j0 < n; j0 += 256) {
+    const uint32_t j = j0 + tid;
+    uint32_t cls = FC_NONE, x = 0;
+    uint4 rc = make_uint4(0u, 0u, 0u, 0u);
+    uint2 rc2 = make_uint2(0u, 0u);
     if (j < n) {
-      const uint32_t x = f_id[j];
-      xs[q] = x;
-      rc[q] = a.rec[x];
-      rc2[q] = a.rec2[x];
+      x = f_id[j];
+      rc = a.rec[x];
+      rc2 = a.rec2[x];
       // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
-      cls[q] = a.B[x] ? rc[q].w : (uint32_t)FC_NONE;
+      cls = a.B[x] ? rc.w : (uint32_t)FC_NONE;
     }
-    const uint64_t one = cls[q] == FC_SMALL ? 1ull : cls[q] == FC_LARGE ? (1ull << 16) : cls[q] == FC_SPLIT ? (1ull << 32) : 0ull;
-    uint64_t tot = 0;
-    if (q * 256u < n) pos[q] = run + wg_excl_scan(one, s_w, &tot);
-    else pos[q] = run;
-    run += tot;
-  }
-  const uint32_t nsmall = (uint32_t)(run & 0xFFFFu), nlarge = (uint32_t)((run >> 16) & 0xFFFFu);
-  const uint32_t nsplit_all = (uint32_t)((run >> 32) & 0xFFFFu);
-  const uint32_t nsplit = nsplit_all < a.vrows ? nsplit_all : a.vrows;
-  for (uint32_t q = 0; q < 4; q++) {
-    const uint32_t j = q * 256u + tid;
-    if (j >= n) break;
+    const uint64_t bs = __ballot(cls == FC_SMALL), bl = __ballot(cls == FC_LARGE), bp = __ballot(cls == FC_SPLIT);
+    __syncthreads();  // s_cc of the previous step has been read
+    if (lane == 0) {
+      s_cc[wid][0] = (uint32_t)__popcll(bs);
+      s_cc[wid][1] = (uint32_t)__popcll(bl);
+      s_cc[wid][2] = (uint32_t)__popcll(bp);
+    }
+    __syncthreads();
+    uint32_t ps = nsmall, pl = nlarge, pp = nsplit_all;
+    for (uint32_t w = 0; w < 4; w++) {
+      if (w < wid) { ps += s_cc[w][0]; pl += s_cc[w][1]; pp += s_cc[w][2]; }
+      nsmall += s_cc[w][0];
+      nlarge += s_cc[w][1];
+      nsplit_all += s_cc[w][2];
+    }
     bool keep = true;
-    if (cls[q] == FC_SMALL) f_list[(uint32_t)(pos[q] & 0xFFFFu)] = xs[q];
-    else if (cls[q] == FC_LARGE) f_list[nsmall + (uint32_t)((pos[q] >> 16) & 0xFFFFu)] = xs[q];
-    else if (cls[q] == FC_SPLIT) {
-      const uint32_t r = (uint32_t)((pos[q] >> 32) & 0xFFFFu);
-      if (r < nsplit) { s_row[r] = rc[q]; s_row2[r] = rc2[q]; s_rowu[r] = xs[q]; }
+    if (cls == FC_SMALL) {
+      f_list[ps + (uint32_t)__popcll(bs & lt)] = x;
+    } else if (cls == FC_LARGE) {
+      f_list[FR_CAP - 1u - (pl + (uint32_t)__popcll(bl & lt))] = x;  // large items from the end
+    } else if (cls == FC_SPLIT) {
+      const uint32_t r = pp + (uint32_t)__popcll(bp & lt);
+      if (r < a.vrows) { s_row[r] = rc; s_row2[r] = rc2; s_rowu[r] = x; }
       else keep = false;
     }
-    if (blockIdx.x == 0) a.fr[j] = keep ? xs[q] : ~0u;
+    if (blockIdx.x == 0 && j < n) a.fr[j] = keep ? x : ~0u;
   }
+  const uint32_t nsplit = nsplit_all < a.vrows ? nsplit_all : a.vrows;
   if (lead) { a.ctl[0] = n; a.ctl[1] = nsplit; }
   __syncthreads();
   // split rows and their units
   if (tid < 64) {
     const uint32_t un = tid < nsplit ? s_row2[tid].y : 0u;
-    const uint32_t incl = pbf_wave_incl_scan(un);
-    if (tid < nsplit) s_upfx[tid + 1] = incl;
+    const uint32_t ui = pbf_wave_incl_scan(un);
+    if (tid < nsplit) s_upfx[tid + 1] = ui;
     if (tid == 0) s_upfx[0] = 0;
   }
   __syncthreads();
   const uint32_t nunits = s_upfx[nsplit];
-  const uint32_t ntasks = nlarge + nunits + (nsmall + 3u) / 4u;
+  const uint32_t nut = nunits < gridDim.x ? nunits : gridDim.x;  // unit ranges
+  const uint32_t ntasks = nlarge + nut + (nsmall + 3u) / 4u;
   for (uint32_t task = blockIdx.x; task < ntasks; task += gridDim.x) {
     __syncthreads();  // LDS of the previous task
     if (task < nlarge) {
-      const uint32_t u = f_list[nsmall + task];
+      const uint32_t u = f_list[FR_CAP - 1u - task];
       vwalk_item<V_LARGE_TEAM, V_LARGE_CAP, false>(a, lds.l, u, a.rec[u], k, tid);
-    } else if (task < nlarge + nunits) {
-      const uint32_t q = task - nlarge;
-      uint32_t r = 0;
-      while (s_upfx[r + 1] <= q) r++;
-      vsplit_unit(a, lds.sh, r, s_rowu[r], s_row[r], s_row2[r], q - s_upfx[r]);
+    } else if (task < nlarge + nut) {
+      const uint32_t t = task - nlarge;
+      uint32_t q0 = (uint32_t)((uint64_t)nunits * t / nut), q1 = (uint32_t)((uint64_t)nunits * (t + 1) / nut);
+      while (q0 < q1) {
+        uint32_t r = 0;
+        while (s_upfx[r + 1] <= q0) r++;
+        const uint32_t e = q1 < s_upfx[r + 1] ? q1 : s_upfx[r + 1];
+        vsplit_units(a, lds.sp, r, s_rowu[r], s_row[r], s_row2[r], q0 - s_upfx[r], e - s_upfx[r]);
+        q0 = e;
+        __syncthreads();
+      }
     } else {
-      const uint32_t i = (task - nlarge - nunits) * 4u + wid;
+      const uint32_t i = (task - nlarge - nut) * 4u + wid;
       if (i < nsmall) {
         const uint32_t u = f_list[i];
         vwalk_item<V_SMALL_TEAM, V_SMALL_CAP, true>(a, lds.s[wid], u, a.rec[u], k, lane);
