@@ -7,9 +7,9 @@
 //   the peeled items' butterflies are subtracted from the live items they share them with;
 //   every updated live item gets D = max(B, k).  The tip / wing number is D at removal.
 // The reference's bucket structure is replaced by min-count frontiers: per round a min
-// reduction, a selection (stream compaction of D == k), a two-hop walk of the frontier with
-// 64-bit atomic decrements, and an update of the touched items.  Rounds run in batches on the
-// device; the host reads the live count once per batch.
+// reduction, a selection of the items with D == k, a walk of the frontier with 64-bit atomic
+// decrements, and an update of the touched items.  Rounds run in batches on the device; the
+// host reads the live count once per batch.
 //
 // Vertex peel walk (PeelOrigParallel :670-702): for a peeled u, every u2 != u reached through a
 // centre v loses C(c(u,u2), 2), c counted in an LDS hash (same team enumeration as counting).
@@ -26,13 +26,13 @@ constexpr unsigned long long KNONE = ~0ull;
 // work[x] of a peeled item: its walk cost; bit 31 is the side an edge walk starts from
 constexpr uint32_t WORK_MASK = 0x7FFFFFFFu, WORK_SSIDE = 0x80000000u;
 
-// Control words, list entries and walk targets are read with relaxed agent-scope atomic loads
-// (vector loads; a plain load of a uniform address would be a scalar-cache load).  Every such
-// value is written by an earlier launch on the stream: no kernel here reads what another
-// workgroup of the same launch stored, since within a launch workgroups share data only through
-// device-scope atomics whose returned values carry it (queue claims, list reservations).  The
-// end of the writing launch and the start of the reading one already order those writes before
-// any load, plain or atomic; the atomic form is the one these kernels have been tested with.
+// Every control word, list entry and walk target a kernel here reads was written by an earlier
+// launch on the stream: no kernel reads what another workgroup of the same launch stored, since
+// within a launch workgroups share data only through device-scope atomics whose returned values
+// carry it (queue claims, list reservations).  The end of the writing launch and the start of
+// the reading one order those writes before any load, plain or atomic.  The edge peel reads
+// them with relaxed agent-scope atomic loads (vector loads; a plain load of a uniform address
+// would be a scalar-cache load), the form it has been tested with.
 __device__ __forceinline__ uint32_t dev_ld(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
