@@ -136,3 +136,26 @@ def test_tip_peel_many_blocks(ctx, nv, nu, m, seed, nb, bs_log):
     ref = O.peel_vert(g, use_v, v)
     assert len(np.unique(ref)) > 1000
     assert np.array_equal(tip, ref)
+
+
+def test_tip_peel_level_beyond_frontier_cap(ctx):
+    # 3000 S vertices with no butterfly (a matching) and 4000 in one butterfly each (2000
+    # disjoint C4s), plus one K_{3,3}: the levels 0 and 1 hold more items than a round's
+    # frontier takes (1024), so each is peeled over several rounds at the same level; peeling
+    # one vertex of a C4 leaves its partner at level 1 with its butterfly count untouched
+    c4 = 2000
+    s_m = np.arange(3000, dtype=np.uint32)  # the matching: V j - U j, j < 3000
+    o_m = np.arange(3000, dtype=np.uint32)
+    # C4 k: V vertices 3000 + 2k, 3000 + 2k + 1 both on U vertices 3000 + 2k, 3000 + 2k + 1
+    s_c = 3000 + np.repeat(np.arange(2 * c4, dtype=np.uint32), 2)
+    o_c = 3000 + 2 * (np.arange(4 * c4, dtype=np.uint32) // 4) + (np.arange(4 * c4, dtype=np.uint32) % 2)
+    s_k = 3000 + 2 * c4 + np.repeat(np.arange(3, dtype=np.uint32), 3)
+    o_k = 3000 + 2 * c4 + np.tile(np.arange(3, dtype=np.uint32), 3)
+    nv = nu = 3000 + 2 * c4 + 3
+    g = pb.Graph.from_edges(nv, nu, np.concatenate([s_m, s_c, s_k]), np.concatenate([o_m, o_c, o_k]))
+    for use_v in (0, 1):
+        v = O.count_vert(g, use_v)
+        assert (v > 1).sum() == 3 and (v == 1).sum() == 2 * c4  # the K_{3,3}, the C4s
+        tip = pb.as_u64(ctx.peel(g, use_v, v))
+        assert np.array_equal(tip, O.peel_vert(g, use_v, v))
+        assert ctx.timing()["rounds"] >= 8  # 3 rounds at level 0, 4 at level 1, the K_{3,3}
