@@ -416,8 +416,9 @@ __device__ __forceinline__ void credit(const CountArgs& a, uint32_t x, unsigned 
 // Most endpoint credits go to the hubs (ranks below HOT_N): on a power-law graph over 90 % of
 // them.  Scattered 64-bit device atomics run at a small fraction of the rate of coalesced
 // ones, so a wavefront appends its hub credits to an HBM log instead: consecutive entries of
-// a chunk of HLOG_CHUNK it reserved with one counter atomic (coalesced stores), and
-// k_hlog_reduce sums the log per rank in LDS.  Entries are (rank << 52) | value; a value of
+// a chunk of HLOG_CHUNK it reserved with one counter atomic (coalesced stores);
+// k_hlog_reduce sums each workgroup's share of the log per rank in LDS into a row of partial
+// sums and k_hlog_sum adds the rows' columns to B.  Entries are (rank << 52) | value; a value of
 // 2^52 or more, an entry past the log's capacity, and every credit to another rank take the
 // direct atomic.  An abandoned chunk's unused entries are zeroed (credits of 0).
 constexpr uint32_t HLOG_CHUNK = 1024;
@@ -462,7 +463,7 @@ __device__ __forceinline__ void credit_wave(const CountArgs& a, HotLog& hl, bool
 // Sums the log per rank in LDS and adds each rank's sum to its count (coalesced atomics).
 __global__ __launch_bounds__(256) void k_hlog_reduce(const unsigned long long* __restrict__ hlog,
                                                      const uint32_t* __restrict__ len, uint32_t cap,
-                                                     unsigned long long* __restrict__ B, int64_t n) {
+                                                     unsigned long long* __restrict__ part) {
   __shared__ unsigned long long acc[HOT_N];
   for (uint32_t x = threadIdx.x; x < HOT_N; x += blockDim.x) acc[x] = 0;
   __syncthreads();
@@ -472,8 +473,23 @@ __global__ __launch_bounds__(256) void k_hlog_reduce(const unsigned long long* _
     if (e) atomicAdd(&acc[e >> 52], e & (HLOG_VMAX - 1ull));
   }
   __syncthreads();
-  for (uint32_t x = threadIdx.x; x < HOT_N && x < n; x += blockDim.x)
-    if (acc[x]) atomicAdd(&B[x], acc[x]);
+  // row blockIdx.x of the partial sums (plain stores: the ranks' atomics from every workgroup
+  // would pile up on the same HOT_N words)
+  for (uint32_t x = threadIdx.x; x < HOT_N; x += blockDim.x) part[(size_t)blockIdx.x * HOT_N + x] = acc[x];
+}
+
+// Rank x of the hub log: the sum of column x of the partial sums over rows [HLOG_ROWS_PER * by,
+// + HLOG_ROWS_PER), added to B[x].
+constexpr uint32_t HLOG_ROWS_PER = 64;
+__global__ __launch_bounds__(256) void k_hlog_sum(const unsigned long long* __restrict__ part, uint32_t rows,
+                                                  unsigned long long* __restrict__ B, int64_t n) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t r0 = blockIdx.y * HLOG_ROWS_PER, r1 = min(r0 + HLOG_ROWS_PER, rows);
+  if (x >= HOT_N || (int64_t)x >= n) return;
+  unsigned long long s = 0;
+#pragma unroll 8
+  for (uint32_t r = r0; r < r1; r++) s += part[(size_t)r * HOT_N + x];
+  if (s) atomicAdd(&B[x], s);
 }
 
 // ---------------- windowed wedge enumeration ----------------
@@ -1312,14 +1328,19 @@ struct ClassOut {
   uint32_t rlo, rhi;    // only endpoints with ranks in [rlo, rhi) are listed (a batch)
 };
 
-__device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, const ClassOut& co) {
+// Classifies endpoint i (a thread of a 256-thread workgroup, every thread calling) and appends
+// it to its list.  The appends of the workgroup take one counter atomic per list (lists[w][c]:
+// wavefront w's count of class c; per-wavefront atomics on the three counters serialise), then
+// every lane places its entry.
+__device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t T, uint32_t deg, const ClassOut& co,
+                                                uint32_t (*cnt)[4], uint32_t* base) {
   uint32_t* __restrict__ bound = co.bound;
   uint32_t* __restrict__ wlist = co.wlist;
   uint32_t* __restrict__ blist = co.blist;
   uint32_t* __restrict__ olist = co.olist;
   uint32_t* __restrict__ lens = co.lens;
   const uint32_t cb = co.cb;
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   uint32_t cls = 0;  // 0 none, 1 wavefront, 2 workgroup, 3 HBM
   valid = valid && i >= co.rlo && i < co.rhi;
   if (valid && T) {
@@ -1335,19 +1356,22 @@ __device__ __forceinline__ void classify_append(uint32_t i, bool valid, uint64_t
     }
   }
   const uint64_t m1 = __ballot(cls == 1), m2 = __ballot(cls == 2), m3 = __ballot(cls == 3);
-  uint32_t base1 = 0, base2 = 0, base3 = 0;
-  if (lane == 0) {
-    if (m1) base1 = atomicAdd(&lens[0], (uint32_t)__popcll(m1));
-    if (m2) base2 = atomicAdd(&lens[1], (uint32_t)__popcll(m2));
-    if (m3) base3 = atomicAdd(&lens[11], (uint32_t)__popcll(m3));
+  if (lane < 3) cnt[wid][lane] = (uint32_t)__popcll(lane == 0 ? m1 : lane == 1 ? m2 : m3);
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const uint32_t c = threadIdx.x;
+    const uint32_t tot = cnt[0][c] + cnt[1][c] + cnt[2][c] + cnt[3][c];
+    base[c] = tot ? atomicAdd(&lens[c == 0 ? 0 : c == 1 ? 1 : 11], tot) : 0u;
   }
-  base1 = __shfl(base1, 0, 64);
-  base2 = __shfl(base2, 0, 64);
-  base3 = __shfl(base3, 0, 64);
-  const uint64_t lt = pbf_lanemask_lt();
-  if (cls == 1) wlist[base1 + __popcll(m1 & lt)] = i;
-  if (cls == 2) blist[base2 + __popcll(m2 & lt)] = i;
-  if (cls == 3) olist[base3 + __popcll(m3 & lt)] = i;
+  __syncthreads();
+  if (cls) {
+    const uint32_t c = cls - 1u;
+    uint32_t pos = base[c];
+    for (uint32_t w = 0; w < wid; w++) pos += cnt[w][c];
+    pos += (uint32_t)__popcll((c == 0 ? m1 : c == 1 ? m2 : m3) & pbf_lanemask_lt());
+    (c == 0 ? wlist : c == 1 ? blist : olist)[pos] = i;
+  }
+  __syncthreads();  // cnt and base are reused by the next call
 }
 
 // Short lists: classified here.  Long lists: listed in `big` with their chunk counts.
@@ -1356,9 +1380,11 @@ __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ r
                                                   int64_t n, ClassOut co, uint32_t* __restrict__ big,
                                                   uint32_t* __restrict__ big_chunks, unsigned long long* __restrict__ w64) {
   __shared__ unsigned long long s_T[4][64];
+  __shared__ uint32_t s_cnt[4][4], s_base[4];
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wid) * 64; base < n; base += nwaves * 64) {
+  // every thread of a workgroup runs the same steps (the appends are per workgroup)
+  for (int64_t b0 = (int64_t)blockIdx.x * 256; b0 < n; b0 += (int64_t)gridDim.x * 256) {
+    const int64_t base = b0 + wid * 64;
     const int64_t i = base + lane;
     uint32_t io = 0, deg = 0;
     if (i < n) { io = roff[i]; deg = roff[i + 1] - io; }
@@ -1386,8 +1412,8 @@ __global__ __launch_bounds__(256) void k_classify(const uint32_t* __restrict__ r
     }
     wave_lds_fence();
     const uint64_t T = s_T[wid][lane];
-    classify_append((uint32_t)i, i < n && small, T, deg, co);
-    const bool big_item = !small && i >= (int64_t)co.rlo && i < (int64_t)co.rhi;
+    classify_append((uint32_t)i, i < n && small, T, deg, co, s_cnt, s_base);
+    const bool big_item = i < n && !small && i >= (int64_t)co.rlo && i < (int64_t)co.rhi;
     const uint64_t mb = __ballot(big_item);
     uint32_t bb = 0;
     if (lane == 0 && mb) bb = atomicAdd(&co.lens[2], (uint32_t)__popcll(mb));
@@ -1441,12 +1467,13 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
                                                       const unsigned long long* __restrict__ w64, ClassOut co,
                                                       uint32_t* __restrict__ split, uint32_t* __restrict__ split_chunks,
                                                       uint32_t split_cap) {
+  __shared__ uint32_t s_cnt[4][4], s_base[4];
   const uint32_t nbig = lens_in[2];
   const uint32_t lane = threadIdx.x & 63;
-  // every wavefront runs the same number of iterations (the ballots inside need the whole wave)
+  // every thread of a workgroup runs the same iterations (the appends are per workgroup)
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nbig; base += stride) {
-    const uint32_t t = base + lane;
+  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < nbig; b0 += stride) {
+    const uint32_t t = b0 + threadIdx.x;
     const bool valid = t < nbig;
     const uint32_t i = valid ? big[t] : 0u;
     const uint32_t deg = valid ? roff[i + 1] - roff[i] : 0u;
@@ -1461,7 +1488,7 @@ __global__ __launch_bounds__(256) void k_classify_big(const uint32_t* __restrict
     // past the capacity (several graphs of a union, each with its own low ranks) an endpoint
     // is classified as an ordinary heavy one; the consumers clamp the count to split_cap
     const bool sp = cand && slot < split_cap;
-    classify_append(i, valid && !sp, T, deg, co);
+    classify_append(i, valid && !sp, T, deg, co, s_cnt, s_base);
     if (sp) {
       split[slot] = i;
       split_chunks[slot] = (deg + SPLIT_CHUNK - 1) / SPLIT_CHUNK;
@@ -1720,7 +1747,7 @@ __global__ void k_oidx(const uint32_t* __restrict__ redg, const uint32_t* __rest
 // beside the slot, which keeps two dependent loads and the search off their critical path.
 // rinfo[r] = (roff[r], in(r)): one 8-byte gather per slot instead of two 4-byte ones.
 // With maxdeg (cleared beforehand), also the largest degree.
-__global__ void k_rinfo(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ inpref, int64_t n,
+__global__ __launch_bounds__(256) void k_rinfo(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ inpref, int64_t n,
                         uint2* __restrict__ rinfo, uint32_t* __restrict__ maxdeg) {
   uint32_t md = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
@@ -1729,8 +1756,15 @@ __global__ void k_rinfo(const uint32_t* __restrict__ roff, const uint32_t* __res
     md = max(md, roff[r + 1] - o);
   }
   if (maxdeg) {
+    // one atomic per workgroup: atomics of every wavefront on one address serialise
+    __shared__ uint32_t part[4];
     md = wave_incl_max(md);
-    if (__lane_id() == 63u && md) atomicMax(maxdeg, md);
+    if (__lane_id() == 63u) part[threadIdx.x >> 6] = md;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      md = max(max(part[0], part[1]), max(part[2], part[3]));
+      if (md) atomicMax(maxdeg, md);
+    }
   }
 }
 
@@ -1931,9 +1965,13 @@ int run_count(pbf_context* ctx, const RankedGraph& R, CountArgs a0, uint32_t* li
     a.rerun = 1;
     hipLaunchKernelGGL((k_count2<MODE, 1>), dim3(grid), dim3(TPB), 0, ctx->stream, a);
     // the log's reducer: four workgroups per CU (32 KB of accumulators each)
-    if (MODE == MODE_VERT && a.hlog_cap)
-      hipLaunchKernelGGL(k_hlog_reduce, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, a.hlog, a.hlog_len, a.hlog_cap,
-                         a.B, n);
+    if (MODE == MODE_VERT && a.hlog_cap) {
+      const uint32_t rows = (uint32_t)ctx->num_cus * 4;
+      unsigned long long* part = a.hlog + a.hlog_cap;  // rows x HOT_N partial sums after the log
+      hipLaunchKernelGGL(k_hlog_reduce, dim3(rows), dim3(256), 0, ctx->stream, a.hlog, a.hlog_len, a.hlog_cap, part);
+      hipLaunchKernelGGL(k_hlog_sum, dim3(HOT_N / 256, (rows + HLOG_ROWS_PER - 1) / HLOG_ROWS_PER), dim3(256), 0, ctx->stream,
+                         part, rows, a.B, n);
+    }
     PBF_HIP(hipGetLastError());
     uint32_t novf = 0;
     // each block needs 2 kn words of dense counters: with small key ranges (a union of small
@@ -2024,7 +2062,8 @@ int pbf_dev_count(pbf_context* ctx, const DevGraph& g, int use_v, const RankedGr
     if (pbf_scratch_room(ctx) == SIZE_MAX && n > 0) {
       const int64_t cap = std::min<int64_t>(int64_t(1) << 27, 16 * m2 + (1 << 16));
       void* hl;
-      PBF_CHECK(pbf_scratch(ctx, S_HLOG, 8 * (size_t)cap, &hl));
+      // the log, then the reducer's rows of partial sums (one per workgroup, HOT_N each)
+      PBF_CHECK(pbf_scratch(ctx, S_HLOG, 8 * ((size_t)cap + (size_t)ctx->num_cus * 4 * HOT_N), &hl));
       a.hlog = static_cast<unsigned long long*>(hl);
       a.hlog_cap = (uint32_t)cap;
     }

@@ -20,14 +20,26 @@ constexpr int KEYS = 33;  // key values 0..32
 constexpr uint32_t RL_CHUNK = 1024;  // lists longer than this are relabelled in chunks
 constexpr uint32_t RL_SMALL = 16;    // lists up to this long are relabelled by one thread each
 
-__global__ void k_side_partial(const uint32_t* __restrict__ off, int64_t n, unsigned long long* acc) {
+// Adds a 256-thread workgroup's values into *acc with one atomic: atomics of every wavefront
+// on one address serialise, thousands of them cost more than the sum they carry.
+__device__ __forceinline__ void block_add(uint64_t s, unsigned long long* acc) {
+  __shared__ uint64_t part[4];
+  s = pbf_wave_sum(s);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(acc, (unsigned long long)t);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_side_partial(const uint32_t* __restrict__ off, int64_t n, unsigned long long* acc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t s = 0;
   for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     s += (uint64_t)pbf_side_term(off[i + 1] - off[i]);
   }
-  s = pbf_wave_sum(s);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, (unsigned long long)s);
+  block_add(s, acc);
 }
 
 __device__ __forceinline__ uint32_t vertex_degree(const DevGraph& g, int64_t x) {
@@ -382,15 +394,14 @@ __global__ void k_rl_find_long(DevGraph g, uint32_t* __restrict__ list, uint32_t
 }
 
 // W = sum_r in(r) * out(r) + C(in(r), 2) (rankWedgeF, butterfly_utils.h:216-237).
-__global__ void k_rank_wedges(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ inpref, int64_t n,
-                              unsigned long long* acc) {
+__global__ __launch_bounds__(256) void k_rank_wedges(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ inpref,
+                                                     int64_t n, unsigned long long* acc) {
   uint64_t s = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     uint64_t d = roff[r + 1] - roff[r], in = inpref[r], out = d - in;
     s += in * out + in * (in - 1) / 2;
   }
-  s = pbf_wave_sum(s);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, (unsigned long long)s);
+  block_add(s, acc);
 }
 
 // edgeToIdxs (butterfly_utils.h:2034-2062): one thread per S-CSR position, binary search.
@@ -532,8 +543,8 @@ int pbf_dev_side(pbf_context* ctx, const DevGraph& g, int* use_v, int64_t* wedge
   PBF_CHECK(pbf_scratch(ctx, S_MISC, 64 * sizeof(uint64_t), &mp));
   unsigned long long* acc = static_cast<unsigned long long*>(mp);
   PBF_CHECK(pbf_fill_u64(ctx, reinterpret_cast<uint64_t*>(acc), 0, 2));
-  if (g.nv) hipLaunchKernelGGL(k_side_partial, dim3(grid_for(g.nv, 256, 4096)), dim3(256), 0, ctx->stream, g.offV, g.nv, acc);
-  if (g.nu) hipLaunchKernelGGL(k_side_partial, dim3(grid_for(g.nu, 256, 4096)), dim3(256), 0, ctx->stream, g.offU, g.nu, acc + 1);
+  if (g.nv) hipLaunchKernelGGL(k_side_partial, dim3(grid_for(g.nv, 256, ctx->num_cus * 4)), dim3(256), 0, ctx->stream, g.offV, g.nv, acc);
+  if (g.nu) hipLaunchKernelGGL(k_side_partial, dim3(grid_for(g.nu, 256, ctx->num_cus * 4)), dim3(256), 0, ctx->stream, g.offU, g.nu, acc + 1);
   PBF_HIP(hipGetLastError());
   PBF_HIP(hipMemcpyAsync(ctx->pinned, acc, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   PBF_HIP(hipStreamSynchronize(ctx->stream));
@@ -653,7 +664,7 @@ int pbf_dev_relabel(pbf_context* ctx, const DevGraph& g, int use_v, bool want_po
   }
   unsigned long long* acc = static_cast<unsigned long long*>(mp) + 8;
   PBF_CHECK(pbf_fill_u64(ctx, reinterpret_cast<uint64_t*>(acc), 0, 1));
-  hipLaunchKernelGGL(k_rank_wedges, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, R->roff, R->inpref, n, acc);
+  hipLaunchKernelGGL(k_rank_wedges, dim3(grid_for(n, 256, ctx->num_cus * 4)), dim3(256), 0, ctx->stream, R->roff, R->inpref, n, acc);
   PBF_HIP(hipGetLastError());
   R->wedges_dev = acc;
   R->wedges = -1;
