@@ -302,23 +302,30 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
   const int64_t lo = (int64_t)b << a.bs_log;
   const int64_t hi = lo + (int64_t(1) << a.bs_log) < a.ns ? lo + (int64_t(1) << a.bs_log) : a.ns;
   const bool one = hi - lo <= 256 * (int64_t)PREP_PER;
+  // the block's first 256 * PREP_PER items (state, D, B, row 0's count, whether the last round
+  // had split rows or not), at clamped indices (hi > lo) and before the control words, so that
+  // every load is in flight at once; they are masked only once all are issued
+  uint8_t st[PREP_PER];
+  unsigned long long d[PREP_PER], bx[PREP_PER], sub[PREP_PER];
+  uint32_t c0[PREP_PER];
+#pragma unroll
+  for (uint32_t i = 0; i < PREP_PER; i++) {
+    const int64_t xx = lo + 256 * i + tid;
+    const int64_t xc = xx < hi ? xx : hi - 1;
+    st[i] = a.state[xc];
+    d[i] = a.D[xc];
+    bx[i] = a.B[xc];
+    c0[i] = a.rows[xc];
+  }
   if (tid < PREP_PER * 8) s_ret[tid] = 0u;
   __syncthreads();
   const unsigned long long k = ctl_ld(a.k);
   const uint32_t nfr = ctl_ld(a.ctl + 0), nrows = ctl_ld(a.ctl + 1), flag = ctl_ld(a.vflag + b);
   const uint32_t f0 = a.fr[tid];  // FR_CAP >= 256 entries
-  uint8_t st[PREP_PER];
-  unsigned long long d[PREP_PER], bx[PREP_PER], sub[PREP_PER];
-  uint32_t c0[PREP_PER];
   if (one) {
 #pragma unroll
     for (uint32_t i = 0; i < PREP_PER; i++) {
-      const int64_t xx = lo + 256 * i + tid;
-      const bool in = xx < hi;
-      st[i] = in ? a.state[xx] : 0;
-      d[i] = in ? a.D[xx] : 0ull;
-      bx[i] = in ? a.B[xx] : 0ull;
-      c0[i] = in ? a.rows[xx] : 0u;  // row 0, whether the last round had split rows or not
+      if (lo + 256 * i + tid >= hi) { st[i] = 0; d[i] = 0ull; bx[i] = 0ull; c0[i] = 0u; }
     }
     // the previous round's frontier items of this block retire (their D is their tip number)
     uint32_t ret = 0;
@@ -351,7 +358,8 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
 #pragma unroll
         for (uint32_t i = 0; i < PREP_PER; i++) {
           const int64_t xx = lo + 256 * i + tid;
-          c[i] = xx < hi ? row[xx] : 0u;
+          const uint32_t cv = row[xx < hi ? xx : hi - 1];
+          c[i] = xx < hi ? cv : 0u;
         }
 #pragma unroll
         for (uint32_t i = 0; i < PREP_PER; i++) {
@@ -424,9 +432,12 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
     for (uint32_t i = 0; i < PREP_PER; i++) {
       const int64_t xx = x0 + 256 * i + tid;
       const bool in = xx < hi;
-      st[i] = in ? a.state[xx] : 0;
-      d[i] = in ? a.D[xx] : 0ull;
-      bx[i] = in ? a.B[xx] : 0ull;
+      const int64_t xc = in ? xx : hi - 1;
+      const uint8_t s0 = a.state[xc];
+      const unsigned long long d0 = a.D[xc], b0 = a.B[xc];
+      st[i] = in ? s0 : 0;
+      d[i] = in ? d0 : 0ull;
+      bx[i] = in ? b0 : 0ull;
       sub[i] = 0;
     }
     for (uint32_t r = 0; r < nrows; r++) {
@@ -435,7 +446,8 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
 #pragma unroll
       for (uint32_t i = 0; i < PREP_PER; i++) {
         const int64_t xx = x0 + 256 * i + tid;
-        c[i] = xx < hi ? row[xx] : 0u;
+        const uint32_t cv = row[xx < hi ? xx : hi - 1];
+        c[i] = xx < hi ? cv : 0u;
       }
 #pragma unroll
       for (uint32_t i = 0; i < PREP_PER; i++) {
@@ -467,8 +479,10 @@ __global__ __launch_bounds__(256) void k_vpeel_prepare(VPeel a) {
   if (mn != KNONE) {
     for (int64_t x0 = lo; x0 < hi && found <= NARG; x0 += 256) {
       const int64_t xx = x0 + tid;
-      bool sel = false;
-      if (xx < hi) sel = a.state[xx] == 1 && a.D[xx] == mn;
+      const int64_t xc = xx < hi ? xx : hi - 1;
+      const uint8_t s0 = a.state[xc];
+      const unsigned long long d0 = a.D[xc];
+      const bool sel = (xx < hi) & (s0 == 1) & (d0 == mn);  // no short circuit: both loads in flight
       const uint64_t bm = __ballot(sel);
       __syncthreads();  // s_wc of the previous step has been read
       if (lane == 0) s_wc[0][wid] = (uint32_t)__popcll(bm);
@@ -513,17 +527,47 @@ __device__ __forceinline__ void walk_barrier() {
   }
 }
 
+// The keys of PER paths of a thread, w + i * stride for i < PER, among the paths [0, lim) of a
+// step whose N centres start at the paths P[0 .. N) (nondecreasing, P[0] = 0, ~0u past the last
+// centre) with centre j's path p at edgO[vs[j] + p]; u where the path is not below lim.  The
+// centre is found by a search of fixed depth, one level for all PER paths at a time, and a path
+// past lim is searched as the last one, so that every LDS read and every key load is
+// unconditional and each level's reads (and all the key loads) are in flight together.
+template <uint32_t N, uint32_t PER>
+__device__ __forceinline__ void path_keys(const uint32_t* P, const uint32_t* vs, const uint32_t* __restrict__ edgO,
+                                          uint32_t w, uint32_t stride, uint32_t lim, uint32_t u, uint32_t (&key)[PER]) {
+  uint32_t wq[PER], l[PER];
+#pragma unroll
+  for (uint32_t i = 0; i < PER; i++) {
+    const uint32_t wi = w + i * stride;
+    wq[i] = wi < lim ? wi : lim - 1u;
+    l[i] = 0;
+  }
+#pragma unroll
+  for (uint32_t step = N / 2; step; step >>= 1) {
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) l[i] = P[l[i] + step] <= wq[i] ? l[i] + step : l[i];
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < PER; i++) {
+    const uint32_t kv = edgO[vs[l[i]] + wq[i]];
+    key[i] = w + i * stride < lim ? kv : u;
+  }
+}
+
 // The walk of one frontier item u (its record r) by one team (thread tid of TEAM): every
 // two-hop path u -> v -> u2 (u2 != u) counted in an LDS hash (PeelOrigParallel,
-// butterfly_peel.h:670-702), then C(c, 2) comes off B[u2] for every u2 with D[u2] > k, and u2's
-// block is flagged for the next prepare pass.  USED tables keep a list of their used slots; the
+// butterfly_peel.h:670-702), then C(c, 2) comes off B[u2] for every u2, and u2's block is flagged
+// for the next prepare pass.  The reference decrements only the u2 with D[u2] > k; the others
+// need no test here: a live u2 at the level keeps D = max(B, k) = k however B falls (B stays the
+// exact count of its butterflies, never below zero), and a retired u2's B is never read again,
+// so the sweep issues its atomics without first waiting for a load of D.  USED tables keep a list of their used slots; the
 // others are sized from the item's bound on distinct keys, min(T(u), ns), and swept whole.  The
 // classes keep every item's distinct keys below the table's fill limit, so a table cannot fill
 // up; if one did, the round's error flag is set and the peel fails.  A step takes TEAM centres,
 // whose paths are numbered by the slot records: the centre j of the step starts at sv[j].x.
 template <int TEAM, int CAP, bool USED>
-__device__ void vwalk_item(const VPeel& a, WalkLds<TEAM, CAP, USED>& L, uint32_t u, uint4 r, unsigned long long k,
-                           uint32_t tid) {
+__device__ void vwalk_item(const VPeel& a, WalkLds<TEAM, CAP, USED>& L, uint32_t u, uint4 r, uint32_t tid) {
   TeamShared<TEAM>& sh = L.sh;
   const uint32_t uo = r.x, deg = r.y, T = r.z;
   uint32_t cap = CAP;
@@ -555,20 +599,7 @@ __device__ void vwalk_item(const VPeel& a, WalkLds<TEAM, CAP, USED>& L, uint32_t
     // WALK_PER paths per thread at a time, every key loaded before any is inserted
     for (uint32_t w0 = 0; w0 < ce - cs; w0 += TEAM * WALK_PER) {
       uint32_t key[WALK_PER];
-#pragma unroll
-      for (uint32_t i = 0; i < WALK_PER; i++) {
-        const uint32_t w = w0 + i * TEAM + tid;
-        key[i] = u;
-        if (w < ce - cs) {
-          // the last centre of the step starting at or before w
-          uint32_t lo = 0, hi = TEAM - 1;
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (sh.P[mid] <= w) lo = mid; else hi = mid - 1;
-          }
-          key[i] = a.edgO[sh.vs[lo] + w];
-        }
-      }
+      path_keys<TEAM, WALK_PER>(sh.P, sh.vs, a.edgO, w0 + tid, TEAM, ce - cs, u, key);
 #pragma unroll
       for (uint32_t i = 0; i < WALK_PER; i++)
         if (key[i] != u && !sh.ovf) lds_insert<TEAM>(t, sh, key[i]);
@@ -585,7 +616,7 @@ __device__ void vwalk_item(const VPeel& a, WalkLds<TEAM, CAP, USED>& L, uint32_t
     const uint32_t slot = USED ? L.used[q] : q;
     const uint32_t key = L.keys[slot];
     const uint32_t c = key == EMPTY ? 0u : L.cnts[slot];
-    if (c > 1u && a.D[key] > k) {
+    if (c > 1u) {
       atomicAdd(&a.B[key], (unsigned long long)(-(long long)((unsigned long long)c * (c - 1u) / 2u)));
       a.vflag[key >> a.bs_log] = 1u;  // the next prepare pass updates D over the block
     }
@@ -643,26 +674,15 @@ __device__ void vsplit_paths(const VPeel& a, SplitLds& L, uint32_t r, uint32_t u
   uint32_t* row = a.rows + (size_t)r * (size_t)a.ns;
   for (uint32_t w0 = wa; w0 < wb; w0 += 256u * VS_BATCH) {
     uint32_t key[VS_BATCH];
-#pragma unroll
-    for (uint32_t i = 0; i < VS_BATCH; i++) {
-      const uint32_t w = w0 + i * 256u + tid;
-      key[i] = u;
-      if (w < wb) {
-        // the last centre of the chunk starting at or before w
-        uint32_t l = 0, h = 255;
-        while (l < h) {
-          const uint32_t mid = (l + h + 1) >> 1;
-          if (sh.P[mid] <= w) l = mid; else h = mid - 1;
-        }
-        key[i] = a.edgO[sh.vs[l] + w];
-      }
-    }
+    path_keys<256, VS_BATCH>(sh.P, sh.vs, a.edgO, w0 + tid, 256u, wb, u, key);
+    // unconditional loads (key[i] is u where there is no path): the VS_BATCH loads are in
+    // flight together, where a guarded load would wait for each in turn
     uint8_t live[VS_BATCH];
 #pragma unroll
-    for (uint32_t i = 0; i < VS_BATCH; i++) live[i] = key[i] != u ? a.state[key[i]] : 0;
+    for (uint32_t i = 0; i < VS_BATCH; i++) live[i] = a.state[key[i]];
 #pragma unroll
     for (uint32_t i = 0; i < VS_BATCH; i++)
-      if (live[i] == 1 && !agg_insert(L, key[i])) atomicAdd(row + key[i], 1u);
+      if (key[i] != u && live[i] == 1 && !agg_insert(L, key[i])) atomicAdd(row + key[i], 1u);
   }
 }
 
@@ -784,8 +804,10 @@ __global__ __launch_bounds__(256) void k_vpeel_round(VPeel a) {
         const int64_t hi = lo + (int64_t(1) << a.bs_log) < a.ns ? lo + (int64_t(1) << a.bs_log) : a.ns;
         for (int64_t x0 = lo; x0 < hi && n < FR_CAP; x0 += 256) {
           const int64_t xx = x0 + tid;
-          bool sel = false;
-          if (xx < hi) sel = a.state[xx] == 1 && a.D[xx] == k;
+          const int64_t xc = xx < hi ? xx : hi - 1;
+          const uint8_t s0 = a.state[xc];
+          const unsigned long long d0 = a.D[xc];
+          const bool sel = (xx < hi) & (s0 == 1) & (d0 == k);
           const uint64_t bm = __ballot(sel);
           __syncthreads();
           if (lane == 0) s_wc[wid] = (uint32_t)__popcll(bm);
@@ -865,7 +887,7 @@ __global__ __launch_bounds__(256) void k_vpeel_round(VPeel a) {
     __syncthreads();  // LDS of the previous task
     if (task < nlarge) {
       const uint32_t u = f_list[FR_CAP - 1u - task];
-      vwalk_item<V_LARGE_TEAM, V_LARGE_CAP, false>(a, lds.l, u, a.rec[u], k, tid);
+      vwalk_item<V_LARGE_TEAM, V_LARGE_CAP, false>(a, lds.l, u, a.rec[u], tid);
     } else if (task < nlarge + nut) {
       const uint32_t t = task - nlarge;
       uint32_t q0 = (uint32_t)((uint64_t)nunits * t / nut), q1 = (uint32_t)((uint64_t)nunits * (t + 1) / nut);
@@ -881,7 +903,7 @@ __global__ __launch_bounds__(256) void k_vpeel_round(VPeel a) {
       const uint32_t i = (task - nlarge - nut) * 4u + wid;
       if (i < nsmall) {
         const uint32_t u = f_list[i];
-        vwalk_item<V_SMALL_TEAM, V_SMALL_CAP, true>(a, lds.s[wid], u, a.rec[u], k, lane);
+        vwalk_item<V_SMALL_TEAM, V_SMALL_CAP, true>(a, lds.s[wid], u, a.rec[u], lane);
       }
     }
   }
