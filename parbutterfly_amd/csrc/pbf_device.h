@@ -229,6 +229,21 @@ __device__ __forceinline__ uint64_t pbf_match_key6(uint32_t k, uint64_t active) 
   return m;
 }
 
+// Makes the compiler wait for x here (an empty asm that reads it): placed after a batch of
+// loads and before the atomics that depend on them, so that no wait lands between atomics
+// (on this ISA a wait for a load also waits for every atomic issued before it).
+__device__ __forceinline__ void pbf_ready(uint32_t x) { asm volatile("" ::"v"(x)); }
+
+template <typename T>
+__device__ __forceinline__ T pbf_wave_max(T x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const T y = __shfl_xor(x, d, 64);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
 // Inclusive wave scan (64 lanes).
 template <typename T>
 __device__ __forceinline__ T pbf_wave_incl_scan(T x) {

@@ -19,6 +19,7 @@ namespace {
 constexpr int KEYS = 33;  // key values 0..32
 constexpr uint32_t RL_CHUNK = 1024;  // lists longer than this are relabelled in chunks
 constexpr uint32_t RL_SMALL = 16;    // lists up to this long are relabelled by one thread each
+constexpr uint32_t RL_REG = 4;       // chunks of 64 entries of a list that the relabel keeps in registers
 
 // Adds a 256-thread workgroup's values into *acc with one atomic: atomics of every wavefront
 // on one address serialise, thousands of them cost more than the sum they carry.
@@ -137,15 +138,34 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
     if (deg <= RL_SMALL || deg > RL_CHUNK) continue;  // short lists: k_relabel_small; long: k_rl_*
     const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
     const int64_t nbr_shift = xv ? g.nv : 0;  // combined id of neighbour = adj[j] + shift
-    const uint32_t flag = entry_flag(use_v, sflag, x, xv);
     const uint2 rkx = rk[x];
+    // the first RL_REG chunks of 64 entries (the whole list, for most lists) are loaded once, at
+    // indices clamped to the last entry, then their neighbours' rank and key (one 8-byte gather
+    // each), every load of a stage in flight together, and both passes use them from registers;
+    // the chunks of a longer list past those are loaded in each pass
+    const uint32_t nt = (uint32_t)__builtin_amdgcn_readfirstlane((int)((deg + 63u) / 64u));
+    uint32_t nb[RL_REG];
+    uint2 ry[RL_REG];
+#pragma unroll
+    for (uint32_t t = 0; t < RL_REG; t++) {
+      const uint32_t j = t * 64u + lane;
+      nb[t] = adj[j < deg ? j : deg - 1u];
+    }
+    // x's rank arrives first: its list offset is loaded with the gathers
     const uint32_t r = rkx.x;
+    const uint32_t rbase = roff[r];
     const uint32_t lox = lo ? lo[r] : 0u;  // graph-local entries (a union ranked graph by graph)
+#pragma unroll
+    for (uint32_t t = 0; t < RL_REG; t++) ry[t] = rk[nb[t] + nbr_shift];
+    const uint32_t flag = entry_flag(use_v, sflag, x, xv);
     if (lane < KEYS + 1) { cnt[wid][lane] = 0; run[wid][lane] = 0; }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     // pass A: key histogram of the neighbours
-    for (uint32_t j = lane; j < deg; j += 64) atomicAdd(&cnt[wid][rk[adj[j] + nbr_shift].y], 1u);
+#pragma unroll
+    for (uint32_t t = 0; t < RL_REG; t++)
+      if (t * 64u + lane < deg) atomicAdd(&cnt[wid][ry[t].y], 1u);
+    for (uint32_t j = RL_REG * 64u + lane; j < deg; j += 64) atomicAdd(&cnt[wid][rk[adj[j] + nbr_shift].y], 1u);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     // exclusive prefix in descending key order: start[k] = sum_{k' > k} cnt[k']
@@ -159,29 +179,32 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, int use_v, const ui
       // in(r): neighbours of strictly higher key, plus same-key neighbours when x is in U
       inpref[r] = run[wid][kself] + (xv ? 0u : cnt[wid][kself]);
     }
-    const uint32_t rbase = roff[r];
     // pass B: stable scatter, 64 neighbours at a time
-    for (uint32_t j0 = 0; j0 < deg; j0 += 64) {
-      const uint32_t j = j0 + lane;
+    auto scatter = [&](uint32_t t, uint2 v) {
+      const uint32_t j = t * 64u + lane;
       const bool act = j < deg;
-      uint2 ry = make_uint2(0u, 0u);
-      if (act) ry = rk[adj[j] + nbr_shift];  // the neighbour's rank and key, one 8-byte gather
-      const uint32_t ky = ry.y;
+      const uint32_t kj = act ? v.y : 0u;
       const uint64_t active = __ballot(act);
-      const uint64_t mm = pbf_match_key6(ky, active);
+      const uint64_t mm = pbf_match_key6(kj, active);
       const uint32_t rkn = __popcll(mm & pbf_lanemask_lt());
-      uint32_t slot = 0;
       if (act) {
-        slot = rbase + run[wid][ky] + rkn;
-        redg[slot] = ((ry.x - lox) << 1) | flag;
+        const uint32_t slot = rbase + run[wid][kj] + rkn;
+        redg[slot] = ((v.x - lox) << 1) | flag;
         rown[slot] = r;
         if (rpos) rpos[slot] = obase + j;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      if (act && rkn == 0) run[wid][ky] += __popcll(mm);
+      if (act && rkn == 0) run[wid][kj] += __popcll(mm);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
+    };
+#pragma unroll
+    for (uint32_t t = 0; t < RL_REG; t++)
+      if (t < nt) scatter(t, ry[t]);  // uniform
+    for (uint32_t t = RL_REG; t < nt; t++) {
+      const uint32_t j = t * 64u + lane;
+      scatter(t, rk[adj[j < deg ? j : deg - 1u] + nbr_shift]);
     }
   }
 }
@@ -199,29 +222,53 @@ __global__ __launch_bounds__(256) void k_relabel_small(DevGraph g, int use_v, co
     const bool xv = x < g.nv;
     const uint32_t obase = xv ? g.offV[x] : g.offU[x - g.nv];
     const uint32_t deg = (xv ? g.offV[x + 1] : g.offU[x - g.nv + 1]) - obase;
+    // the largest short list of the wavefront (before the long lists leave)
+    const uint32_t dmax = (uint32_t)__builtin_amdgcn_readfirstlane((int)pbf_wave_max(deg <= RL_SMALL ? deg : 0u));
     if (deg > RL_SMALL) continue;
     const uint32_t* adj = xv ? g.edgV + obase : g.edgU + obase;
     const int64_t sh = xv ? g.nv : 0;
-    const uint32_t flag = entry_flag(use_v, sflag, x, xv);
     const uint2 rkx = rk[x];
-    const uint32_t r = rkx.x;
-    const uint32_t lox = lo ? lo[r] : 0u;
-    const uint32_t kself = rkx.y;
-    // each neighbour's rank and key: one 8-byte gather
-    uint32_t yr[RL_SMALL], ky[RL_SMALL];
+    // each neighbour's rank and key (one 8-byte gather): the list's entries first, at indices
+    // clamped to the last one, then the gathers (of x's own record for an empty list), so that
+    // each stage's loads are in flight together; entries past deg are never used
+    // in batches of 4 entries up to the largest list of the wavefront; an empty list reads a
+    // valid word (offV[0]) and gathers x's own record
+    const uint32_t* src = deg ? adj : g.offV;
+    uint32_t nb[RL_SMALL], yr[RL_SMALL], ky[RL_SMALL];
 #pragma unroll
-    for (uint32_t j = 0; j < RL_SMALL; j++) {
-      const uint2 ry = j < deg ? rk[adj[j] + sh] : make_uint2(0u, 0u);
-      yr[j] = ry.x;
-      ky[j] = ry.y;
+    for (uint32_t b = 0; b < RL_SMALL; b += 4) {
+      if (b == 0 || b < dmax) {
+#pragma unroll
+        for (uint32_t j = b; j < b + 4; j++) {
+          const uint32_t v = src[j < deg ? j : (deg ? deg - 1u : 0u)];
+          nb[j] = deg ? v + (uint32_t)sh : (uint32_t)x;
+        }
+      }
     }
+    // x's rank arrives first: its list offset is loaded with the gathers
+    const uint32_t r = rkx.x;
+    const uint32_t rbase = roff[r];
+    const uint32_t lox = lo ? lo[r] : 0u;
+#pragma unroll
+    for (uint32_t b = 0; b < RL_SMALL; b += 4) {
+      if (b == 0 || b < dmax) {
+#pragma unroll
+        for (uint32_t j = b; j < b + 4; j++) {
+          const uint2 ry = rk[nb[j]];
+          yr[j] = ry.x;
+          ky[j] = ry.y;
+        }
+      }
+    }
+    const uint32_t flag = entry_flag(use_v, sflag, x, xv);
+    if (deg == 0) { inpref[r] = 0u; continue; }
+    const uint32_t kself = rkx.y;
     // in(r): neighbours of strictly higher key, plus same-key neighbours when x is in U
     uint32_t in = 0;
 #pragma unroll
     for (uint32_t j = 0; j < RL_SMALL; j++)
       in += (j < deg && (ky[j] > kself || (!xv && ky[j] == kself))) ? 1u : 0u;
     inpref[r] = in;
-    const uint32_t rbase = roff[r];
 #pragma unroll
     for (uint32_t j = 0; j < RL_SMALL; j++) {
       if (j < deg) {
@@ -473,28 +520,71 @@ __global__ __launch_bounds__(256) void k_acore_remove(DevGraph g, const uint32_t
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t n = g.nv + g.nu;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  __shared__ uint32_t s_inc[4][64], s_ob[4][64];
   for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wid) * 64; base < n; base += nwaves * 64) {
-    // each wave scans 64 ids for actives, then walks their lists cooperatively
-    int64_t x = base + lane;
-    bool a = x < n && !done[x] && lvl[x] == k;
-    uint64_t am = __ballot(a);
+    // each wave scans 64 ids for actives, then walks all their lists together: entry e of the
+    // wave's lists belongs to the lane whose inclusive degree prefix first exceeds e
+    const int64_t x = base + lane;
+    const int64_t xc = x < n ? x : n - 1;
+    const uint8_t dn = done[xc], lv = lvl[xc];  // both loads in flight
+    const bool a = (x < n) & (dn == 0) & (lv == k);
+    const uint64_t am = __ballot(a);
     if (a) { done[x] = 1; }
     if (am && lane == 0) atomicAdd(&nrem, (uint32_t)__popcll(am));
-    while (am) {
-      int src = __ffsll((long long)am) - 1;
-      am &= am - 1;
-      int64_t xa = base + src;
-      const bool xv = xa < g.nv;
-      const uint32_t ob = xv ? g.offV[xa] : g.offU[xa - g.nv];
-      const uint32_t d = xv ? g.offV[xa + 1] - ob : g.offU[xa - g.nv + 1] - ob;
-      const uint32_t* adj = xv ? g.edgV + ob : g.edgU + ob;
-      const int64_t sh = xv ? g.nv : 0;
-      for (uint32_t j = lane; j < d; j += 64) {
-        const uint32_t y = (uint32_t)(adj[j] + sh);
-        // unfinished and below k (finished / active vertices have level >= k)
-        if (lvl[y] < k) atomicAdd(&removed[y], 1u);
-      }
+    if (!am) continue;
+    const bool xv = xc < g.nv;
+    const uint64_t vm = __ballot(xv);
+    uint32_t ob = 0, d = 0;
+    if (a) {
+      ob = xv ? g.offV[xc] : g.offU[xc - g.nv];
+      d = (xv ? g.offV[xc + 1] : g.offU[xc - g.nv + 1]) - ob;
     }
+    const uint32_t inc = pbf_wave_incl_scan(d);
+    s_inc[wid][lane] = inc;
+    s_ob[wid][lane] = ob;
+    const uint32_t E = (uint32_t)__shfl((int)inc, 63);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr uint32_t PER = 4;
+    for (uint32_t e0 = 0; e0 < E; e0 += 64u * PER) {
+      uint32_t ec[PER], own[PER], y[PER];
+#pragma unroll
+      for (uint32_t i = 0; i < PER; i++) {
+        const uint32_t e = e0 + i * 64u + lane;
+        ec[i] = e < E ? e : E - 1u;
+        own[i] = 0;
+      }
+      // owner: the number of lanes whose inclusive prefix is at most e (fixed depth, every
+      // path's LDS read of a level in flight together)
+#pragma unroll
+      for (uint32_t step = 32; step; step >>= 1) {
+#pragma unroll
+        for (uint32_t i = 0; i < PER; i++) own[i] = s_inc[wid][own[i] + step - 1u] <= ec[i] ? own[i] + step : own[i];
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < PER; i++) {
+        const uint32_t o = own[i];
+        const uint32_t first = o ? s_inc[wid][o - 1u] : 0u;
+        const bool ov = (vm >> o) & 1ull;
+        const uint32_t* edg = ov ? g.edgV : g.edgU;
+        y[i] = edg[s_ob[wid][o] + (ec[i] - first)] + (ov ? (uint32_t)g.nv : 0u);
+      }
+      uint32_t ly[PER];
+#pragma unroll
+      for (uint32_t i = 0; i < PER; i++) ly[i] = lvl[y[i]];
+      // every level has arrived before the first atomic is issued (a wait placed after an
+      // atomic would wait for the atomic as well)
+#pragma unroll
+      for (uint32_t i = 0; i < PER; i++) pbf_ready(ly[i]);
+#pragma unroll
+      for (uint32_t i = 0; i < PER; i++)
+        // unfinished and below k (finished / active vertices have level >= k)
+        if ((e0 + i * 64u + lane < E) & (ly[i] < k)) atomicAdd(&removed[y[i]], 1u);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the LDS rows are rewritten by the next step
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   __syncthreads();
   if (threadIdx.x == 0 && nrem) atomicSub(&lvl_cnt[k], nrem);
