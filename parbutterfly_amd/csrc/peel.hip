@@ -221,10 +221,6 @@ __global__ void k_update_retire(const uint32_t* __restrict__ touched, uint32_t* 
   }
 }
 
-__device__ __forceinline__ void touch(uint32_t x, uint32_t* tflag, uint32_t* touched, uint32_t* ntouched) {
-  wave_append(atomicExch(&tflag[x], 1u) == 0u, x, touched, ntouched);
-}
-
 // ---------------- vertex (tip) peel ----------------
 // A round is two launches.
 //   k_vpeel_prepare, one workgroup per block of S items (at most 256 blocks of at least 1024):
@@ -995,18 +991,78 @@ struct PeelEArgs {
   const unsigned long long* k;
 };
 
-// Edge f is charged by peeled edge e when it is live and, if peeled too, has a larger index.
-__device__ __forceinline__ bool usable(const uint8_t* state, uint32_t f, uint32_t e) {
-  const uint8_t s = state[f];
-  return s == 1 || (s == 2 && f > e);
-}
-
 // Walk of a peeled edge e = (v, u).  Every butterfly through e is {v, v2} x {u, u2}; it is found
 // either from v (u2 in N(v), v2 in N(u2), test v2 in N(u)) or, symmetrically, from u (v2 in
 // N(u), u2 in N(v2), test u2 in N(v)), whichever end has the smaller two-hop work.  Below, a is
 // the starting end, b the other one: c runs over N(a) \ {b}, d over N(c) \ {a}, and d must lie in
 // N(b) (staged in LDS).  The edge (a, c) loses the butterflies of its segment at once; (c, d)
 // and (b, d) lose one each.
+// Items x of the lanes with add set are appended to the touched list unless already flagged:
+// every lane's flag exchanges are in flight together, then the wavefront takes one slot range.
+// Every lane of the wavefront calls it together.
+template <uint32_t N>
+__device__ __forceinline__ void touch_n(const uint32_t (&x)[N], const bool (&add)[N], uint32_t* tflag, uint32_t* touched,
+                                        uint32_t* ntouched) {
+  uint32_t old[N];
+#pragma unroll
+  for (uint32_t i = 0; i < N; i++) old[i] = add[i] ? atomicExch(&tflag[x[i]], 1u) : 1u;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < N; i++) cnt += old[i] == 0u ? 1u : 0u;
+  const uint32_t incl = pbf_wave_incl_scan(cnt);
+  const uint32_t tot = (uint32_t)__shfl((int)incl, 63, 64);
+  if (tot == 0u) return;
+  uint32_t base = 0;
+  if (__lane_id() == 0) base = atomicAdd(ntouched, tot);
+  base = (uint32_t)__shfl((int)base, 0, 64) + incl - cnt;
+#pragma unroll
+  for (uint32_t i = 0; i < N; i++)
+    if (old[i] == 0u) touched[base++] = x[i];
+}
+
+// The number of entries of the sorted list arr[0, n) that are at most d (binary lifting from the
+// largest power of two top <= n; every probe of a level in flight together for the PER
+// searches), and whether d is among them (the last accepted probe equals d).
+template <uint32_t PER, typename Arr>
+__device__ __forceinline__ void count_le(const Arr& arr, uint32_t n, uint32_t top, const uint32_t (&d)[PER],
+                                         uint32_t (&pos)[PER], bool (&hit)[PER]) {
+  uint32_t last[PER];
+#pragma unroll
+  for (uint32_t i = 0; i < PER; i++) { pos[i] = 0; last[i] = 0xFFFFFFFFu; }
+  for (uint32_t step = top; step; step >>= 1) {
+    uint32_t v[PER];
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+      const uint32_t q = pos[i] + step - 1u;
+      v[i] = arr[q < n ? q : n - 1u];
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+      if (pos[i] + step <= n && v[i] <= d[i]) { pos[i] += step; last[i] = v[i]; }
+    }
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < PER; i++) hit[i] = pos[i] > 0u && last[i] == d[i];
+}
+
+struct GlobalList {
+  const uint32_t* p;
+  __device__ __forceinline__ uint32_t operator[](uint32_t q) const { return p[q]; }
+};
+struct LdsList {
+  const uint32_t* p;
+  __device__ __forceinline__ uint32_t operator[](uint32_t q) const { return p[q]; }
+};
+
+// Walk of a peeled edge e = (v, u).  Every butterfly through e is {v, v2} x {u, u2}; it is found
+// either from v (u2 in N(v), v2 in N(u2), test v2 in N(u)) or, symmetrically, from u (v2 in
+// N(u), u2 in N(v2), test u2 in N(v)), whichever end has the smaller two-hop work.  Below, a is
+// the starting end, b the other one: c runs over N(a) \ {b}, d over N(c) \ {a}, and d must lie in
+// N(b) (staged in LDS when it fits).  The edge (a, c) loses the butterflies of its segment at
+// once; (c, d) and (b, d) lose one each.  A thread takes EW_PER paths at a time, and each stage
+// (the path's entry and edge id, the search of N(b), the two edges' states, the touched flags)
+// has all of its loads in flight together.
+constexpr uint32_t EW_PER = 4;
 template <int TEAM, int UCAP>
 __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   __shared__ TeamShared<TEAM> sh;
@@ -1040,7 +1096,9 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     const uint32_t* idY = from_s ? a.idO : a.idS;
     const uint32_t ao = offX[va], adeg = offX[va + 1] - ao;
     const uint32_t bo = offY[vb], bdeg = offY[vb + 1] - bo;
+    if (bdeg == 0) continue;  // no butterfly (uniform; e itself lies in N(b), so this never happens)
     const bool cached = bdeg <= (uint32_t)UCAP;
+    const uint32_t top = 1u << (31 - __clz(bdeg));  // the largest power of two <= bdeg
     if (cached) {
       for (uint32_t q = tid; q < (1u << BM_LOG) / 32; q += TEAM) nb_bits[q] = 0;
       __syncthreads();
@@ -1061,11 +1119,13 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
     for (uint32_t c0 = 0; c0 < adeg; c0 += TEAM) {
       const uint32_t j = c0 + tid;
       uint32_t L = 0, so = 0, fc = 0;
+      uint8_t sc = 0;
       if (j < adeg) {
         const uint32_t q = ao + j;  // edge (a, c)
         const uint32_t c = nbrX[q];
         fc = idX[q];
-        if (c != vb && usable(a.state, fc, e)) {
+        sc = a.state[fc];
+        if (c != vb && (sc == 1 || (sc == 2 && fc > e))) {
           so = offY[c];
           L = offY[c + 1] - so;
         }
@@ -1076,47 +1136,91 @@ __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
       sh.vs[tid] = so;
       sh.acc[tid] = 0;
       __syncthreads();
-      for (uint32_t w = tid; w < T; w += TEAM) {
-        if (*found >= target) break;
-        const uint32_t s = seg_of<TEAM>(sh.P, w);
-        const uint32_t kk = w - (s ? sh.P[s - 1] : 0u);
-        const uint32_t p2 = sh.vs[s] + kk;
-        const uint32_t d = nbrY[p2];
-        if (d == va) continue;
-        // edge (b, d): d in N(b)?  And edge (c, d) still there?
-        uint32_t lo = 0, hi = bdeg, fd;
-        uint8_t sd;
-        if (cached) {
-          const uint32_t h = (d * 0x9E3779B1u) >> (32 - BM_LOG);
-          if (!((nb_bits[h >> 5] >> (h & 31)) & 1u)) continue;
-          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nb_v[mid] < d) lo = mid + 1; else hi = mid; }
-          if (lo == bdeg || nb_v[lo] != d) continue;
-          fd = idY[p2];
-          sd = a.state[fd];
-          if (sd == 0) continue;
-        } else {
-          fd = idY[p2];
-          sd = a.state[fd];
-          if (sd == 0) continue;
-          while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (nbrY[bo + mid] < d) lo = mid + 1; else hi = mid; }
-          if (lo == bdeg || nbrY[bo + lo] != d) continue;
+      for (uint32_t w0 = 0; w0 < T; w0 += TEAM * EW_PER) {
+        // wavefront-uniform early exit (every lane runs the wavefront operations below)
+        if ((uint32_t)__builtin_amdgcn_readfirstlane((int)*found) >= target) break;
+        uint32_t wq[EW_PER], sg[EW_PER], d[EW_PER], fd[EW_PER];
+        bool ok[EW_PER];
+#pragma unroll
+        for (uint32_t i = 0; i < EW_PER; i++) {
+          const uint32_t w = w0 + i * TEAM + tid;
+          ok[i] = w < T;
+          wq[i] = w < T ? w : T - 1u;
+          sg[i] = 0;
         }
-        const uint32_t fb = cached ? nb_f[lo] : idY[bo + lo];
-        const uint8_t sb = a.state[fb];
-        if (sb == 0) continue;
-        atomicAdd((uint32_t*)found, 1u);  // a butterfly through e
-        if (!(sd == 1 || fd > e) || !(sb == 1 || fb > e)) continue;  // charged by a smaller peeled edge
-        atomicAdd(&sh.acc[s], 1ull);
-        if (a.state[fd] == 1) { atomicAdd(&a.B[fd], ~0ull); touch(fd, a.tflag, a.touched, a.ntouched); }
-        if (a.state[fb] == 1) { atomicAdd(&a.B[fb], ~0ull); touch(fb, a.tflag, a.touched, a.ntouched); }
+        // segment of each path: the number of P entries at most w (fixed depth)
+#pragma unroll
+        for (uint32_t step = TEAM / 2; step; step >>= 1) {
+#pragma unroll
+          for (uint32_t i = 0; i < EW_PER; i++) sg[i] = sh.P[sg[i] + step - 1u] <= wq[i] ? sg[i] + step : sg[i];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < EW_PER; i++) {
+          const uint32_t s = sg[i];
+          const uint32_t p2 = sh.vs[s] + wq[i] - (s ? sh.P[s - 1u] : 0u);
+          d[i] = nbrY[p2];
+          fd[i] = idY[p2];
+        }
+        // edge (b, d): d in N(b)?
+        uint32_t pos[EW_PER];
+        bool hit[EW_PER];
+#pragma unroll
+        for (uint32_t i = 0; i < EW_PER; i++) ok[i] = ok[i] & (d[i] != va);
+        if (cached) {
+          bool any = false;
+#pragma unroll
+          for (uint32_t i = 0; i < EW_PER; i++) {
+            const uint32_t h = (d[i] * 0x9E3779B1u) >> (32 - BM_LOG);
+            ok[i] = ok[i] & (((nb_bits[h >> 5] >> (h & 31)) & 1u) != 0u);
+            any |= ok[i];
+          }
+          if (__ballot(any) == 0ull) continue;
+          count_le<EW_PER>(LdsList{nb_v}, bdeg, top, d, pos, hit);
+        } else {
+          count_le<EW_PER>(GlobalList{nbrY + bo}, bdeg, top, d, pos, hit);
+        }
+        uint32_t fb[EW_PER];
+#pragma unroll
+        for (uint32_t i = 0; i < EW_PER; i++) {
+          ok[i] = ok[i] & hit[i];
+          const uint32_t q = pos[i] ? pos[i] - 1u : 0u;
+          fb[i] = cached ? nb_f[q] : idY[bo + q];
+        }
+        // edge (c, d) and edge (b, d) still there?
+        uint32_t sd[EW_PER], sb[EW_PER];
+#pragma unroll
+        for (uint32_t i = 0; i < EW_PER; i++) {
+          sd[i] = a.state[fd[i]];
+          sb[i] = a.state[fb[i]];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < EW_PER; i++) { pbf_ready(sd[i]); pbf_ready(sb[i]); }
+        uint32_t tx[2 * EW_PER];
+        bool tadd[2 * EW_PER];
+#pragma unroll
+        for (uint32_t i = 0; i < EW_PER; i++) {
+          tx[2 * i] = fd[i];
+          tx[2 * i + 1] = fb[i];
+          tadd[2 * i] = false;
+          tadd[2 * i + 1] = false;
+          if (!(ok[i] && sd[i] != 0 && sb[i] != 0)) continue;
+          atomicAdd((uint32_t*)found, 1u);  // a butterfly through e
+          // charged by a smaller peeled edge?
+          if (!(sd[i] == 1 || fd[i] > e) || !(sb[i] == 1 || fb[i] > e)) continue;
+          atomicAdd(&sh.acc[sg[i]], 1ull);
+          if (sd[i] == 1) { atomicAdd(&a.B[fd[i]], ~0ull); tadd[2 * i] = true; }
+          if (sb[i] == 1) { atomicAdd(&a.B[fb[i]], ~0ull); tadd[2 * i + 1] = true; }
+        }
+        touch_n<2 * EW_PER>(tx, tadd, a.tflag, a.touched, a.ntouched);
       }
       __syncthreads();
-      if (j < adeg) {
-        const unsigned long long cnt = sh.acc[tid];
-        if (cnt && a.state[fc] == 1) {
-          atomicAdd(&a.B[fc], (unsigned long long)(-(long long)cnt));
-          touch(fc, a.tflag, a.touched, a.ntouched);
-        }
+      {
+        const unsigned long long cnt = j < adeg ? sh.acc[tid] : 0ull;
+        const bool add = cnt && sc == 1;
+        if (add) atomicAdd(&a.B[fc], (unsigned long long)(-(long long)cnt));
+        const uint32_t tx1[1] = {fc};
+        const bool ta1[1] = {add};
+        touch_n<1>(tx1, ta1, a.tflag, a.touched, a.ntouched);
       }
       __syncthreads();
       if (*found >= target) break;  // uniform: read by every thread after the barrier
