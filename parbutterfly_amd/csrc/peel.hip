@@ -96,49 +96,86 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x)
   return x;
 }
 
-// The round kernels visit items ids[0, n) (every item that may still be live), or 0..n-1
-// when ids is null.
-// Appends x to a list when flag is set, one counter atomic per wavefront (the lanes active at
-// the call take part; the lowest of them reserves the room).  A per-lane atomic on one shared
-// counter would serialise every append of the launch.
-__device__ __forceinline__ void wave_append(bool flag, uint32_t x, uint32_t* list, uint32_t* len) {
-  const uint64_t fm = __ballot(flag);
-  if (fm == 0ull) return;
-  const uint32_t leader = (uint32_t)__ffsll((long long)__ballot(1)) - 1u;
+// Appends the x[i] whose f[i] is set, all N of every lane with one counter atomic per
+// wavefront.  Every lane of the wavefront calls it together.
+template <uint32_t N>
+__device__ __forceinline__ void wave_append_n(const uint32_t (&x)[N], const bool (&f)[N], uint32_t* list, uint32_t* len) {
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < N; i++) cnt += f[i] ? 1u : 0u;
+  const uint32_t incl = pbf_wave_incl_scan(cnt);
+  const uint32_t tot = (uint32_t)__shfl((int)incl, 63, 64);
+  if (tot == 0u) return;
   uint32_t base = 0;
-  if (__lane_id() == leader) base = atomicAdd(len, (uint32_t)__popcll(fm));
-  base = (uint32_t)__shfl((int)base, (int)leader, 64);
-  if (flag) list[base + __popcll(fm & pbf_lanemask_lt())] = x;
+  if (__lane_id() == 0) base = atomicAdd(len, tot);
+  base = (uint32_t)__shfl((int)base, 0, 64) + incl - cnt;
+#pragma unroll
+  for (uint32_t i = 0; i < N; i++)
+    if (f[i]) list[base++] = x[i];
+}
+
+// The round kernels visit items ids[0, n) (every item that may still be live), or 0..n-1
+// when ids is null.  The selection kernels below take SEL_PER items per thread at a time (items p0 + 256 i + tid
+// of a workgroup step), every load of a stage in flight together; a step's loop bound is the
+// same for the whole workgroup, so every lane takes part in the wavefront appends.
+constexpr uint32_t SEL_PER = 4;
+
+// Marks the selected items current (state 2) and lists them by work class; returns how many.
+__device__ __forceinline__ uint32_t select_items(const uint32_t (&x)[SEL_PER], const bool (&in)[SEL_PER],
+                                                 unsigned long long k, const unsigned long long* __restrict__ D,
+                                                 const unsigned long long* __restrict__ B, uint8_t* __restrict__ state,
+                                                 const uint32_t* __restrict__ work, uint32_t small_max,
+                                                 uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
+                                                 uint32_t* len) {
+  uint8_t st[SEL_PER];
+  unsigned long long d[SEL_PER], bx[SEL_PER];
+  uint32_t wk[SEL_PER];
+#pragma unroll
+  for (uint32_t i = 0; i < SEL_PER; i++) {
+    st[i] = state[x[i]];
+    d[i] = D[x[i]];
+    bx[i] = B[x[i]];
+    wk[i] = work[x[i]];
+  }
+  uint32_t nsel = 0;
+  bool fs[SEL_PER], fl[SEL_PER];
+#pragma unroll
+  for (uint32_t i = 0; i < SEL_PER; i++) {
+    const bool sel = in[i] & (st[i] == 1) & (d[i] == k);
+    if (sel) { state[x[i]] = 2; nsel++; }
+    // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
+    const uint32_t w = sel && bx[i] ? (wk[i] & WORK_MASK) : 0u;
+    fs[i] = w != 0u && w <= small_max;
+    fl[i] = w > small_max;
+  }
+  wave_append_n<SEL_PER>(x, fs, small_list, len + 0);
+  wave_append_n<SEL_PER>(x, fl, large_list, len + 1);
+  return nsel;
 }
 
 // Frontier from the previous round's touched items.  After a round at level k every live item
 // has D >= k, and the live items with D == k are all touched ones (an untouched item keeps its
 // D, which was not k or it would have been peeled).  So when a touched item has D == k the new
 // minimum is k again and the frontier is exactly those items; else the full scan below runs.
-__global__ void k_select_touched(const unsigned long long* __restrict__ D, const unsigned long long* __restrict__ B,
-                                 uint8_t* __restrict__ state,
-                                 const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
-                                 uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
-                                 Ctl c) {
-  // control words, state and D are loaded before any of them is tested
+__global__ __launch_bounds__(256) void k_select_touched(const unsigned long long* __restrict__ D,
+                                                        const unsigned long long* __restrict__ B, uint8_t* __restrict__ state,
+                                                        const uint32_t* __restrict__ touched, const uint32_t* __restrict__ work,
+                                                        uint32_t small_max, uint32_t* __restrict__ small_list,
+                                                        uint32_t* __restrict__ large_list, Ctl c) {
   const unsigned long long k = dev_ld(c.kprev);
   const uint32_t nt = dev_ld(c.len + 4);
   if (k == KNONE) return;
   uint32_t nsel = 0;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-    const uint32_t x = touched[t];
-    const uint8_t st = state[x];
-    const unsigned long long d = D[x];
-    const bool sel = st == 1 && d == k;
-    uint32_t w = 0;
-    if (sel) {
-      state[x] = 2;
-      nsel++;
-      // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
-      w = B[x] ? (work[x] & WORK_MASK) : 0u;
+  for (uint32_t t0 = blockIdx.x * 256u * SEL_PER; t0 < nt; t0 += gridDim.x * 256u * SEL_PER) {
+    uint32_t x[SEL_PER];
+    bool in[SEL_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) {
+      const uint32_t t = t0 + 256u * i + threadIdx.x;
+      in[i] = t < nt;
+      x[i] = touched[t < nt ? t : nt - 1u];
     }
-    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
-    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
+    nsel += select_items(x, in, k, D, B, state, work, small_max, small_list, large_list, c.len);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) {
@@ -154,11 +191,20 @@ __global__ __launch_bounds__(256) void k_min_live(const unsigned long long* __re
   __shared__ unsigned long long s_min[4];
   if (dev_ld(c.len + 5)) return;
   unsigned long long mn = KNONE;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t x = ids ? ids[p] : (uint32_t)p;
-    const uint8_t st = state[x];
-    const unsigned long long d = D[x];  // with the state, not after it
-    if (st) mn = d < mn ? d : mn;
+  for (int64_t p0 = (int64_t)blockIdx.x * 256 * SEL_PER; p0 < n; p0 += (int64_t)gridDim.x * 256 * SEL_PER) {
+    uint32_t x[SEL_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) {
+      const int64_t p = p0 + 256 * i + threadIdx.x;
+      const int64_t pc = p < n ? p : n - 1;
+      x[i] = ids ? ids[pc] : (uint32_t)pc;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) {
+      const uint8_t st = state[x[i]];
+      const unsigned long long d = D[x[i]];
+      if (st) mn = d < mn ? d : mn;  // an index past n repeats item n - 1
+    }
   }
   // one atomic per workgroup: per-wavefront atomics on the one word serialise the launch
   mn = wave_min_u64(mn);
@@ -172,29 +218,26 @@ __global__ __launch_bounds__(256) void k_min_live(const unsigned long long* __re
 
 // Frontier selection: live items with D == k become current (state 2) and are listed by work
 // class (small / large by `work`); `live` is decremented by the number selected.
-__global__ void k_select(const unsigned long long* __restrict__ D, const unsigned long long* __restrict__ B,
-                         uint8_t* __restrict__ state,
-                         const uint32_t* __restrict__ ids, int64_t n, const uint32_t* __restrict__ work,
-                         uint32_t small_max, uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list,
-                         Ctl c) {
+__global__ __launch_bounds__(256) void k_select(const unsigned long long* __restrict__ D,
+                                                const unsigned long long* __restrict__ B, uint8_t* __restrict__ state,
+                                                const uint32_t* __restrict__ ids, int64_t n,
+                                                const uint32_t* __restrict__ work, uint32_t small_max,
+                                                uint32_t* __restrict__ small_list, uint32_t* __restrict__ large_list, Ctl c) {
   const unsigned long long k = dev_ld(c.k);
   const uint32_t found = dev_ld(c.len + 5);
   if (k == KNONE || found) return;
   uint32_t nsel = 0;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t x = ids ? ids[p] : (uint32_t)p;
-    const uint8_t st = state[x];
-    const unsigned long long d = D[x];  // with the state, not after it
-    const bool sel = st == 1 && d == k;
-    uint32_t w = 0;
-    if (sel) {
-      state[x] = 2;
-      nsel++;
-      // an item in no remaining butterfly (B == 0) has nothing to decrement: no walk
-      w = B[x] ? (work[x] & WORK_MASK) : 0u;
+  for (int64_t p0 = (int64_t)blockIdx.x * 256 * SEL_PER; p0 < n; p0 += (int64_t)gridDim.x * 256 * SEL_PER) {
+    uint32_t x[SEL_PER];
+    bool in[SEL_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) {
+      const int64_t p = p0 + 256 * i + threadIdx.x;
+      in[i] = p < n;
+      const int64_t pc = p < n ? p : n - 1;
+      x[i] = ids ? ids[pc] : (uint32_t)pc;
     }
-    wave_append(sel && w != 0 && w <= small_max, x, small_list, &c.len[0]);
-    wave_append(sel && w > small_max, x, large_list, &c.len[1]);
+    nsel += select_items(x, in, k, D, B, state, work, small_max, small_list, large_list, c.len);
   }
   nsel = pbf_wave_sum(nsel);
   if ((threadIdx.x & 63) == 0 && nsel) atomicAdd(c.live, (unsigned long long)(-(long long)nsel));
@@ -203,21 +246,45 @@ __global__ void k_select(const unsigned long long* __restrict__ D, const unsigne
 // Touched live items: D = max(B, k) (updateBuckets, butterfly_putils.h:1036-1043).
 // End of a round, one launch for two independent passes: touched items get D[x] = max(B[x], k)
 // (and lose their flag), current items (state 2) die.  A touched item is live, never current.
-__global__ void k_update_retire(const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
-                                const unsigned long long* __restrict__ B, unsigned long long* __restrict__ D,
-                                uint8_t* __restrict__ state, const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
+__global__ __launch_bounds__(256) void k_update_retire(const uint32_t* __restrict__ touched, uint32_t* __restrict__ tflag,
+                                                       const unsigned long long* __restrict__ B,
+                                                       unsigned long long* __restrict__ D, uint8_t* __restrict__ state,
+                                                       const uint32_t* __restrict__ ids, int64_t n, Ctl c) {
   const unsigned long long k = dev_ld(c.k);
   const uint32_t nt = dev_ld(c.len + 3);
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-    const uint32_t x = touched[t];
-    tflag[x] = 0;
-    const unsigned long long b = B[x];
-    D[x] = b > k ? b : k;
+  for (uint32_t t0 = blockIdx.x * 256u * SEL_PER; t0 < nt; t0 += gridDim.x * 256u * SEL_PER) {
+    uint32_t x[SEL_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) {
+      const uint32_t t = t0 + 256u * i + threadIdx.x;
+      x[i] = touched[t < nt ? t : nt - 1u];
+    }
+    unsigned long long bx[SEL_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) bx[i] = B[x[i]];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) {
+      if (t0 + 256u * i + threadIdx.x < nt) {
+        tflag[x[i]] = 0;
+        D[x[i]] = bx[i] > k ? bx[i] : k;
+      }
+    }
   }
   if (k == KNONE) return;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t x = ids ? ids[p] : (uint32_t)p;
-    if (state[x] == 2) state[x] = 0;
+  for (int64_t p0 = (int64_t)blockIdx.x * 256 * SEL_PER; p0 < n; p0 += (int64_t)gridDim.x * 256 * SEL_PER) {
+    uint32_t x[SEL_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) {
+      const int64_t p = p0 + 256 * i + threadIdx.x;
+      const int64_t pc = p < n ? p : n - 1;
+      x[i] = ids ? ids[pc] : (uint32_t)pc;
+    }
+    uint8_t st[SEL_PER];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++) st[i] = state[x[i]];
+#pragma unroll
+    for (uint32_t i = 0; i < SEL_PER; i++)
+      if (st[i] == 2) state[x[i]] = 0;  // repeated items (past n) write the same 0
   }
 }
 
@@ -1062,7 +1129,7 @@ struct LdsList {
 // once; (c, d) and (b, d) lose one each.  A thread takes EW_PER paths at a time, and each stage
 // (the path's entry and edge id, the search of N(b), the two edges' states, the touched flags)
 // has all of its loads in flight together.
-constexpr uint32_t EW_PER = 4;
+constexpr uint32_t EW_PER = 8;
 template <int TEAM, int UCAP>
 __global__ __launch_bounds__(TEAM) void k_peele_walk(PeelEArgs a) {
   __shared__ TeamShared<TEAM> sh;
@@ -1529,7 +1596,7 @@ int pbf_dev_peel_edge(pbf_context* ctx, const DevGraph& g, int use_v, const uint
   uint32_t* large_list = pb.lists + m;
   int64_t entries = m;  // entries of the current view: every edge that may still be live
   auto round = [&]() -> int {
-    const unsigned sel_grid = grid_for(entries, 256, 8192);
+    const unsigned sel_grid = grid_for(entries, 256 * SEL_PER, 8192);
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, ctx->stream, pb.c);
     hipLaunchKernelGGL(k_select_touched, dim3(ctx->num_cus * 4), dim3(256), 0, ctx->stream, pb.D, pb.B, pb.state, pb.touched,
                        pb.work, small_max, small_list, large_list, pb.c);
