@@ -155,6 +155,24 @@ def batched_bench(pb, ctx, count=4096, first_seed=1000, cpu_sample=512):
     res["autotuned_graphs_s"] = round(count / t_tuned, 1)
     res["bit_exact_autotuned_vs_plain"] = same(out_t, out_p)
     res["speedup_vs_plain"] = round(t_plain / t_tuned, 2)
+    # the whole set as one union pass under each selector variant (ranking x knob setting):
+    # what a batch-level choice of variant could gain over the fixed one (variant 3)
+    var_ms, var_exact = [], True
+    for v in range(9):
+        rank, lm, lw, bpc = pb.variant_tuning(v)
+        ctx.set_tuning(light_max=lm, light_work=lw, blocks_per_cu=bpc)
+        ctx.count_batch(dgs, rank, mode="union", outputs=out_u)  # warms this variant's scratch
+        bt = None
+        for _ in range(2):
+            ctx.synchronize()
+            t = time.perf_counter()
+            out_u = ctx.count_batch(dgs, rank, mode="union", outputs=out_u)
+            dt = time.perf_counter() - t
+            bt = dt if bt is None else min(bt, dt)
+        var_ms.append(round(bt * 1e3, 2))
+        var_exact = var_exact and same(out_u, out_p)
+    res["union_ms_by_variant"] = var_ms
+    res["bit_exact_union_variants_vs_plain"] = var_exact
     ctx.set_tuning()
     # batched + learned (pbf_count_batch_learned): features, MLP selection, one union pass per
     # chosen ranking
