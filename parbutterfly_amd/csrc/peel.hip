@@ -1058,12 +1058,6 @@ struct PeelEArgs {
   const unsigned long long* k;
 };
 
-// Walk of a peeled edge e = (v, u).  Every butterfly through e is {v, v2} x {u, u2}; it is found
-// either from v (u2 in N(v), v2 in N(u2), test v2 in N(u)) or, symmetrically, from u (v2 in
-// N(u), u2 in N(v2), test u2 in N(v)), whichever end has the smaller two-hop work.  Below, a is
-// the starting end, b the other one: c runs over N(a) \ {b}, d over N(c) \ {a}, and d must lie in
-// N(b) (staged in LDS).  The edge (a, c) loses the butterflies of its segment at once; (c, d)
-// and (b, d) lose one each.
 // Items x of the lanes with add set are appended to the touched list unless already flagged:
 // every lane's flag exchanges are in flight together, then the wavefront takes one slot range.
 // Every lane of the wavefront calls it together.
