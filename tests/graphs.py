@@ -65,3 +65,16 @@ def hbm_graph(keys, pad):
     v += list(range(1, 1 + leaves))
     u += [i + 1] * leaves
     return pb.Graph.from_edges(1 + leaves, i + 2, np.array(v, np.uint32), np.array(u, np.uint32))
+
+
+def two_hubs_with_leaves(hub, leaves):
+    """V vertices 0 and 1 adjacent to U vertices 0..hub-1; U vertex j also adjacent to `leaves`
+    private V vertices 2 + leaves * j + t (degree 1).  The butterflies are {0, 1} x {j, k}: a hub
+    edge lies in hub - 1 of them, a leaf edge in none."""
+    j = np.arange(hub, dtype=np.uint32)
+    v = [np.zeros(hub, np.uint32), np.ones(hub, np.uint32)]
+    u = [j, j]
+    for t in range(leaves):
+        v.append(2 + leaves * j + t)
+        u.append(j)
+    return pb.Graph.from_edges(2 + leaves * hub, hub, np.concatenate(v).astype(np.uint32), np.concatenate(u).astype(np.uint32))

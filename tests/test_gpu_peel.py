@@ -4,7 +4,7 @@ import pytest
 
 import parbutterfly_amd as pb
 from oracle import oracle as O
-from tests.graphs import complete_bipartite, small_graphs
+from tests.graphs import complete_bipartite, small_graphs, two_hubs_with_leaves
 
 pytestmark = pytest.mark.gpu
 
@@ -32,6 +32,23 @@ def test_peel_closed_form(ctx):
         e = pb.as_u64(ctx.count(g, use_v, "EDGE"))
         wing = pb.as_u64(ctx.peel_edge(g, use_v, e))
         assert (wing == (a - 1) * (b - 1)).all()
+
+
+@pytest.mark.parametrize("hub,leaves", [(300, 2), (3000, 2)])
+def test_wing_walks_unstaged_lists(ctx, hub, leaves):
+    # two V hubs on the same `hub` U vertices, each U vertex with `leaves` private V leaves: a
+    # hub edge is walked from its U end (the smaller two-hop work), so N(b) is a hub's list, too
+    # long to stage in LDS for the wavefront walks (hub 300 > 256) or the workgroup walks
+    # (hub 3000 > 2048); closed form: every hub edge lies in hub - 1 butterflies, a leaf edge in none
+    g = two_hubs_with_leaves(hub, leaves)
+    use_v, _ = ctx.side(g)
+    e = pb.as_u64(ctx.count(g, use_v, "EDGE"))
+    wing = pb.as_u64(ctx.peel_edge(g, use_v, e))
+    assert np.array_equal(wing, O.peel_edge(g, use_v, e))
+    assert sorted(set(wing.tolist())) == [0, hub - 1] and int((wing == hub - 1).sum()) == 2 * hub
+    v = pb.as_u64(ctx.count(g, use_v, "VERT"))
+    tip = pb.as_u64(ctx.peel(g, use_v, v))
+    assert np.array_equal(tip, O.peel_vert(g, use_v, v))
 
 
 def test_peel_medium(ctx):
