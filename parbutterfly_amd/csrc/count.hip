@@ -206,6 +206,8 @@ constexpr uint32_t POOL_WORDS = 8192;             // 32 KB of LDS per workgroup
 constexpr uint32_t D32_MAX = POOL_WORDS;          // dense u32: i < 8192
 constexpr uint32_t D16_MAX = 2 * POOL_WORDS;      // dense u16: i < 16384
 constexpr uint32_t LT_WORDS = POOL_WORDS / NWAVE; // wavefront table: 2048 packed slots
+constexpr uint32_t LG_CLAIM = 2;                  // light groups a wavefront claims at once
+constexpr uint32_t HB_CLAIM = 1;                  // heavy endpoints a workgroup claims at once
 constexpr uint32_t LIGHT_MAX = LT_WORDS / 4 * 3 - 1;  // light endpoint: <= 1535 distinct keys ...
 constexpr uint32_t LIGHT_WORK = 1u << 14;         // ... and at most 16384 wedges
 static_assert(LIGHT_MAX + LIGHT_MAX / 3 + 1 <= LT_WORDS, "a light table must fit its wavefront's slots");
@@ -1198,9 +1200,22 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   const uint32_t nb = *a.nblock;
   uint32_t n1 = 0, n2 = 0;  // thread 0: the next item and the one after it
   HRec r1{};                // thread 0: the next item's record
+  // thread 0 claims items HB_CLAIM at a time, the next batch as soon as the current one is
+  // taken up (one claim per item would serialise on the queue's counter, about 12 ns each)
+  uint32_t cb = 0, cleft = 0, pend = 0;
+  auto take = [&]() -> uint32_t {
+    if (cleft == 0) {
+      cb = pend;
+      cleft = HB_CLAIM;
+      pend = atomicAdd(&a.queues[0], HB_CLAIM);
+    }
+    cleft--;
+    return cb++;
+  };
   if (threadIdx.x == 0) {
-    const uint32_t q0 = atomicAdd(&a.queues[0], 1u);
-    n1 = atomicAdd(&a.queues[0], 1u);
+    pend = atomicAdd(&a.queues[0], HB_CLAIM);
+    const uint32_t q0 = take();
+    n1 = take();
     HRec r{};
     if (q0 < nb) {
       r.i = a.blist[q0];
@@ -1217,7 +1232,7 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
     __syncthreads();  // every thread holds the record before thread 0 stores the next one
     if (idx >= nb) break;
     if (threadIdx.x == 0) {
-      n2 = atomicAdd(&a.queues[0], 1u);
+      n2 = take();  // a workgroup's items increase, so after the first past nb all are
       if (n1 < nb) r1.i = a.blist[n1];
     }
     auto after_pass1 = [&](int stage) {
@@ -1248,10 +1263,22 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   // Software pipeline over groups k, k+1, k+2: while group k runs, the records of k+1 are
   // loaded (after its pass 1) and the bounds of k+2 (at its end), whose claim was issued at
   // its start; the dependent loads of a group's setup thus leave the critical path.
-  uint32_t g = 0, gn = 0;
-  if (lane == 0) { g = atomicAdd(&a.queues[1], 1u); gn = atomicAdd(&a.queues[1], 1u); }
-  g = __shfl(g, 0, 64);
-  gn = __shfl(gn, 0, 64);
+  // Groups are claimed LG_CLAIM at a time, the next batch claimed as soon as the current one is
+  // taken up (its atomic has LG_CLAIM groups to return): same-address atomics serialise at
+  // about 12 ns each, so one claim per group would bound the launch at that rate.
+  uint32_t cb = 0, cleft = LG_CLAIM, pend = 0;
+  if (lane == 0) { cb = atomicAdd(&a.queues[1], LG_CLAIM); pend = atomicAdd(&a.queues[1], LG_CLAIM); }
+  cb = (uint32_t)__shfl((int)cb, 0, 64);
+  auto take = [&]() -> uint32_t {  // wave-uniform
+    if (cleft == 0) {
+      cb = (uint32_t)__shfl((int)pend, 0, 64);
+      cleft = LG_CLAIM;
+      if (lane == 0) pend = atomicAdd(&a.queues[1], LG_CLAIM);
+    }
+    cleft--;
+    return cb++;
+  };
+  uint32_t g = take(), gn = take();
   uint32_t gs = 0, ge = 0, ns = 0, ne = 0;
   uint4 rc = make_uint4(0u, 0u, 0u, 0u);
   if (g < ng) {
@@ -1260,15 +1287,13 @@ __global__ __launch_bounds__(TPB) void k_count2(CountArgs a) {
   }
   if (gn < ng) { ns = a.gstart[gn]; ne = a.gstart[gn + 1]; }
   while (g < ng) {
-    uint32_t claim = 0;
-    if (lane == 0) claim = atomicAdd(&a.queues[1], 1u);
+    const uint32_t gnn = take();  // a wave's groups increase, so after the first past ng all are
     uint4 nrc = make_uint4(0u, 0u, 0u, 0u);
-    uint32_t gnn = 0, nns = 0, nne = 0;
+    uint32_t nns = 0, nne = 0;
     auto at_b = [&]() {
       if (gn < ng && lane < ne - ns) nrc = a.lrec[ns + lane];
     };
     auto at_c = [&]() {
-      gnn = __shfl(claim, 0, 64);
       if (gnn < ng) { nns = a.gstart[gnn]; nne = a.gstart[gnn + 1]; }
     };
     const uint32_t nk = ge - gs;
